@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput of the fused FP64 Langevin step on one MI355X.
+
+Workload (the headline configuration): a 2+1D hydro-like medium on a 200 x 200 grid over
+[-15, 15]^2 fm with 120 tau slices from 0.6 to 12 fm/c (T Gaussian, sigma_x = 3, sigma_y = 4 fm,
+Bjorken-cooled peak T0 = 0.45 GeV, radial flow v_r = tanh(0.08 r (tau - 0.6))), 10^7 quarks
+(70% charm m = 1.5, 30% bottom m = 4.75 GeV) with positions ~ T^4 at tau0, p_T ~ p_T/(1+p_T^2/4)^3
+on [0, 20] GeV and rapidity U[-1, 1]; D_s(2 pi T) = 4, RTA with tau_R = 5 (eta/s)/T, eta/s = 0.16,
+dt = 0.01 fm/c, Philox noise.  One step = one pass of the fused step over all 10^7 quarks.
+
+The state is resident in HBM before the timed region; the timed region is bracketed by device
+synchronisations and timed with HIP events on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "langevinmedium.jl_amd")
+for _p in (PKG, os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
+BYTES_PER_PARTICLE_STEP = 80   # read + write x, y, px, py, pz (FP64)
+
+
+def build_workload(N, seed=3):
+    import numpy as np
+    from lmhip import media
+
+    rng = np.random.default_rng(seed)
+    g = media.hydro_grid(nx=200, ny=200, nt=120, L=15.0, tau0=0.6, tau1=12.0, T0=0.45)
+    nc = int(round(0.7 * N))
+    counts = [nc, N - nc]
+    masses = [1.5, 4.75]
+    pos = media.hydro_positions(N, rng)
+    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], masses[0], rng),
+                                       media.pt_rapidity_momenta(counts[1], masses[1], rng)]))
+    return g, masses, counts, pos, mom
+
+
+def tables_for(g, masses, DsT, pdim):
+    import numpy as np
+    from lmhip import media, transport
+
+    eps = np.finfo(float).eps
+    Tmin = max(float(np.min(g.T)), 0.0)
+    Tmax = max(float(np.max(g.T)), Tmin + eps)
+    out = []
+    for m in masses:
+        drag = transport.build_tau_drag_spline(m, DsT, Tmin=Tmin, Tmax=Tmax, nT=1024)
+        rta = transport.build_relaxation_spline(media.tau_R_law(0.16), Tmin=Tmin, Tmax=Tmax, nT=1024)
+        jt = transport.build_juttner_invcdf(m, pdim, Tmin=Tmin, Tmax=Tmax, nT=256, nU=2048, np_=8192)
+        out.append((drag, rta, jt))
+    return out
+
+
+def make_params(DsT, noise_fixed=False):
+    from lmhip import _ffi
+    return _ffi.make_params(dt=0.01, t0=0.6, DsT=DsT, seed=3, stream=0,
+                            collision=_ffi.LMH_LANGEVIN_RTA,
+                            noise=_ffi.LMH_NOISE_FIXED if noise_fixed else _ffi.LMH_NOISE_PHILOX,
+                            check_finite=False)
+
+
+def cpu_baseline(g, masses, tabs, DsT, budget_s=12.0, n=20000):
+    """The oracle (C restatement of the reference CPU kernels, one pass per kernel, 1 thread) on
+    a bounded sample of the same workload: n quarks of the same mix, the Philox noise record it
+    reads generated inside the timed loop, as the reference draws randn every step."""
+    import ctypes
+    import numpy as np
+    import _oracle as O
+    from lmhip import _ffi
+
+    rng = np.random.default_rng(99)
+    nc = int(round(0.7 * n))
+    counts = [nc, n - nc]
+    from lmhip import media
+    pos = media.hydro_positions(n, rng)
+    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], masses[0], rng),
+                                       media.pt_rapidity_momenta(counts[1], masses[1], rng)]))
+    keep = O.Keep()
+    gm = O.grid_medium(keep, g)
+    sp = O.species(keep, masses, counts, drag=[t[0] for t in tabs], rta=[t[1] for t in tabs],
+                   jt=[(t[2][0], t[2][1], t[2][2], t[2][3], t[2][4]) for t in tabs])
+    p = make_params(DsT, noise_fixed=True)
+    C = O.lib().lmo_noise_channels(ctypes.byref(p), 2, 3)
+    rec = np.zeros((1, C, n))
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        O.lib().lmo_noise_record(ctypes.byref(p), 2, 3, n, 1 + steps, O.dp(rec[0]))
+        rc, _ = O.run_grid(p, gm, sp, pos, mom, 1 + steps, 1, rec)
+        assert rc == 0
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} quarks (70% c / 30% b) x {steps} steps of the same workload, oracle "
+                      f"(C restatement of kernels_cpu.jl, one pass per kernel), 1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--spl", type=int, default=1, help="steps per kernel launch")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--slice", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the step kernel from rocprofv3 counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(particles shard with no per-step exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import lmhip
+    from lmhip import _ffi, media
+
+    N = args.n
+    DsT = media.DsT_from_2piT(4.0)
+    t_setup = time.perf_counter()
+    g, masses, counts, pos, mom = build_workload(N)
+    tabs = tables_for(g, masses, DsT, 3)
+    params = make_params(DsT)
+    ctx = lmhip.Context(0)
+    ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice)
+    ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
+    for k, (drag, rta, jt) in enumerate(tabs):
+        ctx.set_time_table(_ffi.LMH_TABLE_DRAG, k, *drag)
+        ctx.set_time_table(_ffi.LMH_TABLE_RTA, k, *rta)
+        ctx.set_juttner_table(k, jt[0], jt[1], jt[2], jt[3], jt[4])
+    ctx.set_particles(pos, mom, masses, counts)
+    del pos, mom
+    ctx.boost_to_lab(params, 0)
+    ctx.synchronize()
+    print(f"setup {time.perf_counter() - t_setup:.1f} s; N={N} medium {g.nbytes() / 1e6:.0f} MB", file=sys.stderr, flush=True)
+
+    ctx.run(params, 1, args.warmup, None, args.spl)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    ctx.run(params, 1 + args.warmup, args.steps, None, args.spl)
+    ev1.record(stream)
+    ev1.synchronize()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = ev0.elapsed_time(ev1)
+    bad = ctx.nonfinite_step()
+    if bad >= 0:
+        raise RuntimeError(f"non-finite state at step {bad}")
+    obs = ctx.observables(40, 0.0, 10.0, 0.0, -1)
+    ctx.close()
+
+    launches = math.ceil(args.steps / max(1, args.spl)) * (2 if args.slice and args.spl == 1 else 1)
+    kernel_launches = math.ceil(args.steps / max(1, args.spl))
+    value = N * args.steps / (ms / 1e3)
+    t_launch = ms / 1e3 / kernel_launches
+    bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
+    achieved = bytes_per_launch / t_launch / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": args.traffic,
+            "kernel": "k_cart_grid<3,PHILOX>", "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(t_launch * 1e3, 4), "steps_per_launch": max(1, args.spl)}
+    out = {
+        "metric": "particle-steps/s at 1e7 quarks, 2+1D medium",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2+1D hydro grid 200x200x120 over [-15,15]^2 fm, tau 0.6-12 fm/c; "
+                               f"{N} quarks (70% charm m=1.5, 30% bottom m=4.75); Langevin D_s(2piT)=4 + "
+                               "RTA tau_R=5(eta/s)/T; dt=0.01 fm/c; Philox noise; FP64",
+                   "quarks": N, "grid": [200, 200, 120], "block": args.block,
+                   "steps_per_launch": max(1, args.spl), "slice": args.slice,
+                   "wall_s_timed": round(wall, 4), "mean_pT": round(obs[1] / obs[0], 6)},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(g, masses, tabs, DsT, budget_s=args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,217 @@
+/*
+ * lmhip.h -- C ABI of the MI355X (gfx950) backend for LangevInMedium.jl's per-particle
+ * Langevin step.
+ *
+ * The library (liblmhip.so) owns the particle state in HBM as FP64 structure-of-arrays, the
+ * medium and the host-built transport tables, and one HIP stream per context.  The reference
+ * reaches its CPU and CUDA kernels through `simulate_ensemble_bulk(::Backend, ...)`; a new
+ * backend type whose method calls these entry points replaces the CUDA driver
+ * (src/simulate_gpu.jl:26-462) and its kernels (src/kernels_gpu.jl, src/kernels_gpu_GC.jl),
+ * and follows the CPU kernels (src/kernels_cpu.jl, src/kernels_cpu_GC.jl) as the numerical
+ * contract.
+ *
+ * Conventions
+ *   - Host arrays are borrowed for the duration of a call only.
+ *   - Matrices use Julia's column-major layout: a (rows, N) particle matrix holds element
+ *     (d, i) at [d + rows*i]; an (n_r, n_t) medium matrix holds (i, j) at [i + n_r*j]; a
+ *     2+1D grid field (nx, ny, nt) holds (ix, iy, it) at [ix + nx*(iy + ny*it)].
+ *   - Every entry point returns LMH_OK (0) or a negative status; lmh_last_error() gives the
+ *     message.  No C++ exception crosses the ABI.
+ *   - A context is not thread-safe; use one per host thread (the reference runs one Julia task).
+ */
+#ifndef LMHIP_H
+#define LMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define LMH_OK            0
+#define LMH_E_INVALID    -1  /* an argument is out of range or inconsistent                   */
+#define LMH_E_STATE      -2  /* call order: no medium, particles or table set yet             */
+#define LMH_E_HIP        -3  /* HIP runtime error (message carries hipGetErrorString)         */
+#define LMH_E_NONFINITE  -4  /* NaN/Inf in the state after a step (simulate_cpu.jl:293-303)   */
+#define LMH_E_NOMEM      -5  /* device allocation failed                                      */
+
+/* ---- enumerations ---------------------------------------------------------------------- */
+/* coordinates: Cartesian step (simulate_cpu.jl:215-316) or the general-coordinate (tau, r)
+ * step (simulate_cpu_general_coords.jl:83-126)                                             */
+#define LMH_CARTESIAN          0
+#define LMH_GENERAL_COORDS     1
+
+/* momentum update branch (simulate_cpu.jl:229-278):
+ *   LANGEVIN      forces + update + boost back                     (default :langevin)
+ *   RTA           BGK redraw with probability dt/tau_n + boost back   (collision_mode=:rta)
+ *   FLUID         glue to the flow, no boost back  (momentum_langevin=false or DsT == 0)
+ *   LANGEVIN_RTA  the Langevin kick followed by the BGK redraw, then the boost back; an
+ *                 extension with no reference counterpart (the reference's modes are
+ *                 exclusive)                                                              */
+#define LMH_LANGEVIN           0
+#define LMH_RTA                1
+#define LMH_FLUID              2
+#define LMH_LANGEVIN_RTA       3
+
+/* noise source: FIXED reads a host- or device-supplied record per step; PHILOX generates the
+ * same record in registers from Philox4x64-10 keyed by (seed, stream), counter
+ * (particle, step, block, 0).                                                               */
+#define LMH_NOISE_FIXED        0
+#define LMH_NOISE_PHILOX       1
+
+/* integrator of the Ornstein-Uhlenbeck kick:
+ *   PREPOINT   exact OU with the pre-point drag (kernels_cpu.jl:329-346)
+ *   MIDPOINT   drift-midpoint drag (kernels_gpu.jl:368-381, integrator_mode=1)
+ *   POSTPOINT  drag evaluated at the updated momentum by three fixed-point sweeps
+ *              (extension; reduces to PREPOINT when the drag does not depend on p)         */
+#define LMH_PREPOINT           0
+#define LMH_MIDPOINT           1
+#define LMH_POSTPOINT          2
+
+/* drag model:
+ *   TABLE    eta_D = 1/tau_drag(T) from the host-built table, times m/E when relativistic
+ *            (kernels_cpu.jl:281-290,339)
+ *   GAMMA_P  eta(p,T) = gamma0*T^2/(1+p^2/Lambda^2) with kappa = 2*m*T*gamma0*T^2
+ *            (extension)                                                                  */
+#define LMH_DRAG_TABLE         0
+#define LMH_DRAG_GAMMA_P       1
+
+/* time tables built on the host by the reference's transport.jl builders */
+#define LMH_TABLE_DRAG         0   /* build_tau_drag_spline (transport.jl:208-209)          */
+#define LMH_TABLE_RTA          1   /* build_taun_current_spline (transport.jl:221-222), or
+                                      any relaxation-time law on the same uniform T grid    */
+
+#define LMH_MAX_SPECIES        4
+
+typedef struct lmh_params {
+    double   dt;                  /* Δt [fm/c]                                               */
+    double   t0;                  /* initial_time; the kick of step s samples s*dt + t0      */
+    double   DsT;                 /* D_s·T: gates the drag (kernels_cpu.jl:270,281) and the
+                                     RTA time (kernels_cpu.jl:488); position diffusion uses it */
+    double   psi2;                /* Ψ₂ of the optional v₂ flow modulation (kernels_cpu.jl:86) */
+    double   gamma0;              /* LMH_DRAG_GAMMA_P coefficient [fm^-1 GeV^-2]             */
+    double   Lambda;              /* LMH_DRAG_GAMMA_P momentum scale [GeV]                   */
+    uint64_t seed;                /* Philox key word 0                                       */
+    uint64_t stream;              /* Philox key word 1                                       */
+    int32_t  coords;              /* LMH_CARTESIAN | LMH_GENERAL_COORDS                      */
+    int32_t  collision;           /* LMH_LANGEVIN | LMH_RTA | LMH_FLUID | LMH_LANGEVIN_RTA    */
+    int32_t  noise;               /* LMH_NOISE_FIXED | LMH_NOISE_PHILOX                      */
+    int32_t  integrator;          /* LMH_PREPOINT | LMH_MIDPOINT | LMH_POSTPOINT             */
+    int32_t  drag_model;          /* LMH_DRAG_TABLE | LMH_DRAG_GAMMA_P                       */
+    int32_t  relativistic;        /* Lorentz vs Galilean kinematics (simulate.jl:54-61)      */
+    int32_t  bjorken_redshift;    /* p_z <- p_z*tau_a/(tau_a+dt) (kernels_cpu.jl:427-435)   */
+    int32_t  position_diffusion;  /* overdamped position noise (kernels_cpu.jl:553-565,591-607) */
+    int32_t  reflecting_boundary; /* wall at r_max = last(xgrid) (kernels_cpu.jl:574-581,609-634) */
+    int32_t  check_finite;        /* 1: flag the first step whose state is non-finite        */
+} lmh_params;
+
+/* Fixed-noise record of one step: a (C, N) row-major block of doubles, channel c of
+ * particle i at [c*N + i]; a run of nsteps reads (nsteps, C, N).  Channels, in order:
+ *   xi[pdim]            momentum kick normals            (LANGEVIN, LANGEVIN_RTA; the GC
+ *                                                         step uses one channel, for p^r)
+ *   u_col, u_samp       collision decision and |p*| quantile, U[0,1)   (RTA, LANGEVIN_RTA)
+ *   g[pdim]             raw Gaussian direction of the redraw           (RTA, LANGEVIN_RTA)
+ *   xpos[dims]          position-diffusion normals                     (position_diffusion)
+ * lmh_noise_channels() returns C for a configuration.                                      */
+int32_t lmh_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim);
+
+/* observables over the current state (no reference counterpart: the reference returns raw
+ * histories, simulate_cpu.jl:318-322).  out receives, in order:
+ *   [0] N_sel  [1] sum p_T  [2] sum p_T^2  [3] sum cos(2 phi)  [4] sum p_T*cos(2 phi)
+ *   [5 .. 5+nbins)  p_T histogram on [pt_min, pt_max), equal bins, or its Gaussian-smoothed
+ *                   counterpart when sigma > 0 (each particle adds
+ *                   Phi((hi-pT)/sigma) - Phi((lo-pT)/sigma) to every bin)
+ * cos(2 phi) = (px^2 - py^2)/p_T^2; particles with p_T == 0 add 0 to it.  species < 0
+ * selects all species.                                                                      */
+typedef struct lmh_obs_spec {
+    int32_t nbins;
+    int32_t species;
+    double  pt_min, pt_max;
+    double  sigma;
+} lmh_obs_spec;
+
+/* launch shape, chosen per context (by the host autotuner or by hand) */
+typedef struct lmh_tuning {
+    int32_t block;             /* threads per block: 64, 128 or 256 (default 256)               */
+    int32_t steps_per_launch;  /* steps a particle stays in registers when lmh_run gets 0 (1)  */
+    int32_t slice;             /* 2+1D grid, one-step launches: 1 interpolates the medium in tau
+                                  once per step into a slice the particles read (default 0)    */
+    int32_t _pad;
+} lmh_tuning;
+
+typedef struct lmh_ctx lmh_ctx;
+
+/* ---- lifetime ---------------------------------------------------------------------------- */
+int         lmh_create(lmh_ctx** out, int32_t device);
+void        lmh_destroy(lmh_ctx* ctx);
+const char* lmh_last_error(const lmh_ctx* ctx);
+/* the context's HIP stream (a hipStream_t), for events and host synchronisation */
+void*       lmh_stream(lmh_ctx* ctx);
+int         lmh_synchronize(lmh_ctx* ctx);
+
+int         lmh_set_tuning(lmh_ctx* ctx, const lmh_tuning* t);
+int         lmh_get_tuning(lmh_ctx* ctx, lmh_tuning* t);
+
+/* ---- medium ------------------------------------------------------------------------------ */
+/* Radial (r, t) medium of the reference: T, v and the optional v2 are (nr, nt) matrices on
+ * sorted grids (kernels_cpu.jl:22-51, simulate.jl:21-27).  v2 may be NULL.                  */
+int lmh_set_medium_radial(lmh_ctx* ctx,
+                          const double* rgrid, int32_t nr,
+                          const double* tgrid, int32_t nt,
+                          const double* T, const double* v, const double* v2);
+
+/* 2+1D (x, y, tau) medium (extension): uniform x and y nodes x0 + k*dx, sorted tgrid, fields
+ * T, vx, vy as (nx, ny, nt) column-major.  Queries are clamped to the grid.                */
+int lmh_set_medium_grid2p1(lmh_ctx* ctx,
+                           double x0, double dx, int32_t nx,
+                           double y0, double dy, int32_t ny,
+                           const double* tgrid, int32_t nt,
+                           const double* T, const double* vx, const double* vy);
+
+/* ---- host-built transport tables (transport.jl:165-196,277-365) -------------------------- */
+/* uniform-grid linear time table: node k holds the time at Tmin + k/invdT                   */
+int lmh_set_time_table(lmh_ctx* ctx, int32_t which, int32_t species,
+                       double Tmin, double invdT, const double* vals, int32_t n);
+/* inverse-CDF Jüttner table, invcdf[jT*nU + iu] = p*(u_iu; T_jT) (transport.jl:268-270)   */
+int lmh_set_juttner_table(lmh_ctx* ctx, int32_t species,
+                          const double* invcdf, int32_t nU, int32_t nT,
+                          double Tmin, double invdT);
+
+/* ---- particles --------------------------------------------------------------------------- */
+/* pos (dims, N) and mom (pdim, N), Julia layout.  Species are contiguous segments:
+ * counts[k] particles of mass masses[k], in order; sum(counts) == N.
+ * Cartesian: (dims, pdim) in {(1,1),(2,2),(3,3),(2,3)}, the check_momentum_dims contract
+ * (utils.jl:561-573).  General coordinates: dims == pdim == 2, rows (tau, r), (p^tau, p^r). */
+int lmh_set_particles(lmh_ctx* ctx,
+                      const double* pos, int32_t dims,
+                      const double* mom, int32_t pdim,
+                      int64_t N, int32_t nspecies,
+                      const int64_t* counts, const double* masses);
+int lmh_get_particles(lmh_ctx* ctx, double* pos, double* mom);
+
+/* ---- stepping ---------------------------------------------------------------------------- */
+/* The driver's initial boost of the LRF momenta to the lab frame at `step`
+ * (simulate_cpu.jl:160-163; step 0).  Cartesian only.                                       */
+int lmh_boost_to_lab(lmh_ctx* ctx, const lmh_params* p, int64_t step);
+
+/* Steps step_begin .. step_begin+nsteps-1 of the driver loop (simulate_cpu.jl:215-303 or
+ * simulate_cpu_general_coords.jl:83-115).  noise: NULL for LMH_NOISE_PHILOX, else a host
+ * array (nsteps, C, N).  steps_per_launch: 0 lets the library choose; k >= 1 keeps the state
+ * in registers for k steps per kernel launch (the results do not depend on k).             */
+int lmh_run(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+            const double* noise, int32_t steps_per_launch);
+/* as lmh_run, with the fixed-noise record already in device memory                         */
+int lmh_run_device_noise(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin,
+                         int64_t nsteps, const double* d_noise, int32_t steps_per_launch);
+/* step of the first non-finite state seen since the last lmh_set_particles, or -1           */
+int64_t lmh_nonfinite_step(lmh_ctx* ctx);
+
+/* ---- observables ------------------------------------------------------------------------- */
+/* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */
+int lmh_observables(lmh_ctx* ctx, const lmh_obs_spec* spec, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LMHIP_H */

@@ -1,0 +1,1406 @@
+// lmhip.hip -- fused FP64 Langevin step kernels for gfx950 and the C ABI of liblmhip.so.
+//
+// One thread owns one particle.  The state lives in HBM as structure-of-arrays rows (x, y[, z],
+// px, py[, pz]), read once and written once per launch; a launch may keep a particle in
+// registers for several steps.  The reference runs each step as 4-6 separate passes over all
+// particles (src/simulate_cpu.jl:215-303, src/simulate_gpu.jl:283-392); here the whole step is
+// one pass:
+//     medium sample -> boost to the LRF -> [Bjorken redshift] -> Langevin kick | BGK redraw |
+//     glue to the flow -> boost back -> position push -> non-finite check.
+// Noise is either a fixed record read from HBM (for per-particle parity with the CPU kernels)
+// or Philox4x64-10 generated in registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lmhip_device.h"
+
+using namespace lmh;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct SpeciesArgs {
+    double m;
+    TimeTable drag;
+    TimeTable rta;
+    JuttnerTable jt;
+};
+
+// Species are contiguous particle segments; one launch covers all of them: block b belongs to
+// the segment whose first block is the largest block0[k] <= b.
+struct StepArgs {
+    double* pos[3];
+    double* mom[3];
+    int64_t N;                  // particles (the noise row stride)
+    int64_t step_begin;
+    int32_t nsteps;
+    int32_t C, oxi, orta, opos; // fixed-noise record layout
+    int32_t nseg;
+    int64_t begin[LMH_MAX_SPECIES + 1];
+    int64_t block0[LMH_MAX_SPECIES + 1];
+    const double* noise;
+    unsigned long long* bad;    // first non-finite step (atomicMin), ULLONG_MAX if none
+    lmh_params p;
+    SpeciesArgs sp[LMH_MAX_SPECIES];
+    double vmax;                // sqrt(1 - 1e-12), kernels_cpu.jl:102
+    double sqrt_dt;
+};
+
+// this block's species segment and this thread's particle (-1 past the segment's end)
+__device__ __forceinline__ int64_t particle_of_thread(const StepArgs& a, int* seg) {
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < LMH_MAX_SPECIES; ++j)
+        if (j < a.nseg && (int64_t)blockIdx.x >= a.block0[j]) k = j;
+    *seg = k;
+    const int64_t i = a.begin[k] + ((int64_t)blockIdx.x - a.block0[k]) * blockDim.x + threadIdx.x;
+    return i < a.begin[k + 1] ? i : -1;
+}
+
+struct RadialMedium {
+    const double* xg;
+    const double* tg;
+    const double* T;
+    const double* V;
+    const double* V2;
+    int32_t nx, nt;
+    double r_axis_eps, r_max;   // kernels_cpu.jl:528-534
+};
+
+struct GridMedium {
+    const double4* nodes;       // (T, vx, vy, 0) at ix + nx*(iy + ny*it)
+    const double4* slice;       // the medium at one time, (T, vx, vy, 0) at ix + nx*iy
+    const double* tg;
+    int32_t nx, ny, nt;
+    double x0, inv_dx, xlast;
+    double y0, inv_dy, ylast;
+};
+
+// ---------------------------------------------------------------------------------------------
+// noise of one particle-step (the record layout of include/lmhip.h)
+template <int DIMS, int PDIM, int NOISE>
+__device__ __forceinline__ void fetch_noise(const StepArgs& a, int64_t i, int64_t step, int s_off,
+                                            double* xi, double& ucol, double& usamp, double* g,
+                                            double* xp) {
+    if (NOISE == LMH_NOISE_FIXED) {
+        const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
+        if (a.oxi >= 0) {
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) xi[d] = rec[(int64_t)(a.oxi + d) * a.N];
+        }
+        if (a.orta >= 0) {
+            ucol = rec[(int64_t)a.orta * a.N];
+            usamp = rec[(int64_t)(a.orta + 1) * a.N];
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
+        }
+        if (a.opos >= 0) {
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) xp[d] = rec[(int64_t)(a.opos + d) * a.N];
+        }
+    } else {
+        uint64_t w[4];
+        const uint64_t k0 = a.p.seed, k1 = a.p.stream;
+        if (a.oxi >= 0) {
+            philox4x64((uint64_t)i, (uint64_t)step, 0, 0, k0, k1, w);
+            double n0, n1;
+            box_muller(w[0], w[1], &n0, &n1);
+            xi[0] = n0;
+            if (PDIM >= 2) xi[1 % PDIM] = n1;
+            if (PDIM >= 3) {
+                double n2, n3;
+                box_muller(w[2], w[3], &n2, &n3);
+                xi[2 % PDIM] = n2;
+            }
+        }
+        if (a.orta >= 0) {
+            philox4x64((uint64_t)i, (uint64_t)step, 1, 0, k0, k1, w);
+            ucol = u_co(w[0]);
+            usamp = u_co(w[1]);
+            double g0, g1;
+            box_muller(w[2], w[3], &g0, &g1);
+            g[0] = g0;
+            if (PDIM >= 2) g[1 % PDIM] = g1;
+        }
+        if ((a.orta >= 0 && PDIM == 3) || a.opos >= 0) {
+            philox4x64((uint64_t)i, (uint64_t)step, 2, 0, k0, k1, w);
+            double b0, b1, b2, b3;
+            box_muller(w[0], w[1], &b0, &b1);
+            box_muller(w[2], w[3], &b2, &b3);
+            if (PDIM == 3) g[2 % PDIM] = b0;
+            xp[0] = b1;
+            if (DIMS >= 2) xp[1 % DIMS] = b2;
+            if (DIMS >= 3) xp[2 % DIMS] = b3;
+        }
+    }
+}
+
+// kernel_boost_to_rest_frame_cpu! (sign -1, kernels_cpu.jl:89-145) and
+// kernel_boost_to_lab_frame_cpu! (sign +1, :188-241) for one particle with r >= eps
+template <int DIMS, int PDIM, bool RADIAL>
+__device__ __forceinline__ void radial_boost(double* q, const double* x, double r, double v,
+                                             double m, int sign, int relativistic, double vmax) {
+    if (!relativistic) {
+        if (RADIAL) {
+            q[0] = sign > 0 ? q[0] + m * v : q[0] - m * v;
+        } else {
+            double s0 = m * v * (x[0] / r);
+            double s1 = m * v * (x[1 % DIMS] / r);
+            q[0] = sign > 0 ? q[0] + s0 : q[0] - s0;
+            q[1 % PDIM] = sign > 0 ? q[1 % PDIM] + s1 : q[1 % PDIM] - s1;
+        }
+        return;
+    }
+    v = jl_clamp(v, -vmax, vmax);
+    double v2 = v * v;
+    double gam = 1.0 / sqrt(1.0 - v2 + 1e-10);
+    double p2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
+    double E = sqrt(p2 + m * m);
+    if (RADIAL) {
+        q[0] = sign > 0 ? gam * (q[0] + v * E) : gam * (q[0] - v * E);
+    } else {
+        double rhatx = x[0] / r, rhaty = x[1 % DIMS] / r;
+        double px = q[0], py = q[1 % PDIM];
+        double p_par = px * rhatx + py * rhaty;
+        double p_perp_x = px - p_par * rhatx;
+        double p_perp_y = py - p_par * rhaty;
+        double p_par_new = sign > 0 ? gam * (p_par + v * E) : gam * (p_par - v * E);
+        q[0] = p_par_new * rhatx + p_perp_x;
+        q[1 % PDIM] = p_par_new * rhaty + p_perp_y;
+    }
+}
+
+__device__ __forceinline__ void bjorken(const lmh_params& p, int64_t step, double* q2) {
+    // kernel_bjorken_redshift_cpu!, kernels_cpu.jl:427-435
+    double tau_a = p.t0 + (double)(step - 1) * p.dt;
+    if (tau_a > 0.0) {
+        double f = tau_a / (tau_a + p.dt);
+        *q2 *= f;
+    }
+}
+
+__device__ __forceinline__ bool finite_all(const double* a, int n) {
+    bool ok = true;
+    for (int d = 0; d < n; ++d) ok = ok && isfinite(a[d]);
+    return ok;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cartesian step on the radial (r, t) medium: the reference's own medium (simulate_cpu.jl).
+template <int DIMS, int PDIM, int NOISE>
+__global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium med) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs& sp = a.sp[seg];
+    constexpr bool RADIAL = DIMS == 1;
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    double x[DIMS], q[PDIM];
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) x[d] = a.pos[d][i];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    int64_t bad = -1;
+    for (int s_off = 0; s_off < a.nsteps; ++s_off) {
+        const int64_t step = a.step_begin + s_off;
+        const double t_now = (double)step * p.dt + p.t0;
+        double yd;
+        const int j = axis_cell(med.tg, med.nt, t_now, &yd);
+        double xi[PDIM], g[PDIM], xp[DIMS];
+        double ucol = 1.0, usamp = 0.0;
+        fetch_noise<DIMS, PDIM, NOISE>(a, i, step, s_off, xi, ucol, usamp, g, xp);
+
+        double r2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) r2 += x[d] * x[d];
+        const double r = sqrt(r2);
+        double xd;
+        const int ir = axis_cell(med.xg, med.nx, r, &xd);
+        const bool skip = r < kEps;
+        double v = 0.0;
+        if (!skip) {
+            v = bilinear(med.V, med.nx, ir, j, xd, yd);
+            if (DIMS >= 2 && med.V2 != nullptr) {
+                double v2m = bilinear(med.V2, med.nx, ir, j, xd, yd);
+                double phi = atan2(x[1 % DIMS], x[0]);
+                v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - p.psi2)));
+            }
+            radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, -1, p.relativistic, a.vmax);
+        }
+        if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
+
+        if (p.collision == LMH_FLUID) {
+            // kernel_set_to_fluid_velocity_cpu!, kernels_cpu.jl:683-733: r from rows 1-2
+            double rf = RADIAL ? x[0] : sqrt(x[0] * x[0] + x[1 % DIMS] * x[1 % DIMS]);
+            if (!(rf < kEps)) {
+                double xdf;
+                int irf = axis_cell(med.xg, med.nx, rf, &xdf);
+                double vf = bilinear(med.V, med.nx, irf, j, xdf, yd);
+                if (!p.relativistic) {
+                    if (RADIAL) {
+                        q[0] = m * vf;
+                    } else {
+                        q[0] = m * vf * (x[0] / rf);
+                        q[1 % PDIM] = m * vf * (x[1 % DIMS] / rf);
+                    }
+                } else {
+                    vf = jl_clamp(vf, -a.vmax, a.vmax);
+                    double gam = 1.0 / sqrt(1.0 - vf * vf + 1e-10);
+                    if (RADIAL) {
+                        q[0] = m * gam * vf;
+                    } else {
+                        q[0] = m * gam * vf * (x[0] / rf);
+                        q[1 % PDIM] = m * gam * vf * (x[1 % DIMS] / rf);
+                    }
+                }
+            }
+        } else {
+            const double T = jl_max(bilinear(med.T, med.nx, ir, j, xd, yd), 0.0);
+            if (p.collision == LMH_LANGEVIN || p.collision == LMH_LANGEVIN_RTA) {
+                double etaD, kT;
+                drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+                ou_kick<PDIM, RADIAL>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+            }
+            if (p.collision == LMH_RTA || p.collision == LMH_LANGEVIN_RTA)
+                rta_redraw<PDIM, RADIAL>(p, sp.rta, sp.jt, T, q, ucol, usamp, g);
+            if (!skip) radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, +1, p.relativistic, a.vmax);
+        }
+
+        // kernel_update_positions_cpu!, kernels_cpu.jl:535-636
+        double E2 = m * m;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
+        const double E = p.relativistic ? sqrt(E2) : m;
+        if (RADIAL) {
+            double r0 = x[0];
+            double pr = q[0];
+            double r_abs = fabs(r0);
+            double r_safe = (r_abs < med.r_axis_eps) ? med.r_axis_eps : r_abs;
+            double dr = (pr / E) * p.dt;
+            if (p.position_diffusion) {
+                double xdd;
+                int ird = axis_cell(med.xg, med.nx, r_safe, &xdd);
+                double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
+                double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
+                if (D > 0.0) dr += (D / r_safe) * p.dt + sqrt(2.0 * D * p.dt) * xp[0];
+            }
+            double r_new = r0 + dr;
+            if (r_new < 0.0) {
+                r_new = -r_new;
+                q[0] = -q[0];
+            }
+            if (p.reflecting_boundary && med.r_max > 0.0 && r_new > med.r_max) {
+                r_new = 2.0 * med.r_max - r_new;
+                q[0] = -q[0];
+                if (r_new < 0.0) {
+                    r_new = -r_new;
+                    q[0] = -q[0];
+                }
+            }
+            x[0] = r_new;
+        } else {
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) x[d] = x[d] + p.dt * q[d % PDIM] / E;
+            if (p.position_diffusion) {
+                double rr = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIMS; ++d) rr += x[d] * x[d];
+                double r_now = sqrt(rr);
+                double xdd;
+                int ird = axis_cell(med.xg, med.nx, r_now, &xdd);
+                double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
+                double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
+                if (D > 0.0) {
+                    double sigma = sqrt(2.0 * D * p.dt);
+#pragma unroll
+                    for (int d = 0; d < DIMS; ++d) x[d] = x[d] + sigma * xp[d];
+                }
+            }
+            if (p.reflecting_boundary && med.r_max > 0.0) {
+                double rr = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIMS; ++d) rr += x[d] * x[d];
+                double r_now = sqrt(rr);
+                if (r_now > med.r_max) {
+                    double invr = 1.0 / jl_max(r_now, med.r_axis_eps);
+                    double ppar = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIMS; ++d) ppar += q[d % PDIM] * (x[d] * invr);
+#pragma unroll
+                    for (int d = 0; d < DIMS; ++d) q[d % PDIM] = q[d % PDIM] - 2.0 * ppar * (x[d] * invr);
+                    double scale = (2.0 * med.r_max - r_now) * invr;
+#pragma unroll
+                    for (int d = 0; d < DIMS; ++d) x[d] *= scale;
+                }
+            }
+        }
+        if (bad < 0 && !(finite_all(x, DIMS) && finite_all(q, PDIM))) bad = step;
+    }
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) a.pos[d][i] = x[d];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
+}
+
+// The driver's initial boost to the lab at `step` (simulate_cpu.jl:160-163), radial medium
+template <int DIMS, int PDIM>
+__global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialMedium med, int64_t step) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs& sp = a.sp[seg];
+    constexpr bool RADIAL = DIMS == 1;
+    const lmh_params& p = a.p;
+    double x[DIMS], q[PDIM];
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) x[d] = a.pos[d][i];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) r2 += x[d] * x[d];
+    const double r = sqrt(r2);
+    if (r < kEps) return;
+    const double t_now = (double)step * p.dt + p.t0;
+    double yd, xd;
+    const int j = axis_cell(med.tg, med.nt, t_now, &yd);
+    const int ir = axis_cell(med.xg, med.nx, r, &xd);
+    double v = bilinear(med.V, med.nx, ir, j, xd, yd);
+    if (DIMS >= 2 && med.V2 != nullptr) {
+        double v2m = bilinear(med.V2, med.nx, ir, j, xd, yd);
+        double phi = atan2(x[1 % DIMS], x[0]);
+        v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - p.psi2)));
+    }
+    radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, sp.m, +1, p.relativistic, a.vmax);
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2+1D grid medium (extension): sample at (x, y) with the time cell (j, td) of the step.  Each
+// corner is interpolated in tau first, then the corners in x, then in y; SLICE reads corners
+// already interpolated in tau by k_grid_slice, which rounds identically.
+__device__ __forceinline__ int grid_cell(double f, int n) {
+    double fl = floor(f);
+    if (!(fl >= 0.0)) return 0;
+    if (fl > (double)(n - 2)) return n - 2;
+    return (int)fl;
+}
+
+__device__ __forceinline__ double4 lerp4(const double4& a, const double4& b, double td) {
+    double4 c;
+    c.x = a.x * (1.0 - td) + b.x * td;
+    c.y = a.y * (1.0 - td) + b.y * td;
+    c.z = a.z * (1.0 - td) + b.z * td;
+    c.w = 0.0;
+    return c;
+}
+
+template <bool SLICE>
+__device__ __forceinline__ void grid_sample(const GridMedium& g, double x, double y, int j,
+                                            double td, double* T, double* vx, double* vy) {
+    double xc = jl_clamp(x, g.x0, g.xlast);
+    double yc = jl_clamp(y, g.y0, g.ylast);
+    double fx = (xc - g.x0) * g.inv_dx;
+    double fy = (yc - g.y0) * g.inv_dy;
+    int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
+    double xd = fx - (double)ix, yd = fy - (double)iy;
+    int64_t n00 = ix + (int64_t)g.nx * iy;
+    double4 c00, c10, c01, c11;
+    if (SLICE) {
+        c00 = g.slice[n00];
+        c10 = g.slice[n00 + 1];
+        c01 = g.slice[n00 + g.nx];
+        c11 = g.slice[n00 + g.nx + 1];
+    } else {
+        const int64_t nxy = (int64_t)g.nx * g.ny;
+        const double4* s0 = g.nodes + nxy * j + n00;
+        const double4* s1 = s0 + nxy;
+        c00 = lerp4(s0[0], s1[0], td);
+        c10 = lerp4(s0[1], s1[1], td);
+        c01 = lerp4(s0[g.nx], s1[g.nx], td);
+        c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
+    }
+    double a0 = c00.x * (1.0 - xd) + c10.x * xd;
+    double a1 = c01.x * (1.0 - xd) + c11.x * xd;
+    *T = a0 * (1.0 - yd) + a1 * yd;
+    double b0 = c00.y * (1.0 - xd) + c10.y * xd;
+    double b1 = c01.y * (1.0 - xd) + c11.y * xd;
+    *vx = b0 * (1.0 - yd) + b1 * yd;
+    double e0 = c00.z * (1.0 - xd) + c10.z * xd;
+    double e1 = c01.z * (1.0 - xd) + c11.z * xd;
+    *vy = e0 * (1.0 - yd) + e1 * yd;
+}
+
+// boost along v_hat = (vx, vy)/|v| with speed |v| (the reference's r_hat boost with
+// r_hat -> v_hat), skipped when |v| < eps
+template <int PDIM>
+__device__ __forceinline__ void grid_boost(double* q, double m, double vx, double vy, int sign,
+                                           int relativistic, double vmax) {
+    double vm2 = vx * vx + vy * vy;
+    double vmag = sqrt(vm2);
+    if (vmag < kEps) return;
+    double nx = vx / vmag, ny = vy / vmag;
+    if (!relativistic) {
+        double s = sign > 0 ? 1.0 : -1.0;
+        q[0] += s * (m * vmag * nx);
+        q[1] += s * (m * vmag * ny);
+        return;
+    }
+    double v = vmag > vmax ? vmax : vmag;
+    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+    double p2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
+    double E = sqrt(p2 + m * m);
+    double px = q[0], py = q[1];
+    double p_par = px * nx + py * ny;
+    double p_perp_x = px - p_par * nx;
+    double p_perp_y = py - p_par * ny;
+    double p_par_new = sign > 0 ? gam * (p_par + v * E) : gam * (p_par - v * E);
+    q[0] = p_par_new * nx + p_perp_x;
+    q[1] = p_par_new * ny + p_perp_y;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    if (k >= nxy) return;
+    slice[k] = lerp4(g.nodes[nxy * j + k], g.nodes[nxy * (j + 1) + k], td);
+}
+
+template <int PDIM, int NOISE, bool SLICE>
+__global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs& sp = a.sp[seg];
+    constexpr int DIMS = 2;
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    double x0 = a.pos[0][i], x1 = a.pos[1][i];
+    double q[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    int64_t bad = -1;
+    for (int s_off = 0; s_off < a.nsteps; ++s_off) {
+        const int64_t step = a.step_begin + s_off;
+        const double t_now = (double)step * p.dt + p.t0;
+        double td;
+        const int j = axis_cell(g.tg, g.nt, t_now, &td);
+        double xi[PDIM], gg[PDIM], xp[DIMS];
+        double ucol = 1.0, usamp = 0.0;
+        fetch_noise<DIMS, PDIM, NOISE>(a, i, step, s_off, xi, ucol, usamp, gg, xp);
+
+        double Tm, vx, vy;
+        grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
+        grid_boost<PDIM>(q, m, vx, vy, -1, p.relativistic, a.vmax);
+        if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
+        if (p.collision == LMH_FLUID) {
+            double vmag = sqrt(vx * vx + vy * vy);
+            if (vmag < kEps) {
+                q[0] = 0.0;
+                q[1] = 0.0;
+            } else {
+                double nx = vx / vmag, ny = vy / vmag;
+                if (!p.relativistic) {
+                    q[0] = m * vmag * nx;
+                    q[1] = m * vmag * ny;
+                } else {
+                    double v = vmag > a.vmax ? a.vmax : vmag;
+                    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+                    q[0] = m * gam * v * nx;
+                    q[1] = m * gam * v * ny;
+                }
+            }
+        } else {
+            const double T = jl_max(Tm, 0.0);
+            if (p.collision == LMH_LANGEVIN || p.collision == LMH_LANGEVIN_RTA) {
+                double etaD, kT;
+                drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+                ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+            }
+            if (p.collision == LMH_RTA || p.collision == LMH_LANGEVIN_RTA)
+                rta_redraw<PDIM, false>(p, sp.rta, sp.jt, T, q, ucol, usamp, gg);
+            grid_boost<PDIM>(q, m, vx, vy, +1, p.relativistic, a.vmax);
+        }
+        double E2 = m * m;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
+        const double E = p.relativistic ? sqrt(E2) : m;
+        x0 = x0 + p.dt * q[0] / E;
+        x1 = x1 + p.dt * q[1] / E;
+        if (p.position_diffusion) {
+            double Td, ux, uy;
+            grid_sample<SLICE>(g, x0, x1, j, td, &Td, &ux, &uy);
+            double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
+            if (D > 0.0) {
+                double sigma = sqrt(2.0 * D * p.dt);
+                x0 = x0 + sigma * xp[0];
+                x1 = x1 + sigma * xp[1];
+            }
+        }
+        if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
+    }
+    a.pos[0][i] = x0;
+    a.pos[1][i] = x1;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
+}
+
+template <int PDIM>
+__global__ __launch_bounds__(kBlock) void k_boost_lab_grid(StepArgs a, GridMedium g, int64_t step) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs& sp = a.sp[seg];
+    const lmh_params& p = a.p;
+    double q[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    const double t_now = (double)step * p.dt + p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    double Tm, vx, vy;
+    grid_sample<false>(g, a.pos[0][i], a.pos[1][i], j, td, &Tm, &vx, &vy);
+    grid_boost<PDIM>(q, sp.m, vx, vy, +1, p.relativistic, a.vmax);
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+}
+
+// ---------------------------------------------------------------------------------------------
+// general-coordinate (tau, r) step: kernels_cpu_GC.jl:64-102 (boosts, regulators 1e-10 and
+// 1e-8), :110-174 (forces: Gamma == 0, no m/E, T unclamped), :244-256 (mass shell), :259-274
+// (push); order of simulate_cpu_general_coords.jl:83-115.
+template <int NOISE>
+__global__ __launch_bounds__(kBlock) void k_gc(StepArgs a, RadialMedium med) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs& sp = a.sp[seg];
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    double tau = a.pos[0][i], rr = a.pos[1][i];
+    double pt = a.mom[0][i], pr = a.mom[1][i];
+    int64_t bad = -1;
+    for (int s_off = 0; s_off < a.nsteps; ++s_off) {
+        const int64_t step = a.step_begin + s_off;
+        const double t_now = (double)step * p.dt + p.t0;
+        double xi;
+        if (NOISE == LMH_NOISE_FIXED) {
+            xi = a.noise[(int64_t)s_off * a.N + i];
+        } else {
+            uint64_t w[4];
+            philox4x64((uint64_t)i, (uint64_t)step, 0, 0, p.seed, p.stream, w);
+            double n1;
+            box_muller(w[0], w[1], &xi, &n1);
+        }
+        double yd, xd;
+        const int j = axis_cell_gc(med.tg, med.nt, t_now, &yd);
+        const int ir = axis_cell_gc(med.xg, med.nx, fabs(rr), &xd);
+        const double v = bilinear(med.V, med.nx, ir, j, xd, yd);
+        double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+        double npt = gam * (pt - v * pr);
+        double npr = gam * (pr - v * pt);
+        pt = npt;
+        pr = npr;
+        const double T = bilinear(med.T, med.nx, ir, j, xd, yd);
+        double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
+        double tau_n = DsT_safe == 0.0 ? 0.0 : eval_time_table(sp.drag, T);
+        bool ok = tau_n > 0.0 && isfinite(tau_n);
+        double etaD = ok ? (1.0 / tau_n) : 0.0;
+        double kappa = ok ? ((2.0 * m * T) / tau_n) : 0.0;
+        double kT = sqrt(kappa);
+        double det, sto;
+        if (etaD > 0.0 && p.dt > 0.0) {
+            double aa = exp(-etaD * p.dt);
+            double noise_pref = kT * sqrt((1.0 - aa * aa) / (2.0 * etaD * p.dt));
+            det = (aa - 1.0) * pr + 0.0;
+            sto = noise_pref * xi;
+        } else {
+            det = 0.0;
+            sto = kT * xi;
+        }
+        pr = pr + (det + a.sqrt_dt * sto);
+        pt = sqrt(m * m + pr * pr);
+        double gam2 = 1.0 / sqrt(1.0 - v * v + 1e-8);
+        npt = gam2 * (pt + v * pr);
+        npr = gam2 * (pr + v * pt);
+        pt = npt;
+        pr = npr;
+        const double E = pt;
+        tau = tau + p.dt * pt / E;
+        rr = rr + p.dt * pr / E;
+        if (bad < 0 && !(isfinite(tau) && isfinite(rr) && isfinite(pt) && isfinite(pr))) bad = step;
+    }
+    a.pos[0][i] = tau;
+    a.pos[1][i] = rr;
+    a.mom[0][i] = pt;
+    a.mom[1][i] = pr;
+    if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
+}
+
+// ---------------------------------------------------------------------------------------------
+// observables: per-block partial sums (each thread accumulates its own column in LDS, columns
+// are then summed in a fixed tree), then one block sums the partials in block order.  The
+// result does not depend on scheduling.
+constexpr int kObsBlock = 64;
+constexpr int kObsMaxBins = 120;
+
+__global__ __launch_bounds__(kObsBlock) void k_obs_partial(const double* __restrict__ px,
+                                                            const double* __restrict__ py,
+                                                            int64_t i0, int64_t i1, lmh_obs_spec s,
+                                                            double* __restrict__ partial) {
+    extern __shared__ double acc[];           // [5 + nbins][kObsBlock]
+    const int t = threadIdx.x;
+    const int K = 5 + s.nbins;
+    for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] = 0.0;
+    const double width = (s.pt_max - s.pt_min) / (double)(s.nbins > 0 ? s.nbins : 1);
+    const double inv_w = (double)s.nbins / (s.pt_max - s.pt_min);
+    const double inv_s = s.sigma > 0.0 ? 1.0 / (s.sigma * sqrt(2.0)) : 0.0;
+    for (int64_t i = i0 + (int64_t)blockIdx.x * kObsBlock + t; i < i1;
+         i += (int64_t)gridDim.x * kObsBlock) {
+        double a = px[i], b = py[i];
+        double pt2 = a * a + b * b;
+        double pt = sqrt(pt2);
+        double c2 = pt2 > 0.0 ? (a * a - b * b) / pt2 : 0.0;
+        acc[0 * kObsBlock + t] += 1.0;
+        acc[1 * kObsBlock + t] += pt;
+        acc[2 * kObsBlock + t] += pt2;
+        acc[3 * kObsBlock + t] += c2;
+        acc[4 * kObsBlock + t] += pt * c2;
+        if (s.nbins == 0) continue;
+        if (s.sigma > 0.0) {
+            for (int k = 0; k < s.nbins; ++k) {
+                double lo = s.pt_min + (double)k * width;
+                double hi = lo + width;
+                acc[(5 + k) * kObsBlock + t] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
+            }
+        } else {
+            double f = (pt - s.pt_min) * inv_w;
+            double fl = floor(f);
+            if (fl >= 0.0 && fl < (double)s.nbins) acc[(5 + (int)fl) * kObsBlock + t] += 1.0;
+        }
+    }
+    __syncthreads();
+    for (int stride = kObsBlock / 2; stride > 0; stride >>= 1) {
+        if (t < stride)
+            for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] += acc[k * kObsBlock + t + stride];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < K; ++k) partial[(int64_t)blockIdx.x * K + k] = acc[k * kObsBlock];
+}
+
+__global__ __launch_bounds__(kObsBlock) void k_obs_final(const double* __restrict__ partial, int nblk,
+                                                          int K, double* __restrict__ out) {
+    for (int k = threadIdx.x; k < K; k += kObsBlock) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * K + k];
+        out[k] = s;
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+// host side
+struct lmh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // particles
+    int dims = 0, pdim = 0;
+    int64_t N = 0;
+    int nspecies = 0;
+    int64_t counts[LMH_MAX_SPECIES] = {0, 0, 0, 0};
+    double mass[LMH_MAX_SPECIES] = {0, 0, 0, 0};
+    double* d_state = nullptr;      // (dims + pdim) rows of N doubles
+    unsigned long long* d_bad = nullptr;
+
+    // medium
+    int medium = -1;                // 0 radial, 1 grid
+    double* d_rg = nullptr;
+    double* d_tg = nullptr;
+    std::vector<double> h_tg;       // host copy of the time grid (slice path)
+    double* d_T = nullptr;
+    double* d_V = nullptr;
+    double* d_V2 = nullptr;
+    int nr = 0, nt = 0;
+    double r_axis_eps = 1e-6, r_max = 0.0;
+    double4* d_nodes = nullptr;
+    double4* d_slice = nullptr;
+    int gx = 0, gy = 0;
+    double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
+
+    lmh_tuning tune = {256, 1, 0, 0};
+
+    // tables
+    struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
+    struct JTab { double* d = nullptr; double Tmin = 0, invdT = 1; int nU = 0, nT = 0; };
+    Tab drag[LMH_MAX_SPECIES], rta[LMH_MAX_SPECIES];
+    JTab jt[LMH_MAX_SPECIES];
+
+    // scratch
+    double* d_noise = nullptr;
+    size_t noise_cap = 0;
+    double* d_obs = nullptr;
+    size_t obs_cap = 0;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace {
+
+#define LMH_HIP(ctx, expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return (ctx)->fail(LMH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int upload(lmh_ctx* c, double** dst, const double* src, size_t n) {
+    dfree(*dst);
+    if (hipMalloc((void**)dst, n * sizeof(double)) != hipSuccess) {
+        *dst = nullptr;
+        return c->fail(LMH_E_NOMEM, "device allocation failed");
+    }
+    LMH_HIP(c, hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+    return LMH_OK;
+}
+
+bool sorted(const double* v, int n) {
+    for (int k = 1; k < n; ++k)
+        if (!(v[k] >= v[k - 1])) return false;
+    return true;
+}
+
+int noise_layout(const lmh_params* p, int dims, int pdim, int* oxi, int* orta, int* opos) {
+    int c = 0;
+    *oxi = *orta = *opos = -1;
+    if (p->coords == LMH_GENERAL_COORDS) {
+        *oxi = 0;
+        return 1;
+    }
+    if (p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA) { *oxi = c; c += pdim; }
+    if (p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA) { *orta = c; c += 2 + pdim; }
+    if (p->position_diffusion) { *opos = c; c += dims; }
+    return c;
+}
+
+int64_t species_begin(const lmh_ctx* c, int k) {
+    int64_t b = 0;
+    for (int j = 0; j < k; ++j) b += c->counts[j];
+    return b;
+}
+
+// kernel arguments for all species segments; *nblocks receives the grid size
+StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks) {
+    StepArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < 3; ++d) {
+        a.pos[d] = d < c->dims ? c->d_state + (int64_t)d * c->N : nullptr;
+        a.mom[d] = d < c->pdim ? c->d_state + (int64_t)(c->dims + d) * c->N : nullptr;
+    }
+    a.N = c->N;
+    a.p = *p;
+    a.bad = c->d_bad;
+    a.nseg = c->nspecies;
+    int64_t b = 0, blk = 0;
+    for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
+        a.begin[k] = b;
+        a.block0[k] = blk;
+        if (k < c->nspecies) {
+            b += c->counts[k];
+            blk += (c->counts[k] + block - 1) / block;
+        }
+        a.sp[k].m = c->mass[k];
+        a.sp[k].drag = TimeTable{c->drag[k].d, c->drag[k].Tmin, c->drag[k].invdT, c->drag[k].n};
+        a.sp[k].rta = TimeTable{c->rta[k].d, c->rta[k].Tmin, c->rta[k].invdT, c->rta[k].n};
+        a.sp[k].jt = JuttnerTable{c->jt[k].d, c->jt[k].Tmin, c->jt[k].invdT, c->jt[k].nU, c->jt[k].nT};
+    }
+    a.begin[LMH_MAX_SPECIES] = b;
+    a.block0[LMH_MAX_SPECIES] = blk;
+    // segments past nseg are empty and never selected
+    a.vmax = std::sqrt(1.0 - 1e-12);
+    a.sqrt_dt = std::sqrt(p->dt);
+    a.C = noise_layout(p, c->dims, c->pdim, &a.oxi, &a.orta, &a.opos);
+    *nblocks = (unsigned)blk;
+    return a;
+}
+
+RadialMedium make_radial(const lmh_ctx* c) {
+    RadialMedium m;
+    m.xg = c->d_rg;
+    m.tg = c->d_tg;
+    m.T = c->d_T;
+    m.V = c->d_V;
+    m.V2 = c->d_V2;
+    m.nx = c->nr;
+    m.nt = c->nt;
+    m.r_axis_eps = c->r_axis_eps;
+    m.r_max = c->r_max;
+    return m;
+}
+
+GridMedium make_grid(const lmh_ctx* c) {
+    GridMedium g;
+    g.nodes = c->d_nodes;
+    g.slice = c->d_slice;
+    g.tg = c->d_tg;
+    g.nx = c->gx;
+    g.ny = c->gy;
+    g.nt = c->nt;
+    g.x0 = c->gx0;
+    g.inv_dx = 1.0 / c->gdx;
+    g.xlast = c->gx0 + (double)(c->gx - 1) * c->gdx;
+    g.y0 = c->gy0;
+    g.inv_dy = 1.0 / c->gdy;
+    g.ylast = c->gy0 + (double)(c->gy - 1) * c->gdy;
+    return g;
+}
+
+// host copy of the medium time cell of interpolate_2d_cpu (kernels_cpu.jl:28-45)
+void host_time_cell(const std::vector<double>& tg, double t, int* j, double* td) {
+    int n = (int)tg.size();
+    double tc = t > tg[n - 1] ? tg[n - 1] : (t < tg[0] ? tg[0] : t);
+    int lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        int m = lo + ((hi - lo) >> 1);
+        if (tc < tg[m - 1]) hi = m; else lo = m;
+    }
+    lo = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+    double a = tg[lo - 1], b = tg[lo];
+    double d = b - a;
+    *td = std::fabs(d) < kEps ? 0.0 : (tc - a) / d;
+    *j = lo - 1;
+}
+
+unsigned grid_blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+extern "C" {
+
+int32_t lmh_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
+    int a, b, c;
+    if (!p) return 0;
+    return noise_layout(p, dims, pdim, &a, &b, &c);
+}
+
+int lmh_create(lmh_ctx** out, int32_t device) {
+    if (!out) return LMH_E_INVALID;
+    *out = nullptr;
+    lmh_ctx* c = new (std::nothrow) lmh_ctx();
+    if (!c) return LMH_E_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LMH_E_HIP;
+    }
+    if (hipMalloc((void**)&c->d_bad, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return LMH_E_NOMEM;
+    }
+    unsigned long long none = ULLONG_MAX;
+    if (hipMemcpy(c->d_bad, &none, sizeof(none), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(c->d_bad);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return LMH_E_HIP;
+    }
+    *out = c;
+    return LMH_OK;
+}
+
+void lmh_destroy(lmh_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_state);
+    dfree(c->d_bad);
+    dfree(c->d_rg);
+    dfree(c->d_tg);
+    dfree(c->d_T);
+    dfree(c->d_V);
+    dfree(c->d_V2);
+    dfree(c->d_nodes);
+    dfree(c->d_slice);
+    for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
+        dfree(c->drag[k].d);
+        dfree(c->rta[k].d);
+        dfree(c->jt[k].d);
+    }
+    dfree(c->d_noise);
+    dfree(c->d_obs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lmh_last_error(const lmh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* lmh_stream(lmh_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int lmh_synchronize(lmh_ctx* c) {
+    if (!c) return LMH_E_INVALID;
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
+
+int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const double* tgrid,
+                          int32_t nt, const double* T, const double* v, const double* v2) {
+    if (!c) return LMH_E_INVALID;
+    if (!rgrid || !tgrid || !T || !v || nr < 2 || nt < 2)
+        return c->fail(LMH_E_INVALID, "set_medium_radial: grids need >= 2 nodes and T, v must be given");
+    if (!sorted(rgrid, nr) || !sorted(tgrid, nt))
+        return c->fail(LMH_E_INVALID, "set_medium_radial: grids must be sorted");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    size_t n = (size_t)nr * (size_t)nt;
+    int rc;
+    if ((rc = upload(c, &c->d_rg, rgrid, nr)) || (rc = upload(c, &c->d_tg, tgrid, nt)) ||
+        (rc = upload(c, &c->d_T, T, n)) || (rc = upload(c, &c->d_V, v, n)))
+        return rc;
+    if (v2) {
+        if ((rc = upload(c, &c->d_V2, v2, n))) return rc;
+    } else {
+        dfree(c->d_V2);
+    }
+    dfree(c->d_nodes);
+    dfree(c->d_slice);
+    c->h_tg.assign(tgrid, tgrid + nt);
+    c->nr = nr;
+    c->nt = nt;
+    c->r_axis_eps = std::max(1e-12, 0.5 * std::fabs(rgrid[1] - rgrid[0]));
+    c->r_max = rgrid[nr - 1];
+    c->medium = 0;
+    return LMH_OK;
+}
+
+int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double y0, double dy,
+                           int32_t ny, const double* tgrid, int32_t nt, const double* T,
+                           const double* vx, const double* vy) {
+    if (!c) return LMH_E_INVALID;
+    if (!tgrid || !T || !vx || !vy || nx < 2 || ny < 2 || nt < 2 || !(dx > 0.0) || !(dy > 0.0) ||
+        !std::isfinite(x0) || !std::isfinite(y0))
+        return c->fail(LMH_E_INVALID, "set_medium_grid2p1: need nx, ny, nt >= 2, dx, dy > 0 and all fields");
+    if (!sorted(tgrid, nt)) return c->fail(LMH_E_INVALID, "set_medium_grid2p1: tgrid must be sorted");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    size_t nxy = (size_t)nx * (size_t)ny, n = nxy * (size_t)nt;
+    std::vector<double4> nodes;
+    try {
+        nodes.resize(n);
+    } catch (...) {
+        return c->fail(LMH_E_NOMEM, "set_medium_grid2p1: host allocation failed");
+    }
+    for (size_t k = 0; k < n; ++k) nodes[k] = make_double4(T[k], vx[k], vy[k], 0.0);
+    dfree(c->d_nodes);
+    dfree(c->d_slice);
+    if (hipMalloc((void**)&c->d_nodes, n * sizeof(double4)) != hipSuccess ||
+        hipMalloc((void**)&c->d_slice, nxy * sizeof(double4)) != hipSuccess)
+        return c->fail(LMH_E_NOMEM, "set_medium_grid2p1: device allocation failed");
+    LMH_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    int rc = upload(c, &c->d_tg, tgrid, nt);
+    if (rc) return rc;
+    dfree(c->d_rg);
+    dfree(c->d_T);
+    dfree(c->d_V);
+    dfree(c->d_V2);
+    c->h_tg.assign(tgrid, tgrid + nt);
+    c->gx = nx;
+    c->gy = ny;
+    c->nt = nt;
+    c->gx0 = x0;
+    c->gdx = dx;
+    c->gy0 = y0;
+    c->gdy = dy;
+    c->medium = 1;
+    return LMH_OK;
+}
+
+int lmh_set_time_table(lmh_ctx* c, int32_t which, int32_t species, double Tmin, double invdT,
+                       const double* vals, int32_t n) {
+    if (!c) return LMH_E_INVALID;
+    if (species < 0 || species >= LMH_MAX_SPECIES || (which != LMH_TABLE_DRAG && which != LMH_TABLE_RTA) ||
+        !vals || n < 1)
+        return c->fail(LMH_E_INVALID, "set_time_table: bad species, table kind or size");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    lmh_ctx::Tab& t = which == LMH_TABLE_DRAG ? c->drag[species] : c->rta[species];
+    int rc = upload(c, &t.d, vals, (size_t)n);
+    if (rc) return rc;
+    t.Tmin = Tmin;
+    t.invdT = invdT;
+    t.n = n;
+    return LMH_OK;
+}
+
+int lmh_set_juttner_table(lmh_ctx* c, int32_t species, const double* invcdf, int32_t nU,
+                          int32_t nT, double Tmin, double invdT) {
+    if (!c) return LMH_E_INVALID;
+    if (species < 0 || species >= LMH_MAX_SPECIES || !invcdf || nU < 2 || nT < 2)
+        return c->fail(LMH_E_INVALID, "set_juttner_table: bad species or size");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    lmh_ctx::JTab& t = c->jt[species];
+    int rc = upload(c, &t.d, invcdf, (size_t)nU * (size_t)nT);
+    if (rc) return rc;
+    t.Tmin = Tmin;
+    t.invdT = invdT;
+    t.nU = nU;
+    t.nT = nT;
+    return LMH_OK;
+}
+
+int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double* mom, int32_t pdim,
+                      int64_t N, int32_t nspecies, const int64_t* counts, const double* masses) {
+    if (!c) return LMH_E_INVALID;
+    if (!pos || !mom || N < 1 || dims < 1 || dims > 3 || pdim < 1 || pdim > 3)
+        return c->fail(LMH_E_INVALID, "set_particles: need N >= 1 and 1 <= dims, pdim <= 3");
+    if (nspecies < 1 || nspecies > LMH_MAX_SPECIES || !counts || !masses)
+        return c->fail(LMH_E_INVALID, "set_particles: need 1..4 species with counts and masses");
+    int64_t sum = 0;
+    for (int k = 0; k < nspecies; ++k) {
+        if (counts[k] < 0 || !std::isfinite(masses[k]))
+            return c->fail(LMH_E_INVALID, "set_particles: species counts must be >= 0 and masses finite");
+        sum += counts[k];
+    }
+    if (sum != N) return c->fail(LMH_E_INVALID, "set_particles: species counts must sum to N");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    size_t rows = (size_t)(dims + pdim);
+    std::vector<double> soa;
+    try {
+        soa.resize(rows * (size_t)N);
+    } catch (...) {
+        return c->fail(LMH_E_NOMEM, "set_particles: host allocation failed");
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < dims; ++d) soa[(size_t)d * N + i] = pos[d + (int64_t)dims * i];
+        for (int d = 0; d < pdim; ++d) soa[(size_t)(dims + d) * N + i] = mom[d + (int64_t)pdim * i];
+    }
+    dfree(c->d_state);
+    if (hipMalloc((void**)&c->d_state, rows * (size_t)N * sizeof(double)) != hipSuccess) {
+        c->d_state = nullptr;
+        c->N = 0;
+        return c->fail(LMH_E_NOMEM, "set_particles: device allocation failed");
+    }
+    LMH_HIP(c, hipMemcpy(c->d_state, soa.data(), rows * (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+    unsigned long long none = ULLONG_MAX;
+    LMH_HIP(c, hipMemcpy(c->d_bad, &none, sizeof(none), hipMemcpyHostToDevice));
+    c->dims = dims;
+    c->pdim = pdim;
+    c->N = N;
+    c->nspecies = nspecies;
+    for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
+        c->counts[k] = k < nspecies ? counts[k] : 0;
+        c->mass[k] = k < nspecies ? masses[k] : 0.0;
+    }
+    return LMH_OK;
+}
+
+int lmh_get_particles(lmh_ctx* c, double* pos, double* mom) {
+    if (!c) return LMH_E_INVALID;
+    if (!c->d_state) return c->fail(LMH_E_STATE, "get_particles: no particles set");
+    if (!pos || !mom) return c->fail(LMH_E_INVALID, "get_particles: null output");
+    LMH_HIP(c, hipSetDevice(c->device));
+    size_t rows = (size_t)(c->dims + c->pdim);
+    std::vector<double> soa;
+    try {
+        soa.resize(rows * (size_t)c->N);
+    } catch (...) {
+        return c->fail(LMH_E_NOMEM, "get_particles: host allocation failed");
+    }
+    LMH_HIP(c, hipMemcpyAsync(soa.data(), c->d_state, rows * (size_t)c->N * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t N = c->N;
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < c->dims; ++d) pos[d + (int64_t)c->dims * i] = soa[(size_t)d * N + i];
+        for (int d = 0; d < c->pdim; ++d) mom[d + (int64_t)c->pdim * i] = soa[(size_t)(c->dims + d) * N + i];
+    }
+    return LMH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_run(lmh_ctx* c, const lmh_params* p) {
+    if (!p) return c->fail(LMH_E_INVALID, "null params");
+    if (!c->d_state) return c->fail(LMH_E_STATE, "no particles set");
+    if (c->medium < 0) return c->fail(LMH_E_STATE, "no medium set");
+    if (!(p->dt == p->dt)) return c->fail(LMH_E_INVALID, "dt is NaN");
+    if (p->noise != LMH_NOISE_FIXED && p->noise != LMH_NOISE_PHILOX)
+        return c->fail(LMH_E_INVALID, "unknown noise mode");
+    if (p->collision < LMH_LANGEVIN || p->collision > LMH_LANGEVIN_RTA)
+        return c->fail(LMH_E_INVALID, "unknown collision mode");
+    if (p->integrator < LMH_PREPOINT || p->integrator > LMH_POSTPOINT)
+        return c->fail(LMH_E_INVALID, "unknown integrator");
+    if (p->drag_model != LMH_DRAG_TABLE && p->drag_model != LMH_DRAG_GAMMA_P)
+        return c->fail(LMH_E_INVALID, "unknown drag model");
+    if (p->coords == LMH_GENERAL_COORDS) {
+        if (c->medium != 0) return c->fail(LMH_E_STATE, "general coordinates need the radial medium");
+        if (c->dims != 2 || c->pdim != 2)
+            return c->fail(LMH_E_INVALID, "general coordinates need dims == pdim == 2 (tau, r)");
+        if (p->collision != LMH_LANGEVIN || p->drag_model != LMH_DRAG_TABLE ||
+            p->integrator != LMH_PREPOINT || p->position_diffusion || p->reflecting_boundary ||
+            p->bjorken_redshift)
+            return c->fail(LMH_E_INVALID, "general coordinates support the pre-point Langevin step only");
+    } else if (p->coords == LMH_CARTESIAN) {
+        bool ok = (c->dims == c->pdim) || (c->dims == 2 && c->pdim == 3);
+        if (!ok) return c->fail(LMH_E_INVALID, "momentum_dimensions with dimensions is not supported (utils.jl:561-573)");
+        if (p->bjorken_redshift && c->pdim != 3)
+            return c->fail(LMH_E_INVALID, "bjorken_redshift requires momentum row 3");
+        if (c->medium == 1 && (c->dims != 2 || p->reflecting_boundary))
+            return c->fail(LMH_E_INVALID, "the 2+1D grid needs dims == 2 and no reflecting boundary");
+    } else {
+        return c->fail(LMH_E_INVALID, "unknown coordinates");
+    }
+    bool uses_drag = p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA;
+    bool uses_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
+    for (int k = 0; k < c->nspecies; ++k) {
+        if (c->counts[k] == 0) continue;
+        if (uses_drag && p->drag_model == LMH_DRAG_TABLE && p->DsT > 0.0 && !c->drag[k].d)
+            return c->fail(LMH_E_STATE, "no drag table for species " + std::to_string(k));
+        if (uses_rta && (!c->rta[k].d || !c->jt[k].d))
+            return c->fail(LMH_E_STATE, "RTA needs a relaxation-time table and a Juttner table for species " + std::to_string(k));
+    }
+    return LMH_OK;
+}
+
+template <int DIMS, int PDIM>
+void launch_radial(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int noise) {
+    const int B = c->tune.block;
+    if (noise == LMH_NOISE_FIXED)
+        hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, LMH_NOISE_FIXED>), dim3(nb), dim3(B), 0, c->stream, a, med);
+    else
+        hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, LMH_NOISE_PHILOX>), dim3(nb), dim3(B), 0, c->stream, a, med);
+}
+
+template <int PDIM>
+void launch_grid(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice) {
+    const int B = c->tune.block;
+    if (noise == LMH_NOISE_FIXED) {
+        if (slice) hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_FIXED, true>), dim3(nb), dim3(B), 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_FIXED, false>), dim3(nb), dim3(B), 0, c->stream, a, g);
+    } else {
+        if (slice) hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_PHILOX, true>), dim3(nb), dim3(B), 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_PHILOX, false>), dim3(nb), dim3(B), 0, c->stream, a, g);
+    }
+}
+
+int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+             const double* d_noise, int32_t spl) {
+    const bool grid = c->medium == 1 && p->coords == LMH_CARTESIAN;
+    const int32_t k = spl > 0 ? spl : std::max(1, c->tune.steps_per_launch);
+    const std::vector<double>& tg = c->h_tg;
+    RadialMedium rmed = make_radial(c);
+    GridMedium gmed = make_grid(c);
+    unsigned nb = 0;
+    StepArgs a = make_args(c, p, c->tune.block, &nb);
+    if (nb == 0) return LMH_OK;
+    for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
+        const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        const bool slice = grid && n == 1 && c->tune.slice;
+        if (slice) {
+            int j;
+            double td;
+            const double t_now = (double)(step_begin + s0) * p->dt + p->t0;
+            host_time_cell(tg, t_now, &j, &td);
+            const int64_t nxy = (int64_t)c->gx * c->gy;
+            hipLaunchKernelGGL(k_grid_slice, dim3(grid_blocks(nxy)), dim3(kBlock), 0, c->stream, gmed, c->d_slice, j, td);
+        }
+        a.step_begin = step_begin + s0;
+        a.nsteps = n;
+        a.noise = d_noise ? d_noise + s0 * (int64_t)a.C * c->N : nullptr;
+        if (p->coords == LMH_GENERAL_COORDS) {
+            if (p->noise == LMH_NOISE_FIXED)
+                hipLaunchKernelGGL((k_gc<LMH_NOISE_FIXED>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
+            else
+                hipLaunchKernelGGL((k_gc<LMH_NOISE_PHILOX>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
+        } else if (grid) {
+            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice);
+            else launch_grid<3>(c, a, nb, gmed, p->noise, slice);
+        } else {
+            if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise);
+            else if (c->dims == 2 && c->pdim == 2) launch_radial<2, 2>(c, a, nb, rmed, p->noise);
+            else if (c->dims == 2 && c->pdim == 3) launch_radial<2, 3>(c, a, nb, rmed, p->noise);
+            else launch_radial<3, 3>(c, a, nb, rmed, p->noise);
+        }
+        LMH_HIP(c, hipGetLastError());
+    }
+    if (p->check_finite) {
+        unsigned long long bad = ULLONG_MAX;
+        LMH_HIP(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+        LMH_HIP(c, hipStreamSynchronize(c->stream));
+        if (bad != ULLONG_MAX)
+            return c->fail(LMH_E_NONFINITE, "Breaking simulation due to NaN/Inf at step " + std::to_string((long long)bad));
+    }
+    return LMH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmh_set_tuning(lmh_ctx* c, const lmh_tuning* t) {
+    if (!c) return LMH_E_INVALID;
+    if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
+        (t->slice != 0 && t->slice != 1))
+        return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice in {0, 1}");
+    c->tune = *t;
+    return LMH_OK;
+}
+
+int lmh_get_tuning(lmh_ctx* c, lmh_tuning* t) {
+    if (!c || !t) return LMH_E_INVALID;
+    *t = c->tune;
+    return LMH_OK;
+}
+
+int lmh_boost_to_lab(lmh_ctx* c, const lmh_params* p, int64_t step) {
+    if (!c) return LMH_E_INVALID;
+    int rc = check_run(c, p);
+    if (rc) return rc;
+    if (p->coords != LMH_CARTESIAN) return c->fail(LMH_E_INVALID, "boost_to_lab: Cartesian only");
+    LMH_HIP(c, hipSetDevice(c->device));
+    RadialMedium rmed = make_radial(c);
+    GridMedium gmed = make_grid(c);
+    unsigned nb = 0;
+    StepArgs a = make_args(c, p, c->tune.block, &nb);
+    if (nb == 0) return LMH_OK;
+    const int B = c->tune.block;
+    if (c->medium == 1) {
+        if (c->pdim == 2) hipLaunchKernelGGL((k_boost_lab_grid<2>), dim3(nb), dim3(B), 0, c->stream, a, gmed, step);
+        else hipLaunchKernelGGL((k_boost_lab_grid<3>), dim3(nb), dim3(B), 0, c->stream, a, gmed, step);
+    } else {
+        if (c->dims == 1) hipLaunchKernelGGL((k_boost_lab_radial<1, 1>), dim3(nb), dim3(B), 0, c->stream, a, rmed, step);
+        else if (c->pdim == 2) hipLaunchKernelGGL((k_boost_lab_radial<2, 2>), dim3(nb), dim3(B), 0, c->stream, a, rmed, step);
+        else if (c->dims == 2) hipLaunchKernelGGL((k_boost_lab_radial<2, 3>), dim3(nb), dim3(B), 0, c->stream, a, rmed, step);
+        else hipLaunchKernelGGL((k_boost_lab_radial<3, 3>), dim3(nb), dim3(B), 0, c->stream, a, rmed, step);
+    }
+    LMH_HIP(c, hipGetLastError());
+    return LMH_OK;
+}
+
+int lmh_run(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+            const double* noise, int32_t steps_per_launch) {
+    if (!c) return LMH_E_INVALID;
+    int rc = check_run(c, p);
+    if (rc) return rc;
+    if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
+    if (nsteps == 0) return LMH_OK;
+    LMH_HIP(c, hipSetDevice(c->device));
+    const double* d_noise = nullptr;
+    if (p->noise == LMH_NOISE_FIXED) {
+        if (!noise) return c->fail(LMH_E_INVALID, "fixed-noise mode needs a noise record");
+        int a, b, d;
+        size_t C = (size_t)noise_layout(p, c->dims, c->pdim, &a, &b, &d);
+        size_t n = C * (size_t)nsteps * (size_t)c->N;
+        if (n > c->noise_cap) {
+            LMH_HIP(c, hipStreamSynchronize(c->stream));
+            dfree(c->d_noise);
+            c->noise_cap = 0;
+            if (hipMalloc((void**)&c->d_noise, n * sizeof(double)) != hipSuccess) {
+                c->d_noise = nullptr;
+                return c->fail(LMH_E_NOMEM, "run: noise allocation failed");
+            }
+            c->noise_cap = n;
+        }
+        if (n) {
+            LMH_HIP(c, hipMemcpyAsync(c->d_noise, noise, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            LMH_HIP(c, hipStreamSynchronize(c->stream));
+        }
+        d_noise = c->d_noise;
+    }
+    return run_impl(c, p, step_begin, nsteps, d_noise, steps_per_launch);
+}
+
+int lmh_run_device_noise(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+                         const double* d_noise, int32_t steps_per_launch) {
+    if (!c) return LMH_E_INVALID;
+    int rc = check_run(c, p);
+    if (rc) return rc;
+    if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
+    if (p->noise == LMH_NOISE_FIXED && !d_noise) return c->fail(LMH_E_INVALID, "fixed-noise mode needs a noise record");
+    if (nsteps == 0) return LMH_OK;
+    LMH_HIP(c, hipSetDevice(c->device));
+    return run_impl(c, p, step_begin, nsteps, d_noise, steps_per_launch);
+}
+
+int64_t lmh_nonfinite_step(lmh_ctx* c) {
+    if (!c || !c->d_bad) return -1;
+    unsigned long long bad = ULLONG_MAX;
+    if (hipSetDevice(c->device) != hipSuccess) return -1;
+    if (hipMemcpyAsync(&bad, c->d_bad, sizeof(bad), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    return bad == ULLONG_MAX ? -1 : (int64_t)bad;
+}
+
+int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
+    if (!c) return LMH_E_INVALID;
+    if (!s || !out) return c->fail(LMH_E_INVALID, "observables: null spec or output");
+    if (!c->d_state) return c->fail(LMH_E_STATE, "observables: no particles set");
+    if (c->pdim < 2) return c->fail(LMH_E_INVALID, "observables: need >= 2 momentum rows");
+    if (s->nbins < 0 || s->nbins > kObsMaxBins || (s->nbins > 0 && !(s->pt_max > s->pt_min)))
+        return c->fail(LMH_E_INVALID, "observables: 0 <= nbins <= 120 and pt_max > pt_min");
+    if (s->species >= c->nspecies) return c->fail(LMH_E_INVALID, "observables: species out of range");
+    LMH_HIP(c, hipSetDevice(c->device));
+    int64_t i0 = 0, i1 = c->N;
+    if (s->species >= 0) {
+        i0 = species_begin(c, s->species);
+        i1 = i0 + c->counts[s->species];
+    }
+    const int K = 5 + s->nbins;
+    const int nblk = 1024;
+    size_t need = (size_t)K * (nblk + 1);
+    if (need > c->obs_cap) {
+        LMH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c->d_obs);
+        c->obs_cap = 0;
+        if (hipMalloc((void**)&c->d_obs, need * sizeof(double)) != hipSuccess) {
+            c->d_obs = nullptr;
+            return c->fail(LMH_E_NOMEM, "observables: allocation failed");
+        }
+        c->obs_cap = need;
+    }
+    const double* px = c->d_state + (int64_t)c->dims * c->N;
+    const double* py = px + c->N;
+    size_t shmem = (size_t)K * kObsBlock * sizeof(double);
+    hipLaunchKernelGGL(k_obs_partial, dim3(nblk), dim3(kObsBlock), shmem, c->stream, px, py, i0, i1, *s, c->d_obs);
+    hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(kObsBlock), 0, c->stream, c->d_obs, nblk, K, c->d_obs + (size_t)K * nblk);
+    LMH_HIP(c, hipGetLastError());
+    LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// lmhip_device.h -- device building blocks of the fused Langevin step (gfx950, FP64).
+//
+// Every function restates one piece of the reference's CPU kernels (LangevInMedium.jl,
+// src/kernels_cpu.jl and src/kernels_cpu_GC.jl) in the reference's order of operations, so that
+// the fused kernels reproduce the CPU path to rounding.  The library is compiled with
+// -ffp-contract=off: no multiply-add is fused that the reference rounds twice.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lmhip.h"
+
+namespace lmh {
+
+constexpr double kEps = 2.220446049250313e-16;          // Julia eps()
+constexpr double kFmGeV = 1.0 / 0.197327;                // constants.jl:13,23
+constexpr double kTwoPi = 6.283185307179586;
+
+__device__ __forceinline__ double jl_clamp(double x, double lo, double hi) {
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+__device__ __forceinline__ double jl_max(double a, double b) { return b > a ? b : a; }
+
+// Base.searchsortedlast: last 1-based index k with v[k] <= x (0 if none)
+__device__ __forceinline__ int searchsortedlast(const double* __restrict__ v, int n, double x) {
+    int lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        int m = lo + ((hi - lo) >> 1);
+        if (x < v[m - 1]) hi = m; else lo = m;
+    }
+    return lo;
+}
+
+// Cell and fraction of interpolate_2d_cpu along one axis (kernels_cpu.jl:27-45): clamp the
+// query, searchsortedlast, clamp the cell to [1, n-1], eps-guarded fraction.  Returns the
+// 0-based left node.
+__device__ __forceinline__ int axis_cell(const double* __restrict__ g, int n, double q, double* frac) {
+    double qc = jl_clamp(q, g[0], g[n - 1]);
+    int i = searchsortedlast(g, n, qc);
+    i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
+    double a = g[i - 1], b = g[i];
+    double d = b - a;
+    *frac = fabs(d) < kEps ? 0.0 : (qc - a) / d;
+    return i - 1;
+}
+
+// The same without the query clamp and the eps guard (kernels_cpu_GC.jl:38-51)
+__device__ __forceinline__ int axis_cell_gc(const double* __restrict__ g, int n, double q, double* frac) {
+    int i = searchsortedlast(g, n, q);
+    i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
+    double a = g[i - 1], b = g[i];
+    *frac = (q - a) / (b - a);
+    return i - 1;
+}
+
+// bilinear combination of interpolate_2d_cpu (kernels_cpu.jl:39-50), column-major (nx, nt)
+__device__ __forceinline__ double bilinear(const double* __restrict__ f, int nx, int i, int j,
+                                           double xd, double yd) {
+    const double* c = f + i + (int64_t)nx * j;
+    double v00 = c[0], v10 = c[1];
+    double v01 = c[nx], v11 = c[nx + 1];
+    double c0 = v00 * (1.0 - xd) + v10 * xd;
+    double c1 = v01 * (1.0 - xd) + v11 * xd;
+    return c0 * (1.0 - yd) + c1 * yd;
+}
+
+// eval_tau_n_spline, transport.jl:231-253 (index clamped, fraction not clamped)
+struct TimeTable {
+    const double* vals;
+    double Tmin, invdT;
+    int n;
+};
+
+__device__ __forceinline__ double eval_time_table(const TimeTable& t, double T) {
+    if (t.n < 2) return t.vals[0];
+    if (!isfinite(T)) return 0.0;
+    double u = (T - t.Tmin) * t.invdT;
+    double fu = floor(u);
+    int i;
+    if (!(fu >= 0.0)) i = 1;
+    else if (fu > (double)(t.n - 2)) i = t.n - 1;
+    else i = (int)fu + 1;
+    double w = u - (double)(i - 1);
+    double y0 = t.vals[i - 1];
+    double y1 = t.vals[i];
+    return (1.0 - w) * y0 + w * y1;
+}
+
+// inverse-CDF Jüttner table and its bilinear lookup, kernels_gpu.jl:88-117
+struct JuttnerTable {
+    const double* vals;
+    double Tmin, invdT;
+    int nU, nT;
+};
+
+__device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double u, double T) {
+    double uu = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double fu = uu * (double)(jt.nU - 1);
+    int iu = (int)floor(fu);
+    iu = iu < 0 ? 0 : (iu > jt.nU - 2 ? jt.nU - 2 : iu);
+    double tu = fu - (double)iu;
+    double fT = (T - jt.Tmin) * jt.invdT;
+    double fl = floor(fT);
+    int jT;
+    if (!(fl >= 0.0)) jT = 0;
+    else if (fl > (double)(jt.nT - 2)) jT = jt.nT - 2;
+    else jT = (int)fl;
+    double tT = fT - (double)jT;
+    tT = tT < 0.0 ? 0.0 : (tT > 1.0 ? 1.0 : tT);
+    const double* b0 = jt.vals + (int64_t)jT * jt.nU + iu;
+    const double* b1 = b0 + jt.nU;
+    double v00 = b0[0], v10 = b0[1];
+    double v01 = b1[0], v11 = b1[1];
+    double c0 = v00 * (1.0 - tu) + v10 * tu;
+    double c1 = v01 * (1.0 - tu) + v11 * tu;
+    return c0 * (1.0 - tT) + c1 * tT;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Philox4x64-10 (Salmon et al., SC'11), counter (particle, step, block, 0), key (seed, stream)
+__device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                                           uint64_t k0, uint64_t k1, uint64_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
+        uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c0;
+        uint64_t hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c0);
+        uint64_t lo1 = 0xCA5A826395121157ULL * c2;
+        uint64_t hi1 = __umul64hi(0xCA5A826395121157ULL, c2);
+        uint64_t n0 = hi1 ^ c1 ^ k0;
+        uint64_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+__device__ __forceinline__ double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
+
+__device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
+    double r = sqrt(-2.0 * log(u_oc(wa)));
+    double th = kTwoPi * u_co(wb);
+    double s, c;
+    sincos(th, &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// drag/noise coefficients eta_D and kT from the local (clamped) T: kernels_cpu.jl:270-290, or
+// the GAMMA_P model eta_D = gamma0*T^2, kappa = 2 m T eta_D.
+__device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable& tab, double m,
+                                            double T, double* etaD, double* kT) {
+    if (p.drag_model == LMH_DRAG_GAMMA_P) {
+        double e = p.gamma0 * T * T;
+        *etaD = e;
+        *kT = sqrt(2.0 * m * T * e);
+        return;
+    }
+    double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
+    double tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(tab, T);
+    bool ok = tau > 0.0 && isfinite(tau);
+    *etaD = ok ? (1.0 / tau) : 0.0;
+    double kappa = ok ? ((2.0 * m * T) / tau) : 0.0;
+    *kT = sqrt(kappa);
+}
+
+// effective drag at momentum q: eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
+// or eta_D/(1+|q|^2/Lambda^2) for GAMMA_P
+template <int PDIM>
+__device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, double m, const double* q) {
+    double q2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q2 += q[d] * q[d];
+    if (p.drag_model == LMH_DRAG_GAMMA_P) return etaD / (1.0 + q2 / (p.Lambda * p.Lambda));
+    double E = sqrt(q2 + m * m);
+    return p.relativistic ? etaD * m / E : etaD;
+}
+
+// The LRF momentum kick: kernel_compute_all_forces_cpu! (kernels_cpu.jl:293-352) followed by
+// kernel_update_momenta_LRF_cpu! (:406-417), p += det + sqrt(dt)*sto.  RADIAL selects the Euler
+// form of radial mode (:293-323, sto = kT*xi since kL == kT).  MIDPOINT follows
+// kernels_gpu.jl:368-381; POSTPOINT re-evaluates the drag at the updated momentum by three
+// fixed-point sweeps.
+template <int PDIM, bool RADIAL>
+__device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double kT, double m,
+                                        double sqrt_dt, double* q, const double* xi) {
+    const double dt = p.dt;
+    if (RADIAL) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) s += q[d] * q[d];
+        double pm = sqrt(s);
+        double E_r = sqrt(pm * pm + m * m);
+        double eta_r = p.relativistic ? etaD * m / E_r : etaD;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) {
+            double det = -eta_r * q[d] * dt;
+            double sto = kT * 1.0 * xi[d];
+            q[d] = q[d] + (det + sqrt_dt * sto);
+        }
+        return;
+    }
+    if (p.integrator == LMH_POSTPOINT && etaD > 0.0 && dt > 0.0) {
+        double qk[PDIM], dk[PDIM], sk[PDIM];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qk[d] = q[d];
+        for (int it = 0; it < 3; ++it) {
+            double eta_eff = eta_of<PDIM>(p, etaD, m, qk);
+            double a = exp(-eta_eff * dt);
+            double noise_pref = eta_eff > 0.0 ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) {
+                dk[d] = (a - 1.0) * q[d];
+                sk[d] = noise_pref * xi[d];
+            }
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) qk[d] = q[d] + (dk[d] + sqrt_dt * sk[d]);
+        }
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = qk[d];
+        return;
+    }
+    if (etaD > 0.0) {
+        double eta_eff = eta_of<PDIM>(p, etaD, m, q);
+        if (p.integrator == LMH_MIDPOINT && dt > 0.0) {
+            double a_half = exp(-eta_eff * (0.5 * dt));
+            double qm[PDIM];
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) qm[d] = a_half * q[d];
+            eta_eff = eta_of<PDIM>(p, etaD, m, qm);
+        }
+        double a = exp(-eta_eff * dt);
+        double noise_pref = (dt > 0.0 && eta_eff > 0.0) ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) {
+            double det = (a - 1.0) * q[d];
+            double sto = noise_pref * xi[d];
+            q[d] = q[d] + (det + sqrt_dt * sto);
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) {
+            double sto = kT * xi[d];
+            q[d] = q[d] + (0.0 + sqrt_dt * sto);
+        }
+    }
+}
+
+// BGK redraw in the LRF: kernel_rta_collision_cpu! (kernels_cpu.jl:485-505) with the
+// inverse-CDF draw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701)
+template <int PDIM, bool RADIAL>
+__device__ __forceinline__ void rta_redraw(const lmh_params& p, const TimeTable& taun,
+                                           const JuttnerTable& jt, double T, double* q,
+                                           double ucol, double usamp, const double* g) {
+    double tau_n = (p.DsT > 0.0 && T > 0.0) ? eval_time_table(taun, T) : 0.0;
+    double Pcol = (tau_n > 0.0 && isfinite(tau_n)) ? jl_clamp(p.dt / tau_n, 0.0, 1.0) : 1.0;
+    if (!(ucol < Pcol)) return;
+    double pstar = juttner_lookup(jt, usamp, T);
+    if (RADIAL) {
+        q[0] = g[0] < 0.0 ? -pstar : pstar;
+    } else {
+        double nrm2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) nrm2 += g[d] * g[d];
+        double inv = pstar / sqrt(jl_max(nrm2, 1e-300));
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = g[d] * inv;
+    }
+}
+
+}  // namespace lmh

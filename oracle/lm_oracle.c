@@ -1,0 +1,1279 @@
+/*
+ * lm_oracle.c -- CPU restatement of LangevInMedium.jl's per-particle Langevin step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that the HIP backend is compared against.  Only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it.
+ *
+ * The reference is Julia and cannot run in this environment, so every function below restates
+ * one reference function, in the reference's own order of operations, and cites its file:line
+ * (paths relative to the LangevInMedium.jl repository root).  The step keeps the reference's
+ * structure: one pass over all particles per kernel, with the noise the driver would draw
+ * (`randn`, `rand`) supplied instead as a fixed-noise record.  Built with
+ * -ffp-contract=off so that no multiply-add is fused that Julia would round twice.
+ *
+ * Functions with no reference counterpart (the 2+1D grid medium, species segments, the
+ * LANGEVIN_RTA composite, the GAMMA_P drag, the post-point integrator, Philox noise and the
+ * observables) are build extensions; their definitions are stated where they appear.
+ */
+#include "lm_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define JL_EPS DBL_EPSILON            /* Julia eps() */
+static const double FMGEV = 1.0 / 0.197327;   /* constants.jl:13 (GevInvTofm), :23 (fmGeV) */
+static const double TWO_PI = 6.283185307179586;
+
+double lmo_fmGeV(void) { return FMGEV; }
+
+/* Julia's clamp(x, lo, hi) = x > hi ? hi : (x < lo ? lo : x) */
+static inline double jl_clamp(double x, double lo, double hi) {
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+/* Julia's max/min for the finite, unsigned-zero cases used here */
+static inline double jl_max(double a, double b) { return (b > a) ? b : a; }
+
+/* Base.searchsortedlast on a sorted vector: the last 1-based index k with v[k] <= x, 0 if none. */
+int64_t lmo_searchsortedlast(const double* v, int64_t n, double x) {
+    int64_t lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        int64_t m = lo + ((hi - lo) >> 1);
+        if (x < v[m - 1]) hi = m; else lo = m;
+    }
+    return lo;
+}
+
+static inline int64_t clamp_i64(int64_t i, int64_t lo, int64_t hi) {
+    return i > hi ? hi : (i < lo ? lo : i);
+}
+
+/* interpolate_2d_cpu, kernels_cpu.jl:22-51 (query clamp, searchsortedlast, eps guard) */
+double lmo_interpolate_2d(const double* x, int32_t nx, const double* y, int32_t ny,
+                          const double* values, double xi, double yi) {
+    xi = jl_clamp(xi, x[0], x[nx - 1]);
+    yi = jl_clamp(yi, y[0], y[ny - 1]);
+    int64_t i = lmo_searchsortedlast(x, nx, xi);
+    int64_t j = lmo_searchsortedlast(y, ny, yi);
+    i = clamp_i64(i, 1, nx - 1);
+    j = clamp_i64(j, 1, ny - 1);
+    double x0 = x[i - 1], x1 = x[i];
+    double y0 = y[j - 1], y1 = y[j];
+    double v00 = values[(i - 1) + (int64_t)nx * (j - 1)], v10 = values[i + (int64_t)nx * (j - 1)];
+    double v01 = values[(i - 1) + (int64_t)nx * j], v11 = values[i + (int64_t)nx * j];
+    double dx = x1 - x0;
+    double dy = y1 - y0;
+    double xd = fabs(dx) < JL_EPS ? 0.0 : (xi - x0) / dx;
+    double yd = fabs(dy) < JL_EPS ? 0.0 : (yi - y0) / dy;
+    double c0 = v00 * (1.0 - xd) + v10 * xd;
+    double c1 = v01 * (1.0 - xd) + v11 * xd;
+    return c0 * (1.0 - yd) + c1 * yd;
+}
+
+/* interpolate_2d_cpu of the GC module, kernels_cpu_GC.jl:37-57: no query clamp, no eps guard,
+ * so queries off the grid extrapolate the edge cell linearly. */
+double lmo_interpolate_2d_gc(const double* x, int32_t nx, const double* y, int32_t ny,
+                             const double* values, double xi, double yi) {
+    int64_t i = lmo_searchsortedlast(x, nx, xi);
+    int64_t j = lmo_searchsortedlast(y, ny, yi);
+    i = clamp_i64(i, 1, nx - 1);
+    j = clamp_i64(j, 1, ny - 1);
+    double x0 = x[i - 1], x1 = x[i];
+    double y0 = y[j - 1], y1 = y[j];
+    double v00 = values[(i - 1) + (int64_t)nx * (j - 1)], v10 = values[i + (int64_t)nx * (j - 1)];
+    double v01 = values[(i - 1) + (int64_t)nx * j], v11 = values[i + (int64_t)nx * j];
+    double xd = (xi - x0) / (x1 - x0);
+    double yd = (yi - y0) / (y1 - y0);
+    double c0 = v00 * (1.0 - xd) + v10 * xd;
+    double c1 = v01 * (1.0 - xd) + v11 * xd;
+    return c0 * (1.0 - yd) + c1 * yd;
+}
+
+/* eval_tau_n_spline, transport.jl:231-253: linear, index clamped, fraction NOT clamped. */
+double lmo_eval_time_table(double T, double Tmin, double invdT, const double* vals, int32_t n) {
+    if (n < 2) return vals[0];
+    if (!isfinite(T)) return 0.0;
+    double u = (T - Tmin) * invdT;
+    double fu = floor(u);
+    int64_t i;                                   /* 1-based, clamp(Int(floor(u)) + 1, 1, n-1) */
+    if (!(fu >= 0.0)) i = 1;
+    else if (fu > (double)(n - 2)) i = n - 1;
+    else i = (int64_t)fu + 1;
+    double t = u - (double)(i - 1);
+    double y0 = vals[i - 1];
+    double y1 = vals[i];
+    return (1.0 - t) * y0 + t * y1;
+}
+
+/* tau_drag, transport.jl:132-137: m*DsT/T^2 in GeV^-1, converted to fm. */
+double lmo_tau_drag(double T, double m, double DsT) {
+    if (!isfinite(T) || !isfinite(m) || !isfinite(DsT) || T <= 0.0 || m <= 0.0 || DsT <= 0.0)
+        return 0.0;
+    return m * DsT / (T * T) / FMGEV;
+}
+
+/* effective_DsT, transport.jl:139-156 */
+double lmo_effective_DsT(double T, double DsT, int32_t linear, double slope, double offset,
+                         double Tfo, int32_t quad, double Tref, int32_t* err) {
+    if (err) *err = 0;
+    if (quad) {
+        if (linear || !(Tref > 0.0)) { if (err) *err = 1; return 0.0; }
+        double x = T / Tref;
+        return DsT * x * x;
+    }
+    if (!linear) return DsT;
+    double Tf = jl_max(T, Tfo);
+    return slope * Tf + offset;
+}
+
+/*
+ * Exponentially scaled modified Bessel function K_nu(z)*exp(z), the quantity Bessels.besselkx
+ * returns (transport.jl:77-78,87-88).  Own implementation from the integral representation
+ *     K_nu(z) e^z = int_0^inf exp(-z (cosh t - 1)) cosh(nu t) dt ,   z > 0,
+ * by the trapezoid rule, which converges geometrically for this analytic, doubly-exponentially
+ * decaying integrand; cosh t - 1 = 2 sinh^2(t/2) avoids cancellation near t = 0.  The step is
+ * kept below a quarter of the integrand's width 1/sqrt(z) for large z.
+ */
+double lmo_besselkx(double nu, double z) {
+    if (!(z > 0.0)) return NAN;
+    double h = 1.0 / 32.0;
+    double hz = 0.25 / sqrt(z);
+    if (hz < h) h = hz;
+    double sum = 0.5;                               /* t = 0, half weight */
+    for (int64_t k = 1; k < 2000000; ++k) {
+        double t = (double)k * h;
+        double s = sinh(0.5 * t);
+        double e = -2.0 * z * s * s;
+        double term = 0.5 * (exp(e + nu * t) + exp(e - nu * t));
+        sum += term;
+        if (term < 1e-20 * sum && e < -50.0) break;
+    }
+    return sum * h;
+}
+
+static inline double jl_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+
+/* tau_n_main3, transport.jl:58-104 (z > 50 fast path at :81-85) */
+double lmo_tau_n_main3(double T, double m, double DsT) {
+    if (!isfinite(T) || !isfinite(m) || !isfinite(DsT) || T <= 0.0 || m <= 0.0 || DsT <= 0.0)
+        return 0.0;
+    double z = m / T;
+    if (z <= 0.0) return 0.0;
+    double K2x = lmo_besselkx(2.0, z);
+    double K3x = lmo_besselkx(3.0, z);
+    double denominator = jl_max(fabs(K2x), 1e-300);
+    if (z > 50.0) {
+        double ratio_K3K2 = K3x / denominator * jl_sign(K2x == 0.0 ? 1.0 : K2x);
+        double tau_GeVinv = (DsT / T) * z * ratio_K3K2;
+        double r = tau_GeVinv / FMGEV;
+        return r < 1e20 ? r : 1e20;
+    }
+    double K1x = lmo_besselkx(1.0, z);
+    double K5x = lmo_besselkx(5.0, z);
+    double numerator = 2.0 * K1x - 3.0 * K3x + K5x;
+    double ratio = numerator / denominator * jl_sign(K2x == 0.0 ? 1.0 : K2x);
+    double z3_over_Tm = z * z * z / T;
+    if (!isfinite(z3_over_Tm) || z3_over_Tm > 1e50) z3_over_Tm = 1e50;
+    double tau_GeVinv = (DsT / 48.0) * z3_over_Tm * ratio;
+    if (!isfinite(tau_GeVinv) || fabs(tau_GeVinv) > 1e50) return 1e50 * jl_sign(tau_GeVinv);
+    return tau_GeVinv / FMGEV;
+}
+
+/* _build_time_spline, transport.jl:165-196; which = LMH_TABLE_DRAG uses tau_drag
+ * (build_tau_drag_spline, :208-209), LMH_TABLE_RTA uses tau_n_main3
+ * (build_taun_current_spline, :221-222).  scale is LV_TAUN_SCALE (:9-13,193). */
+int lmo_build_time_spline(int32_t which, double m, double DsT, double Tmin, double Tmax,
+                          int32_t nT, int32_t linear, double slope, double offset, double Tfo,
+                          int32_t quad, double Tref, double scale,
+                          double* Tmin_out, double* invdT_out, double* vals) {
+    if (nT < 2) return LMO_E_INVALID;
+    if (!(Tmin > 0.0)) return LMO_E_INVALID;
+    if (!(Tmax > Tmin)) return LMO_E_INVALID;
+    double dT = (Tmax - Tmin) / (double)(nT - 1);
+    double invdT = 1.0 / dT;
+    for (int32_t i = 0; i < nT; ++i) {
+        double Ti = Tmin + (double)i * dT;
+        int32_t err = 0;
+        double DsT_eff = lmo_effective_DsT(Ti, DsT, linear, slope, offset, Tfo, quad, Tref, &err);
+        if (err) return LMO_E_INVALID;
+        double t = which == LMH_TABLE_DRAG ? lmo_tau_drag(Ti, m, DsT_eff)
+                                           : lmo_tau_n_main3(Ti, m, DsT_eff);
+        vals[i] = t * scale;
+    }
+    *Tmin_out = Tmin;
+    *invdT_out = invdT;
+    return LMO_OK;
+}
+
+/* p^a for the small non-negative integer exponents used here; Julia's Float64^Int is
+ * correctly rounded for a <= 2, which plain products reproduce. */
+static inline double ipow(double p, int a) {
+    if (a == 0) return 1.0;
+    if (a == 1) return p;
+    if (a == 2) return p * p;
+    return p * p * p;
+}
+
+/* build_juttner_invcdf, transport.jl:277-365 */
+int lmo_build_juttner_invcdf(double m, int32_t d, double Tmin, double Tmax, int32_t nT,
+                             int32_t nU, int32_t np, double quantile_cap,
+                             double* invcdf, double* Tmin_out, double* invdT_out) {
+    double M = m;
+    int a = d - 1;
+    if (nT < 2 || nU < 2 || np < 2 || a < 0 || a > 3) return LMO_E_INVALID;
+    double Tmin_f = jl_max(Tmin, 1e-4);
+    double Tmax_f = jl_max(Tmax, Tmin_f * (1.0 + 1e-6));
+    double span = Tmax_f - Tmin_f;
+    Tmin_f = jl_max(Tmin_f - 0.02 * span, 1e-4);
+    Tmax_f = Tmax_f + 0.02 * span;
+    double dT = (Tmax_f - Tmin_f) / (double)(nT - 1);
+    double invdT = 1.0 / dT;
+    double* pgrid = (double*)malloc(sizeof(double) * (size_t)np);
+    double* cdf = (double*)malloc(sizeof(double) * (size_t)np);
+    if (!pgrid || !cdf) { free(pgrid); free(cdf); return LMO_E_INVALID; }
+    for (int32_t jT = 0; jT < nT; ++jT) {
+        double T = Tmin_f + (double)jT * dT;
+        double pmax = 16.0 * sqrt(M * T) + 16.0 * T;
+        double dp = pmax / (double)(np - 1);
+        pgrid[0] = 0.0;
+        double wprev = (a == 0) ? 1.0 : 0.0;
+        cdf[0] = 0.0;
+        for (int32_t k = 1; k < np; ++k) {
+            double p = (double)k * dp;
+            pgrid[k] = p;
+            double E = sqrt(M * M + p * p);
+            double w = (a == 0) ? exp(-(E - M) / T) : ipow(p, a) * exp(-(E - M) / T);
+            cdf[k] = cdf[k - 1] + 0.5 * (wprev + w) * dp;
+            wprev = w;
+        }
+        double norm = cdf[np - 1];
+        double invnorm = norm > 0.0 ? 1.0 / norm : 0.0;
+        for (int32_t k = 0; k < np; ++k) cdf[k] *= invnorm;
+        cdf[np - 1] = 1.0;
+        double p_hi = pgrid[np - 1];
+        double thr = 1.0 - quantile_cap;
+        for (int32_t k = 0; k < np; ++k) {
+            if (cdf[k] >= thr) { p_hi = pgrid[k]; break; }
+        }
+        int64_t base = (int64_t)jT * nU;
+        int32_t k = 0;
+        for (int32_t iu = 0; iu < nU; ++iu) {
+            double u = (double)iu / (double)(nU - 1);
+            while (k < np - 1 && cdf[k + 1] < u) ++k;
+            double c0 = cdf[k], c1 = cdf[k + 1];
+            double p0 = pgrid[k], p1 = pgrid[k + 1];
+            double frac = (c1 > c0) ? (u - c0) / (c1 - c0) : 0.0;
+            frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+            double pval = p0 + frac * (p1 - p0);
+            invcdf[base + iu] = pval < p_hi ? pval : p_hi;
+        }
+    }
+    free(pgrid);
+    free(cdf);
+    *Tmin_out = Tmin_f;
+    *invdT_out = invdT;
+    return LMO_OK;
+}
+
+/* _juttner_invcdf_lookup, kernels_gpu.jl:88-117 (bilinear in (u, T), both indices clamped) */
+double lmo_juttner_invcdf_lookup(const double* invcdf, int32_t nU, int32_t nT,
+                                 double Ttab_min, double Ttab_invdT, double u, double T) {
+    double uu = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double fu = uu * (double)(nU - 1);
+    int64_t iu = (int64_t)floor(fu);
+    iu = iu < 0 ? 0 : (iu > nU - 2 ? nU - 2 : iu);
+    double tu = fu - (double)iu;
+    double fT = (T - Ttab_min) * Ttab_invdT;
+    double fl = floor(fT);
+    int64_t jT;
+    if (!(fl >= 0.0)) jT = 0;
+    else if (fl > (double)(nT - 2)) jT = nT - 2;
+    else jT = (int64_t)fl;
+    double tT = fT - (double)jT;
+    tT = tT < 0.0 ? 0.0 : (tT > 1.0 ? 1.0 : tT);
+    int64_t b0 = jT * nU;
+    int64_t b1 = (jT + 1) * nU;
+    double v00 = invcdf[b0 + iu];
+    double v10 = invcdf[b0 + iu + 1];
+    double v01 = invcdf[b1 + iu];
+    double v11 = invcdf[b1 + iu + 1];
+    double c0 = v00 * (1.0 - tu) + v10 * tu;
+    double c1 = v01 * (1.0 - tu) + v11 * tu;
+    return c0 * (1.0 - tT) + c1 * tT;
+}
+
+/* splitmix64: the private uniform stream of the rejection sampler below */
+static inline uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static inline double u01(uint64_t* s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+/* _draw_juttner_pstar_lib, kernels_cpu.jl:442-458: rejection with the exact-mode envelope.
+ * Julia's rand() is not reproducible here; a splitmix64 stream replaces it, so this sampler
+ * is compared statistically only. */
+int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int64_t n,
+                               double* out) {
+    uint64_t s = seed;
+    for (int64_t j = 0; j < n; ++j) {
+        if (T <= 0.0) { out[j] = 0.0; continue; }
+        int a = d - 1;
+        double E_peak = (a * T + sqrt((double)(a * a) * T * T + 4.0 * m * m)) / 2.0;
+        double p_peak = sqrt(jl_max(E_peak * E_peak - m * m, 1e-12));
+        double gpeak = ipow(p_peak, a) * exp(-(E_peak - m) / T);
+        double wmax = 1.3 * jl_max(gpeak, 1e-300);
+        double pmax = 16.0 * sqrt(m * T) + 16.0 * T;
+        double res = p_peak;
+        for (int it = 0; it < 100000; ++it) {
+            double p = pmax * u01(&s);
+            double E = sqrt(m * m + p * p);
+            double w = ipow(p, a) * exp(-(E - m) / T);
+            if (u01(&s) * wmax <= w) { res = p; break; }
+        }
+        out[j] = res;
+    }
+    return LMO_OK;
+}
+
+/* ============================== Philox4x64-10 and the noise record ==========================
+ * Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11): multipliers 0xD2E7470EE14C6C93 and
+ * 0xCA5A826395121157, Weyl key increments 0x9E3779B97F4A7C15 and 0xBB67AE8584CAA73B, ten
+ * rounds.  numpy.random.Philox implements the same bijection (its first block uses
+ * counter + 1).                                                                              */
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+    unsigned __int128 p = (unsigned __int128)a * (unsigned __int128)b;
+    *hi = (uint64_t)(p >> 64);
+    *lo = (uint64_t)p;
+}
+
+void lmo_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+    uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint64_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ULL, c0, &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ULL, c2, &hi1, &lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0;
+        uint64_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* uniforms from a 64-bit word: [0,1) and (0,1], 53 bits each */
+static inline double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+static inline double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
+/* Box-Muller: radius from the (0,1] uniform, angle from the [0,1) one */
+static inline void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
+    double r = sqrt(-2.0 * log(u_oc(wa)));
+    double th = TWO_PI * u_co(wb);
+    *n0 = r * cos(th);
+    *n1 = r * sin(th);
+}
+
+static int noise_layout(const lmh_params* p, int32_t dims, int32_t pdim,
+                        int32_t* off_xi, int32_t* off_rta, int32_t* off_pos) {
+    int32_t c = 0;
+    *off_xi = *off_rta = *off_pos = -1;
+    if (p->coords == LMH_GENERAL_COORDS) {
+        *off_xi = 0;
+        return 1;
+    }
+    if (p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA) { *off_xi = c; c += pdim; }
+    if (p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA) { *off_rta = c; c += 2 + pdim; }
+    if (p->position_diffusion) { *off_pos = c; c += dims; }
+    return c;
+}
+
+int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
+    int32_t a, b, c;
+    return noise_layout(p, dims, pdim, &a, &b, &c);
+}
+
+/* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
+ * Philox4x64-10 of counter (i, s, b, 0) under key (seed, stream).
+ *   block 0: two Box-Muller pairs n0..n3; xi[d] = n_d (GC: xi = n0)
+ *   block 1: u_col = U[0,1)(w0), u_samp = U[0,1)(w1), (g0, g1) = BoxMuller(w2, w3)
+ *   block 2: (g2, xpos0) = BoxMuller(w0, w1), (xpos1, xpos2) = BoxMuller(w2, w3)           */
+void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
+                      double* rec) {
+    int32_t oxi, orta, opos;
+    noise_layout(p, dims, pdim, &oxi, &orta, &opos);
+    uint64_t key[2] = {p->seed, p->stream};
+    for (int64_t i = 0; i < N; ++i) {
+        uint64_t w[4];
+        double n[4];
+        if (oxi >= 0) {
+            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 0, 0};
+            lmo_philox4x64_10(ctr, key, w);
+            box_muller(w[0], w[1], &n[0], &n[1]);
+            box_muller(w[2], w[3], &n[2], &n[3]);
+            int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
+            for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
+        }
+        double g[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
+        if (orta >= 0) {
+            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 1, 0};
+            lmo_philox4x64_10(ctr, key, w);
+            rec[(int64_t)orta * N + i] = u_co(w[0]);
+            rec[(int64_t)(orta + 1) * N + i] = u_co(w[1]);
+            box_muller(w[2], w[3], &g[0], &g[1]);
+        }
+        if ((orta >= 0 && pdim == 3) || opos >= 0) {
+            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 2, 0};
+            lmo_philox4x64_10(ctr, key, w);
+            box_muller(w[0], w[1], &g[2], &xp[0]);
+            box_muller(w[2], w[3], &xp[1], &xp[2]);
+        }
+        if (orta >= 0)
+            for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = g[d];
+        if (opos >= 0)
+            for (int32_t d = 0; d < dims; ++d) rec[(int64_t)(opos + d) * N + i] = xp[d];
+    }
+}
+
+/* ===================== Cartesian step on the radial (r, t) medium =========================== */
+
+static inline double interp_med(const lmo_radial_medium* med, const double* f, double r, double t) {
+    return lmo_interpolate_2d(med->xgrid, med->nx, med->tgrid, med->nt, f, r, t);
+}
+
+/* kernel_boost_to_rest_frame_cpu!, kernels_cpu.jl:53-148 */
+void lmo_kernel_boost_to_rest(double* mom, const double* pos, const lmo_radial_medium* med,
+                              double m, int64_t N, int64_t step, double dt, double t0,
+                              int32_t dims, int32_t pdim, int32_t radial_mode, double psi2,
+                              int32_t relativistic) {
+    for (int64_t i = 0; i < N; ++i) {
+        double t_now = (double)step * dt + t0;
+        double r2 = 0.0;
+        for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+        double r = sqrt(r2);
+        if (r < JL_EPS) continue;
+        double v = interp_med(med, med->v, r, t_now);
+        if (med->v2 && dims >= 2) {
+            double v2m = interp_med(med, med->v2, r, t_now);
+            double phi = atan2(pos[1 + dims * i], pos[0 + dims * i]);
+            v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - psi2)));
+        }
+        if (!relativistic) {
+            if (radial_mode) {
+                mom[0 + pdim * i] -= m * v;
+            } else {
+                double x = pos[0 + dims * i], y = pos[1 + dims * i];
+                mom[0 + pdim * i] -= m * v * (x / r);
+                mom[1 + pdim * i] -= m * v * (y / r);
+            }
+            continue;
+        }
+        double vmax = sqrt(1.0 - 1e-12);
+        v = jl_clamp(v, -vmax, vmax);
+        double v2 = v * v;
+        double gam = 1.0 / sqrt(1.0 - v2 + 1e-10);
+        double p2 = 0.0;
+        for (int32_t d = 0; d < pdim; ++d) p2 += mom[d + pdim * i] * mom[d + pdim * i];
+        double E = sqrt(p2 + m * m);
+        if (radial_mode) {
+            mom[0 + pdim * i] = gam * (mom[0 + pdim * i] - v * E);
+        } else {
+            double x = pos[0 + dims * i], y = pos[1 + dims * i];
+            double rhatx = x / r, rhaty = y / r;
+            double px = mom[0 + pdim * i], py = mom[1 + pdim * i];
+            double p_par = px * rhatx + py * rhaty;
+            double p_perp_x = px - p_par * rhatx;
+            double p_perp_y = py - p_par * rhaty;
+            double p_par_new = gam * (p_par - v * E);
+            mom[0 + pdim * i] = p_par_new * rhatx + p_perp_x;
+            mom[1 + pdim * i] = p_par_new * rhaty + p_perp_y;
+        }
+    }
+}
+
+/* kernel_boost_to_lab_frame_cpu!, kernels_cpu.jl:159-244 */
+void lmo_kernel_boost_to_lab(double* mom, const double* pos, const lmo_radial_medium* med,
+                             double m, int64_t N, int64_t step, double dt, double t0,
+                             int32_t dims, int32_t pdim, int32_t radial_mode, double psi2,
+                             int32_t relativistic) {
+    for (int64_t i = 0; i < N; ++i) {
+        double r2 = 0.0;
+        for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+        double r = sqrt(r2);
+        if (r < JL_EPS) continue;
+        double t_now = (double)step * dt + t0;
+        double v = interp_med(med, med->v, r, t_now);
+        if (med->v2 && dims >= 2) {
+            double v2m = interp_med(med, med->v2, r, t_now);
+            double phi = atan2(pos[1 + dims * i], pos[0 + dims * i]);
+            v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - psi2)));
+        }
+        if (!relativistic) {
+            if (radial_mode) {
+                mom[0 + pdim * i] += m * v;
+            } else {
+                double x = pos[0 + dims * i], y = pos[1 + dims * i];
+                mom[0 + pdim * i] += m * v * (x / r);
+                mom[1 + pdim * i] += m * v * (y / r);
+            }
+            continue;
+        }
+        double vmax = sqrt(1.0 - 1e-12);
+        v = jl_clamp(v, -vmax, vmax);
+        double v2 = v * v;
+        double gam = 1.0 / sqrt(1.0 - v2 + 1e-10);
+        double p2 = 0.0;
+        for (int32_t d = 0; d < pdim; ++d) p2 += mom[d + pdim * i] * mom[d + pdim * i];
+        double E = sqrt(p2 + m * m);
+        if (radial_mode) {
+            mom[0 + pdim * i] = gam * (mom[0 + pdim * i] + v * E);
+        } else {
+            double x = pos[0 + dims * i], y = pos[1 + dims * i];
+            double rhatx = x / r, rhaty = y / r;
+            double px = mom[0 + pdim * i], py = mom[1 + pdim * i];
+            double p_par = px * rhatx + py * rhaty;
+            double p_perp_x = px - p_par * rhatx;
+            double p_perp_y = py - p_par * rhaty;
+            double p_par_new = gam * (p_par + v * E);
+            mom[0 + pdim * i] = p_par_new * rhatx + p_perp_x;
+            mom[1 + pdim * i] = p_par_new * rhaty + p_perp_y;
+        }
+    }
+}
+
+/* kernel_bjorken_redshift_cpu!, kernels_cpu.jl:427-435 (p_z is row 3) */
+static void kernel_bjorken(double* mom, int32_t pdim, int64_t step, double dt, double t0,
+                           int64_t N) {
+    double tau_a = t0 + (double)(step - 1) * dt;
+    if (!(tau_a > 0.0)) return;
+    double f = tau_a / (tau_a + dt);
+    for (int64_t i = 0; i < N; ++i) mom[2 + pdim * i] *= f;
+}
+
+/*
+ * The drag/noise coefficients of one particle in the LRF and its momentum update.
+ *   Reference (PREPOINT, TABLE): kernel_compute_all_forces_cpu!, kernels_cpu.jl:326-352
+ *   (exact OU with eta_eff = eta_D*m/E when relativistic), then
+ *   kernel_update_momenta_LRF_cpu!, kernels_cpu.jl:406-417: p += det + sqrt(dt)*sto.
+ *   MIDPOINT: kernels_gpu.jl:366-381 (noise-free half step, eta_eff re-evaluated there).
+ *   POSTPOINT (extension): eta_eff evaluated at the updated momentum by three fixed-point
+ *   sweeps q_k = p + ((a_k - 1) p + sqrt(dt) noise_pref_k xi), q_0 = p, eta_k = eta(q_{k-1}).
+ *   GAMMA_P (extension): eta(q) = eta_D / (1 + |q|^2/Lambda^2) with eta_D = gamma0*T^2 and
+ *   kappa = 2 m T eta_D; the m/E factor is not applied.
+ * ηD and kT are the per-particle coefficients of kernels_cpu.jl:281-290.                     */
+static inline double eta_of(const lmh_params* p, double etaD, double m, const double* q,
+                            int32_t pdim) {
+    double q2 = 0.0;
+    for (int32_t d = 0; d < pdim; ++d) q2 += q[d] * q[d];
+    if (p->drag_model == LMH_DRAG_GAMMA_P) {
+        return etaD / (1.0 + q2 / (p->Lambda * p->Lambda));
+    }
+    double E = sqrt(q2 + m * m);
+    return p->relativistic ? etaD * m / E : etaD;
+}
+
+static void ou_forces(const lmh_params* p, double etaD, double kT, double m, const double* q,
+                      const double* xi, int32_t pdim, double* det, double* sto) {
+    double dt = p->dt;
+    if (etaD > 0.0) {
+        double eta_eff = eta_of(p, etaD, m, q, pdim);
+        if (p->integrator == LMH_MIDPOINT && dt > 0.0) {
+            double a_half = exp(-eta_eff * (0.5 * dt));
+            double qm[3];
+            for (int32_t d = 0; d < pdim; ++d) qm[d] = a_half * q[d];
+            eta_eff = eta_of(p, etaD, m, qm, pdim);
+        }
+        double a = exp(-eta_eff * dt);
+        double noise_pref = (dt > 0.0 && eta_eff > 0.0)
+                                ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+        for (int32_t d = 0; d < pdim; ++d) {
+            det[d] = (a - 1.0) * q[d];
+            sto[d] = noise_pref * xi[d];
+        }
+    } else {
+        for (int32_t d = 0; d < pdim; ++d) {
+            det[d] = 0.0;
+            sto[d] = kT * xi[d];
+        }
+    }
+}
+
+/* coefficients eta_D, kT from the local T (kernels_cpu.jl:263-290) or the GAMMA_P model */
+static void drag_coeffs(const lmh_params* p, const lmo_table* tab, double m, double T,
+                        double* etaD, double* kT) {
+    if (p->drag_model == LMH_DRAG_GAMMA_P) {
+        double e = p->gamma0 * T * T;
+        double kappa = 2.0 * m * T * e;
+        *etaD = e;
+        *kT = sqrt(kappa);
+        return;
+    }
+    double DsT_safe = p->DsT <= 0.0 ? 0.0 : p->DsT;
+    double tau = DsT_safe == 0.0 ? 0.0 : lmo_eval_time_table(T, tab->Tmin, tab->invdT, tab->vals, tab->n);
+    double etaD_ = (tau > 0.0 && isfinite(tau)) ? (1.0 / tau) : 0.0;
+    double kappa = (tau > 0.0 && isfinite(tau)) ? ((2.0 * m * T) / tau) : 0.0;
+    *etaD = etaD_;
+    *kT = sqrt(kappa);
+}
+
+/* kernel_compute_all_forces_cpu! + kernel_update_momenta_LRF_cpu! over all particles, with
+ * T[i] the clamped local temperature of each particle.  Radial mode (kernels_cpu.jl:293-323):
+ * Euler drag det = -eta_eff p dt and sto = kT xi (the (kL-kT) projector term is exactly zero
+ * because kL == kT, :288-289). */
+static void forces_and_update(const lmh_params* p, const lmo_table* tab, double m,
+                              const double* Tloc, double* mom, const double* xi /* (pdim,N) */,
+                              int32_t pdim, int32_t radial_mode, int64_t N,
+                              double* det /* (pdim,N) */, double* sto) {
+    double dt = p->dt;
+    for (int64_t i = 0; i < N; ++i) {
+        double T = Tloc[i];
+        double etaD, kT;
+        drag_coeffs(p, tab, m, T, &etaD, &kT);
+        double* q = mom + (int64_t)pdim * i;
+        const double* x = xi + (int64_t)pdim * i;
+        double* de = det + (int64_t)pdim * i;
+        double* st = sto + (int64_t)pdim * i;
+        if (radial_mode) {
+            double s = 0.0;
+            for (int32_t d = 0; d < pdim; ++d) s += q[d] * q[d];
+            double pm = sqrt(s);
+            double E_r = sqrt(pm * pm + m * m);
+            double eta_r = p->relativistic ? etaD * m / E_r : etaD;
+            for (int32_t d = 0; d < pdim; ++d) {
+                de[d] = -eta_r * q[d] * dt;
+                st[d] = kT * 1.0 * x[d];
+            }
+        } else if (p->integrator == LMH_POSTPOINT && etaD > 0.0 && dt > 0.0) {
+            double qk[3], dk[3], sk[3];
+            for (int32_t d = 0; d < pdim; ++d) qk[d] = q[d];
+            double sqrt_dt = sqrt(dt);
+            for (int it = 0; it < 3; ++it) {
+                double eta_eff = eta_of(p, etaD, m, qk, pdim);
+                double a = exp(-eta_eff * dt);
+                double noise_pref = eta_eff > 0.0
+                                        ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+                for (int32_t d = 0; d < pdim; ++d) {
+                    dk[d] = (a - 1.0) * q[d];
+                    sk[d] = noise_pref * x[d];
+                }
+                for (int32_t d = 0; d < pdim; ++d) qk[d] = q[d] + (dk[d] + sqrt_dt * sk[d]);
+            }
+            for (int32_t d = 0; d < pdim; ++d) { de[d] = dk[d]; st[d] = sk[d]; }
+        } else {
+            ou_forces(p, etaD, kT, m, q, x, pdim, de, st);
+        }
+    }
+    double sqrt_dt = sqrt(dt);
+    for (int64_t i = 0; i < N; ++i)
+        for (int32_t d = 0; d < pdim; ++d) {
+            int64_t k = d + (int64_t)pdim * i;
+            mom[k] += det[k] + sqrt_dt * sto[k];
+        }
+}
+
+/* kernel_rta_collision_cpu!, kernels_cpu.jl:474-508, with the redraw of
+ * kernel_rta_collision_gpu!, kernels_gpu.jl:653-704: |p*| from the inverse-CDF table at the
+ * supplied quantile, direction from the supplied Gaussians (radial mode: the sign of g0). */
+static void rta_collision(const lmh_params* p, const lmo_table* taun, const lmo_jtable* jt,
+                          const double* Tloc, double* mom, int32_t pdim, int32_t radial_mode,
+                          int64_t N, const double* ucol, const double* usamp,
+                          const double* g /* (pdim,N) */) {
+    double dt = p->dt;
+    for (int64_t i = 0; i < N; ++i) {
+        double T = Tloc[i];
+        double tau_n = (p->DsT > 0.0 && T > 0.0)
+                           ? lmo_eval_time_table(T, taun->Tmin, taun->invdT, taun->vals, taun->n)
+                           : 0.0;
+        double Pcol = (tau_n > 0.0 && isfinite(tau_n)) ? jl_clamp(dt / tau_n, 0.0, 1.0) : 1.0;
+        if (!(ucol[i] < Pcol)) continue;
+        double pstar = lmo_juttner_invcdf_lookup(jt->vals, jt->nU, jt->nT, jt->Tmin, jt->invdT,
+                                                 usamp[i], T);
+        if (radial_mode) {
+            mom[0 + pdim * i] = g[0 + pdim * i] < 0.0 ? -pstar : pstar;
+        } else {
+            double nrm2 = 0.0;
+            for (int32_t d = 0; d < pdim; ++d) nrm2 += g[d + pdim * i] * g[d + pdim * i];
+            double inv = pstar / sqrt(jl_max(nrm2, 1e-300));
+            for (int32_t d = 0; d < pdim; ++d) mom[d + pdim * i] = g[d + pdim * i] * inv;
+        }
+    }
+}
+
+/* kernel_set_to_fluid_velocity_cpu!, kernels_cpu.jl:671-735 */
+static void set_to_fluid(double* mom, const double* pos, const lmo_radial_medium* med, double m,
+                         int64_t N, int64_t step, double dt, double t0, int32_t dims,
+                         int32_t pdim, int32_t radial_mode, int32_t relativistic) {
+    for (int64_t i = 0; i < N; ++i) {
+        double r, x = 0.0, y = 0.0;
+        if (radial_mode) {
+            r = pos[0 + dims * i];
+        } else {
+            x = pos[0 + dims * i];
+            y = pos[1 + dims * i];
+            r = sqrt(x * x + y * y);
+        }
+        if (r < JL_EPS) continue;
+        double tau_now = (double)step * dt + t0;
+        double v = interp_med(med, med->v, r, tau_now);
+        if (!relativistic) {
+            if (radial_mode) {
+                mom[0 + pdim * i] = m * v;
+            } else {
+                mom[0 + pdim * i] = m * v * (x / r);
+                mom[1 + pdim * i] = m * v * (y / r);
+            }
+            continue;
+        }
+        double vmax = sqrt(1.0 - 1e-12);
+        v = jl_clamp(v, -vmax, vmax);
+        double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+        if (radial_mode) {
+            mom[0 + pdim * i] = m * gam * v;
+        } else {
+            mom[0 + pdim * i] = m * gam * v * (x / r);
+            mom[1 + pdim * i] = m * gam * v * (y / r);
+        }
+    }
+}
+
+/* kernel_update_positions_cpu!, kernels_cpu.jl:510-637, with the position-diffusion randn()
+ * draws (:562, :604) supplied as xpos (dims, N). */
+static void update_positions(const lmh_params* p, const lmo_radial_medium* med, double* pos,
+                             double* mom, double m, int64_t N, int64_t step, int32_t dims,
+                             int32_t pdim, int32_t radial_mode, const double* xpos) {
+    double dt = p->dt, t0 = p->t0;
+    const double* xg = med->xgrid;
+    double r_axis_eps = med->nx >= 2 ? jl_max(1e-12, 0.5 * fabs(xg[1] - xg[0])) : 1e-6;
+    double r_max = med->nx >= 1 ? xg[med->nx - 1] : 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        double E2 = m * m;
+        for (int32_t d = 0; d < pdim; ++d) E2 += mom[d + pdim * i] * mom[d + pdim * i];
+        double E = p->relativistic ? sqrt(E2) : m;
+        if (radial_mode) {
+            double r = pos[0 + dims * i];
+            double pr = mom[0 + pdim * i];
+            double r_abs = fabs(r);
+            double r_safe = (r_abs < r_axis_eps) ? r_axis_eps : r_abs;
+            double dr = (pr / E) * dt;
+            if (p->position_diffusion) {
+                double T = interp_med(med, med->T, r_safe, (double)step * dt + t0);
+                double D = (p->DsT / jl_max(T, JL_EPS)) / FMGEV;
+                if (D > 0.0) {
+                    double xi = xpos[0 + dims * i];
+                    dr += (D / r_safe) * dt + sqrt(2.0 * D * dt) * xi;
+                }
+            }
+            double r_new = r + dr;
+            if (r_new < 0.0) {
+                r_new = -r_new;
+                mom[0 + pdim * i] = -mom[0 + pdim * i];
+            }
+            if (p->reflecting_boundary && r_max > 0.0 && r_new > r_max) {
+                r_new = 2.0 * r_max - r_new;
+                mom[0 + pdim * i] = -mom[0 + pdim * i];
+                if (r_new < 0.0) {
+                    r_new = -r_new;
+                    mom[0 + pdim * i] = -mom[0 + pdim * i];
+                }
+            }
+            pos[0 + dims * i] = r_new;
+        } else {
+            for (int32_t d = 0; d < dims; ++d)
+                pos[d + dims * i] += dt * mom[d + pdim * i] / E;
+            if (p->position_diffusion) {
+                double r2 = 0.0;
+                for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+                double r_now = sqrt(r2);
+                double T = interp_med(med, med->T, r_now, (double)step * dt + t0);
+                double D = (p->DsT / jl_max(T, JL_EPS)) / FMGEV;
+                if (D > 0.0) {
+                    double sigma = sqrt(2.0 * D * dt);
+                    for (int32_t d = 0; d < dims; ++d)
+                        pos[d + dims * i] += sigma * xpos[d + dims * i];
+                }
+            }
+            if (p->reflecting_boundary && r_max > 0.0) {
+                double r2 = 0.0;
+                for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+                double r_now = sqrt(r2);
+                if (r_now > r_max) {
+                    double invr = 1.0 / jl_max(r_now, r_axis_eps);
+                    double ppar = 0.0;
+                    for (int32_t d = 0; d < dims; ++d)
+                        ppar += mom[d + pdim * i] * (pos[d + dims * i] * invr);
+                    for (int32_t d = 0; d < dims; ++d)
+                        mom[d + pdim * i] -= 2.0 * ppar * (pos[d + dims * i] * invr);
+                    double scale = (2.0 * r_max - r_now) * invr;
+                    for (int32_t d = 0; d < dims; ++d) pos[d + dims * i] *= scale;
+                }
+            }
+        }
+    }
+}
+
+/* scratch of one segment */
+typedef struct {
+    double *xi, *det, *sto, *T, *ucol, *usamp, *g, *xpos, *aux;
+} work_t;
+
+static int work_alloc(work_t* w, int64_t N, int32_t pdim, int32_t dims) {
+    size_t n = (size_t)N;
+    memset(w, 0, sizeof(*w));
+    w->xi = (double*)calloc(n * (size_t)pdim, sizeof(double));
+    w->det = (double*)calloc(n * (size_t)pdim, sizeof(double));
+    w->sto = (double*)calloc(n * (size_t)pdim, sizeof(double));
+    w->T = (double*)calloc(n, sizeof(double));
+    w->ucol = (double*)calloc(n, sizeof(double));
+    w->usamp = (double*)calloc(n, sizeof(double));
+    w->g = (double*)calloc(n * (size_t)pdim, sizeof(double));
+    w->xpos = (double*)calloc(n * (size_t)dims, sizeof(double));
+    w->aux = (double*)calloc(n * 3, sizeof(double));
+    return (w->xi && w->det && w->sto && w->T && w->ucol && w->usamp && w->g && w->xpos && w->aux)
+               ? LMO_OK : LMO_E_INVALID;
+}
+static void work_free(work_t* w) {
+    free(w->xi); free(w->det); free(w->sto); free(w->T); free(w->ucol); free(w->usamp);
+    free(w->g); free(w->xpos); free(w->aux);
+}
+
+/* channels of the record (row-major (C, Ntot), segment starting at i0) into Julia-layout
+ * (rows, n) scratch: the driver's `ξ .= randn(pdim, N)` (simulate_cpu.jl:216) and the draws of
+ * the RTA and diffusion kernels, read from the fixed-noise record instead. */
+static void gather_channels(const double* rec, int64_t Ntot, int64_t i0, int64_t n, int32_t c0,
+                            int32_t rows, double* out) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t d = 0; d < rows; ++d)
+            out[d + (int64_t)rows * i] = rec[(int64_t)(c0 + d) * Ntot + i0 + i];
+}
+
+static int all_finite(const double* a, int64_t n) {
+    for (int64_t k = 0; k < n; ++k)
+        if (!isfinite(a[k])) return 0;
+    return 1;
+}
+
+static int64_t species_begin(const lmo_species* sp, int32_t k) {
+    int64_t b = 0;
+    for (int32_t j = 0; j < k; ++j) b += sp->counts[j];
+    return b;
+}
+
+/* One step of simulate_cpu.jl:215-290 on one species segment (pos/mom point at its start). */
+static void step_cartesian_segment(const lmh_params* p, const lmo_radial_medium* med,
+                                   const lmo_species* sp, int32_t k, int32_t dims,
+                                   int32_t pdim, double* pos, double* mom, int64_t n,
+                                   int64_t step, const double* rec, int64_t Ntot, int64_t i0,
+                                   work_t* w) {
+    int32_t oxi, orta, opos;
+    noise_layout(p, dims, pdim, &oxi, &orta, &opos);
+    double m = sp->mass[k];
+    int32_t radial_mode = dims == 1;
+    double dt = p->dt, t0 = p->t0;
+    if (oxi >= 0) gather_channels(rec, Ntot, i0, n, oxi, pdim, w->xi);
+    if (orta >= 0) {
+        gather_channels(rec, Ntot, i0, n, orta, 1, w->ucol);
+        gather_channels(rec, Ntot, i0, n, orta + 1, 1, w->usamp);
+        gather_channels(rec, Ntot, i0, n, orta + 2, pdim, w->g);
+    }
+    if (opos >= 0) gather_channels(rec, Ntot, i0, n, opos, dims, w->xpos);
+
+    /* 1. boost to the LRF (simulate_cpu.jl:219-222) */
+    lmo_kernel_boost_to_rest(mom, pos, med, m, n, step, dt, t0, dims, pdim, radial_mode,
+                             p->psi2, p->relativistic);
+    /* 1b. Bjorken redshift (simulate_cpu.jl:225-227) */
+    if (p->bjorken_redshift) kernel_bjorken(mom, pdim, step, dt, t0, n);
+
+    if (p->collision == LMH_FLUID) {
+        /* simulate_cpu.jl:229-233 */
+        set_to_fluid(mom, pos, med, m, n, step, dt, t0, dims, pdim, radial_mode, p->relativistic);
+    } else {
+        /* local temperature, clamped at 0 (kernels_cpu.jl:264-267, :486-487) */
+        for (int64_t i = 0; i < n; ++i) {
+            double r2 = 0.0;
+            for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+            double r = sqrt(r2);
+            double T = interp_med(med, med->T, r, (double)step * dt + t0);
+            w->T[i] = jl_max(T, 0.0);
+        }
+        if (p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA) {
+            /* 2.-3. forces and update (simulate_cpu.jl:255-271) */
+            forces_and_update(p, &sp->drag[k], m, w->T, mom, w->xi, pdim, radial_mode, n,
+                              w->det, w->sto);
+        }
+        if (p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA) {
+            /* simulate_cpu.jl:240-245 */
+            rta_collision(p, &sp->rta[k], &sp->jt[k], w->T, mom, pdim, radial_mode, n, w->ucol,
+                          w->usamp, w->g);
+        }
+        /* 4. boost back to the lab (simulate_cpu.jl:248-251, :274-277) */
+        lmo_kernel_boost_to_lab(mom, pos, med, m, n, step, dt, t0, dims, pdim, radial_mode,
+                                p->psi2, p->relativistic);
+    }
+    /* 5. positions (simulate_cpu.jl:281-290) */
+    update_positions(p, med, pos, mom, m, n, step, dims, pdim, radial_mode, w->xpos);
+}
+
+static int check_cartesian_dims(int32_t dims, int32_t pdim) {
+    if (dims == pdim && dims >= 1 && dims <= 3) return 1;
+    return dims == 2 && pdim == 3;
+}
+
+/* The driver's initial boost at step 0, simulate_cpu.jl:160-163 */
+int lmo_boost_to_lab_all(const lmh_params* p, const lmo_radial_medium* med,
+                         const lmo_species* sp, int32_t dims, int32_t pdim,
+                         double* pos, double* mom, int64_t N, int64_t step) {
+    if (!check_cartesian_dims(dims, pdim)) return LMO_E_INVALID;
+    if (species_begin(sp, sp->nspecies) != N) return LMO_E_INVALID;
+    for (int32_t k = 0; k < sp->nspecies; ++k) {
+        int64_t b = species_begin(sp, k);
+        lmo_kernel_boost_to_lab(mom + (int64_t)pdim * b, pos + (int64_t)dims * b, med,
+                                sp->mass[k], sp->counts[k], step, p->dt, p->t0, dims, pdim,
+                                dims == 1, p->psi2, p->relativistic);
+    }
+    return LMO_OK;
+}
+
+/* Steps step_begin .. step_begin+nsteps-1 of simulate_cpu.jl:215-303; noise is the fixed-noise
+ * record (nsteps, C, N).  The NaN/Inf scan of :293-303 stops the run at the first bad step. */
+int lmo_run_cartesian(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+                      int32_t dims, int32_t pdim, double* pos, double* mom, int64_t N,
+                      int64_t step_begin, int64_t nsteps, const double* noise,
+                      int64_t* bad_step) {
+    if (!check_cartesian_dims(dims, pdim)) return LMO_E_INVALID;
+    if (bad_step) *bad_step = -1;
+    int32_t C = lmo_noise_channels(p, dims, pdim);
+    work_t w;
+    if (work_alloc(&w, N, pdim, dims) != LMO_OK) { work_free(&w); return LMO_E_INVALID; }
+    int rc = LMO_OK;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        int64_t step = step_begin + s;
+        const double* rec = noise ? noise + s * (int64_t)C * N : NULL;
+        for (int32_t k = 0; k < sp->nspecies; ++k) {
+            int64_t b = species_begin(sp, k);
+            step_cartesian_segment(p, med, sp, k, dims, pdim, pos + (int64_t)dims * b,
+                                   mom + (int64_t)pdim * b, sp->counts[k], step, rec, N, b, &w);
+        }
+        if (!all_finite(mom, (int64_t)pdim * N) || !all_finite(pos, (int64_t)dims * N)) {
+            if (bad_step) *bad_step = step;
+            rc = LMO_E_NONFINITE;
+            break;
+        }
+    }
+    work_free(&w);
+    return rc;
+}
+
+/* ======================= general-coordinate (tau, r) step ==================================== */
+
+/* One step of simulate_cpu_general_coords.jl:83-115 on one segment, dims == 2:
+ *   kernel_boost_to_rest_frame_general_coords_cpu!   kernels_cpu_GC.jl:64-82   (regulator 1e-10)
+ *   kernel_compute_all_forces_general_coords_cpu!     kernels_cpu_GC.jl:110-174 (Gamma == 0,
+ *        rows 2..dims, eta_D without the m/E factor, T not clamped)
+ *   kernel_update_momenta_LRF_general_coords_cpu!     kernels_cpu_GC.jl:244-256 (mass shell)
+ *   kernel_boost_to_lab_frame_general_coords_cpu!     kernels_cpu_GC.jl:85-102  (regulator 1e-8)
+ *   kernel_update_positions_general_coords_cpu!       kernels_cpu_GC.jl:259-274
+ * The per-particle p_mags/p_units/eta outputs of the force kernel feed nothing later and are
+ * not formed.  Returns LMO_E_DOMAIN where Julia's sqrt would throw (kappa < 0 from T < 0). */
+static int step_gc_segment(const lmh_params* p, const lmo_radial_medium* med, double m,
+                           const lmo_table* tab, double* pos, double* mom, int64_t n,
+                           int64_t step, const double* rec, int64_t Ntot, int64_t i0, work_t* w) {
+    double dt = p->dt, t0 = p->t0;
+    const int32_t dims = 2;
+    /* 1. boost to the LRF */
+    for (int64_t i = 0; i < n; ++i) {
+        double r = fabs(pos[1 + dims * i]);
+        double v = lmo_interpolate_2d_gc(med->xgrid, med->nx, med->tgrid, med->nt, med->v, r,
+                                         (double)step * dt + t0);
+        double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+        double ptau = mom[0 + dims * i], pr = mom[1 + dims * i];
+        mom[0 + dims * i] = gam * (ptau - v * pr);
+        mom[1 + dims * i] = gam * (pr - v * ptau);
+    }
+    /* 2. forces (row 2 only) */
+    for (int64_t i = 0; i < n; ++i) {
+        double T = lmo_interpolate_2d_gc(med->xgrid, med->nx, med->tgrid, med->nt, med->T,
+                                         fabs(pos[1 + dims * i]), (double)step * dt + t0);
+        double DsT_safe = p->DsT <= 0.0 ? 0.0 : p->DsT;
+        double tau_n = DsT_safe == 0.0 ? 0.0 : lmo_eval_time_table(T, tab->Tmin, tab->invdT,
+                                                                    tab->vals, tab->n);
+        double etaD = (tau_n > 0.0 && isfinite(tau_n)) ? (1.0 / tau_n) : 0.0;
+        double kappa = (tau_n > 0.0 && isfinite(tau_n)) ? ((2.0 * m * T) / tau_n) : 0.0;
+        if (kappa < 0.0) return LMO_E_DOMAIN;
+        double kT = sqrt(kappa);
+        double xi = rec[(int64_t)0 * Ntot + i0 + i];
+        double geo_term = 0.0;          /* Christoffel drift with Gamma == 0 (:105-108, :146-154) */
+        double det, sto;
+        if (etaD > 0.0 && dt > 0.0) {
+            double a = exp(-etaD * dt);
+            double noise_pref = kT * sqrt((1.0 - a * a) / (2.0 * etaD * dt));
+            det = (a - 1.0) * mom[1 + dims * i] + geo_term;
+            sto = noise_pref * xi;
+        } else {
+            det = geo_term;
+            sto = kT * xi;
+        }
+        w->det[i] = det;
+        w->sto[i] = sto;
+    }
+    /* 3. update + mass shell */
+    for (int64_t i = 0; i < n; ++i) {
+        mom[1 + dims * i] += w->det[i] + sqrt(dt) * w->sto[i];
+        double psq = mom[1 + dims * i] * mom[1 + dims * i];
+        mom[0 + dims * i] = sqrt(m * m + psq);
+    }
+    /* 4. boost back */
+    for (int64_t i = 0; i < n; ++i) {
+        double r = fabs(pos[1 + dims * i]);
+        double v = lmo_interpolate_2d_gc(med->xgrid, med->nx, med->tgrid, med->nt, med->v, r,
+                                         (double)step * dt + t0);
+        double gam = 1.0 / sqrt(1.0 - v * v + 1e-8);
+        double ptau = mom[0 + dims * i], pr = mom[1 + dims * i];
+        mom[0 + dims * i] = gam * (ptau + v * pr);
+        mom[1 + dims * i] = gam * (pr + v * ptau);
+    }
+    /* 5. push x^mu += dt p^mu / p^tau */
+    for (int64_t i = 0; i < n; ++i) {
+        double E = mom[0 + dims * i];
+        for (int32_t mu = 0; mu < dims; ++mu) pos[mu + dims * i] += dt * mom[mu + dims * i] / E;
+    }
+    return LMO_OK;
+}
+
+int lmo_run_gc(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+               double* pos, double* mom, int64_t N, int64_t step_begin, int64_t nsteps,
+               const double* noise, int64_t* bad_step) {
+    if (bad_step) *bad_step = -1;
+    if (!noise) return LMO_E_INVALID;
+    work_t w;
+    if (work_alloc(&w, N, 2, 2) != LMO_OK) { work_free(&w); return LMO_E_INVALID; }
+    int rc = LMO_OK;
+    for (int64_t s = 0; s < nsteps && rc == LMO_OK; ++s) {
+        int64_t step = step_begin + s;
+        const double* rec = noise + s * N;
+        for (int32_t k = 0; k < sp->nspecies && rc == LMO_OK; ++k) {
+            int64_t b = species_begin(sp, k);
+            rc = step_gc_segment(p, med, sp->mass[k], &sp->drag[k], pos + 2 * b, mom + 2 * b,
+                                 sp->counts[k], step, rec, N, b, &w);
+        }
+        if (rc == LMO_OK && (!all_finite(mom, 2 * N) || !all_finite(pos, 2 * N))) {
+            if (bad_step) *bad_step = step;
+            rc = LMO_E_NONFINITE;
+        }
+    }
+    work_free(&w);
+    return rc;
+}
+
+/* =========================== 2+1D grid medium (extension) ===================================
+ * Medium: uniform x and y nodes x0 + k*dx, a sorted tau grid, fields (nx, ny, nt).
+ * Sample at (x, y, t): the query is clamped to the grid; the cell index is
+ * clamp(floor(f), 0, n-2) with f = (x_c - x0)*(1/dx), and the fraction is f - index; the tau
+ * cell and fraction follow interpolate_2d_cpu (kernels_cpu.jl:27-45).  Each of the four corner
+ * values is first interpolated in tau, then in x, then in y, in interpolate_2d_cpu's form.   */
+static inline int64_t grid_cell(double f, int32_t n) {
+    double fl = floor(f);
+    if (!(fl >= 0.0)) return 0;
+    if (fl > (double)(n - 2)) return n - 2;
+    return (int64_t)fl;
+}
+
+void lmo_grid_sample(const lmo_grid_medium* g, double x, double y, double t, double out[3]) {
+    double xlast = g->x0 + (double)(g->nx - 1) * g->dx;
+    double ylast = g->y0 + (double)(g->ny - 1) * g->dy;
+    double inv_dx = 1.0 / g->dx, inv_dy = 1.0 / g->dy;
+    double xc = jl_clamp(x, g->x0, xlast);
+    double yc = jl_clamp(y, g->y0, ylast);
+    double fx = (xc - g->x0) * inv_dx;
+    double fy = (yc - g->y0) * inv_dy;
+    int64_t ix = grid_cell(fx, g->nx), iy = grid_cell(fy, g->ny);
+    double xd = fx - (double)ix, yd = fy - (double)iy;
+    const double* tg = g->tgrid;
+    double tc = jl_clamp(t, tg[0], tg[g->nt - 1]);
+    int64_t j = clamp_i64(lmo_searchsortedlast(tg, g->nt, tc), 1, g->nt - 1);
+    double ta = tg[j - 1], tb = tg[j];
+    double dtt = tb - ta;
+    double td = fabs(dtt) < JL_EPS ? 0.0 : (tc - ta) / dtt;
+    int64_t nx = g->nx, nxy = (int64_t)g->nx * g->ny;
+    int64_t s0 = nxy * (j - 1), s1 = nxy * j;
+    const double* F[3] = {g->T, g->vx, g->vy};
+    for (int f = 0; f < 3; ++f) {
+        const double* A = F[f];
+        int64_t n00 = ix + nx * iy, n10 = n00 + 1, n01 = n00 + nx, n11 = n01 + 1;
+        double c00 = A[s0 + n00] * (1.0 - td) + A[s1 + n00] * td;
+        double c10 = A[s0 + n10] * (1.0 - td) + A[s1 + n10] * td;
+        double c01 = A[s0 + n01] * (1.0 - td) + A[s1 + n01] * td;
+        double c11 = A[s0 + n11] * (1.0 - td) + A[s1 + n11] * td;
+        double cx0 = c00 * (1.0 - xd) + c10 * xd;
+        double cx1 = c01 * (1.0 - xd) + c11 * xd;
+        out[f] = cx0 * (1.0 - yd) + cx1 * yd;
+    }
+}
+
+/*
+ * The frame change on the grid: the reference boosts along r_hat with the radial flow speed
+ * (kernels_cpu.jl:101-144, 199-241); the grid medium carries the flow vector (vx, vy), so the
+ * boost runs along v_hat = (vx, vy)/|v| with speed |v|, in the same operation order (r_hat ->
+ * v_hat, v -> |v|), skipped when |v| < eps as the reference skips r < eps.
+ */
+static void grid_boost(double* q, int32_t pdim, double m, double vx, double vy, int32_t sign,
+                       int32_t relativistic) {
+    double vm2 = vx * vx + vy * vy;
+    double vmag = sqrt(vm2);
+    if (vmag < JL_EPS) return;
+    double nx = vx / vmag, ny = vy / vmag;
+    if (!relativistic) {
+        double s = sign > 0 ? 1.0 : -1.0;
+        q[0] += s * (m * vmag * nx);
+        q[1] += s * (m * vmag * ny);
+        return;
+    }
+    double vmax = sqrt(1.0 - 1e-12);
+    double v = vmag > vmax ? vmax : vmag;
+    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+    double p2 = 0.0;
+    for (int32_t d = 0; d < pdim; ++d) p2 += q[d] * q[d];
+    double E = sqrt(p2 + m * m);
+    double px = q[0], py = q[1];
+    double p_par = px * nx + py * ny;
+    double p_perp_x = px - p_par * nx;
+    double p_perp_y = py - p_par * ny;
+    double p_par_new = sign > 0 ? gam * (p_par + v * E) : gam * (p_par - v * E);
+    q[0] = p_par_new * nx + p_perp_x;
+    q[1] = p_par_new * ny + p_perp_y;
+}
+
+static void step_grid_segment(const lmh_params* p, const lmo_grid_medium* g,
+                              const lmo_species* sp, int32_t k, int32_t pdim, double* pos,
+                              double* mom, int64_t n, int64_t step, const double* rec,
+                              int64_t Ntot, int64_t i0, work_t* w) {
+    const int32_t dims = 2;
+    int32_t oxi, orta, opos;
+    noise_layout(p, dims, pdim, &oxi, &orta, &opos);
+    double m = sp->mass[k];
+    double dt = p->dt, t0 = p->t0;
+    double t_now = (double)step * dt + t0;
+    if (oxi >= 0) gather_channels(rec, Ntot, i0, n, oxi, pdim, w->xi);
+    if (orta >= 0) {
+        gather_channels(rec, Ntot, i0, n, orta, 1, w->ucol);
+        gather_channels(rec, Ntot, i0, n, orta + 1, 1, w->usamp);
+        gather_channels(rec, Ntot, i0, n, orta + 2, pdim, w->g);
+    }
+    if (opos >= 0) gather_channels(rec, Ntot, i0, n, opos, dims, w->xpos);
+    /* medium at each particle: aux = (T, vx, vy) */
+    for (int64_t i = 0; i < n; ++i) {
+        double s3[3];
+        lmo_grid_sample(g, pos[0 + dims * i], pos[1 + dims * i], t_now, s3);
+        w->aux[3 * i] = s3[0]; w->aux[3 * i + 1] = s3[1]; w->aux[3 * i + 2] = s3[2];
+    }
+    for (int64_t i = 0; i < n; ++i)
+        grid_boost(mom + (int64_t)pdim * i, pdim, m, w->aux[3 * i + 1], w->aux[3 * i + 2], -1,
+                   p->relativistic);
+    if (p->bjorken_redshift) kernel_bjorken(mom, pdim, step, dt, t0, n);
+    if (p->collision == LMH_FLUID) {
+        /* kernels_cpu.jl:713-731 with r_hat -> v_hat; a fluid at rest sets p_perp to 0 */
+        for (int64_t i = 0; i < n; ++i) {
+            double vx = w->aux[3 * i + 1], vy = w->aux[3 * i + 2];
+            double vmag = sqrt(vx * vx + vy * vy);
+            double* q = mom + (int64_t)pdim * i;
+            if (vmag < JL_EPS) { q[0] = 0.0; q[1] = 0.0; continue; }
+            double nx = vx / vmag, ny = vy / vmag;
+            if (!p->relativistic) { q[0] = m * vmag * nx; q[1] = m * vmag * ny; continue; }
+            double vmax = sqrt(1.0 - 1e-12);
+            double v = vmag > vmax ? vmax : vmag;
+            double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+            q[0] = m * gam * v * nx;
+            q[1] = m * gam * v * ny;
+        }
+    } else {
+        for (int64_t i = 0; i < n; ++i) w->T[i] = jl_max(w->aux[3 * i], 0.0);
+        if (p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA)
+            forces_and_update(p, &sp->drag[k], m, w->T, mom, w->xi, pdim, 0, n, w->det, w->sto);
+        if (p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA)
+            rta_collision(p, &sp->rta[k], &sp->jt[k], w->T, mom, pdim, 0, n, w->ucol, w->usamp,
+                          w->g);
+        for (int64_t i = 0; i < n; ++i)
+            grid_boost(mom + (int64_t)pdim * i, pdim, m, w->aux[3 * i + 1], w->aux[3 * i + 2], +1,
+                       p->relativistic);
+    }
+    /* push (kernels_cpu.jl:537-541, 586-607) */
+    for (int64_t i = 0; i < n; ++i) {
+        double E2 = m * m;
+        for (int32_t d = 0; d < pdim; ++d) E2 += mom[d + pdim * i] * mom[d + pdim * i];
+        double E = p->relativistic ? sqrt(E2) : m;
+        for (int32_t d = 0; d < dims; ++d) pos[d + dims * i] += dt * mom[d + pdim * i] / E;
+        if (p->position_diffusion) {
+            double s3[3];
+            lmo_grid_sample(g, pos[0 + dims * i], pos[1 + dims * i], t_now, s3);
+            double D = (p->DsT / jl_max(s3[0], JL_EPS)) / FMGEV;
+            if (D > 0.0) {
+                double sigma = sqrt(2.0 * D * dt);
+                for (int32_t d = 0; d < dims; ++d) pos[d + dims * i] += sigma * w->xpos[d + dims * i];
+            }
+        }
+    }
+}
+
+int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
+                 int32_t pdim, double* pos, double* mom, int64_t N, int64_t step_begin,
+                 int64_t nsteps, const double* noise, int64_t* bad_step) {
+    if (bad_step) *bad_step = -1;
+    if (pdim != 2 && pdim != 3) return LMO_E_INVALID;
+    if (p->reflecting_boundary) return LMO_E_INVALID;
+    int32_t C = lmo_noise_channels(p, 2, pdim);
+    work_t w;
+    if (work_alloc(&w, N, pdim, 2) != LMO_OK) { work_free(&w); return LMO_E_INVALID; }
+    int rc = LMO_OK;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        int64_t step = step_begin + s;
+        const double* rec = noise ? noise + s * (int64_t)C * N : NULL;
+        for (int32_t k = 0; k < sp->nspecies; ++k) {
+            int64_t b = species_begin(sp, k);
+            step_grid_segment(p, g, sp, k, pdim, pos + 2 * b, mom + (int64_t)pdim * b,
+                              sp->counts[k], step, rec, N, b, &w);
+        }
+        if (!all_finite(mom, (int64_t)pdim * N) || !all_finite(pos, 2 * N)) {
+            if (bad_step) *bad_step = step;
+            rc = LMO_E_NONFINITE;
+            break;
+        }
+    }
+    work_free(&w);
+    return rc;
+}
+
+/* ================================ observables (extension) ===================================
+ * over particles [i0, i1) of a (pdim, N) momentum matrix, see lmh_obs_spec in lmhip.h      */
+int lmo_observables(const double* mom, int32_t pdim, int64_t i0, int64_t i1,
+                    const lmh_obs_spec* spec, double* out) {
+    if (pdim < 2 || spec->nbins < 0) return LMO_E_INVALID;
+    int32_t nb = spec->nbins;
+    for (int32_t b = 0; b < 5 + nb; ++b) out[b] = 0.0;
+    double width = (spec->pt_max - spec->pt_min) / (double)(nb > 0 ? nb : 1);
+    double inv_w = (double)nb / (spec->pt_max - spec->pt_min);
+    for (int64_t i = i0; i < i1; ++i) {
+        double px = mom[0 + (int64_t)pdim * i], py = mom[1 + (int64_t)pdim * i];
+        double pt2 = px * px + py * py;
+        double pt = sqrt(pt2);
+        double c2 = pt2 > 0.0 ? (px * px - py * py) / pt2 : 0.0;
+        out[0] += 1.0;
+        out[1] += pt;
+        out[2] += pt2;
+        out[3] += c2;
+        out[4] += pt * c2;
+        if (nb == 0) continue;
+        if (spec->sigma > 0.0) {
+            double inv_s = 1.0 / (spec->sigma * sqrt(2.0));
+            for (int32_t b = 0; b < nb; ++b) {
+                double lo = spec->pt_min + (double)b * width;
+                double hi = lo + width;
+                out[5 + b] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
+            }
+        } else {
+            double f = (pt - spec->pt_min) * inv_w;
+            double fl = floor(f);
+            if (fl >= 0.0 && fl < (double)nb) out[5 + (int64_t)fl] += 1.0;
+        }
+    }
+    return LMO_OK;
+}

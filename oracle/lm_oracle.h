@@ -1,0 +1,132 @@
+/*
+ * lm_oracle.h -- CPU restatement of LangevInMedium.jl's per-particle Langevin step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP backend is compared against.
+ * Only the tests, the smoke check and the CPU-baseline leg of the benchmark may load it; the
+ * product (liblmhip.so and the Python host package) never does.
+ *
+ * Arrays use the reference's Julia layout: a (rows, N) particle matrix holds (d, i) at
+ * [d + rows*i]; an (n_r, n_t) medium matrix holds (i, j) at [i + n_r*j].  The fixed-noise
+ * record of one step is the (C, N) row-major block described in include/lmhip.h.
+ */
+#ifndef LM_ORACLE_H
+#define LM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/lmhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMO_OK          0
+#define LMO_E_INVALID  -1
+#define LMO_E_NONFINITE -4
+#define LMO_E_DOMAIN   -6   /* a Julia DomainError (sqrt of a negative number) */
+
+typedef struct lmo_radial_medium {
+    const double* xgrid;
+    const double* tgrid;
+    const double* T;       /* (nx, nt) */
+    const double* v;       /* (nx, nt) */
+    const double* v2;      /* (nx, nt) or NULL */
+    int32_t nx, nt;
+} lmo_radial_medium;
+
+typedef struct lmo_grid_medium {
+    double x0, dx, y0, dy;
+    const double* tgrid;
+    const double* T;       /* (nx, ny, nt) */
+    const double* vx;
+    const double* vy;
+    int32_t nx, ny, nt, _pad;
+} lmo_grid_medium;
+
+typedef struct lmo_table {     /* uniform-grid linear time table (transport.jl:231-253) */
+    const double* vals;
+    double Tmin, invdT;
+    int32_t n, _pad;
+} lmo_table;
+
+typedef struct lmo_jtable {    /* inverse-CDF Jüttner table (transport.jl:268-270) */
+    const double* vals;
+    double Tmin, invdT;
+    int32_t nU, nT;
+} lmo_jtable;
+
+typedef struct lmo_species {
+    int32_t nspecies, _pad;
+    int64_t counts[LMH_MAX_SPECIES];
+    double  mass[LMH_MAX_SPECIES];
+    lmo_table  drag[LMH_MAX_SPECIES];
+    lmo_table  rta[LMH_MAX_SPECIES];
+    lmo_jtable jt[LMH_MAX_SPECIES];
+} lmo_species;
+
+/* ---- scalar building blocks ---------------------------------------------------------------- */
+double lmo_fmGeV(void);
+int64_t lmo_searchsortedlast(const double* v, int64_t n, double x);
+double lmo_interpolate_2d(const double* x, int32_t nx, const double* y, int32_t ny,
+                          const double* values, double xi, double yi);
+double lmo_interpolate_2d_gc(const double* x, int32_t nx, const double* y, int32_t ny,
+                             const double* values, double xi, double yi);
+double lmo_eval_time_table(double T, double Tmin, double invdT, const double* vals, int32_t n);
+double lmo_tau_drag(double T, double m, double DsT);
+double lmo_effective_DsT(double T, double DsT, int32_t linear, double slope, double offset,
+                         double Tfo, int32_t quad, double Tref, int32_t* err);
+double lmo_besselkx(double nu, double z);
+double lmo_tau_n_main3(double T, double m, double DsT);
+int lmo_build_time_spline(int32_t which, double m, double DsT, double Tmin, double Tmax,
+                          int32_t nT, int32_t linear, double slope, double offset, double Tfo,
+                          int32_t quad, double Tref, double scale,
+                          double* Tmin_out, double* invdT_out, double* vals);
+int lmo_build_juttner_invcdf(double m, int32_t d, double Tmin, double Tmax, int32_t nT,
+                             int32_t nU, int32_t np, double quantile_cap,
+                             double* invcdf, double* Tmin_out, double* invdT_out);
+double lmo_juttner_invcdf_lookup(const double* invcdf, int32_t nU, int32_t nT,
+                                 double Ttab_min, double Ttab_invdT, double u, double T);
+int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int64_t n,
+                               double* out);
+
+/* ---- Philox4x64-10 and the noise record ------------------------------------------------------ */
+void lmo_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim);
+void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
+                      double* rec);
+
+/* ---- Cartesian step on the radial (r, t) medium: kernels_cpu.jl + simulate_cpu.jl ------------ */
+void lmo_kernel_boost_to_rest(double* mom, const double* pos, const lmo_radial_medium* med,
+                              double m, int64_t N, int64_t step, double dt, double t0,
+                              int32_t dims, int32_t pdim, int32_t radial_mode, double psi2,
+                              int32_t relativistic);
+void lmo_kernel_boost_to_lab(double* mom, const double* pos, const lmo_radial_medium* med,
+                             double m, int64_t N, int64_t step, double dt, double t0,
+                             int32_t dims, int32_t pdim, int32_t radial_mode, double psi2,
+                             int32_t relativistic);
+int lmo_boost_to_lab_all(const lmh_params* p, const lmo_radial_medium* med,
+                         const lmo_species* sp, int32_t dims, int32_t pdim,
+                         double* pos, double* mom, int64_t N, int64_t step);
+int lmo_run_cartesian(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+                      int32_t dims, int32_t pdim, double* pos, double* mom, int64_t N,
+                      int64_t step_begin, int64_t nsteps, const double* noise,
+                      int64_t* bad_step);
+
+/* ---- general-coordinate (tau, r) step: kernels_cpu_GC.jl + simulate_cpu_general_coords.jl ---- */
+int lmo_run_gc(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+               double* pos, double* mom, int64_t N, int64_t step_begin, int64_t nsteps,
+               const double* noise, int64_t* bad_step);
+
+/* ---- 2+1D grid medium (extension) ------------------------------------------------------------ */
+void lmo_grid_sample(const lmo_grid_medium* g, double x, double y, double t, double out[3]);
+int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
+                 int32_t pdim, double* pos, double* mom, int64_t N, int64_t step_begin,
+                 int64_t nsteps, const double* noise, int64_t* bad_step);
+
+/* ---- observables ----------------------------------------------------------------------------- */
+int lmo_observables(const double* mom, int32_t pdim, int64_t i0, int64_t i1,
+                    const lmh_obs_spec* spec, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
