@@ -113,7 +113,8 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--spl", type=int, default=1, help="steps per kernel launch")
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--slice", type=int, default=0)
+    ap.add_argument("--slice", type=int, default=1)
+    ap.add_argument("--defer", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
@@ -136,7 +137,7 @@ def main():
     tabs = tables_for(g, masses, DsT, 3)
     params = make_params(DsT)
     ctx = lmhip.Context(0)
-    ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice)
+    ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice, defer=args.defer)
     ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
     for k, (drag, rta, jt) in enumerate(tabs):
         ctx.set_time_table(_ffi.LMH_TABLE_DRAG, k, *drag)
@@ -163,23 +164,28 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     ms = ev0.elapsed_time(ev1)
+    # the same number of steps again with every launch bracketed by events on the library's
+    # stream: the device time of the step kernel alone
+    ctx.profile_start()
+    ctx.run(params, 1 + args.warmup + args.steps, args.steps, None, args.spl)
+    prof = ctx.profile_stop()
     bad = ctx.nonfinite_step()
     if bad >= 0:
         raise RuntimeError(f"non-finite state at step {bad}")
     obs = ctx.observables(40, 0.0, 10.0, 0.0, -1)
     ctx.close()
 
-    launches = math.ceil(args.steps / max(1, args.spl)) * (2 if args.slice and args.spl == 1 else 1)
-    kernel_launches = math.ceil(args.steps / max(1, args.spl))
     value = N * args.steps / (ms / 1e3)
-    t_launch = ms / 1e3 / kernel_launches
+    kernel_launches = prof["step"]["launches"]
+    t_launch = prof["step"]["ms"] / 1e3 / kernel_launches
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
     achieved = bytes_per_launch / t_launch / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": args.traffic,
-            "kernel": "k_cart_grid<3,PHILOX>", "bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": round(t_launch * 1e3, 4), "steps_per_launch": max(1, args.spl)}
+            "kernel": "k_cart_grid (PDIM=3, Philox, Langevin+RTA)", "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(t_launch * 1e3, 4), "steps_per_launch": max(1, args.spl),
+            "other_kernels_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in prof.items() if k != "step"}}
     out = {
         "metric": "particle-steps/s at 1e7 quarks, 2+1D medium",
         "value": round(value, 1),
@@ -197,7 +203,7 @@ def main():
                                f"{N} quarks (70% charm m=1.5, 30% bottom m=4.75); Langevin D_s(2piT)=4 + "
                                "RTA tau_R=5(eta/s)/T; dt=0.01 fm/c; Philox noise; FP64",
                    "quarks": N, "grid": [200, 200, 120], "block": args.block,
-                   "steps_per_launch": max(1, args.spl), "slice": args.slice,
+                   "steps_per_launch": max(1, args.spl), "slice": args.slice, "defer": args.defer,
                    "wall_s_timed": round(wall, 4), "mean_pT": round(obs[1] / obs[0], 6)},
         "roofline": roof,
     }

@@ -137,8 +137,20 @@ typedef struct lmh_tuning {
     int32_t steps_per_launch;  /* steps a particle stays in registers when lmh_run gets 0 (1)  */
     int32_t slice;             /* 2+1D grid, one-step launches: 1 interpolates the medium in tau
                                   once per step into a slice the particles read (default 0)    */
-    int32_t _pad;
+    int32_t defer;             /* 2+1D grid, one-step launches with RTA: 1 queues the colliding
+                                  particles and finishes them in a second kernel, so the
+                                  redraw runs in colliding lanes only (default 1)              */
 } lmh_tuning;
+
+/* device time per launch class, recorded with HIP events on the context's stream between
+ * lmh_profile_start and lmh_profile_stop                                                     */
+#define LMH_PROF_STEP      0   /* the step kernels                                         */
+#define LMH_PROF_REDRAW    1   /* the deferred-collision kernels                           */
+#define LMH_PROF_SLICE     2   /* the medium-slice kernels                                 */
+typedef struct lmh_profile {
+    int64_t launches[3];
+    double  ms[3];
+} lmh_profile;
 
 typedef struct lmh_ctx lmh_ctx;
 
@@ -206,6 +218,10 @@ int lmh_run_device_noise(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin,
                          int64_t nsteps, const double* d_noise, int32_t steps_per_launch);
 /* step of the first non-finite state seen since the last lmh_set_particles, or -1           */
 int64_t lmh_nonfinite_step(lmh_ctx* ctx);
+
+/* ---- timing of the launches (adds two events per launch while active) -------------------- */
+int lmh_profile_start(lmh_ctx* ctx);
+int lmh_profile_stop(lmh_ctx* ctx, lmh_profile* out);     /* synchronises */
 
 /* ---- observables ------------------------------------------------------------------------- */
 /* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */

@@ -49,11 +49,15 @@ struct StepArgs {
     int64_t block0[LMH_MAX_SPECIES + 1];
     const double* noise;
     unsigned long long* bad;    // first non-finite step (atomicMin), ULLONG_MAX if none
+    const SpeciesArgs* species; // per-species mass and tables, device memory
+    int32_t* queue;             // deferred collisions: particle indices
+    unsigned int* qcount;       // two counters, used by alternate steps
     lmh_params p;
-    SpeciesArgs sp[LMH_MAX_SPECIES];
-    double vmax;                // sqrt(1 - 1e-12), kernels_cpu.jl:102
     double sqrt_dt;
 };
+
+// |v| <= sqrt(1 - 1e-12), kernels_cpu.jl:102
+__device__ __forceinline__ double vmax_() { return sqrt(1.0 - 1e-12); }
 
 // this block's species segment and this thread's particle (-1 past the segment's end)
 __device__ __forceinline__ int64_t particle_of_thread(const StepArgs& a, int* seg) {
@@ -86,32 +90,25 @@ struct GridMedium {
 };
 
 // ---------------------------------------------------------------------------------------------
-// noise of one particle-step (the record layout of include/lmhip.h)
-template <int DIMS, int PDIM, int NOISE>
-__device__ __forceinline__ void fetch_noise(const StepArgs& a, int64_t i, int64_t step, int s_off,
-                                            double* xi, double& ucol, double& usamp, double* g,
-                                            double* xp) {
+// Noise of one particle-step, in three parts generated where they are used: the kick normals
+// and the collision uniform (block 0), the redraw's quantile and direction (block 1, colliding
+// particles only) and the position-diffusion normals (block 2).  FIXED reads the record layout
+// of include/lmhip.h; PHILOX generates the same values.
+template <int PDIM, int NOISE>
+__device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t step, int s_off,
+                                           double* xi, double* ucol) {
     if (NOISE == LMH_NOISE_FIXED) {
         const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
         if (a.oxi >= 0) {
 #pragma unroll
             for (int d = 0; d < PDIM; ++d) xi[d] = rec[(int64_t)(a.oxi + d) * a.N];
         }
-        if (a.orta >= 0) {
-            ucol = rec[(int64_t)a.orta * a.N];
-            usamp = rec[(int64_t)(a.orta + 1) * a.N];
-#pragma unroll
-            for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
-        }
-        if (a.opos >= 0) {
-#pragma unroll
-            for (int d = 0; d < DIMS; ++d) xp[d] = rec[(int64_t)(a.opos + d) * a.N];
-        }
+        if (a.orta >= 0) *ucol = rec[(int64_t)a.orta * a.N];
     } else {
+        if (a.oxi < 0 && a.orta < 0) return;
         uint64_t w[4];
-        const uint64_t k0 = a.p.seed, k1 = a.p.stream;
+        philox4x64((uint64_t)i, (uint64_t)step, 0, 0, a.p.seed, a.p.stream, w);
         if (a.oxi >= 0) {
-            philox4x64((uint64_t)i, (uint64_t)step, 0, 0, k0, k1, w);
             double n0, n1;
             box_muller(w[0], w[1], &n0, &n1);
             xi[0] = n0;
@@ -122,24 +119,52 @@ __device__ __forceinline__ void fetch_noise(const StepArgs& a, int64_t i, int64_
                 xi[2 % PDIM] = n2;
             }
         }
-        if (a.orta >= 0) {
-            philox4x64((uint64_t)i, (uint64_t)step, 1, 0, k0, k1, w);
-            ucol = u_co(w[0]);
-            usamp = u_co(w[1]);
-            double g0, g1;
-            box_muller(w[2], w[3], &g0, &g1);
-            g[0] = g0;
-            if (PDIM >= 2) g[1 % PDIM] = g1;
+        if (a.orta >= 0) *ucol = u_low44(w);
+    }
+}
+
+template <int PDIM, int NOISE>
+__device__ __forceinline__ void noise_redraw(const StepArgs& a, int64_t i, int64_t step, int s_off,
+                                             double* usamp, double* g) {
+    if (NOISE == LMH_NOISE_FIXED) {
+        const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
+        *usamp = rec[(int64_t)(a.orta + 1) * a.N];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
+    } else {
+        uint64_t w[4];
+        philox4x64((uint64_t)i, (uint64_t)step, 1, 0, a.p.seed, a.p.stream, w);
+        double n0, n1;
+        box_muller(w[0], w[1], &n0, &n1);
+        g[0] = n0;
+        if (PDIM >= 2) g[1 % PDIM] = n1;
+        if (PDIM >= 3) {
+            double n2, n3;
+            box_muller(w[2], w[3], &n2, &n3);
+            g[2 % PDIM] = n2;
         }
-        if ((a.orta >= 0 && PDIM == 3) || a.opos >= 0) {
-            philox4x64((uint64_t)i, (uint64_t)step, 2, 0, k0, k1, w);
-            double b0, b1, b2, b3;
-            box_muller(w[0], w[1], &b0, &b1);
-            box_muller(w[2], w[3], &b2, &b3);
-            if (PDIM == 3) g[2 % PDIM] = b0;
-            xp[0] = b1;
-            if (DIMS >= 2) xp[1 % DIMS] = b2;
-            if (DIMS >= 3) xp[2 % DIMS] = b3;
+        *usamp = u_low44(w);
+    }
+}
+
+template <int DIMS, int NOISE>
+__device__ __forceinline__ void noise_diffusion(const StepArgs& a, int64_t i, int64_t step, int s_off,
+                                                double* xp) {
+    if (NOISE == LMH_NOISE_FIXED) {
+        const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) xp[d] = rec[(int64_t)(a.opos + d) * a.N];
+    } else {
+        uint64_t w[4];
+        philox4x64((uint64_t)i, (uint64_t)step, 2, 0, a.p.seed, a.p.stream, w);
+        double n0, n1;
+        box_muller(w[0], w[1], &n0, &n1);
+        xp[0] = n0;
+        if (DIMS >= 2) xp[1 % DIMS] = n1;
+        if (DIMS >= 3) {
+            double n2, n3;
+            box_muller(w[2], w[3], &n2, &n3);
+            xp[2 % DIMS] = n2;
         }
     }
 }
@@ -198,12 +223,12 @@ __device__ __forceinline__ bool finite_all(const double* a, int n) {
 
 // ---------------------------------------------------------------------------------------------
 // Cartesian step on the radial (r, t) medium: the reference's own medium (simulate_cpu.jl).
-template <int DIMS, int PDIM, int NOISE>
+template <int DIMS, int PDIM, int NOISE, int COLL>
 __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium med) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
-    const SpeciesArgs& sp = a.sp[seg];
+    const SpeciesArgs sp = a.species[seg];
     constexpr bool RADIAL = DIMS == 1;
     const lmh_params& p = a.p;
     const double m = sp.m;
@@ -218,9 +243,9 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
         const double t_now = (double)step * p.dt + p.t0;
         double yd;
         const int j = axis_cell(med.tg, med.nt, t_now, &yd);
-        double xi[PDIM], g[PDIM], xp[DIMS];
-        double ucol = 1.0, usamp = 0.0;
-        fetch_noise<DIMS, PDIM, NOISE>(a, i, step, s_off, xi, ucol, usamp, g, xp);
+        double xi[PDIM];
+        double ucol = 1.0;
+        noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
 
         double r2 = 0.0;
 #pragma unroll
@@ -237,11 +262,11 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                 double phi = atan2(x[1 % DIMS], x[0]);
                 v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - p.psi2)));
             }
-            radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, -1, p.relativistic, a.vmax);
+            radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, -1, p.relativistic, vmax_());
         }
         if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
 
-        if (p.collision == LMH_FLUID) {
+        if (COLL == LMH_FLUID) {
             // kernel_set_to_fluid_velocity_cpu!, kernels_cpu.jl:683-733: r from rows 1-2
             double rf = RADIAL ? x[0] : sqrt(x[0] * x[0] + x[1 % DIMS] * x[1 % DIMS]);
             if (!(rf < kEps)) {
@@ -256,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                         q[1 % PDIM] = m * vf * (x[1 % DIMS] / rf);
                     }
                 } else {
-                    vf = jl_clamp(vf, -a.vmax, a.vmax);
+                    vf = jl_clamp(vf, -vmax_(), vmax_());
                     double gam = 1.0 / sqrt(1.0 - vf * vf + 1e-10);
                     if (RADIAL) {
                         q[0] = m * gam * vf;
@@ -268,14 +293,17 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             }
         } else {
             const double T = jl_max(bilinear(med.T, med.nx, ir, j, xd, yd), 0.0);
-            if (p.collision == LMH_LANGEVIN || p.collision == LMH_LANGEVIN_RTA) {
+            if (COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA) {
                 double etaD, kT;
                 drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
                 ou_kick<PDIM, RADIAL>(p, etaD, kT, m, a.sqrt_dt, q, xi);
             }
-            if (p.collision == LMH_RTA || p.collision == LMH_LANGEVIN_RTA)
-                rta_redraw<PDIM, RADIAL>(p, sp.rta, sp.jt, T, q, ucol, usamp, g);
-            if (!skip) radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, +1, p.relativistic, a.vmax);
+            if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) && rta_collides(p, sp.rta, T, ucol)) {
+                double usamp, g[PDIM];
+                noise_redraw<PDIM, NOISE>(a, i, step, s_off, &usamp, g);
+                rta_redraw<PDIM, RADIAL>(sp.jt, T, q, usamp, g);
+            }
+            if (!skip) radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, +1, p.relativistic, vmax_());
         }
 
         // kernel_update_positions_cpu!, kernels_cpu.jl:535-636
@@ -294,7 +322,11 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                 int ird = axis_cell(med.xg, med.nx, r_safe, &xdd);
                 double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
                 double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
-                if (D > 0.0) dr += (D / r_safe) * p.dt + sqrt(2.0 * D * p.dt) * xp[0];
+                if (D > 0.0) {
+                    double xp[DIMS];
+                    noise_diffusion<DIMS, NOISE>(a, i, step, s_off, xp);
+                    dr += (D / r_safe) * p.dt + sqrt(2.0 * D * p.dt) * xp[0];
+                }
             }
             double r_new = r0 + dr;
             if (r_new < 0.0) {
@@ -323,6 +355,8 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                 double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
                 double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
                 if (D > 0.0) {
+                    double xp[DIMS];
+                    noise_diffusion<DIMS, NOISE>(a, i, step, s_off, xp);
                     double sigma = sqrt(2.0 * D * p.dt);
 #pragma unroll
                     for (int d = 0; d < DIMS; ++d) x[d] = x[d] + sigma * xp[d];
@@ -361,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
-    const SpeciesArgs& sp = a.sp[seg];
+    const SpeciesArgs sp = a.species[seg];
     constexpr bool RADIAL = DIMS == 1;
     const lmh_params& p = a.p;
     double x[DIMS], q[PDIM];
@@ -384,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
         double phi = atan2(x[1 % DIMS], x[0]);
         v *= (1.0 + 2.0 * v2m * cos(2.0 * (phi - p.psi2)));
     }
-    radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, sp.m, +1, p.relativistic, a.vmax);
+    radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, sp.m, +1, p.relativistic, vmax_());
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
 }
@@ -445,34 +479,111 @@ __device__ __forceinline__ void grid_sample(const GridMedium& g, double x, doubl
     *vy = e0 * (1.0 - yd) + e1 * yd;
 }
 
-// boost along v_hat = (vx, vy)/|v| with speed |v| (the reference's r_hat boost with
-// r_hat -> v_hat), skipped when |v| < eps
-template <int PDIM>
-__device__ __forceinline__ void grid_boost(double* q, double m, double vx, double vy, int sign,
-                                           int relativistic, double vmax) {
+// The frame change on the grid: the reference's boost along r_hat with the radial speed
+// (kernels_cpu.jl:101-144, 199-241) with r_hat -> v_hat = (vx, vy)/|v| and v -> |v|, skipped when
+// |v| < eps as the reference skips r < eps.  The flow-dependent factors are formed once per step
+// and applied for both directions.
+struct FlowBoost {
+    bool on;
+    double nx, ny, vmag, v, gam;
+};
+
+__device__ __forceinline__ FlowBoost flow_boost(double vx, double vy) {
+    FlowBoost b;
     double vm2 = vx * vx + vy * vy;
-    double vmag = sqrt(vm2);
-    if (vmag < kEps) return;
-    double nx = vx / vmag, ny = vy / vmag;
+    b.vmag = sqrt(vm2);
+    b.on = !(b.vmag < kEps);
+    b.nx = vx / b.vmag;
+    b.ny = vy / b.vmag;
+    double vmax = vmax_();
+    b.v = b.vmag > vmax ? vmax : b.vmag;
+    b.gam = 1.0 / sqrt(1.0 - b.v * b.v + 1e-10);
+    return b;
+}
+
+template <int PDIM>
+__device__ __forceinline__ void apply_boost(double* q, double m, const FlowBoost& b, int sign,
+                                            int relativistic) {
+    if (!b.on) return;
     if (!relativistic) {
         double s = sign > 0 ? 1.0 : -1.0;
-        q[0] += s * (m * vmag * nx);
-        q[1] += s * (m * vmag * ny);
+        q[0] += s * (m * b.vmag * b.nx);
+        q[1] += s * (m * b.vmag * b.ny);
         return;
     }
-    double v = vmag > vmax ? vmax : vmag;
-    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
     double p2 = 0.0;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
     double E = sqrt(p2 + m * m);
     double px = q[0], py = q[1];
-    double p_par = px * nx + py * ny;
-    double p_perp_x = px - p_par * nx;
-    double p_perp_y = py - p_par * ny;
-    double p_par_new = sign > 0 ? gam * (p_par + v * E) : gam * (p_par - v * E);
-    q[0] = p_par_new * nx + p_perp_x;
-    q[1] = p_par_new * ny + p_perp_y;
+    double p_par = px * b.nx + py * b.ny;
+    double p_perp_x = px - p_par * b.nx;
+    double p_perp_y = py - p_par * b.ny;
+    double p_par_new = sign > 0 ? b.gam * (p_par + b.v * E) : b.gam * (p_par - b.v * E);
+    q[0] = p_par_new * b.nx + p_perp_x;
+    q[1] = p_par_new * b.ny + p_perp_y;
+}
+
+// kernel_set_to_fluid_velocity_cpu! (kernels_cpu.jl:713-731) with r_hat -> v_hat; a fluid at
+// rest sets the transverse momentum to 0
+template <int PDIM>
+__device__ __forceinline__ void glue_to_flow(double* q, double m, double vx, double vy, int relativistic) {
+    double vmag = sqrt(vx * vx + vy * vy);
+    if (vmag < kEps) {
+        q[0] = 0.0;
+        q[1] = 0.0;
+        return;
+    }
+    double nx = vx / vmag, ny = vy / vmag;
+    if (!relativistic) {
+        q[0] = m * vmag * nx;
+        q[1] = m * vmag * ny;
+        return;
+    }
+    double vmax = vmax_();
+    double v = vmag > vmax ? vmax : vmag;
+    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
+    q[0] = m * gam * v * nx;
+    q[1] = m * gam * v * ny;
+}
+
+// position push (kernels_cpu.jl:537-541, 586-607) with the optional position diffusion
+template <int PDIM, int NOISE, bool SLICE>
+__device__ __forceinline__ void grid_push(const StepArgs& a, const GridMedium& g, int64_t i,
+                                          int64_t step, int s_off, int j, double td, double m,
+                                          double* x0, double* x1, const double* q) {
+    const lmh_params& p = a.p;
+    double E2 = m * m;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
+    const double E = p.relativistic ? sqrt(E2) : m;
+    *x0 = *x0 + p.dt * q[0] / E;
+    *x1 = *x1 + p.dt * q[1] / E;
+    if (p.position_diffusion) {
+        double Td, ux, uy;
+        grid_sample<SLICE>(g, *x0, *x1, j, td, &Td, &ux, &uy);
+        double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
+        if (D > 0.0) {
+            double xp[2];
+            noise_diffusion<2, NOISE>(a, i, step, s_off, xp);
+            double sigma = sqrt(2.0 * D * p.dt);
+            *x0 = *x0 + sigma * xp[0];
+            *x1 = *x1 + sigma * xp[1];
+        }
+    }
+}
+
+// slot of this lane in the deferred-collision queue (one atomic per wave), -1 if not queued
+__device__ __forceinline__ int64_t queue_slot(unsigned int* count, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ull) return -1;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = mask & ((1ull << lane) - 1ull);
+    return want ? (int64_t)base + __popcll(below) : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
@@ -482,13 +593,17 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* sl
     slice[k] = lerp4(g.nodes[nxy * j + k], g.nodes[nxy * (j + 1) + k], td);
 }
 
-template <int PDIM, int NOISE, bool SLICE>
+// The step on the 2+1D grid.  With DEFER (one step per launch), particles that collide are
+// left in the LRF after the kick and queued; k_redraw_grid finishes them (redraw, boost back,
+// push), so that the redraw's noise and table lookups run only in colliding lanes.
+template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER>
 __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
-    const SpeciesArgs& sp = a.sp[seg];
-    constexpr int DIMS = 2;
+    const SpeciesArgs sp = a.species[seg];
+    constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
+    constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
     const lmh_params& p = a.p;
     const double m = sp.m;
     double x0 = a.pos[0][i], x1 = a.pos[1][i];
@@ -501,58 +616,43 @@ __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) 
         const double t_now = (double)step * p.dt + p.t0;
         double td;
         const int j = axis_cell(g.tg, g.nt, t_now, &td);
-        double xi[PDIM], gg[PDIM], xp[DIMS];
-        double ucol = 1.0, usamp = 0.0;
-        fetch_noise<DIMS, PDIM, NOISE>(a, i, step, s_off, xi, ucol, usamp, gg, xp);
+        double xi[PDIM];
+        double ucol = 1.0;
+        if (HAS_KICK || HAS_RTA) noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
 
         double Tm, vx, vy;
         grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
-        grid_boost<PDIM>(q, m, vx, vy, -1, p.relativistic, a.vmax);
+        const FlowBoost fb = flow_boost(vx, vy);
+        apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
         if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
-        if (p.collision == LMH_FLUID) {
-            double vmag = sqrt(vx * vx + vy * vy);
-            if (vmag < kEps) {
-                q[0] = 0.0;
-                q[1] = 0.0;
-            } else {
-                double nx = vx / vmag, ny = vy / vmag;
-                if (!p.relativistic) {
-                    q[0] = m * vmag * nx;
-                    q[1] = m * vmag * ny;
-                } else {
-                    double v = vmag > a.vmax ? a.vmax : vmag;
-                    double gam = 1.0 / sqrt(1.0 - v * v + 1e-10);
-                    q[0] = m * gam * v * nx;
-                    q[1] = m * gam * v * ny;
-                }
-            }
+        if (COLL == LMH_FLUID) {
+            glue_to_flow<PDIM>(q, m, vx, vy, p.relativistic);
         } else {
             const double T = jl_max(Tm, 0.0);
-            if (p.collision == LMH_LANGEVIN || p.collision == LMH_LANGEVIN_RTA) {
+            if (HAS_KICK) {
                 double etaD, kT;
                 drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
                 ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
             }
-            if (p.collision == LMH_RTA || p.collision == LMH_LANGEVIN_RTA)
-                rta_redraw<PDIM, false>(p, sp.rta, sp.jt, T, q, ucol, usamp, gg);
-            grid_boost<PDIM>(q, m, vx, vy, +1, p.relativistic, a.vmax);
-        }
-        double E2 = m * m;
+            if (HAS_RTA) {
+                const bool col = rta_collides(p, sp.rta, T, ucol);
+                if (DEFER) {
+                    const int64_t slot = queue_slot(&a.qcount[step & 1], col);
+                    if (slot >= 0) {
+                        a.queue[slot] = (int32_t)i;
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
-        const double E = p.relativistic ? sqrt(E2) : m;
-        x0 = x0 + p.dt * q[0] / E;
-        x1 = x1 + p.dt * q[1] / E;
-        if (p.position_diffusion) {
-            double Td, ux, uy;
-            grid_sample<SLICE>(g, x0, x1, j, td, &Td, &ux, &uy);
-            double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
-            if (D > 0.0) {
-                double sigma = sqrt(2.0 * D * p.dt);
-                x0 = x0 + sigma * xp[0];
-                x1 = x1 + sigma * xp[1];
+                        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+                        return;
+                    }
+                } else if (col) {
+                    double usamp, gg[PDIM];
+                    noise_redraw<PDIM, NOISE>(a, i, step, s_off, &usamp, gg);
+                    rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+                }
             }
+            apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
         }
+        grid_push<PDIM, NOISE, SLICE>(a, g, i, step, s_off, j, td, m, &x0, &x1, q);
         if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
     }
     a.pos[0][i] = x0;
@@ -562,12 +662,59 @@ __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) 
     if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
 }
 
+// The deferred collisions of one step: the redraw of kernel_rta_collision_gpu!
+// (kernels_gpu.jl:673-701), the boost back and the push, for every queued particle.  The
+// particle's medium sample and flow factors are recomputed from the same inputs, so the result is
+// the one the fused path gives.  Also clears the other step parity's counter.
+template <int PDIM, int NOISE, bool SLICE>
+__global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g) {
+    const int64_t step = a.step_begin;
+    const unsigned int count = a.qcount[step & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.qcount[(step + 1) & 1] = 0u;
+    const lmh_params& p = a.p;
+    const double t_now = (double)step * p.dt + p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (int64_t)count;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = a.queue[k];
+        int seg = 0;
+#pragma unroll
+        for (int s = 1; s < LMH_MAX_SPECIES; ++s)
+            if (s < a.nseg && i >= a.begin[s]) seg = s;
+        const SpeciesArgs sp = a.species[seg];
+        const double m = sp.m;
+        double x0 = a.pos[0][i], x1 = a.pos[1][i];
+        double q[PDIM];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+        double Tm, vx, vy;
+        grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
+        const FlowBoost fb = flow_boost(vx, vy);
+        const double T = jl_max(Tm, 0.0);
+        double usamp, gg[PDIM];
+        noise_redraw<PDIM, NOISE>(a, i, step, 0, &usamp, gg);
+        rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+        apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
+        grid_push<PDIM, NOISE, SLICE>(a, g, i, step, 0, j, td, m, &x0, &x1, q);
+        a.pos[0][i] = x0;
+        a.pos[1][i] = x1;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+        if (!(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) atomicMin(a.bad, (unsigned long long)step);
+    }
+}
+
+__global__ void k_clear_counts(unsigned int* qcount) {
+    if (threadIdx.x < 2) qcount[threadIdx.x] = 0u;
+}
+
 template <int PDIM>
 __global__ __launch_bounds__(kBlock) void k_boost_lab_grid(StepArgs a, GridMedium g, int64_t step) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
-    const SpeciesArgs& sp = a.sp[seg];
+    const SpeciesArgs sp = a.species[seg];
     const lmh_params& p = a.p;
     double q[PDIM];
 #pragma unroll
@@ -577,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_grid(StepArgs a, GridMediu
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
     double Tm, vx, vy;
     grid_sample<false>(g, a.pos[0][i], a.pos[1][i], j, td, &Tm, &vx, &vy);
-    grid_boost<PDIM>(q, sp.m, vx, vy, +1, p.relativistic, a.vmax);
+    apply_boost<PDIM>(q, sp.m, flow_boost(vx, vy), +1, p.relativistic);
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
 }
@@ -591,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_gc(StepArgs a, RadialMedium med) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
-    const SpeciesArgs& sp = a.sp[seg];
+    const SpeciesArgs sp = a.species[seg];
     const lmh_params& p = a.p;
     const double m = sp.m;
     double tau = a.pos[0][i], rr = a.pos[1][i];
@@ -748,13 +895,27 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 0, 0};
+    lmh_tuning tune = {256, 1, 0, 1};
 
     // tables
     struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
     struct JTab { double* d = nullptr; double Tmin = 0, invdT = 1; int nU = 0, nT = 0; };
     Tab drag[LMH_MAX_SPECIES], rta[LMH_MAX_SPECIES];
     JTab jt[LMH_MAX_SPECIES];
+
+    // per-species mass and tables as the kernels read them
+    SpeciesArgs h_species[LMH_MAX_SPECIES];
+    SpeciesArgs* d_species = nullptr;
+    // deferred collisions: particle indices and the two step-parity counters
+    int32_t* d_queue = nullptr;
+    unsigned int* d_qcount = nullptr;
+
+    // launch timing
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Mark { int cls; hipEvent_t a, b; };
+    std::vector<Mark> marks;
 
     // scratch
     double* d_noise = nullptr;
@@ -818,6 +979,24 @@ int64_t species_begin(const lmh_ctx* c, int k) {
     return b;
 }
 
+// the kernels' view of the species (masses and table pointers), copied to the device whenever a
+// table or the particles change
+int upload_species(lmh_ctx* c) {
+    for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
+        SpeciesArgs& s = c->h_species[k];
+        s.m = c->mass[k];
+        s.drag = TimeTable{c->drag[k].d, c->drag[k].Tmin, c->drag[k].invdT, c->drag[k].n};
+        s.rta = TimeTable{c->rta[k].d, c->rta[k].Tmin, c->rta[k].invdT, c->rta[k].n};
+        s.jt = JuttnerTable{c->jt[k].d, c->jt[k].Tmin, c->jt[k].invdT, c->jt[k].nU, c->jt[k].nT};
+    }
+    if (!c->d_species && hipMalloc((void**)&c->d_species, sizeof(c->h_species)) != hipSuccess) {
+        c->d_species = nullptr;
+        return c->fail(LMH_E_NOMEM, "species table allocation failed");
+    }
+    LMH_HIP(c, hipMemcpy(c->d_species, c->h_species, sizeof(c->h_species), hipMemcpyHostToDevice));
+    return LMH_OK;
+}
+
 // kernel arguments for all species segments; *nblocks receives the grid size
 StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks) {
     StepArgs a;
@@ -838,15 +1017,13 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
             b += c->counts[k];
             blk += (c->counts[k] + block - 1) / block;
         }
-        a.sp[k].m = c->mass[k];
-        a.sp[k].drag = TimeTable{c->drag[k].d, c->drag[k].Tmin, c->drag[k].invdT, c->drag[k].n};
-        a.sp[k].rta = TimeTable{c->rta[k].d, c->rta[k].Tmin, c->rta[k].invdT, c->rta[k].n};
-        a.sp[k].jt = JuttnerTable{c->jt[k].d, c->jt[k].Tmin, c->jt[k].invdT, c->jt[k].nU, c->jt[k].nT};
     }
     a.begin[LMH_MAX_SPECIES] = b;
     a.block0[LMH_MAX_SPECIES] = blk;
     // segments past nseg are empty and never selected
-    a.vmax = std::sqrt(1.0 - 1e-12);
+    a.species = c->d_species;
+    a.queue = c->d_queue;
+    a.qcount = c->d_qcount;
     a.sqrt_dt = std::sqrt(p->dt);
     a.C = noise_layout(p, c->dims, c->pdim, &a.oxi, &a.orta, &a.opos);
     *nblocks = (unsigned)blk;
@@ -901,6 +1078,32 @@ void host_time_cell(const std::vector<double>& tg, double t, int* j, double* td)
 }
 
 unsigned grid_blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+hipEvent_t next_event(lmh_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// brackets one launch with events while profiling
+struct LaunchMark {
+    lmh_ctx* c;
+    int cls;
+    hipEvent_t a = nullptr;
+    LaunchMark(lmh_ctx* ctx, int k) : c(ctx), cls(k) {
+        if (c->profiling && (a = next_event(c))) (void)hipEventRecord(a, c->stream);
+    }
+    ~LaunchMark() {
+        if (!a) return;
+        hipEvent_t b = next_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->marks.push_back({cls, a, b});
+    }
+};
 
 }  // namespace
 
@@ -958,6 +1161,10 @@ void lmh_destroy(lmh_ctx* c) {
     }
     dfree(c->d_noise);
     dfree(c->d_obs);
+    dfree(c->d_species);
+    dfree(c->d_queue);
+    dfree(c->d_qcount);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1059,7 +1266,7 @@ int lmh_set_time_table(lmh_ctx* c, int32_t which, int32_t species, double Tmin, 
     t.Tmin = Tmin;
     t.invdT = invdT;
     t.n = n;
-    return LMH_OK;
+    return upload_species(c);
 }
 
 int lmh_set_juttner_table(lmh_ctx* c, int32_t species, const double* invcdf, int32_t nU,
@@ -1076,7 +1283,7 @@ int lmh_set_juttner_table(lmh_ctx* c, int32_t species, const double* invcdf, int
     t.invdT = invdT;
     t.nU = nU;
     t.nT = nT;
-    return LMH_OK;
+    return upload_species(c);
 }
 
 int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double* mom, int32_t pdim,
@@ -1093,6 +1300,7 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         sum += counts[k];
     }
     if (sum != N) return c->fail(LMH_E_INVALID, "set_particles: species counts must sum to N");
+    if (N > (int64_t)INT32_MAX) return c->fail(LMH_E_INVALID, "set_particles: at most 2^31-1 particles per context");
     LMH_HIP(c, hipSetDevice(c->device));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
     size_t rows = (size_t)(dims + pdim);
@@ -1123,7 +1331,20 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         c->counts[k] = k < nspecies ? counts[k] : 0;
         c->mass[k] = k < nspecies ? masses[k] : 0.0;
     }
-    return LMH_OK;
+    dfree(c->d_queue);
+    if (hipMalloc((void**)&c->d_queue, (size_t)N * sizeof(int32_t)) != hipSuccess) {
+        c->d_queue = nullptr;
+        return c->fail(LMH_E_NOMEM, "set_particles: queue allocation failed");
+    }
+    if (!c->d_qcount) {
+        if (hipMalloc((void**)&c->d_qcount, 2 * sizeof(unsigned int)) != hipSuccess) {
+            c->d_qcount = nullptr;
+            return c->fail(LMH_E_NOMEM, "set_particles: counter allocation failed");
+        }
+    }
+    unsigned int zero2[2] = {0u, 0u};
+    LMH_HIP(c, hipMemcpy(c->d_qcount, zero2, sizeof(zero2), hipMemcpyHostToDevice));
+    return upload_species(c);
 }
 
 int lmh_get_particles(lmh_ctx* c, double* pos, double* mom) {
@@ -1196,24 +1417,57 @@ int check_run(lmh_ctx* c, const lmh_params* p) {
     return LMH_OK;
 }
 
+template <int DIMS, int PDIM, int NOISE>
+void launch_radial_n(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int coll) {
+    const dim3 G(nb), B(c->tune.block);
+    switch (coll) {
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN>), G, B, 0, c->stream, a, med); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_RTA>), G, B, 0, c->stream, a, med); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_FLUID>), G, B, 0, c->stream, a, med); break;
+        default: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN_RTA>), G, B, 0, c->stream, a, med); break;
+    }
+}
+
 template <int DIMS, int PDIM>
-void launch_radial(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int noise) {
-    const int B = c->tune.block;
-    if (noise == LMH_NOISE_FIXED)
-        hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, LMH_NOISE_FIXED>), dim3(nb), dim3(B), 0, c->stream, a, med);
-    else
-        hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, LMH_NOISE_PHILOX>), dim3(nb), dim3(B), 0, c->stream, a, med);
+void launch_radial(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int noise, int coll) {
+    if (noise == LMH_NOISE_FIXED) launch_radial_n<DIMS, PDIM, LMH_NOISE_FIXED>(c, a, nb, med, coll);
+    else launch_radial_n<DIMS, PDIM, LMH_NOISE_PHILOX>(c, a, nb, med, coll);
+}
+
+template <int PDIM, int NOISE, bool SLICE>
+void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll, bool defer) {
+    const dim3 G(nb), B(c->tune.block);
+    {
+    LaunchMark mk(c, LMH_PROF_STEP);
+    if (coll == LMH_LANGEVIN) {
+        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN, false>), G, B, 0, c->stream, a, g);
+    } else if (coll == LMH_FLUID) {
+        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_FLUID, false>), G, B, 0, c->stream, a, g);
+    } else if (coll == LMH_RTA) {
+        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, true>), G, B, 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, false>), G, B, 0, c->stream, a, g);
+    } else {
+        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, true>), G, B, 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, false>), G, B, 0, c->stream, a, g);
+    }
+    }
+    if (defer) {
+        LaunchMark mk(c, LMH_PROF_REDRAW);
+        // enough lanes for a few percent of the particles per pass; the loop covers any count
+        const unsigned nr = (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (a.N / 16 + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL((k_redraw_grid<PDIM, NOISE, SLICE>), dim3(nr), dim3(kBlock), 0, c->stream, a, g);
+    }
 }
 
 template <int PDIM>
-void launch_grid(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice) {
-    const int B = c->tune.block;
+void launch_grid(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice,
+                 int coll, bool defer) {
     if (noise == LMH_NOISE_FIXED) {
-        if (slice) hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_FIXED, true>), dim3(nb), dim3(B), 0, c->stream, a, g);
-        else hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_FIXED, false>), dim3(nb), dim3(B), 0, c->stream, a, g);
+        if (slice) launch_grid_s<PDIM, LMH_NOISE_FIXED, true>(c, a, nb, g, coll, defer);
+        else launch_grid_s<PDIM, LMH_NOISE_FIXED, false>(c, a, nb, g, coll, defer);
     } else {
-        if (slice) hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_PHILOX, true>), dim3(nb), dim3(B), 0, c->stream, a, g);
-        else hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_PHILOX, false>), dim3(nb), dim3(B), 0, c->stream, a, g);
+        if (slice) launch_grid_s<PDIM, LMH_NOISE_PHILOX, true>(c, a, nb, g, coll, defer);
+        else launch_grid_s<PDIM, LMH_NOISE_PHILOX, false>(c, a, nb, g, coll, defer);
     }
 }
 
@@ -1227,6 +1481,9 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     unsigned nb = 0;
     StepArgs a = make_args(c, p, c->tune.block, &nb);
     if (nb == 0) return LMH_OK;
+    const bool has_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
+    const bool defer = grid && k == 1 && has_rta && c->tune.defer;
+    if (defer) hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, c->d_qcount);
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
         const bool slice = grid && n == 1 && c->tune.slice;
@@ -1236,24 +1493,27 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             const double t_now = (double)(step_begin + s0) * p->dt + p->t0;
             host_time_cell(tg, t_now, &j, &td);
             const int64_t nxy = (int64_t)c->gx * c->gy;
+            LaunchMark mk(c, LMH_PROF_SLICE);
             hipLaunchKernelGGL(k_grid_slice, dim3(grid_blocks(nxy)), dim3(kBlock), 0, c->stream, gmed, c->d_slice, j, td);
         }
         a.step_begin = step_begin + s0;
         a.nsteps = n;
         a.noise = d_noise ? d_noise + s0 * (int64_t)a.C * c->N : nullptr;
         if (p->coords == LMH_GENERAL_COORDS) {
+            LaunchMark mk(c, LMH_PROF_STEP);
             if (p->noise == LMH_NOISE_FIXED)
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_FIXED>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
             else
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_PHILOX>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
         } else if (grid) {
-            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice);
-            else launch_grid<3>(c, a, nb, gmed, p->noise, slice);
+            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1);
+            else launch_grid<3>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1);
         } else {
-            if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise);
-            else if (c->dims == 2 && c->pdim == 2) launch_radial<2, 2>(c, a, nb, rmed, p->noise);
-            else if (c->dims == 2 && c->pdim == 3) launch_radial<2, 3>(c, a, nb, rmed, p->noise);
-            else launch_radial<3, 3>(c, a, nb, rmed, p->noise);
+            LaunchMark mk(c, LMH_PROF_STEP);
+            if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise, p->collision);
+            else if (c->dims == 2 && c->pdim == 2) launch_radial<2, 2>(c, a, nb, rmed, p->noise, p->collision);
+            else if (c->dims == 2 && c->pdim == 3) launch_radial<2, 3>(c, a, nb, rmed, p->noise, p->collision);
+            else launch_radial<3, 3>(c, a, nb, rmed, p->noise, p->collision);
         }
         LMH_HIP(c, hipGetLastError());
     }
@@ -1274,8 +1534,8 @@ extern "C" {
 int lmh_set_tuning(lmh_ctx* c, const lmh_tuning* t) {
     if (!c) return LMH_E_INVALID;
     if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
-        (t->slice != 0 && t->slice != 1))
-        return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice in {0, 1}");
+        (t->slice != 0 && t->slice != 1) || (t->defer != 0 && t->defer != 1))
+        return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice and defer in {0, 1}");
     c->tune = *t;
     return LMH_OK;
 }
@@ -1283,6 +1543,32 @@ int lmh_set_tuning(lmh_ctx* c, const lmh_tuning* t) {
 int lmh_get_tuning(lmh_ctx* c, lmh_tuning* t) {
     if (!c || !t) return LMH_E_INVALID;
     *t = c->tune;
+    return LMH_OK;
+}
+
+int lmh_profile_start(lmh_ctx* c) {
+    if (!c) return LMH_E_INVALID;
+    c->profiling = true;
+    c->ev_used = 0;
+    c->marks.clear();
+    return LMH_OK;
+}
+
+int lmh_profile_stop(lmh_ctx* c, lmh_profile* out) {
+    if (!c) return LMH_E_INVALID;
+    if (!out) return c->fail(LMH_E_INVALID, "profile_stop: null output");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    for (const auto& m : c->marks) {
+        float ms = 0.0f;
+        LMH_HIP(c, hipEventElapsedTime(&ms, m.a, m.b));
+        out->launches[m.cls] += 1;
+        out->ms[m.cls] += ms;
+    }
+    c->profiling = false;
+    c->marks.clear();
+    c->ev_used = 0;
     return LMH_OK;
 }
 

@@ -15,7 +15,6 @@ namespace lmh {
 
 constexpr double kEps = 2.220446049250313e-16;          // Julia eps()
 constexpr double kFmGeV = 1.0 / 0.197327;                // constants.jl:13,23
-constexpr double kTwoPi = 6.283185307179586;
 
 __device__ __forceinline__ double jl_clamp(double x, double lo, double hi) {
     return x > hi ? hi : (x < lo ? lo : x);
@@ -138,11 +137,80 @@ __device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2
 __device__ __forceinline__ double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
 __device__ __forceinline__ double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
 
+// a 44-bit uniform in [0,1) from the 11 low bits of four words (the 53-bit uniforms of the
+// Box-Muller pairs use the high bits)
+__device__ __forceinline__ double u_low44(const uint64_t* w) {
+    uint64_t b = ((w[0] & 0x7FFULL) << 33) | ((w[1] & 0x7FFULL) << 22) | ((w[2] & 0x7FFULL) << 11) |
+                 (w[3] & 0x7FFULL);
+    return (double)b * 0x1.0p-44;
+}
+
+// The two transcendental functions of the Box-Muller transform, written with correctly rounded
+// operations only (+, -, *, /, sqrt, fma, floor, exact bit manipulation), so that the same
+// definition gives the same bits on the host and on the device.
+//
+// noise_log(u), u in [2^-53, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with
+// s = (m - 1)/(m + 1), |s| <= 0.1716, by the atanh series through s^19 (truncation < 3e-17).
+constexpr double kLn2Hi = 6.93147180369123816490e-01;   // e * kLn2Hi is exact for |e| < 2^11
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+
+__device__ __forceinline__ double noise_log(double u) {
+    uint64_t b = (uint64_t)__double_as_longlong(u);
+    int e = (int)((b >> 52) & 0x7FF) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL));
+    if (m > 1.4142135623730951) {
+        m = 0.5 * m;
+        e += 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    double lm = 2.0 * s + (2.0 * s * z) * p;
+    double de = (double)e;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
+}
+
+// noise_sincospi(x), x in [0, 2): x = k/2 + r with |r| <= 1/4 (exact), Taylor polynomials of
+// sin(pi r) through r^19 and cos(pi r) through r^20 (truncation < 1e-17), then the quadrant.
+__device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs) {
+    constexpr double SP[10] = {3.141592653589793, -5.16771278004997, 2.5501640398773455,
+                               -0.5992645293207921, 0.08214588661112823, -0.0073704309457143504,
+                               0.00046630280576761255, -2.1915353447830217e-05,
+                               7.952054001475513e-07, -2.2948428997269873e-08};
+    constexpr double CP[11] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895,
+                               0.2353306303588932, -0.02580689139001406, 0.0019295743094039231,
+                               -0.0001046381049248457, 4.303069587032947e-06,
+                               -1.3878952462213771e-07, 3.604730797462501e-09};
+    double k = floor(2.0 * x + 0.5);
+    double r = x - 0.5 * k;
+    double z = r * r;
+    double ps = SP[9];
+#pragma unroll
+    for (int j = 8; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[10];
+#pragma unroll
+    for (int j = 9; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    double sr = r * ps, cr = pc;
+    int q = (int)k & 3;
+    double s1 = (q & 1) ? cr : sr;
+    double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb
 __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
-    double r = sqrt(-2.0 * log(u_oc(wa)));
-    double th = kTwoPi * u_co(wb);
+    double r = sqrt(-2.0 * noise_log(u_oc(wa)));
     double s, c;
-    sincos(th, &s, &c);
+    noise_sincospi(2.0 * u_co(wb), &s, &c);
     *n0 = r * c;
     *n1 = r * s;
 }
@@ -248,15 +316,19 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
     }
 }
 
-// BGK redraw in the LRF: kernel_rta_collision_cpu! (kernels_cpu.jl:485-505) with the
-// inverse-CDF draw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701)
-template <int PDIM, bool RADIAL>
-__device__ __forceinline__ void rta_redraw(const lmh_params& p, const TimeTable& taun,
-                                           const JuttnerTable& jt, double T, double* q,
-                                           double ucol, double usamp, const double* g) {
+// BGK collision in the LRF: kernel_rta_collision_cpu! (kernels_cpu.jl:485-505) with the
+// inverse-CDF draw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701).  The decision and the
+// redraw are split so that the redraw's noise is generated only for colliding particles.
+__device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTable& taun, double T,
+                                             double ucol) {
     double tau_n = (p.DsT > 0.0 && T > 0.0) ? eval_time_table(taun, T) : 0.0;
     double Pcol = (tau_n > 0.0 && isfinite(tau_n)) ? jl_clamp(p.dt / tau_n, 0.0, 1.0) : 1.0;
-    if (!(ucol < Pcol)) return;
+    return ucol < Pcol;
+}
+
+template <int PDIM, bool RADIAL>
+__device__ __forceinline__ void rta_redraw(const JuttnerTable& jt, double T, double* q,
+                                           double usamp, const double* g) {
     double pstar = juttner_lookup(jt, usamp, T);
     if (RADIAL) {
         q[0] = g[0] < 0.0 ? -pstar : pstar;

@@ -1,0 +1,197 @@
+# LangevInMediumHIP.jl -- the MI355X backend as an add-on module of LangevInMedium.jl.
+#
+# It adds two backend types next to CPUBackend/GPUBackend (src/backends.jl:14-44) and methods of
+# LangevInMedium.Simulate.simulate_ensemble_bulk for them, the way src/simulate_gpu_wrapper.jl:15-108
+# adds the CUDA ones.  The host preparation is the reference's own (Utils, Transport); every step runs
+# in liblmhip.so through ccall (C ABI: include/lmhip.h).  Keyword arguments and return shapes follow
+# simulate_cpu.jl:12-44,318-322 and simulate_cpu_general_coords.jl:18-29,128-132.
+#
+#   using LangevInMedium, LangevInMediumHIP
+#   t, mom, pos = simulate_ensemble_bulk(HIPBackend(), r, p, dens, Tevo, Vevo, (xgrid, tgrid);
+#                                        x_init = x0, p_init = p0, N_particles = N, ...)
+module LangevInMediumHIP
+
+using LangevInMedium
+import LangevInMedium.Simulate: simulate_ensemble_bulk
+using LangevInMedium.Backends: AbstractBackend
+using LangevInMedium.Transport: build_tau_drag_spline, build_taun_current_spline, build_juttner_invcdf
+using LangevInMedium.Utils: check_momentum_dims, append_thermal_pz, sample_initial_particles_milne!
+
+export HIPBackend, HIP_GCBackend
+
+const liblmhip = get(ENV, "LMHIP_LIB", joinpath(@__DIR__, "..", "liblmhip.so"))
+
+struct HIPBackend <: AbstractBackend end
+struct HIP_GCBackend <: AbstractBackend end
+
+# lmh_params (include/lmhip.h), field for field
+struct LmhParams
+    dt::Float64; t0::Float64; DsT::Float64; psi2::Float64; gamma0::Float64; Lambda::Float64
+    seed::UInt64; stream::UInt64
+    coords::Int32; collision::Int32; noise::Int32; integrator::Int32; drag_model::Int32
+    relativistic::Int32; bjorken_redshift::Int32; position_diffusion::Int32
+    reflecting_boundary::Int32; check_finite::Int32
+end
+
+const LMH_CARTESIAN, LMH_GENERAL_COORDS = Int32(0), Int32(1)
+const LMH_LANGEVIN, LMH_RTA, LMH_FLUID = Int32(0), Int32(1), Int32(2)
+const LMH_NOISE_PHILOX = Int32(1)
+const LMH_TABLE_DRAG, LMH_TABLE_RTA = Int32(0), Int32(1)
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+end
+
+function Ctx(device::Integer = 0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:lmh_create, liblmhip), Cint, (Ref{Ptr{Cvoid}}, Int32), r, device)
+    rc == 0 || error("lmh_create(device=$device) failed with status $rc")
+    c = Ctx(r[])
+    finalizer(x -> ccall((:lmh_destroy, liblmhip), Cvoid, (Ptr{Cvoid},), x.h), c)
+    return c
+end
+
+check(c::Ctx, rc) = rc == 0 ||
+    error(unsafe_string(ccall((:lmh_last_error, liblmhip), Cstring, (Ptr{Cvoid},), c.h)))
+
+set_medium!(c, xg, tg, T, V, V2) = GC.@preserve xg tg T V V2 check(c, ccall((:lmh_set_medium_radial, liblmhip), Cint,
+    (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+    c.h, xg, length(xg), tg, length(tg), T, V, V2 === nothing ? C_NULL : V2))
+
+set_table!(c, which, sp, Tmin, invdT, vals) = GC.@preserve vals check(c, ccall((:lmh_set_time_table, liblmhip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Float64, Float64, Ptr{Float64}, Int32), c.h, which, sp, Tmin, invdT, vals, length(vals)))
+
+set_juttner!(c, sp, inv, nU, nT, Tmin, invdT) = GC.@preserve inv check(c, ccall((:lmh_set_juttner_table, liblmhip), Cint,
+    (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32, Int32, Float64, Float64), c.h, sp, inv, nU, nT, Tmin, invdT))
+
+function set_particles!(c, pos::Matrix{Float64}, mom::Matrix{Float64}, m::Float64)
+    N = size(pos, 2)
+    counts = Int64[N]; masses = Float64[m]
+    GC.@preserve pos mom counts masses check(c, ccall((:lmh_set_particles, liblmhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32, Int64, Int32, Ptr{Int64}, Ptr{Float64}),
+        c.h, pos, size(pos, 1), mom, size(mom, 1), N, 1, counts, masses))
+end
+
+function get_particles(c, dims, pdim, N)
+    pos = zeros(dims, N); mom = zeros(pdim, N)
+    check(c, ccall((:lmh_get_particles, liblmhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), c.h, pos, mom))
+    return pos, mom
+end
+
+run!(c, p::LmhParams, step0, n) = check(c, ccall((:lmh_run, liblmhip), Cint,
+    (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ptr{Float64}, Int32), c.h, p, step0, n, C_NULL, 0))
+
+boost_to_lab!(c, p::LmhParams, step) = check(c, ccall((:lmh_boost_to_lab, liblmhip), Cint,
+    (Ptr{Cvoid}, Ref{LmhParams}, Int64), c.h, p, step))
+
+# ---- Cartesian: simulate_gpu_wrapper.jl:15-83 signature ----------------------------------------
+function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, heavy_quark_density,
+        TemperatureEvolutionn, VelocityEvolutionn, SpaceTimeGrid;
+        N_particles::Int = 10_000, Δt::Float64 = 0.001, initial_time::Float64 = 0.0,
+        final_time::Float64 = 1.0, save_interval::Float64 = 0.1, m::Float64 = 1.0, DsT::Float64 = 0.2,
+        DsT_linear::Bool = false, DsT_slope::Float64 = 1.765, DsT_offset::Float64 = -0.159,
+        Tfo::Float64 = 0.156, DsT_quad::Bool = false, DsT_Tref::Float64 = 0.0, dimensions::Int = 3,
+        cartesian_spatial_sampling = nothing, antithetic_momenta::Bool = false,
+        position_diffusion::Bool = false, momentum_langevin::Bool = true,
+        reflecting_boundary::Bool = false, collision_mode::Symbol = :langevin,
+        x_init = nothing, p_init = nothing, V2Evolutionn = nothing, psi2::Float64 = 0.0,
+        freezeout_capture::Bool = false, freezeout_interp::Bool = true, integrator_mode::Int = 0,
+        relativistic::Bool = true, momentum_dimensions::Int = 0, bjorken_redshift::Bool = false,
+        seed::Integer = 0, device::Integer = 0)
+    (collision_mode == :langevin || collision_mode == :rta) ||
+        error("simulate_ensemble_bulk(::HIPBackend): collision_mode=$(collision_mode) is not supported on GPU (only :langevin and :rta).")
+    freezeout_capture && error("freezeout_capture is not available on the HIP backend")
+    (x_init === nothing || p_init === nothing) && error("pass x_init and p_init (host samplers run before the backend)")
+    dimensions == 1 && error("radial mode: reduce x_init/p_init to (1, N) as simulate_cpu.jl:102-136 does")
+    steps = floor(Int, (final_time - initial_time) / Δt)
+    save_every = round(Int, save_interval / Δt)
+    num_saves = div(steps, save_every)
+    xgrid, tgrid = SpaceTimeGrid
+    positions = Matrix{Float64}(x_init); momenta = Matrix{Float64}(p_init)
+    pdim = momentum_dimensions <= 0 ? dimensions : momentum_dimensions
+    check_momentum_dims(dimensions, pdim, false, bjorken_redshift, initial_time)
+    if size(momenta, 1) < pdim
+        momenta = append_thermal_pz(momenta, positions, m,
+            r -> LangevInMedium.KernelsCPU.interpolate_2d_cpu(xgrid, tgrid, TemperatureEvolutionn, r, initial_time))
+    end
+    N = size(positions, 2)
+    c = Ctx(device)
+    set_medium!(c, collect(Float64, xgrid), collect(Float64, tgrid), Matrix{Float64}(TemperatureEvolutionn),
+                Matrix{Float64}(VelocityEvolutionn), V2Evolutionn === nothing ? nothing : Matrix{Float64}(V2Evolutionn))
+    Tmin = max(float(minimum(TemperatureEvolutionn)), 0.0)
+    Tmax = max(float(maximum(TemperatureEvolutionn)), Tmin + eps(Float64))
+    if momentum_langevin && DsT > 0.0
+        kw = (Tmin = Tmin, Tmax = Tmax, nT = 1024, DsT_linear = DsT_linear, DsT_slope = DsT_slope,
+              DsT_offset = DsT_offset, Tfo = Tfo, DsT_quad = DsT_quad, DsT_Tref = DsT_Tref)
+        set_table!(c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(m, DsT; kw...)...)
+        if collision_mode == :rta
+            set_table!(c, LMH_TABLE_RTA, 0, build_taun_current_spline(m, DsT; kw...)...)
+            inv, nU, nT, Tt, idt = build_juttner_invcdf(m, pdim; Tmin = Tmin, Tmax = Tmax, nT = 256, nU = 2048, np = 8192)
+            set_juttner!(c, 0, inv, nU, nT, Tt, idt)
+        end
+    else
+        set_table!(c, LMH_TABLE_DRAG, 0, 0.0, 1.0, [0.0, 0.0])
+    end
+    set_particles!(c, positions, momenta, m)
+    coll = (!momentum_langevin || DsT == 0.0) ? LMH_FLUID : (collision_mode == :rta ? LMH_RTA : LMH_LANGEVIN)
+    p = LmhParams(Δt, initial_time, DsT, psi2, 0.0, 1.0, UInt64(seed), 0, LMH_CARTESIAN, coll,
+                  LMH_NOISE_PHILOX, Int32(integrator_mode), 0, relativistic, bjorken_redshift,
+                  position_diffusion, reflecting_boundary, 1)
+    boost_to_lab!(c, p, 0)
+    pos0, mom0 = get_particles(c, dimensions, pdim, N)
+    pos_hist = [pos0]; mom_hist = [mom0]
+    step = 1
+    while step <= steps
+        n = min(save_every, steps - step + 1)
+        run!(c, p, step, n)
+        step += n
+        if n == save_every && length(pos_hist) <= num_saves
+            ps, ms = get_particles(c, dimensions, pdim, N)
+            push!(pos_hist, ps); push!(mom_hist, ms)
+        end
+    end
+    return range(initial_time, final_time, length = num_saves + 1), mom_hist, pos_hist
+end
+
+# ---- general coordinates (tau, r): simulate.jl:117-141 signature ---------------------------------
+function simulate_ensemble_bulk(::HIP_GCBackend, T_profile_MIS, ur_profile_MIS, mu_profile_MIS,
+        TemperatureEvolutionn, VelocityEvolutionn, SpaceTimeGrid;
+        N_particles::Int = 10_000, Δt::Float64 = 0.001, initial_time::Float64 = 0.0,
+        final_time::Float64 = 1.0, save_interval::Float64 = 0.1, m::Float64 = 1.0,
+        DsT::Float64 = 0.2, dimensions::Int = 2, seed::Integer = 0, device::Integer = 0)
+    steps = floor(Int, (final_time - initial_time) / Δt)
+    save_every = Int(save_interval / Δt)
+    num_saves = div(steps, save_every)
+    xgrid, tgrid = SpaceTimeGrid
+    positions, momenta = sample_initial_particles_milne!(m, dimensions, N_particles, initial_time,
+        T_profile_MIS, ur_profile_MIS, mu_profile_MIS, (0., maximum(xgrid)), 200)
+    c = Ctx(device)
+    set_medium!(c, collect(Float64, xgrid), collect(Float64, tgrid), Matrix{Float64}(TemperatureEvolutionn),
+                Matrix{Float64}(VelocityEvolutionn), nothing)
+    if DsT > 0.0
+        Tmin = max(float(minimum(TemperatureEvolutionn)), 0.0)
+        Tmax = max(float(maximum(TemperatureEvolutionn)), Tmin + eps(Float64))
+        set_table!(c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(m, DsT; Tmin = Tmin, Tmax = Tmax, nT = 1024)...)
+    else
+        set_table!(c, LMH_TABLE_DRAG, 0, 0.0, 1.0, [0.0, 0.0])
+    end
+    set_particles!(c, Matrix{Float64}(positions), Matrix{Float64}(momenta), m)
+    p = LmhParams(Δt, initial_time, DsT, 0.0, 0.0, 1.0, UInt64(seed), 0, LMH_GENERAL_COORDS,
+                  LMH_LANGEVIN, LMH_NOISE_PHILOX, 0, 0, 1, 0, 0, 0, 1)
+    mom_hist = [abs.(momenta[2, :])]
+    pos_hist = [reshape(positions[2, :], 1, :)]
+    step = 1
+    while step <= steps
+        n = min(save_every, steps - step + 1)
+        run!(c, p, step, n)
+        step += n
+        if n == save_every && length(mom_hist) <= num_saves
+            ps, ms = get_particles(c, 2, 2, N_particles)
+            push!(mom_hist, sqrt.(sum(ms .^ 2, dims = 1))[:])
+            push!(pos_hist, reshape(ps[2, :], 1, :))
+        end
+    end
+    return range(initial_time, final_time, length = num_saves + 1), mom_hist, pos_hist
+end
+
+end # module

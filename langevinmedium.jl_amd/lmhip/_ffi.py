@@ -50,7 +50,11 @@ class lmh_params(ctypes.Structure):
 
 
 class lmh_tuning(ctypes.Structure):
-    _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("slice", c_int32), ("_pad", c_int32)]
+    _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("slice", c_int32), ("defer", c_int32)]
+
+
+class lmh_profile(ctypes.Structure):
+    _fields_ = [("launches", c_int64 * 3), ("ms", c_double * 3)]
 
 
 class lmh_obs_spec(ctypes.Structure):
@@ -96,6 +100,8 @@ SIGNATURES = {
     "lmh_run_device_noise": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64, c_void_p, c_int32]),
     "lmh_nonfinite_step": (c_int64, [c_void_p]),
     "lmh_observables": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD]),
+    "lmh_profile_start": (c_int, [c_void_p]),
+    "lmh_profile_stop": (c_int, [c_void_p, POINTER(lmh_profile)]),
 }
 
 _LIB = None

@@ -69,14 +69,15 @@ class Context:
         self._check(self._lib.lmh_synchronize(self._h))
 
     # ---- launch shape ----
-    def set_tuning(self, block=256, steps_per_launch=1, slice=0):
-        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), 0)
+    def set_tuning(self, block=256, steps_per_launch=1, slice=0, defer=1):
+        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer))
         self._check(self._lib.lmh_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _ffi.lmh_tuning()
         self._check(self._lib.lmh_get_tuning(self._h, ctypes.byref(t)))
-        return {"block": t.block, "steps_per_launch": t.steps_per_launch, "slice": t.slice}
+        return {"block": t.block, "steps_per_launch": t.steps_per_launch, "slice": t.slice,
+                "defer": t.defer}
 
     # ---- medium ----
     def set_medium_radial(self, rgrid, tgrid, T, v, v2=None):
@@ -169,6 +170,16 @@ class Context:
 
     def nonfinite_step(self) -> int:
         return int(self._lib.lmh_nonfinite_step(self._h))
+
+    # ---- launch timing ----
+    def profile_start(self):
+        self._check(self._lib.lmh_profile_start(self._h))
+
+    def profile_stop(self) -> dict:
+        p = _ffi.lmh_profile()
+        self._check(self._lib.lmh_profile_stop(self._h, ctypes.byref(p)))
+        names = ("step", "redraw", "slice")
+        return {n: {"launches": int(p.launches[k]), "ms": float(p.ms[k])} for k, n in enumerate(names)}
 
     # ---- observables ----
     def observables(self, nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.0, species=-1):

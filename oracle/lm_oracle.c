@@ -24,7 +24,6 @@
 
 #define JL_EPS DBL_EPSILON            /* Julia eps() */
 static const double FMGEV = 1.0 / 0.197327;   /* constants.jl:13 (GevInvTofm), :23 (fmGeV) */
-static const double TWO_PI = 6.283185307179586;
 
 double lmo_fmGeV(void) { return FMGEV; }
 
@@ -367,12 +366,81 @@ void lmo_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t ou
 /* uniforms from a 64-bit word: [0,1) and (0,1], 53 bits each */
 static inline double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
 static inline double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
-/* Box-Muller: radius from the (0,1] uniform, angle from the [0,1) one */
+
+/* The two transcendental functions of the Box-Muller transform, written with correctly
+ * rounded operations only (+, -, *, /, sqrt, fma, floor and exact bit manipulation) so that the
+ * same definition evaluates to the same bits on any IEEE-754 machine.
+ *
+ * noise_log(u), u in [2^-53, 1]: u = 2^e m with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+ * s = (m - 1)/(m + 1), |s| <= 0.1716, by the Taylor series of atanh through s^19 (truncation
+ * < 3e-17 relative); ln 2 split into a head whose products with e are exact and a tail.       */
+static const double LN2_HI = 6.93147180369123816490e-01;
+static const double LN2_LO = 1.90821492927058770002e-10;
+
+static inline double noise_log(double u) {
+    uint64_t b;
+    memcpy(&b, &u, 8);
+    int64_t e = (int64_t)((b >> 52) & 0x7FF) - 1023;
+    uint64_t mb = (b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
+    double m;
+    memcpy(&m, &mb, 8);
+    if (m > 1.4142135623730951) { m = 0.5 * m; e += 1; }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    double lm = 2.0 * s + (2.0 * s * z) * p;
+    double de = (double)e;
+    return fma(de, LN2_HI, fma(de, LN2_LO, lm));
+}
+
+/* noise_sincospi(x): sin(pi x) and cos(pi x) for x in [0, 2): x = k/2 + r with k the nearest
+ * half-integer count and |r| <= 1/4 (exact), the Taylor polynomials of sin(pi r) through r^19
+ * and cos(pi r) through r^20 (truncation < 1e-17), then the quadrant k mod 4.                 */
+static const double SP[10] = {  /* pi^(2j+1) (-1)^j / (2j+1)!, rounded to double */
+    3.141592653589793, -5.16771278004997, 2.5501640398773455, -0.5992645293207921,
+    0.08214588661112823, -0.0073704309457143504, 0.00046630280576761255,
+    -2.1915353447830217e-05, 7.952054001475513e-07, -2.2948428997269873e-08};
+static const double CP[11] = {  /* pi^(2j) (-1)^j / (2j)!, rounded to double */
+    1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895, 0.2353306303588932,
+    -0.02580689139001406, 0.0019295743094039231, -0.0001046381049248457, 4.303069587032947e-06,
+    -1.3878952462213771e-07, 3.604730797462501e-09};
+
+static inline void noise_sincospi(double x, double* sn, double* cs) {
+    double k = floor(2.0 * x + 0.5);
+    double r = x - 0.5 * k;
+    double z = r * r;
+    double ps = SP[9];
+    for (int j = 8; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[10];
+    for (int j = 9; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    double sr = r * ps, cr = pc;
+    int q = (int)k & 3;
+    double s1 = (q & 1) ? cr : sr;
+    double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+/* Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb */
 static inline void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
-    double r = sqrt(-2.0 * log(u_oc(wa)));
-    double th = TWO_PI * u_co(wb);
-    *n0 = r * cos(th);
-    *n1 = r * sin(th);
+    double r = sqrt(-2.0 * noise_log(u_oc(wa)));
+    double s, c;
+    noise_sincospi(2.0 * u_co(wb), &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+void lmo_noise_transforms(double u, double x, double out[3]) {
+    out[0] = noise_log(u);
+    noise_sincospi(x, &out[1], &out[2]);
 }
 
 static int noise_layout(const lmh_params* p, int32_t dims, int32_t pdim,
@@ -394,11 +462,21 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
     return noise_layout(p, dims, pdim, &a, &b, &c);
 }
 
+/* a 44-bit uniform in [0,1) from the 11 low bits of the four words, which the 53-bit uniforms
+ * of the Box-Muller pairs leave unused */
+static inline double u_low44(const uint64_t w[4]) {
+    uint64_t b = ((w[0] & 0x7FFULL) << 33) | ((w[1] & 0x7FFULL) << 22) | ((w[2] & 0x7FFULL) << 11) |
+                 (w[3] & 0x7FFULL);
+    return (double)b * 0x1.0p-44;
+}
+
 /* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
  * Philox4x64-10 of counter (i, s, b, 0) under key (seed, stream).
- *   block 0: two Box-Muller pairs n0..n3; xi[d] = n_d (GC: xi = n0)
- *   block 1: u_col = U[0,1)(w0), u_samp = U[0,1)(w1), (g0, g1) = BoxMuller(w2, w3)
- *   block 2: (g2, xpos0) = BoxMuller(w0, w1), (xpos1, xpos2) = BoxMuller(w2, w3)           */
+ *   block 0: Box-Muller pairs (w0,w1), (w2,w3) -> xi[0..pdim) (GC: xi = n0);
+ *            u_col = the low-bit uniform of the block
+ *   block 1: (g0, g1) = BoxMuller(w0, w1), g2 = first of BoxMuller(w2, w3);
+ *            u_samp = the low-bit uniform of the block
+ *   block 2: (xpos0, xpos1) = BoxMuller(w0, w1), xpos2 = first of BoxMuller(w2, w3)         */
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec) {
     int32_t oxi, orta, opos;
@@ -407,32 +485,32 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
     for (int64_t i = 0; i < N; ++i) {
         uint64_t w[4];
         double n[4];
-        if (oxi >= 0) {
+        if (oxi >= 0 || orta >= 0) {
             uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 0, 0};
             lmo_philox4x64_10(ctr, key, w);
             box_muller(w[0], w[1], &n[0], &n[1]);
             box_muller(w[2], w[3], &n[2], &n[3]);
-            int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
-            for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
+            if (oxi >= 0) {
+                int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
+                for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
+            }
+            if (orta >= 0) rec[(int64_t)orta * N + i] = u_low44(w);
         }
-        double g[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
         if (orta >= 0) {
             uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 1, 0};
             lmo_philox4x64_10(ctr, key, w);
-            rec[(int64_t)orta * N + i] = u_co(w[0]);
-            rec[(int64_t)(orta + 1) * N + i] = u_co(w[1]);
-            box_muller(w[2], w[3], &g[0], &g[1]);
+            box_muller(w[0], w[1], &n[0], &n[1]);
+            box_muller(w[2], w[3], &n[2], &n[3]);
+            rec[(int64_t)(orta + 1) * N + i] = u_low44(w);
+            for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = n[d];
         }
-        if ((orta >= 0 && pdim == 3) || opos >= 0) {
+        if (opos >= 0) {
             uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 2, 0};
             lmo_philox4x64_10(ctr, key, w);
-            box_muller(w[0], w[1], &g[2], &xp[0]);
-            box_muller(w[2], w[3], &xp[1], &xp[2]);
+            box_muller(w[0], w[1], &n[0], &n[1]);
+            box_muller(w[2], w[3], &n[2], &n[3]);
+            for (int32_t d = 0; d < dims; ++d) rec[(int64_t)(opos + d) * N + i] = n[d];
         }
-        if (orta >= 0)
-            for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = g[d];
-        if (opos >= 0)
-            for (int32_t d = 0; d < dims; ++d) rec[(int64_t)(opos + d) * N + i] = xp[d];
     }
 }
 

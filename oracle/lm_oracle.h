@@ -90,6 +90,7 @@ int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int
 
 /* ---- Philox4x64-10 and the noise record ------------------------------------------------------ */
 void lmo_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+void lmo_noise_transforms(double u, double x, double out[3]);
 int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim);
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec);

@@ -182,6 +182,32 @@ def test_grid_parity(gpu_ok, name, noise, spl):
     check(g, o)
 
 
+@pytest.mark.parametrize("slice_", [0, 1])
+def test_grid_deferred_collisions_identical(gpu_ok, slice_):
+    """queueing the colliding particles for a second kernel gives the fused kernel's bits"""
+    g_ = small_grid()
+    rng = np.random.default_rng(41)
+    N = 20000
+    pos = media.hydro_positions(N, rng) * 2.0
+    mom = media.pt_rapidity_momenta(N, 1.5, rng)
+    p = _ffi.make_params(dt=0.05, t0=0.6, DsT=0.6366, noise=_ffi.LMH_NOISE_PHILOX, seed=8,
+                         collision=_ffi.LMH_LANGEVIN_RTA)
+    from helpers import tables, load_tables
+    drag, rt, jt = tables([1.5], g_.T, 0.6366, 3, True, media.tau_R_law(0.16))
+    res = []
+    for defer in (0, 1):
+        with lmhip.Context(0) as ctx:
+            ctx.set_tuning(block=256, steps_per_launch=1, slice=slice_, defer=defer)
+            ctx.set_medium_grid2p1(g_.x0, g_.dx, g_.nx, g_.y0, g_.dy, g_.ny, g_.tgrid, g_.T, g_.vx, g_.vy)
+            load_tables(ctx, drag, rt, jt)
+            ctx.set_particles(pos, mom, [1.5])
+            ctx.run(p, 1, 25, None, 0)
+            res.append(ctx.get_particles())
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    # and collisions did happen
+    assert np.mean(np.any(res[1][1] != mom, axis=0)) > 0.5
+
+
 def test_nonfinite_reported_at_first_bad_step(gpu_ok):
     rng = np.random.default_rng(0)
     N = 1000
@@ -227,11 +253,14 @@ def test_simulate_ensemble_bulk_matches_oracle_driver(gpu_ok):
     xg, tg, Tf, Vf, _ = flowing_medium()
     x0 = np.asfortranarray(rng.normal(0, 3, (2, N)))
     p0 = np.asfortranarray(rng.normal(0, 1, (3, N)))
-    DsT, dt, steps = 0.6366, 0.01, 45
+    DsT, dt = 0.6366, 0.01
+    t0, t1 = 0.5, 0.955
+    steps, save_every, num_saves = lmhip.utils.step_counts(t0, t1, dt, 0.1)
+    assert (steps, save_every, num_saves) == (45, 10, 4)
     noise = rng.standard_normal((steps, 3, N))
     tp, mh, ph = lmhip.simulate_ensemble_bulk(
         lmhip.HIPBackend(), xg, np.linspace(0, 10, 10), None, Tf, Vf, (xg, tg),
-        N_particles=N, Δt=dt, initial_time=0.5, final_time=0.5 + steps * dt, save_interval=0.1,
+        N_particles=N, Δt=dt, initial_time=t0, final_time=t1, save_interval=0.1,
         m=1.5, DsT=DsT, dimensions=2, momentum_dimensions=3, x_init=x0, p_init=p0, noise=noise)
     assert len(mh) == len(ph) == len(tp) == 5
     keep = O.Keep()

@@ -70,21 +70,48 @@ def test_noise_record_box_muller():
         n1 = mp.sqrt(-2 * mp.log(u1(w[0]))) * mp.sin(th)
         assert abs(rec[0, i] - float(n0)) <= 4e-15 * max(1, abs(float(n0)))
         assert abs(rec[1, i] - float(n1)) <= 4e-15 * max(1, abs(float(n1)))
-        # the collision uniforms come from block 1
+        # the collision uniform is the 44 low bits of block 0, the quantile those of block 1
+        low = lambda w: (((w[0] & 0x7FF) << 33) | ((w[1] & 0x7FF) << 22) | ((w[2] & 0x7FF) << 11) | (w[3] & 0x7FF)) * 2.0 ** -44
+        assert rec[3, i] == low(w)
         bg1 = np.random.Philox(key=np.array([99, 3], dtype=np.uint64),
                                counter=np.array([i - 1 if i else 2**64 - 1, 17 if i else 16, 1, 0], dtype=np.uint64))
         w1 = [int(v) for v in bg1.random_raw(4)]
-        assert rec[3, i] == (w1[0] >> 11) * 2.0 ** -53
-        assert rec[4, i] == (w1[1] >> 11) * 2.0 ** -53
+        assert rec[4, i] == low(w1)
+        g0 = mp.sqrt(-2 * mp.log(u1(w1[0]))) * mp.cos(mp.mpf(6.283185307179586) * u(w1[1]))
+        assert abs(rec[5, i] - float(g0)) <= 4e-15 * max(1, abs(float(g0)))
+
+
+def test_noise_transforms_accuracy():
+    # the Box-Muller log and sincospi are built from correctly rounded operations only
+    f = L.lmo_noise_transforms
+    f.argtypes = [ctypes.c_double, ctypes.c_double, O.PD]
+    f.restype = None
+    out = np.zeros(3)
+    rng = np.random.default_rng(0)
+    us = list(rng.random(4000)) + [2.0 ** -53, 1.0, 0.5, 0.7071067811865476, 0.7071067811865475, 1 - 2.0 ** -53]
+    xs = list(2 * rng.random(len(us) - 6)) + [0.0, 0.25, 0.5, 1.0, 1.5, 1.75]
+    for u, x in zip(us, xs):
+        f(u, x, O.dp(out))
+        lu = mp.log(u)
+        assert abs(out[0] - float(lu)) <= 4e-16 * max(abs(float(lu)), 1e-300) or (u == 1.0 and out[0] == 0.0)
+        assert abs(out[1] - float(mp.sinpi(x))) <= 2.3e-16
+        assert abs(out[2] - float(mp.cospi(x))) <= 2.3e-16
 
 
 def test_noise_record_moments():
-    p = _ffi.make_params(dt=0.01, seed=5, noise=_ffi.LMH_NOISE_PHILOX)
-    rec = O.noise_record(p, 3, 3, 200_000, 1, 1)[0]
-    for c in range(3):
-        assert abs(rec[c].mean()) < 5 / math.sqrt(rec.shape[1])
-        assert abs(rec[c].var() - 1) < 6 * math.sqrt(2 / rec.shape[1])
+    p = _ffi.make_params(dt=0.01, seed=5, noise=_ffi.LMH_NOISE_PHILOX, collision=_ffi.LMH_LANGEVIN_RTA)
+    rec = O.noise_record(p, 2, 3, 200_000, 1, 1)[0]
+    n = rec.shape[1]
+    for c in (0, 1, 2, 5, 6, 7):
+        assert abs(rec[c].mean()) < 5 / math.sqrt(n)
+        assert abs(rec[c].var() - 1) < 6 * math.sqrt(2 / n)
+    for c in (3, 4):                                     # the low-bit uniforms
+        assert abs(rec[c].mean() - 0.5) < 5 * math.sqrt(1 / 12 / n)
+        assert abs(rec[c].var() - 1 / 12) < 0.003
+        assert abs(np.corrcoef(rec[c], rec[0])[0, 1]) < 0.01
     assert abs(np.corrcoef(rec[0], rec[1])[0, 1]) < 0.01
+    # u_col is independent of the normals drawn from the same block's high bits
+    assert abs(np.corrcoef(rec[3], rec[0] ** 2)[0, 1]) < 0.01
 
 
 # ---------------------------------------------------------------- interpolation ------------------
