@@ -61,10 +61,13 @@ def tables_for(g, masses, DsT, pdim):
     return out
 
 
-def make_params(DsT, noise_fixed=False):
+COLLISIONS = {"langevin_rta": 3, "langevin": 0, "rta": 1, "fluid": 2}
+
+
+def make_params(DsT, noise_fixed=False, collision="langevin_rta"):
     from lmhip import _ffi
     return _ffi.make_params(dt=0.01, t0=0.6, DsT=DsT, seed=3, stream=0,
-                            collision=_ffi.LMH_LANGEVIN_RTA,
+                            collision=COLLISIONS[collision],
                             noise=_ffi.LMH_NOISE_FIXED if noise_fixed else _ffi.LMH_NOISE_PHILOX,
                             check_finite=False)
 
@@ -115,6 +118,10 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--slice", type=int, default=1)
     ap.add_argument("--defer", type=int, default=1)
+    ap.add_argument("--collision", default="langevin_rta", choices=sorted(COLLISIONS),
+                    help="diagnostic: another momentum update than the workload's")
+    ap.add_argument("--noise", default="philox", choices=["philox", "fixed"],
+                    help="diagnostic: read a device-resident fixed-noise record instead of Philox")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
@@ -135,7 +142,7 @@ def main():
     t_setup = time.perf_counter()
     g, masses, counts, pos, mom = build_workload(N)
     tabs = tables_for(g, masses, DsT, 3)
-    params = make_params(DsT)
+    params = make_params(DsT, args.noise == "fixed", args.collision)
     ctx = lmhip.Context(0)
     ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice, defer=args.defer)
     ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
@@ -149,7 +156,24 @@ def main():
     ctx.synchronize()
     print(f"setup {time.perf_counter() - t_setup:.1f} s; N={N} medium {g.nbytes() / 1e6:.0f} MB", file=sys.stderr, flush=True)
 
-    ctx.run(params, 1, args.warmup, None, args.spl)
+    record = None
+    if args.noise == "fixed":
+        # one record for every step run below, (steps, C, N) on the device
+        total = args.warmup + 2 * args.steps
+        C = ctx.noise_channels(params)
+        record = torch.randn((total, C, N), dtype=torch.float64, device="cuda")
+        orta = 3 if args.collision == "langevin_rta" else 0
+        if args.collision in ("langevin_rta", "rta"):
+            record[:, orta:orta + 2, :].uniform_(0.0, 1.0)
+        torch.cuda.synchronize()
+
+    def run(step0, n):
+        if record is None:
+            ctx.run(params, step0, n, None, args.spl)
+        else:
+            ctx.run_device_noise(params, step0, n, record[step0 - 1].data_ptr(), args.spl)
+
+    run(1, args.warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -157,7 +181,7 @@ def main():
     ev1 = torch.cuda.Event(enable_timing=True)
     w0 = time.perf_counter()
     ev0.record(stream)
-    ctx.run(params, 1 + args.warmup, args.steps, None, args.spl)
+    run(1 + args.warmup, args.steps)
     ev1.record(stream)
     ev1.synchronize()
     ctx.synchronize()
@@ -167,7 +191,7 @@ def main():
     # the same number of steps again with every launch bracketed by events on the library's
     # stream: the device time of the step kernel alone
     ctx.profile_start()
-    ctx.run(params, 1 + args.warmup + args.steps, args.steps, None, args.spl)
+    run(1 + args.warmup + args.steps, args.steps)
     prof = ctx.profile_stop()
     bad = ctx.nonfinite_step()
     if bad >= 0:
@@ -204,6 +228,7 @@ def main():
                                "RTA tau_R=5(eta/s)/T; dt=0.01 fm/c; Philox noise; FP64",
                    "quarks": N, "grid": [200, 200, 120], "block": args.block,
                    "steps_per_launch": max(1, args.spl), "slice": args.slice, "defer": args.defer,
+                   "collision": args.collision, "noise": args.noise,
                    "wall_s_timed": round(wall, 4), "mean_pT": round(obs[1] / obs[0], 6)},
         "roofline": roof,
     }

@@ -50,8 +50,9 @@ struct StepArgs {
     const double* noise;
     unsigned long long* bad;    // first non-finite step (atomicMin), ULLONG_MAX if none
     const SpeciesArgs* species; // per-species mass and tables, device memory
-    int32_t* queue;             // deferred collisions: particle indices
-    unsigned int* qcount;       // two counters, used by alternate steps
+    int32_t* queue;             // deferred collisions: block b's particles at [b*qblock, ...)
+    int32_t* qcount;            // deferred collisions per block of the step kernel
+    int32_t qblock;             // the step kernel's block size
     lmh_params p;
     double sqrt_dt;
 };
@@ -573,17 +574,25 @@ __device__ __forceinline__ void grid_push(const StepArgs& a, const GridMedium& g
     }
 }
 
-// slot of this lane in the deferred-collision queue (one atomic per wave), -1 if not queued
-__device__ __forceinline__ int64_t queue_slot(unsigned int* count, bool want) {
+// Rank of this lane among the block's queued lanes, in lane order (no atomics): the waves'
+// counts are combined through LDS, and thread 0 records the block's count.  Every thread of the
+// block must call it.
+__device__ __forceinline__ int block_rank(bool want, int32_t* block_count) {
+    __shared__ int wave_count[kBlock / 64];
     const unsigned long long mask = __ballot(want);
-    if (mask == 0ull) return -1;
     const int lane = (int)__lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
-    base = __shfl(base, leader);
+    const int wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) wave_count[wave] = __popcll(mask);
+    __syncthreads();
+    int before = 0, total = 0;
+    const int nw = (int)((blockDim.x + 63) >> 6);
+    for (int w = 0; w < nw; ++w) {
+        before += w < wave ? wave_count[w] : 0;
+        total += wave_count[w];
+    }
+    if (threadIdx.x == 0) *block_count = total;
     const unsigned long long below = mask & ((1ull << lane) - 1ull);
-    return want ? (int64_t)base + __popcll(below) : -1;
+    return want ? before + __popcll(below) : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
@@ -599,8 +608,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* sl
 template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER>
 __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) {
     int seg;
-    const int64_t i = particle_of_thread(a, &seg);
-    if (i < 0) return;
+    const int64_t i_thread = particle_of_thread(a, &seg);
+    // with DEFER every thread reaches the block-wide ranking; threads past the segment's end
+    // compute on the segment's first particle and neither store nor queue
+    const bool valid = i_thread >= 0;
+    if (!DEFER && !valid) return;
+    const int64_t i = valid ? i_thread : a.begin[seg];
     const SpeciesArgs sp = a.species[seg];
     constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
     constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
@@ -616,10 +629,6 @@ __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) 
         const double t_now = (double)step * p.dt + p.t0;
         double td;
         const int j = axis_cell(g.tg, g.nt, t_now, &td);
-        double xi[PDIM];
-        double ucol = 1.0;
-        if (HAS_KICK || HAS_RTA) noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
-
         double Tm, vx, vy;
         grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
         const FlowBoost fb = flow_boost(vx, vy);
@@ -629,17 +638,20 @@ __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) 
             glue_to_flow<PDIM>(q, m, vx, vy, p.relativistic);
         } else {
             const double T = jl_max(Tm, 0.0);
-            if (HAS_KICK) {
-                double etaD, kT;
-                drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
-                ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
-            }
+            double etaD = 0.0, kT = 0.0;
+            if (HAS_KICK) drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+            // the noise is drawn where it is used, to keep it out of the live range of the rest
+            double xi[PDIM];
+            double ucol = 1.0;
+            noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
+            if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
             if (HAS_RTA) {
                 const bool col = rta_collides(p, sp.rta, T, ucol);
                 if (DEFER) {
-                    const int64_t slot = queue_slot(&a.qcount[step & 1], col);
-                    if (slot >= 0) {
-                        a.queue[slot] = (int32_t)i;
+                    const int rank = block_rank(valid && col, &a.qcount[blockIdx.x]);
+                    if (!valid) return;
+                    if (rank >= 0) {
+                        a.queue[(int64_t)blockIdx.x * a.qblock + rank] = (int32_t)i;
 #pragma unroll
                         for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
                         return;
@@ -665,48 +677,74 @@ __global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) 
 // The deferred collisions of one step: the redraw of kernel_rta_collision_gpu!
 // (kernels_gpu.jl:673-701), the boost back and the push, for every queued particle.  The
 // particle's medium sample and flow factors are recomputed from the same inputs, so the result is
-// the one the fused path gives.  Also clears the other step parity's counter.
+// the one the fused path gives.  Each wave takes kRedrawSpan consecutive step-kernel blocks and
+// packs their queued particles into its lanes.
+constexpr int kRedrawSpan = 8;
+
 template <int PDIM, int NOISE, bool SLICE>
-__global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g) {
+__global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g, int64_t nblocks) {
     const int64_t step = a.step_begin;
-    const unsigned int count = a.qcount[step & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.qcount[(step + 1) & 1] = 0u;
     const lmh_params& p = a.p;
     const double t_now = (double)step * p.dt + p.t0;
     double td;
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (int64_t)count;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = a.queue[k];
-        int seg = 0;
+    const int lane = (int)__lane_id();
+    const int64_t nwaves = (nblocks + kRedrawSpan - 1) / kRedrawSpan;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nwaves; w += wstride) {
+        const int64_t b0 = w * kRedrawSpan;
+        // inclusive prefix of the span's counts, held by lanes 0..kRedrawSpan-1
+        const int cnt = (lane < kRedrawSpan && b0 + lane < nblocks) ? a.qcount[b0 + lane] : 0;
+        int incl = cnt;
 #pragma unroll
-        for (int s = 1; s < LMH_MAX_SPECIES; ++s)
-            if (s < a.nseg && i >= a.begin[s]) seg = s;
-        const SpeciesArgs sp = a.species[seg];
-        const double m = sp.m;
-        double x0 = a.pos[0][i], x1 = a.pos[1][i];
-        double q[PDIM];
+        for (int o = 1; o < kRedrawSpan; o <<= 1) {
+            const int up = __shfl_up(incl, o);
+            if (lane >= o) incl += up;
+        }
+        // broadcast the span's intervals while every lane of the wave is active
+        int end_of[kRedrawSpan], cnt_of[kRedrawSpan];
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
-        double Tm, vx, vy;
-        grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
-        const FlowBoost fb = flow_boost(vx, vy);
-        const double T = jl_max(Tm, 0.0);
-        double usamp, gg[PDIM];
-        noise_redraw<PDIM, NOISE>(a, i, step, 0, &usamp, gg);
-        rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
-        apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
-        grid_push<PDIM, NOISE, SLICE>(a, g, i, step, 0, j, td, m, &x0, &x1, q);
-        a.pos[0][i] = x0;
-        a.pos[1][i] = x1;
+        for (int t = 0; t < kRedrawSpan; ++t) {
+            end_of[t] = __shfl(incl, t);
+            cnt_of[t] = __shfl(cnt, t);
+        }
+        const int total = end_of[kRedrawSpan - 1];
+        for (int k = lane; k < total; k += 64) {
+            int bsel = 0, start = 0;
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
-        if (!(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) atomicMin(a.bad, (unsigned long long)step);
+            for (int t = 0; t < kRedrawSpan; ++t) {
+                if (k >= end_of[t] - cnt_of[t] && k < end_of[t]) {
+                    bsel = t;
+                    start = end_of[t] - cnt_of[t];
+                }
+            }
+            const int64_t i = a.queue[(b0 + bsel) * a.qblock + (k - start)];
+            int seg = 0;
+#pragma unroll
+            for (int s2 = 1; s2 < LMH_MAX_SPECIES; ++s2)
+                if (s2 < a.nseg && i >= a.begin[s2]) seg = s2;
+            const SpeciesArgs sp = a.species[seg];
+            const double m = sp.m;
+            double x0 = a.pos[0][i], x1 = a.pos[1][i];
+            double q[PDIM];
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+            double Tm, vx, vy;
+            grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
+            const FlowBoost fb = flow_boost(vx, vy);
+            const double T = jl_max(Tm, 0.0);
+            double usamp, gg[PDIM];
+            noise_redraw<PDIM, NOISE>(a, i, step, 0, &usamp, gg);
+            rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+            apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
+            grid_push<PDIM, NOISE, SLICE>(a, g, i, step, 0, j, td, m, &x0, &x1, q);
+            a.pos[0][i] = x0;
+            a.pos[1][i] = x1;
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+            if (!(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) atomicMin(a.bad, (unsigned long long)step);
+        }
     }
-}
-
-__global__ void k_clear_counts(unsigned int* qcount) {
-    if (threadIdx.x < 2) qcount[threadIdx.x] = 0u;
 }
 
 template <int PDIM>
@@ -908,7 +946,7 @@ struct lmh_ctx {
     SpeciesArgs* d_species = nullptr;
     // deferred collisions: particle indices and the two step-parity counters
     int32_t* d_queue = nullptr;
-    unsigned int* d_qcount = nullptr;
+    int32_t* d_qcount = nullptr;      // one count per step-kernel block
 
     // launch timing
     bool profiling = false;
@@ -1024,6 +1062,7 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     a.species = c->d_species;
     a.queue = c->d_queue;
     a.qcount = c->d_qcount;
+    a.qblock = block;
     a.sqrt_dt = std::sqrt(p->dt);
     a.C = noise_layout(p, c->dims, c->pdim, &a.oxi, &a.orta, &a.opos);
     *nblocks = (unsigned)blk;
@@ -1331,19 +1370,14 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         c->counts[k] = k < nspecies ? counts[k] : 0;
         c->mass[k] = k < nspecies ? masses[k] : 0.0;
     }
+    // a full block region per step-kernel block: at most N/64 + one partial block per species
     dfree(c->d_queue);
-    if (hipMalloc((void**)&c->d_queue, (size_t)N * sizeof(int32_t)) != hipSuccess) {
-        c->d_queue = nullptr;
+    dfree(c->d_qcount);
+    const size_t max_blocks = (size_t)(N / 64) + LMH_MAX_SPECIES + 1;
+    if (hipMalloc((void**)&c->d_queue, max_blocks * kBlock * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->d_qcount, max_blocks * sizeof(int32_t)) != hipSuccess) {
         return c->fail(LMH_E_NOMEM, "set_particles: queue allocation failed");
     }
-    if (!c->d_qcount) {
-        if (hipMalloc((void**)&c->d_qcount, 2 * sizeof(unsigned int)) != hipSuccess) {
-            c->d_qcount = nullptr;
-            return c->fail(LMH_E_NOMEM, "set_particles: counter allocation failed");
-        }
-    }
-    unsigned int zero2[2] = {0u, 0u};
-    LMH_HIP(c, hipMemcpy(c->d_qcount, zero2, sizeof(zero2), hipMemcpyHostToDevice));
     return upload_species(c);
 }
 
@@ -1453,9 +1487,10 @@ void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium&
     }
     if (defer) {
         LaunchMark mk(c, LMH_PROF_REDRAW);
-        // enough lanes for a few percent of the particles per pass; the loop covers any count
-        const unsigned nr = (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (a.N / 16 + kBlock - 1) / kBlock));
-        hipLaunchKernelGGL((k_redraw_grid<PDIM, NOISE, SLICE>), dim3(nr), dim3(kBlock), 0, c->stream, a, g);
+        // one wave per kRedrawSpan step-kernel blocks
+        const int64_t nwaves = ((int64_t)nb + kRedrawSpan - 1) / kRedrawSpan;
+        const unsigned nr = (unsigned)std::max<int64_t>(1, (nwaves + (kBlock / 64) - 1) / (kBlock / 64));
+        hipLaunchKernelGGL((k_redraw_grid<PDIM, NOISE, SLICE>), dim3(nr), dim3(kBlock), 0, c->stream, a, g, (int64_t)nb);
     }
 }
 
@@ -1483,7 +1518,6 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     if (nb == 0) return LMH_OK;
     const bool has_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
     const bool defer = grid && k == 1 && has_rta && c->tune.defer;
-    if (defer) hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, c->d_qcount);
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
         const bool slice = grid && n == 1 && c->tune.slice;
