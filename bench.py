@@ -122,6 +122,8 @@ def main():
                     help="diagnostic: another momentum update than the workload's")
     ap.add_argument("--noise", default="philox", choices=["philox", "fixed"],
                     help="diagnostic: read a device-resident fixed-noise record instead of Philox")
+    ap.add_argument("--host-sort", action="store_true",
+                    help="diagnostic: order each species' quarks by grid cell before the upload")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
@@ -141,6 +143,14 @@ def main():
     DsT = media.DsT_from_2piT(4.0)
     t_setup = time.perf_counter()
     g, masses, counts, pos, mom = build_workload(N)
+    if args.host_sort:
+        ix = np.clip(np.floor((pos[0] - g.x0) / g.dx), 0, g.nx - 2).astype(np.int64)
+        iy = np.clip(np.floor((pos[1] - g.y0) / g.dy), 0, g.ny - 2).astype(np.int64)
+        key = iy * g.nx + ix
+        order = np.concatenate([c0 + np.argsort(key[c0:c0 + cn], kind="stable")
+                                for c0, cn in zip(np.cumsum([0] + counts[:-1]), counts)])
+        pos = np.asfortranarray(pos[:, order])
+        mom = np.asfortranarray(mom[:, order])
     tabs = tables_for(g, masses, DsT, 3)
     params = make_params(DsT, args.noise == "fixed", args.collision)
     ctx = lmhip.Context(0)
@@ -228,7 +238,7 @@ def main():
                                "RTA tau_R=5(eta/s)/T; dt=0.01 fm/c; Philox noise; FP64",
                    "quarks": N, "grid": [200, 200, 120], "block": args.block,
                    "steps_per_launch": max(1, args.spl), "slice": args.slice, "defer": args.defer,
-                   "collision": args.collision, "noise": args.noise,
+                   "collision": args.collision, "noise": args.noise, "host_sort": args.host_sort,
                    "wall_s_timed": round(wall, 4), "mean_pT": round(obs[1] / obs[0], 6)},
         "roofline": roof,
     }
