@@ -28,6 +28,9 @@ for _p in (PKG, os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")):
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 BYTES_PER_PARTICLE_STEP = 80   # read + write x, y, px, py, pz (FP64)
+# HBM bytes per launch of the step kernel on the default workload, from rocprofv3 FETCH_SIZE
+# (x2.00) + WRITE_SIZE (x1.00), factors from a 1 GiB device copy: profiles/k_cart_grid_1e7_grid2p1_pmc.md
+MEASURED_TRAFFIC_PER_QUARK = 80.79
 
 
 def build_workload(N, seed=3):
@@ -127,7 +130,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
-                    help="HBM bytes per launch of the step kernel from rocprofv3 counters")
+                    help="HBM bytes per launch of the step kernel from rocprofv3 counters "
+                         "(default: the measured figure when the workload is the default one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -214,9 +218,14 @@ def main():
     t_launch = prof["step"]["ms"] / 1e3 / kernel_launches
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
     achieved = bytes_per_launch / t_launch / 1e9
+    default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox"
+                     and args.defer == 1 and args.slice == 1 and not args.host_sort)
+    traffic = args.traffic
+    if traffic is None and default_shape:
+        traffic = round(MEASURED_TRAFFIC_PER_QUARK * N)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": args.traffic,
+            "traffic": traffic,
             "kernel": "k_cart_grid (PDIM=3, Philox, Langevin+RTA)", "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(t_launch * 1e3, 4), "steps_per_launch": max(1, args.spl),
             "other_kernels_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in prof.items() if k != "step"}}
