@@ -152,6 +152,16 @@ typedef struct lmh_profile {
     double  ms[3];
 } lmh_profile;
 
+/* pathwise adjoint (extension): the forward pass keeps the state every checkpoint_every steps.
+ * dtau_dDsT is (d tau_drag / d DsT) / tau_drag for the drag table in use: 1/DsT when the table
+ * is proportional to DsT (effective_DsT without modifiers or with the quadratic law,
+ * transport.jl:139-156), 0 for the linear law, which does not depend on DsT.                */
+typedef struct lmh_adjoint_spec {
+    int32_t checkpoint_every;
+    int32_t _pad;
+    double  dtau_dDsT;
+} lmh_adjoint_spec;
+
 typedef struct lmh_ctx lmh_ctx;
 
 /* ---- lifetime ---------------------------------------------------------------------------- */
@@ -226,6 +236,22 @@ int lmh_profile_stop(lmh_ctx* ctx, lmh_profile* out);     /* synchronises */
 /* ---- observables ------------------------------------------------------------------------- */
 /* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */
 int lmh_observables(lmh_ctx* ctx, const lmh_obs_spec* spec, double* out);
+
+/* ---- pathwise adjoint (extension; the reference has no differentiation, Project.toml:6-27) -- */
+/* The forward half of a ChainRulesCore.rrule / torch.autograd.Function: runs steps
+ * step_begin .. step_begin+nsteps-1 exactly as lmh_run does (same final state, same noise) and
+ * keeps checkpoints.  The 2+1D grid in Cartesian coordinates, Philox noise, the relativistic
+ * pre-point Langevin kick with or without RTA collisions, drag TABLE or GAMMA_P.            */
+int lmh_adjoint_forward(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+                        const lmh_adjoint_spec* spec);
+/* The pullback: for the loss L = sum_k cotangent[k] * O_k over the observables O of
+ * lmh_observables(spec) of the final state, grad receives dL/d(DsT, gamma0, Lambda) along the
+ * forward pass's noise path (collision decisions held fixed); dpos (dims, N) and dmom
+ * (pdim, N), both or neither, receive dL/d(state before the forward pass).  Re-runs each
+ * checkpoint segment; may be called again with another cotangent.  Fails with LMH_E_STATE
+ * when the particles, medium or tables changed since lmh_adjoint_forward.  Synchronises.   */
+int lmh_adjoint_backward(lmh_ctx* ctx, const lmh_obs_spec* spec, const double* cotangent,
+                         double* grad, double* dpos, double* dmom);
 
 #ifdef __cplusplus
 }

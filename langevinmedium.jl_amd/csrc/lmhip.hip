@@ -900,6 +900,8 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_final(const double* __restric
     }
 }
 
+#include "lmhip_adjoint.h"
+
 }  // namespace
 
 // =============================================================================================
@@ -954,6 +956,25 @@ struct lmh_ctx {
     size_t ev_used = 0;
     struct Mark { int cls; hipEvent_t a, b; };
     std::vector<Mark> marks;
+
+    // bumped by every entry point that changes the particles, the medium or the tables; an
+    // adjoint backward pass needs the state its forward pass left
+    uint64_t gen = 0;
+    // pathwise adjoint: the last adjoint forward pass and its checkpoints, the backward workspace
+    struct AdjRecord {
+        bool valid = false;
+        lmh_params p;
+        int64_t step_begin = 0, nsteps = 0;
+        int32_t K = 1;
+        double dtau_dDsT = 0.0;
+        uint64_t gen = 0;
+    } adj;
+    double* d_ckpt = nullptr;
+    size_t ckpt_cap = 0;
+    double* d_tape = nullptr;
+    size_t tape_cap = 0;
+    double* d_adjw = nullptr;         // lam, per-particle gradients, partial sums, cotangent
+    size_t adjw_cap = 0;
 
     // scratch
     double* d_noise = nullptr;
@@ -1193,6 +1214,9 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_V2);
     dfree(c->d_nodes);
     dfree(c->d_slice);
+    dfree(c->d_ckpt);
+    dfree(c->d_tape);
+    dfree(c->d_adjw);
     for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
         dfree(c->drag[k].d);
         dfree(c->rta[k].d);
@@ -1222,6 +1246,7 @@ int lmh_synchronize(lmh_ctx* c) {
 int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const double* tgrid,
                           int32_t nt, const double* T, const double* v, const double* v2) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     if (!rgrid || !tgrid || !T || !v || nr < 2 || nt < 2)
         return c->fail(LMH_E_INVALID, "set_medium_radial: grids need >= 2 nodes and T, v must be given");
     if (!sorted(rgrid, nr) || !sorted(tgrid, nt))
@@ -1240,6 +1265,9 @@ int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const dou
     }
     dfree(c->d_nodes);
     dfree(c->d_slice);
+    dfree(c->d_ckpt);
+    dfree(c->d_tape);
+    dfree(c->d_adjw);
     c->h_tg.assign(tgrid, tgrid + nt);
     c->nr = nr;
     c->nt = nt;
@@ -1253,6 +1281,7 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
                            int32_t ny, const double* tgrid, int32_t nt, const double* T,
                            const double* vx, const double* vy) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     if (!tgrid || !T || !vx || !vy || nx < 2 || ny < 2 || nt < 2 || !(dx > 0.0) || !(dy > 0.0) ||
         !std::isfinite(x0) || !std::isfinite(y0))
         return c->fail(LMH_E_INVALID, "set_medium_grid2p1: need nx, ny, nt >= 2, dx, dy > 0 and all fields");
@@ -1269,6 +1298,9 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
     for (size_t k = 0; k < n; ++k) nodes[k] = make_double4(T[k], vx[k], vy[k], 0.0);
     dfree(c->d_nodes);
     dfree(c->d_slice);
+    dfree(c->d_ckpt);
+    dfree(c->d_tape);
+    dfree(c->d_adjw);
     if (hipMalloc((void**)&c->d_nodes, n * sizeof(double4)) != hipSuccess ||
         hipMalloc((void**)&c->d_slice, nxy * sizeof(double4)) != hipSuccess)
         return c->fail(LMH_E_NOMEM, "set_medium_grid2p1: device allocation failed");
@@ -1294,6 +1326,7 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
 int lmh_set_time_table(lmh_ctx* c, int32_t which, int32_t species, double Tmin, double invdT,
                        const double* vals, int32_t n) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     if (species < 0 || species >= LMH_MAX_SPECIES || (which != LMH_TABLE_DRAG && which != LMH_TABLE_RTA) ||
         !vals || n < 1)
         return c->fail(LMH_E_INVALID, "set_time_table: bad species, table kind or size");
@@ -1311,6 +1344,7 @@ int lmh_set_time_table(lmh_ctx* c, int32_t which, int32_t species, double Tmin, 
 int lmh_set_juttner_table(lmh_ctx* c, int32_t species, const double* invcdf, int32_t nU,
                           int32_t nT, double Tmin, double invdT) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     if (species < 0 || species >= LMH_MAX_SPECIES || !invcdf || nU < 2 || nT < 2)
         return c->fail(LMH_E_INVALID, "set_juttner_table: bad species or size");
     LMH_HIP(c, hipSetDevice(c->device));
@@ -1328,6 +1362,7 @@ int lmh_set_juttner_table(lmh_ctx* c, int32_t species, const double* invcdf, int
 int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double* mom, int32_t pdim,
                       int64_t N, int32_t nspecies, const int64_t* counts, const double* masses) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     if (!pos || !mom || N < 1 || dims < 1 || dims > 3 || pdim < 1 || pdim > 3)
         return c->fail(LMH_E_INVALID, "set_particles: need N >= 1 and 1 <= dims, pdim <= 3");
     if (nspecies < 1 || nspecies > LMH_MAX_SPECIES || !counts || !masses)
@@ -1608,6 +1643,7 @@ int lmh_profile_stop(lmh_ctx* c, lmh_profile* out) {
 
 int lmh_boost_to_lab(lmh_ctx* c, const lmh_params* p, int64_t step) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     int rc = check_run(c, p);
     if (rc) return rc;
     if (p->coords != LMH_CARTESIAN) return c->fail(LMH_E_INVALID, "boost_to_lab: Cartesian only");
@@ -1634,6 +1670,7 @@ int lmh_boost_to_lab(lmh_ctx* c, const lmh_params* p, int64_t step) {
 int lmh_run(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
             const double* noise, int32_t steps_per_launch) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     int rc = check_run(c, p);
     if (rc) return rc;
     if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
@@ -1667,6 +1704,7 @@ int lmh_run(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
 int lmh_run_device_noise(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
                          const double* d_noise, int32_t steps_per_launch) {
     if (!c) return LMH_E_INVALID;
+    ++c->gen;
     int rc = check_run(c, p);
     if (rc) return rc;
     if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
@@ -1720,6 +1758,157 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     LMH_HIP(c, hipGetLastError());
     LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// pathwise adjoint (no reference counterpart)
+namespace {
+
+int ensure_dev(lmh_ctx* c, double** ptr, size_t* cap, size_t n, const char* what) {
+    if (n <= *cap) return LMH_OK;
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(*ptr);
+    *cap = 0;
+    if (hipMalloc((void**)ptr, n * sizeof(double)) != hipSuccess) {
+        *ptr = nullptr;
+        return c->fail(LMH_E_NOMEM, std::string(what) + ": device allocation failed");
+    }
+    *cap = n;
+    return LMH_OK;
+}
+
+int check_adjoint(lmh_ctx* c, const lmh_params* p) {
+    if (c->medium != 1 || p->coords != LMH_CARTESIAN || c->dims != 2 || (c->pdim != 2 && c->pdim != 3))
+        return c->fail(LMH_E_INVALID, "adjoint: the 2+1D grid medium in Cartesian coordinates (dims 2, pdim 2 or 3)");
+    if (p->noise != LMH_NOISE_PHILOX)
+        return c->fail(LMH_E_INVALID, "adjoint: Philox noise (common random numbers are regenerated from counters)");
+    if (p->collision != LMH_LANGEVIN && p->collision != LMH_LANGEVIN_RTA)
+        return c->fail(LMH_E_INVALID, "adjoint: the Langevin kick, with or without RTA collisions");
+    if (p->integrator != LMH_PREPOINT || !p->relativistic || p->position_diffusion)
+        return c->fail(LMH_E_INVALID, "adjoint: the relativistic pre-point step without position diffusion");
+    if (!(p->dt > 0.0)) return c->fail(LMH_E_INVALID, "adjoint: dt > 0");
+    if (p->drag_model == LMH_DRAG_GAMMA_P && !(p->Lambda > 0.0))
+        return c->fail(LMH_E_INVALID, "adjoint: Lambda > 0");
+    return LMH_OK;
+}
+
+template <int PDIM>
+void launch_adj_segment(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll,
+                        const double* ckpt, int64_t step0, int32_t nk, double* lam, double* gacc,
+                        double dtau) {
+    const dim3 G(nb), B(c->tune.block);
+    if (coll == LMH_LANGEVIN_RTA)
+        hipLaunchKernelGGL((k_adj_segment<PDIM, LMH_LANGEVIN_RTA>), G, B, 0, c->stream, a, g, ckpt, step0, nk, c->d_tape, lam, gacc, dtau);
+    else
+        hipLaunchKernelGGL((k_adj_segment<PDIM, LMH_LANGEVIN>), G, B, 0, c->stream, a, g, ckpt, step0, nk, c->d_tape, lam, gacc, dtau);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
+                        const lmh_adjoint_spec* s) {
+    if (!c) return LMH_E_INVALID;
+    c->adj.valid = false;
+    int rc = check_run(c, p);
+    if (rc) return rc;
+    if (!s || s->checkpoint_every < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: checkpoint_every >= 1");
+    if (nsteps < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: nsteps >= 1");
+    if ((rc = check_adjoint(c, p))) return rc;
+    LMH_HIP(c, hipSetDevice(c->device));
+    const int64_t K = s->checkpoint_every;
+    const int64_t ncp = (nsteps + K - 1) / K;
+    const size_t R = (size_t)(c->dims + c->pdim);
+    const size_t per = R * (size_t)c->N;
+    if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
+    ++c->gen;
+    for (int64_t cp = 0; cp < ncp; ++cp) {
+        LMH_HIP(c, hipMemcpyAsync(c->d_ckpt + per * (size_t)cp, c->d_state, per * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
+        if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 0))) return rc;
+    }
+    c->adj.p = *p;
+    c->adj.step_begin = step_begin;
+    c->adj.nsteps = nsteps;
+    c->adj.K = (int32_t)K;
+    c->adj.dtau_dDsT = s->dtau_dDsT;
+    c->adj.gen = c->gen;
+    c->adj.valid = true;
+    return LMH_OK;
+}
+
+int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotangent, double* grad,
+                         double* dpos, double* dmom) {
+    if (!c) return LMH_E_INVALID;
+    if (!c->adj.valid || c->adj.gen != c->gen)
+        return c->fail(LMH_E_STATE, "adjoint_backward: no adjoint forward pass for the current state "
+                                    "(the particles, medium or tables changed since)");
+    if (!os || !cotangent || !grad) return c->fail(LMH_E_INVALID, "adjoint_backward: null spec, cotangent or gradient");
+    if ((dpos == nullptr) != (dmom == nullptr))
+        return c->fail(LMH_E_INVALID, "adjoint_backward: give both state-gradient outputs or neither");
+    if (os->nbins < 0 || os->nbins > kObsMaxBins || (os->nbins > 0 && !(os->pt_max > os->pt_min)))
+        return c->fail(LMH_E_INVALID, "adjoint_backward: 0 <= nbins <= 120 and pt_max > pt_min");
+    if (os->species >= c->nspecies) return c->fail(LMH_E_INVALID, "adjoint_backward: species out of range");
+    LMH_HIP(c, hipSetDevice(c->device));
+    const lmh_params& p = c->adj.p;
+    const int64_t N = c->N;
+    const int R = c->dims + c->pdim;
+    const int TR = R + c->pdim + 1;
+    const int64_t K = c->adj.K;
+    const int64_t ncp = (c->adj.nsteps + K - 1) / K;
+    const int nw = 5 + os->nbins;
+    int rc;
+    if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, (size_t)K * TR * (size_t)N, "adjoint_backward tape"))) return rc;
+    const size_t o_lam = 0, o_g = (size_t)R * N, o_part = o_g + (size_t)kAdjThetas * N;
+    const size_t o_out = o_part + (size_t)kAdjThetas * kSumBlocks, o_w = o_out + kAdjThetas;
+    if ((rc = ensure_dev(c, &c->d_adjw, &c->adjw_cap, o_w + nw, "adjoint_backward workspace"))) return rc;
+    double* lam = c->d_adjw + o_lam;
+    double* gacc = c->d_adjw + o_g;
+    LMH_HIP(c, hipMemcpyAsync(c->d_adjw + o_w, cotangent, nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int64_t i0 = 0, i1 = N;
+    if (os->species >= 0) {
+        i0 = species_begin(c, os->species);
+        i1 = i0 + c->counts[os->species];
+    }
+    const double* px = c->d_state + (int64_t)c->dims * N;
+    hipLaunchKernelGGL(k_adj_seed, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, px, px + N, N, i0, i1,
+                       c->pdim, *os, c->d_adjw + o_w, lam);
+    LMH_HIP(c, hipMemsetAsync(gacc, 0, (size_t)kAdjThetas * N * sizeof(double), c->stream));
+    unsigned nb = 0;
+    StepArgs a = make_args(c, &p, c->tune.block, &nb);
+    GridMedium g = make_grid(c);
+    const size_t per = (size_t)R * N;
+    for (int64_t cp = ncp - 1; cp >= 0; --cp) {
+        const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
+        const int64_t step0 = c->adj.step_begin + cp * K;
+        LaunchMark mk(c, LMH_PROF_STEP);
+        if (c->pdim == 2) launch_adj_segment<2>(c, a, nb, g, p.collision, c->d_ckpt + per * cp, step0, nk, lam, gacc, c->adj.dtau_dDsT);
+        else launch_adj_segment<3>(c, a, nb, g, p.collision, c->d_ckpt + per * cp, step0, nk, lam, gacc, c->adj.dtau_dDsT);
+        LMH_HIP(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rows_partial, dim3(kSumBlocks), dim3(kSumBlock), 0, c->stream, gacc, kAdjThetas, N, c->d_adjw + o_part);
+    hipLaunchKernelGGL(k_rows_final, dim3(kAdjThetas), dim3(kSumBlock), 0, c->stream, c->d_adjw + o_part, kSumBlocks, c->d_adjw + o_out);
+    LMH_HIP(c, hipGetLastError());
+    LMH_HIP(c, hipMemcpyAsync(grad, c->d_adjw + o_out, kAdjThetas * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    if (dpos) {
+        std::vector<double> h;
+        try {
+            h.resize(per);
+        } catch (...) {
+            return c->fail(LMH_E_NOMEM, "adjoint_backward: host allocation failed");
+        }
+        LMH_HIP(c, hipMemcpy(h.data(), lam, per * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i) {
+            for (int d = 0; d < c->dims; ++d) dpos[d + (int64_t)c->dims * i] = h[(size_t)d * N + i];
+            for (int d = 0; d < c->pdim; ++d) dmom[d + (int64_t)c->pdim * i] = h[(size_t)(c->dims + d) * N + i];
+        }
+    }
     return LMH_OK;
 }
 

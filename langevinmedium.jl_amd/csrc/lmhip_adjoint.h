@@ -1,0 +1,503 @@
+// lmhip_adjoint.h -- pathwise adjoint of the fused 2+1D Langevin step (included by lmhip.hip
+// inside its anonymous namespace, after the forward kernels it reuses).
+//
+// The loss is L = sum_k w_k O_k over the observables O of the final lab momenta (N, sum p_T,
+// sum p_T^2, sum cos 2phi, sum p_T cos 2phi, the [smoothed] p_T histogram); w is the cotangent a
+// ChainRulesCore.rrule / torch.autograd.Function pullback receives.  The parameters are
+// theta = (DsT, gamma0, Lambda): DsT scales the drag-time table, (gamma0, Lambda) are the
+// GAMMA_P drag eta = gamma0 T^2 / (1 + p^2/Lambda^2).
+//
+// Common random numbers: the noise of particle i at step s is Philox4x64-10 of counter
+// (i, s, block), so the reverse sweep regenerates it instead of storing it per step.  The forward
+// pass keeps the state every K steps (checkpoints); the backward pass walks the segments from the
+// last to the first, re-runs each segment forward from its checkpoint while writing the step
+// inputs (state and kick noise) to a segment tape, then runs the reverse steps over the tape.
+// One thread owns one particle throughout, so no adjoint is ever exchanged between threads.
+//
+// The RTA collision decision is a discontinuous function of the state; the pathwise derivative
+// keeps each step's decision fixed (a colliding step forgets the incoming momentum and depends
+// on the state only through T in the inverse-CDF lookup).
+
+constexpr int kAdjThetas = 3;   // d/dDsT, d/dgamma0, d/dLambda
+
+// d/dT of eval_time_table: the slope of the segment the evaluation uses (transport.jl:231-253
+// extrapolates linearly, so the end segments keep their slope outside the table)
+__device__ __forceinline__ double eval_time_table_slope(const TimeTable& t, double T) {
+    if (t.n < 2 || !isfinite(T)) return 0.0;
+    double u = (T - t.Tmin) * t.invdT;
+    double fu = floor(u);
+    int i;
+    if (!(fu >= 0.0)) i = 1;
+    else if (fu > (double)(t.n - 2)) i = t.n - 1;
+    else i = (int)fu + 1;
+    return (t.vals[i] - t.vals[i - 1]) * t.invdT;
+}
+
+// d/dT of juttner_lookup at fixed quantile u (0 where the T fraction is clamped)
+__device__ __forceinline__ double juttner_lookup_dT(const JuttnerTable& jt, double u, double T) {
+    double uu = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double fu = uu * (double)(jt.nU - 1);
+    int iu = (int)floor(fu);
+    iu = iu < 0 ? 0 : (iu > jt.nU - 2 ? jt.nU - 2 : iu);
+    double tu = fu - (double)iu;
+    double fT = (T - jt.Tmin) * jt.invdT;
+    double fl = floor(fT);
+    int jT;
+    if (!(fl >= 0.0)) jT = 0;
+    else if (fl > (double)(jt.nT - 2)) jT = jt.nT - 2;
+    else jT = (int)fl;
+    double tT = fT - (double)jT;
+    if (!(tT >= 0.0 && tT <= 1.0)) return 0.0;
+    const double* b0 = jt.vals + (int64_t)jT * jt.nU + iu;
+    const double* b1 = b0 + jt.nU;
+    double c0 = b0[0] * (1.0 - tu) + b0[1] * tu;
+    double c1 = b1[0] * (1.0 - tu) + b1[1] * tu;
+    return (c1 - c0) * jt.invdT;
+}
+
+// The medium at (x, y) as grid_sample<false> computes it (same operations, same rounding) and
+// its derivatives with respect to x and y (0 along an axis where the position is clamped).
+struct SampleGrad {
+    double T, vx, vy;
+    double dTx, dTy, dvxx, dvxy, dvyx, dvyy;
+};
+
+__device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, int j,
+                                                 double td, SampleGrad* s) {
+    double xc = jl_clamp(x, g.x0, g.xlast);
+    double yc = jl_clamp(y, g.y0, g.ylast);
+    double fx = (xc - g.x0) * g.inv_dx;
+    double fy = (yc - g.y0) * g.inv_dy;
+    int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
+    double xd = fx - (double)ix, yd = fy - (double)iy;
+    int64_t n00 = ix + (int64_t)g.nx * iy;
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const double4* s0 = g.nodes + nxy * j + n00;
+    const double4* s1 = s0 + nxy;
+    double4 c00 = lerp4(s0[0], s1[0], td);
+    double4 c10 = lerp4(s0[1], s1[1], td);
+    double4 c01 = lerp4(s0[g.nx], s1[g.nx], td);
+    double4 c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
+    double a0 = c00.x * (1.0 - xd) + c10.x * xd;
+    double a1 = c01.x * (1.0 - xd) + c11.x * xd;
+    s->T = a0 * (1.0 - yd) + a1 * yd;
+    double b0 = c00.y * (1.0 - xd) + c10.y * xd;
+    double b1 = c01.y * (1.0 - xd) + c11.y * xd;
+    s->vx = b0 * (1.0 - yd) + b1 * yd;
+    double e0 = c00.z * (1.0 - xd) + c10.z * xd;
+    double e1 = c01.z * (1.0 - xd) + c11.z * xd;
+    s->vy = e0 * (1.0 - yd) + e1 * yd;
+    const double gx = (x >= g.x0 && x <= g.xlast) ? g.inv_dx : 0.0;
+    const double gy = (y >= g.y0 && y <= g.ylast) ? g.inv_dy : 0.0;
+    s->dTx = ((c10.x - c00.x) * (1.0 - yd) + (c11.x - c01.x) * yd) * gx;
+    s->dvxx = ((c10.y - c00.y) * (1.0 - yd) + (c11.y - c01.y) * yd) * gx;
+    s->dvyx = ((c10.z - c00.z) * (1.0 - yd) + (c11.z - c01.z) * yd) * gx;
+    s->dTy = (a1 - a0) * gy;
+    s->dvxy = (b1 - b0) * gy;
+    s->dvyy = (e1 - e0) * gy;
+}
+
+// reverse of apply_boost (relativistic) for input momentum q: qb holds the adjoint of the output
+// on entry and of the input on return; the adjoints of the flow factors are accumulated.
+template <int PDIM>
+__device__ __forceinline__ void boost_reverse(const double* q, double m, const FlowBoost& b, int sign,
+                                              double* qb, double* nxb, double* nyb, double* vb,
+                                              double* gamb) {
+    double p2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
+    const double E = sqrt(p2 + m * m);
+    const double s = sign > 0 ? 1.0 : -1.0;
+    const double ppar = q[0] * b.nx + q[1] * b.ny;
+    const double inner = ppar + s * b.v * E;
+    const double delta = b.gam * inner - ppar;          // q' = q + delta * n on rows 0, 1
+    const double deltab = qb[0] * b.nx + qb[1] * b.ny;
+    *nxb += qb[0] * delta;
+    *nyb += qb[1] * delta;
+    *gamb += deltab * inner;
+    *vb += deltab * b.gam * s * E;
+    const double Eb = deltab * b.gam * s * b.v;
+    const double pparb = deltab * (b.gam - 1.0);
+    *nxb += pparb * q[0];
+    *nyb += pparb * q[1];
+    qb[0] += pparb * b.nx;
+    qb[1] += pparb * b.ny;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) qb[d] += Eb * q[d] / E;
+}
+
+// One forward step, exactly the fused k_cart_grid step (non-deferred, Philox, no position
+// diffusion), for the tape recomputation.  xi and ucol are the step's kick noise.
+template <int PDIM, int COLL>
+__device__ __forceinline__ void adj_forward_step(const StepArgs& a, const GridMedium& g,
+                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
+                                                 double* x0, double* x1, double* q,
+                                                 const double* xi, double ucol) {
+    constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    const double t_now = (double)step * p.dt + p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    double Tm, vx, vy;
+    grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    const FlowBoost fb = flow_boost(vx, vy);
+    apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
+    if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
+    const double T = jl_max(Tm, 0.0);
+    double etaD = 0.0, kT = 0.0;
+    drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+    ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+    if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+        double usamp, gg[PDIM];
+        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
+        rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+    }
+    apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
+    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, g, i, step, 0, j, td, m, x0, x1, q);
+}
+
+// One reverse step.  (x0, x1, qin) is the step's input state and (xi, ucol) its kick noise;
+// lam holds the adjoint of the step's output state (x0, x1, q) on entry and of its input state
+// on return; th accumulates dL/dtheta.  The pre-point exact-OU kick, relativistic boosts.
+template <int PDIM, int COLL>
+__device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMedium& g,
+                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
+                                                 double x0, double x1, const double* qin,
+                                                 const double* xi, double ucol,
+                                                 double dtau_dDsT, double* lam, double* th) {
+    constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    const double dt = p.dt;
+    const double t_now = (double)step * dt + p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+
+    // ---- forward values
+    SampleGrad s;
+    grid_sample_grad(g, x0, x1, j, td, &s);
+    const FlowBoost fb = flow_boost(s.vx, s.vy);
+    double qB[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) qB[d] = qin[d];
+    apply_boost<PDIM>(qB, m, fb, -1, 1);
+    double fbj = 1.0;
+    if (PDIM == 3 && p.bjorken_redshift) {
+        const double tau_a = p.t0 + (double)(step - 1) * dt;
+        if (tau_a > 0.0) fbj = tau_a / (tau_a + dt);
+        bjorken(p, step, &qB[2 % PDIM]);
+    }
+    const double T = jl_max(s.T, 0.0);
+    double etaD = 0.0, kT = 0.0, tau = 0.0, dtau_dT = 0.0;
+    bool tab_ok = false;
+    if (p.drag_model == LMH_DRAG_GAMMA_P) {
+        etaD = p.gamma0 * T * T;
+        kT = sqrt(2.0 * m * T * etaD);
+    } else {
+        const double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
+        tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(sp.drag, T);
+        tab_ok = tau > 0.0 && isfinite(tau);
+        etaD = tab_ok ? (1.0 / tau) : 0.0;
+        const double kappa = tab_ok ? ((2.0 * m * T) / tau) : 0.0;
+        kT = sqrt(kappa);
+        if (tab_ok) dtau_dT = eval_time_table_slope(sp.drag, T);
+    }
+    // kick (ou_kick, pre-point branch)
+    double qK[PDIM];
+    double eta = 0.0, aa = 1.0, rr = 0.0, npref = 0.0, E2 = 0.0, den = 1.0, q2B = 0.0;
+    const bool kick = etaD > 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q2B += qB[d] * qB[d];
+    if (kick) {
+        if (p.drag_model == LMH_DRAG_GAMMA_P) {
+            den = 1.0 + q2B / (p.Lambda * p.Lambda);
+            eta = etaD / den;
+        } else {
+            E2 = sqrt(q2B + m * m);
+            eta = p.relativistic ? etaD * m / E2 : etaD;
+        }
+        aa = exp(-eta * dt);
+        rr = (1.0 - aa * aa) / (2.0 * eta * dt);
+        npref = eta > 0.0 ? (kT * sqrt(rr)) : 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qK[d] = qB[d] + ((aa - 1.0) * qB[d] + a.sqrt_dt * (npref * xi[d]));
+    } else {
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qK[d] = qB[d] + (0.0 + a.sqrt_dt * (kT * xi[d]));
+    }
+    // collision (decision fixed)
+    bool col = false;
+    double qC[PDIM], ghat[PDIM], dpstar = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) qC[d] = qK[d];
+    if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+        col = true;
+        double usamp, gg[PDIM];
+        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
+        rta_redraw<PDIM, false>(sp.jt, T, qC, usamp, gg);
+        double nrm2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) nrm2 += gg[d] * gg[d];
+        const double inv = 1.0 / sqrt(jl_max(nrm2, 1e-300));
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) ghat[d] = gg[d] * inv;
+        dpstar = juttner_lookup_dT(sp.jt, usamp, T);
+    }
+    double qF[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) qF[d] = qC[d];
+    apply_boost<PDIM>(qF, m, fb, +1, 1);
+    double E4sq = m * m;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) E4sq += qF[d] * qF[d];
+    const double E4 = sqrt(E4sq);
+
+    // ---- reverse sweep
+    double qb[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) qb[d] = lam[2 + d];
+    double x0b = lam[0], x1b = lam[1];
+    // push: x += dt * q / E
+    qb[0] += lam[0] * dt / E4;
+    qb[1] += lam[1] * dt / E4;
+    {
+        const double E4b = -(lam[0] * qF[0] + lam[1] * qF[1]) * dt / (E4 * E4);
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qb[d] += E4b * qF[d] / E4;
+    }
+    double nxb = 0.0, nyb = 0.0, vb = 0.0, gamb = 0.0, Tb = 0.0;
+    if (fb.on) boost_reverse<PDIM>(qC, m, fb, +1, qb, &nxb, &nyb, &vb, &gamb);
+    if (col) {
+        double s_ = 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) s_ += qb[d] * ghat[d];
+        Tb += s_ * dpstar;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qb[d] = 0.0;
+    }
+    // kick
+    double kTb = 0.0, etaDb = 0.0;
+    {
+        double xq = 0.0;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) xq += qb[d] * xi[d];
+        if (kick) {
+            double ab = 0.0;
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) ab += qb[d] * qB[d];
+            const double npb = a.sqrt_dt * xq;
+            double etab = 0.0;
+            if (eta > 0.0 && rr > 0.0) {
+                const double srr = sqrt(rr);
+                kTb += npb * srr;
+                const double rrb = npb * kT / (2.0 * srr);
+                ab += -rrb * aa / (eta * dt);
+                etab += -rrb * rr / eta;
+            }
+            etab += -ab * dt * aa;
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) qb[d] *= aa;
+            if (p.drag_model == LMH_DRAG_GAMMA_P) {
+                etaDb += etab / den;
+                const double denb = -etab * eta / den;
+                const double L2 = p.Lambda * p.Lambda;
+#pragma unroll
+                for (int d = 0; d < PDIM; ++d) qb[d] += denb * 2.0 * qB[d] / L2;
+                th[2] += denb * (-2.0 * q2B / (L2 * p.Lambda));
+            } else if (p.relativistic) {
+                etaDb += etab * m / E2;
+                const double E2b = -etab * eta / E2;
+#pragma unroll
+                for (int d = 0; d < PDIM; ++d) qb[d] += E2b * qB[d] / E2;
+            } else {
+                etaDb += etab;
+            }
+        } else {
+            kTb += a.sqrt_dt * xq;
+        }
+    }
+    // drag coefficients
+    if (p.drag_model == LMH_DRAG_GAMMA_P) {
+        if (kT > 0.0) {
+            const double kappab = kTb / (2.0 * kT);      // kappa = 2 m T etaD
+            Tb += kappab * 2.0 * m * etaD;
+            etaDb += kappab * 2.0 * m * T;
+        }
+        Tb += etaDb * 2.0 * p.gamma0 * T;
+        th[1] += etaDb * T * T;
+    } else if (tab_ok) {
+        double taub = -etaDb / (tau * tau);
+        if (kT > 0.0) {
+            const double kappa = kT * kT;
+            const double kappab = kTb / (2.0 * kT);      // kappa = 2 m T / tau
+            Tb += kappab * 2.0 * m / tau;
+            taub += -kappab * kappa / tau;
+        }
+        Tb += taub * dtau_dT;
+        th[0] += taub * tau * dtau_dDsT;
+    }
+    const double Tmb = s.T > 0.0 ? Tb : 0.0;
+    // Bjorken redshift, then the boost to the LRF from the input momentum
+    if (PDIM == 3) qb[2 % PDIM] *= fbj;
+    if (fb.on) {
+        boost_reverse<PDIM>(qin, m, fb, -1, qb, &nxb, &nyb, &vb, &gamb);
+        // flow factors: gam(v), v = min(|v|, vmax), n = v_vec / |v|
+        vb += gamb * fb.gam * fb.gam * fb.gam * fb.v;
+        double vmagb = fb.vmag > vmax_() ? 0.0 : vb;
+        vmagb += -(nxb * s.vx + nyb * s.vy) / (fb.vmag * fb.vmag);
+        const double vxb = nxb / fb.vmag + vmagb * s.vx / fb.vmag;
+        const double vyb = nyb / fb.vmag + vmagb * s.vy / fb.vmag;
+        x0b += Tmb * s.dTx + vxb * s.dvxx + vyb * s.dvyx;
+        x1b += Tmb * s.dTy + vxb * s.dvxy + vyb * s.dvyy;
+    } else {
+        x0b += Tmb * s.dTx;
+        x1b += Tmb * s.dTy;
+    }
+    lam[0] = x0b;
+    lam[1] = x1b;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) lam[2 + d] = qb[d];
+}
+
+// dL/dq of the final lab momenta for the cotangent w of the observables (k_obs_partial's
+// definitions); positions and p_z do not enter.
+__global__ __launch_bounds__(kBlock) void k_adj_seed(const double* __restrict__ px,
+                                                      const double* __restrict__ py, int64_t N,
+                                                      int64_t i0, int64_t i1, int pdim,
+                                                      lmh_obs_spec s, const double* __restrict__ w,
+                                                      double* __restrict__ lam) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int R = 2 + pdim;
+    double gx = 0.0, gy = 0.0;
+    if (i >= i0 && i < i1) {
+        const double a = px[i], b = py[i];
+        const double pt2 = a * a + b * b;
+        if (pt2 > 0.0) {
+            const double pt = sqrt(pt2);
+            const double c2 = (a * a - b * b) / pt2;
+            // d pt, d pt^2, d cos2phi
+            const double ptx = a / pt, pty = b / pt;
+            const double c2x = 4.0 * a * b * b / (pt2 * pt2), c2y = -4.0 * b * a * a / (pt2 * pt2);
+            double gpt = w[1] + w[4] * c2;
+            if (s.nbins > 0 && s.sigma > 0.0) {
+                const double width = (s.pt_max - s.pt_min) / (double)s.nbins;
+                const double inv_s = 1.0 / (s.sigma * sqrt(2.0));
+                const double k = inv_s / sqrt(M_PI);
+                for (int bin = 0; bin < s.nbins; ++bin) {
+                    const double lo = s.pt_min + (double)bin * width;
+                    const double hi = lo + width;
+                    const double zl = (pt - lo) * inv_s, zh = (pt - hi) * inv_s;
+                    gpt += w[5 + bin] * k * (exp(-zl * zl) - exp(-zh * zh));
+                }
+            }
+            gx = gpt * ptx + w[2] * 2.0 * a + (w[3] + w[4] * pt) * c2x;
+            gy = gpt * pty + w[2] * 2.0 * b + (w[3] + w[4] * pt) * c2y;
+        }
+    }
+    lam[0 * N + i] = 0.0;
+    lam[1 * N + i] = 0.0;
+    lam[2 * N + i] = gx;
+    lam[3 * N + i] = gy;
+    for (int r = 4; r < R; ++r) lam[(int64_t)r * N + i] = 0.0;
+}
+
+// One backward segment: re-run the segment's nk steps from its checkpoint while writing each
+// step's input state and kick noise to the tape (rows x0, x1, q..., xi..., ucol), then the
+// reverse steps from the last to the first.  lam carries the state adjoint between segments
+// (rows as the state), gacc the per-particle dL/dtheta (3 rows).
+template <int PDIM, int COLL>
+__global__ __launch_bounds__(kBlock) void k_adj_segment(StepArgs a, GridMedium g,
+                                                         const double* __restrict__ ckpt,
+                                                         int64_t seg_step0, int32_t nk,
+                                                         double* __restrict__ tape,
+                                                         double* __restrict__ lam,
+                                                         double* __restrict__ gacc,
+                                                         double dtau_dDsT) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs sp = a.species[seg];
+    const int64_t N = a.N;
+    constexpr int R = 2 + PDIM;
+    constexpr int TR = R + PDIM + 1;
+    double x0 = ckpt[i], x1 = ckpt[N + i];
+    double q[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = ckpt[(int64_t)(2 + d) * N + i];
+    for (int k = 0; k < nk; ++k) {
+        const int64_t step = seg_step0 + k;
+        double* tk = tape + (int64_t)k * TR * N + i;
+        tk[0] = x0;
+        tk[N] = x1;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) tk[(int64_t)(2 + d) * N] = q[d];
+        double xi[PDIM];
+        double ucol = 1.0;
+        noise_kick<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, xi, &ucol);
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) tk[(int64_t)(R + d) * N] = xi[d];
+        tk[(int64_t)(R + PDIM) * N] = ucol;
+        adj_forward_step<PDIM, COLL>(a, g, sp, i, step, &x0, &x1, q, xi, ucol);
+    }
+    double lm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) lm[r] = lam[(int64_t)r * N + i];
+    double th[kAdjThetas] = {0.0, 0.0, 0.0};
+    for (int k = nk - 1; k >= 0; --k) {
+        const int64_t step = seg_step0 + k;
+        const double* tk = tape + (int64_t)k * TR * N + i;
+        double qin[PDIM], xi[PDIM];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) {
+            qin[d] = tk[(int64_t)(2 + d) * N];
+            xi[d] = tk[(int64_t)(R + d) * N];
+        }
+        adj_reverse_step<PDIM, COLL>(a, g, sp, i, step, tk[0], tk[N], qin, xi,
+                                     tk[(int64_t)(R + PDIM) * N], dtau_dDsT, lm, th);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) lam[(int64_t)r * N + i] = lm[r];
+#pragma unroll
+    for (int t = 0; t < kAdjThetas; ++t) gacc[(int64_t)t * N + i] += th[t];
+}
+
+// Deterministic sums of `rows` rows of n doubles: per-block partials in a fixed tree, then one
+// block per row sums the partials in a fixed tree.
+constexpr int kSumBlock = 256;
+constexpr int kSumBlocks = 1024;
+
+__global__ __launch_bounds__(kSumBlock) void k_rows_partial(const double* __restrict__ x, int rows,
+                                                            int64_t n, double* __restrict__ partial) {
+    __shared__ double acc[kSumBlock];
+    for (int r = 0; r < rows; ++r) {
+        double s = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * kSumBlock + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kSumBlock)
+            s += x[(int64_t)r * n + i];
+        acc[threadIdx.x] = s;
+        __syncthreads();
+        for (int st = kSumBlock / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) acc[threadIdx.x] += acc[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)r * gridDim.x + blockIdx.x] = acc[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kSumBlock) void k_rows_final(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ out) {
+    __shared__ double acc[kSumBlock];
+    const int r = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kSumBlock) s += partial[(int64_t)r * nblk + b];
+    acc[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kSumBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) acc[threadIdx.x] += acc[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = acc[0];
+}

@@ -62,6 +62,10 @@ class lmh_obs_spec(ctypes.Structure):
                 ("pt_max", c_double), ("sigma", c_double)]
 
 
+class lmh_adjoint_spec(ctypes.Structure):
+    _fields_ = [("checkpoint_every", c_int32), ("_pad", c_int32), ("dtau_dDsT", c_double)]
+
+
 def make_params(*, dt, t0=0.0, DsT=0.0, psi2=0.0, gamma0=0.0, Lambda=1.0, seed=0, stream=0,
                 coords=LMH_CARTESIAN, collision=LMH_LANGEVIN, noise=LMH_NOISE_PHILOX,
                 integrator=LMH_PREPOINT, drag_model=LMH_DRAG_TABLE, relativistic=True,
@@ -102,6 +106,9 @@ SIGNATURES = {
     "lmh_observables": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD]),
     "lmh_profile_start": (c_int, [c_void_p]),
     "lmh_profile_stop": (c_int, [c_void_p, POINTER(lmh_profile)]),
+    "lmh_adjoint_forward": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64,
+                                    POINTER(lmh_adjoint_spec)]),
+    "lmh_adjoint_backward": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD, _PD, _PD, _PD]),
 }
 
 _LIB = None

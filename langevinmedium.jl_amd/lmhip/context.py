@@ -187,3 +187,32 @@ class Context:
         out = np.zeros(5 + int(nbins), dtype=np.float64)
         self._check(self._lib.lmh_observables(self._h, ctypes.byref(spec), dptr(out)))
         return out
+
+    # ---- pathwise adjoint ----
+    def adjoint_forward(self, params, step_begin: int, nsteps: int, checkpoint_every: int = 50,
+                        dtau_dDsT: float | None = None):
+        """Steps as run() does, keeping a checkpoint every `checkpoint_every` steps.  dtau_dDsT is
+        (d tau_drag/d DsT)/tau_drag of the drag table: 1/DsT (the default) for tables
+        proportional to DsT, 0 for the linear effective-D_s law."""
+        if dtau_dDsT is None:
+            dtau_dDsT = 1.0 / params.DsT if params.DsT > 0.0 else 0.0
+        spec = _ffi.lmh_adjoint_spec(int(checkpoint_every), 0, float(dtau_dDsT))
+        self._check(self._lib.lmh_adjoint_forward(self._h, ctypes.byref(params), int(step_begin),
+                                                  int(nsteps), ctypes.byref(spec)))
+
+    def adjoint_backward(self, cotangent, nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.0, species=-1,
+                         state_grad=False):
+        """dL/d(DsT, gamma0, Lambda) for L = cotangent . observables(nbins, ...) of the final
+        state; with state_grad also dL/d(positions, momenta) before the forward pass."""
+        spec = _ffi.lmh_obs_spec(int(nbins), int(species), float(pt_min), float(pt_max), float(sigma))
+        w = np.ascontiguousarray(cotangent, dtype=np.float64)
+        if w.shape != (5 + int(nbins),):
+            raise ValueError(f"cotangent must have 5 + nbins = {5 + int(nbins)} entries")
+        grad = np.zeros(3, dtype=np.float64)
+        dpos = dmom = None
+        if state_grad:
+            dpos = np.empty((self.dims, self.N), dtype=np.float64, order="F")
+            dmom = np.empty((self.pdim, self.N), dtype=np.float64, order="F")
+        self._check(self._lib.lmh_adjoint_backward(self._h, ctypes.byref(spec), dptr(w), dptr(grad),
+                                                   dptr(dpos), dptr(dmom)))
+        return (grad, dpos, dmom) if state_grad else grad
