@@ -1265,9 +1265,6 @@ int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const dou
     }
     dfree(c->d_nodes);
     dfree(c->d_slice);
-    dfree(c->d_ckpt);
-    dfree(c->d_tape);
-    dfree(c->d_adjw);
     c->h_tg.assign(tgrid, tgrid + nt);
     c->nr = nr;
     c->nt = nt;
@@ -1298,9 +1295,6 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
     for (size_t k = 0; k < n; ++k) nodes[k] = make_double4(T[k], vx[k], vy[k], 0.0);
     dfree(c->d_nodes);
     dfree(c->d_slice);
-    dfree(c->d_ckpt);
-    dfree(c->d_tape);
-    dfree(c->d_adjw);
     if (hipMalloc((void**)&c->d_nodes, n * sizeof(double4)) != hipSuccess ||
         hipMalloc((void**)&c->d_slice, nxy * sizeof(double4)) != hipSuccess)
         return c->fail(LMH_E_NOMEM, "set_medium_grid2p1: device allocation failed");
