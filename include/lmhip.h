@@ -162,7 +162,30 @@ typedef struct lmh_adjoint_spec {
     double  dtau_dDsT;
 } lmh_adjoint_spec;
 
+/* one run of a batch (extension): its particle count, mass and D_s T, the index of its medium
+ * (lmh_batch_add_medium_grid2p1) and of its tables (lmh_batch_add_time_table,
+ * lmh_batch_add_juttner_table; -1 for none)                                                  */
+typedef struct lmh_run_desc {
+    int64_t count;
+    double  mass;
+    double  DsT;
+    int32_t medium;
+    int32_t drag_table;
+    int32_t rta_table;
+    int32_t juttner_table;
+} lmh_run_desc;
+
+/* launch shape of a batch, chosen by lmh_batch_autotune or by hand                          */
+typedef struct lmh_batch_tuning {
+    int32_t block;             /* threads per block (= particles per tile): 64, 128 or 256     */
+    int32_t steps_per_launch;  /* steps a particle stays in registers per launch               */
+    int32_t persistent;        /* 1: a fixed grid of blocks walks the tiles; 0: a block a tile */
+    int32_t graph_steps;       /* > 0: lmh_batch_run replays a captured HIP graph of this many
+                                  steps (the step index advances on the device); 0: direct    */
+} lmh_batch_tuning;
+
 typedef struct lmh_ctx lmh_ctx;
+typedef struct lmh_batch lmh_batch;
 
 /* ---- lifetime ---------------------------------------------------------------------------- */
 int         lmh_create(lmh_ctx** out, int32_t device);
@@ -252,6 +275,44 @@ int lmh_adjoint_forward(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, i
  * when the particles, medium or tables changed since lmh_adjoint_forward.  Synchronises.   */
 int lmh_adjoint_backward(lmh_ctx* ctx, const lmh_obs_spec* spec, const double* cotangent,
                          double* grad, double* dpos, double* dmom);
+
+/* ---- batches of independent runs (extension; the reference runs one ensemble per call with
+ * scalar m and DsT, simulate_cpu.jl:20-21) -------------------------------------------------- */
+/* Runs r = 0..nruns-1 on the 2+1D grid, each with its own medium, tables, mass, D_s T and
+ * particle count, advanced together by one launch per step (or per steps_per_launch steps).
+ * Run r draws Philox noise with key (seed, stream + r) and counter (local index, step, block):
+ * it follows the trajectories lmh_run gives for the same particles alone with stream + r.   */
+int         lmh_batch_create(lmh_batch** out, int32_t device);
+void        lmh_batch_destroy(lmh_batch* b);
+const char* lmh_batch_last_error(const lmh_batch* b);
+void*       lmh_batch_stream(lmh_batch* b);
+int lmh_batch_add_medium_grid2p1(lmh_batch* b, double x0, double dx, int32_t nx,
+                                 double y0, double dy, int32_t ny,
+                                 const double* tgrid, int32_t nt,
+                                 const double* T, const double* vx, const double* vy,
+                                 int32_t* index);
+int lmh_batch_add_time_table(lmh_batch* b, double Tmin, double invdT, const double* vals,
+                             int32_t n, int32_t* index);
+int lmh_batch_add_juttner_table(lmh_batch* b, const double* invcdf, int32_t nU, int32_t nT,
+                                double Tmin, double invdT, int32_t* index);
+/* pos (2, N) and mom (pdim, N) of all runs, run after run; N = sum of the counts            */
+int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns,
+                       const double* pos, const double* mom, int32_t pdim);
+int lmh_batch_get_particles(lmh_batch* b, double* pos, double* mom);
+int lmh_batch_set_tuning(lmh_batch* b, const lmh_batch_tuning* t);
+int lmh_batch_get_tuning(lmh_batch* b, lmh_batch_tuning* t);
+/* the initial boost of every run to the lab at `step` (simulate_cpu.jl:160-163)            */
+int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step);
+/* steps step_begin .. step_begin+nsteps-1 of every run; p->DsT is ignored (each run's own)  */
+int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps);
+/* times trial_steps steps of every launch shape (block 64/128/256 x persistent 0/1 x
+ * steps_per_launch 1/2/4) on this batch, keeps the fastest (graph_steps unchanged) and
+ * restores the state; the results do not depend on the shape.  Synchronises.              */
+int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps,
+                       lmh_batch_tuning* best, double* best_ms_per_step);
+/* out (nruns, 5 + nbins): lmh_observables of every run (spec->species is ignored)           */
+int lmh_batch_observables(lmh_batch* b, const lmh_obs_spec* spec, double* out);
+int64_t lmh_batch_nonfinite_step(lmh_batch* b);
 
 #ifdef __cplusplus
 }

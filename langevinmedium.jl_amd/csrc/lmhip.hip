@@ -901,6 +901,7 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_final(const double* __restric
 }
 
 #include "lmhip_adjoint.h"
+#include "lmhip_batch.h"
 
 }  // namespace
 
@@ -1904,6 +1905,604 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
         }
     }
     return LMH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// batches of independent runs (no reference counterpart: the reference runs one ensemble per
+// call with scalar m and DsT, simulate_cpu.jl:20-21)
+struct lmh_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    struct Med { double4* nodes = nullptr; double* tg = nullptr; int nx = 0, ny = 0, nt = 0; double x0 = 0, dx = 1, y0 = 0, dy = 1; };
+    struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
+    struct JTab { double* d = nullptr; double Tmin = 0, invdT = 1; int nU = 0, nT = 0; };
+    std::vector<Med> media;
+    std::vector<Tab> tabs;
+    std::vector<JTab> jtabs;
+
+    std::vector<lmh_run_desc> runs;
+    std::vector<int64_t> begin;       // nruns + 1
+    int pdim = 0;
+    int64_t N = 0;
+    double* d_state = nullptr;        // x, y, px, py[, pz] rows of N doubles
+    unsigned long long* d_bad = nullptr;
+    int64_t* d_step = nullptr;
+    int64_t* d_begin = nullptr;
+    BatchRun* d_runs = nullptr;
+    int32_t* d_tile_run = nullptr;
+    int64_t ntiles = 0;
+    int tiles_block = 0;              // block size the tile table was built for
+
+    lmh_batch_tuning tune = {256, 1, 0, 0};
+
+    // captured step loop: valid for `gkey` (parameters, tuning, steps per replay)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
+
+    double* d_obs = nullptr;
+    size_t obs_cap = 0;
+    double* d_save = nullptr;
+    size_t save_cap = 0;
+    int cus = 0;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace {
+
+#define LMB_HIP(b, expr)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return (b)->fail(LMH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void batch_drop_graph(lmh_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gkey.clear();
+}
+
+int batch_dev(lmh_batch* b, double** ptr, size_t* cap, size_t n, const char* what) {
+    if (n <= *cap) return LMH_OK;
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    dfree(*ptr);
+    *cap = 0;
+    if (hipMalloc((void**)ptr, n * sizeof(double)) != hipSuccess) {
+        *ptr = nullptr;
+        return b->fail(LMH_E_NOMEM, std::string(what) + ": device allocation failed");
+    }
+    *cap = n;
+    return LMH_OK;
+}
+
+// tile table and per-run records for the current block size
+int batch_layout(lmh_batch* b, int block) {
+    if (b->tiles_block == block && b->d_runs) return LMH_OK;
+    const int R = (int)b->runs.size();
+    std::vector<BatchRun> h(R);
+    std::vector<int32_t> tr;
+    int64_t t0 = 0;
+    for (int r = 0; r < R; ++r) {
+        const lmh_run_desc& d = b->runs[r];
+        const lmh_batch::Med& md = b->media[d.medium];
+        BatchRun& o = h[r];
+        memset(&o, 0, sizeof(o));
+        o.begin = b->begin[r];
+        o.count = d.count;
+        o.tile0 = t0;
+        o.g.nodes = md.nodes;
+        o.g.slice = nullptr;
+        o.g.tg = md.tg;
+        o.g.nx = md.nx;
+        o.g.ny = md.ny;
+        o.g.nt = md.nt;
+        o.g.x0 = md.x0;
+        o.g.inv_dx = 1.0 / md.dx;
+        o.g.xlast = md.x0 + (double)(md.nx - 1) * md.dx;
+        o.g.y0 = md.y0;
+        o.g.inv_dy = 1.0 / md.dy;
+        o.g.ylast = md.y0 + (double)(md.ny - 1) * md.dy;
+        o.sp.m = d.mass;
+        if (d.drag_table >= 0) {
+            const lmh_batch::Tab& t = b->tabs[d.drag_table];
+            o.sp.drag = TimeTable{t.d, t.Tmin, t.invdT, t.n};
+        }
+        if (d.rta_table >= 0) {
+            const lmh_batch::Tab& t = b->tabs[d.rta_table];
+            o.sp.rta = TimeTable{t.d, t.Tmin, t.invdT, t.n};
+        }
+        if (d.juttner_table >= 0) {
+            const lmh_batch::JTab& t = b->jtabs[d.juttner_table];
+            o.sp.jt = JuttnerTable{t.d, t.Tmin, t.invdT, t.nU, t.nT};
+        }
+        o.DsT = d.DsT;
+        const int64_t nt = (d.count + block - 1) / block;
+        for (int64_t k = 0; k < nt; ++k) tr.push_back(r);
+        t0 += nt;
+    }
+    batch_drop_graph(b);
+    dfree(b->d_runs);
+    dfree(b->d_tile_run);
+    if (hipMalloc((void**)&b->d_runs, std::max<size_t>(1, R) * sizeof(BatchRun)) != hipSuccess ||
+        hipMalloc((void**)&b->d_tile_run, std::max<size_t>(1, tr.size()) * sizeof(int32_t)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch: device allocation failed");
+    if (R) LMB_HIP(b, hipMemcpy(b->d_runs, h.data(), R * sizeof(BatchRun), hipMemcpyHostToDevice));
+    if (!tr.empty()) LMB_HIP(b, hipMemcpy(b->d_tile_run, tr.data(), tr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    b->ntiles = t0;
+    b->tiles_block = block;
+    return LMH_OK;
+}
+
+BatchArgs batch_args(lmh_batch* b, const lmh_params* p) {
+    BatchArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < 3; ++d) {
+        a.base.pos[d] = d < 2 ? b->d_state + (int64_t)d * b->N : nullptr;
+        a.base.mom[d] = d < b->pdim ? b->d_state + (int64_t)(2 + d) * b->N : nullptr;
+    }
+    a.base.N = b->N;
+    a.base.p = *p;
+    a.base.bad = b->d_bad;
+    a.base.nseg = 1;
+    a.base.sqrt_dt = std::sqrt(p->dt);
+    a.base.C = noise_layout(p, 2, b->pdim, &a.base.oxi, &a.base.orta, &a.base.opos);
+    a.runs = b->d_runs;
+    a.tile_run = b->d_tile_run;
+    a.ntiles = b->ntiles;
+    a.step_dev = b->d_step;
+    return a;
+}
+
+int batch_check(lmh_batch* b, const lmh_params* p) {
+    if (!p) return b->fail(LMH_E_INVALID, "null params");
+    if (b->runs.empty() || !b->d_state) return b->fail(LMH_E_STATE, "batch: no runs set");
+    if (p->coords != LMH_CARTESIAN || p->noise != LMH_NOISE_PHILOX)
+        return b->fail(LMH_E_INVALID, "batch: Cartesian coordinates and Philox noise");
+    if (p->collision < LMH_LANGEVIN || p->collision > LMH_LANGEVIN_RTA)
+        return b->fail(LMH_E_INVALID, "unknown collision mode");
+    if (p->integrator < LMH_PREPOINT || p->integrator > LMH_POSTPOINT)
+        return b->fail(LMH_E_INVALID, "unknown integrator");
+    if (p->drag_model != LMH_DRAG_TABLE && p->drag_model != LMH_DRAG_GAMMA_P)
+        return b->fail(LMH_E_INVALID, "unknown drag model");
+    if (p->reflecting_boundary) return b->fail(LMH_E_INVALID, "the 2+1D grid has no reflecting boundary");
+    if (p->bjorken_redshift && b->pdim != 3) return b->fail(LMH_E_INVALID, "bjorken_redshift requires momentum row 3");
+    const bool drag = p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA;
+    const bool rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
+    for (size_t r = 0; r < b->runs.size(); ++r) {
+        const lmh_run_desc& d = b->runs[r];
+        if (drag && p->drag_model == LMH_DRAG_TABLE && d.DsT > 0.0 && d.drag_table < 0)
+            return b->fail(LMH_E_STATE, "batch: run " + std::to_string(r) + " has no drag table");
+        if (rta && (d.rta_table < 0 || d.juttner_table < 0))
+            return b->fail(LMH_E_STATE, "batch: RTA needs a relaxation-time and a Juttner table for run " + std::to_string(r));
+    }
+    return LMH_OK;
+}
+
+template <int PDIM, bool PERSIST>
+void batch_launch_p(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, int32_t n, unsigned grid) {
+    const dim3 G(grid), B(b->tune.block);
+    switch (coll) {
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_RTA, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_FLUID, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
+        default: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN_RTA, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
+    }
+}
+
+// steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps
+void batch_enqueue(lmh_batch* b, const BatchArgs& a, int coll, int64_t nsteps) {
+    const int32_t k = std::max(1, b->tune.steps_per_launch);
+    unsigned grid = (unsigned)b->ntiles;
+    if (b->tune.persistent) {
+        // enough blocks to fill every CU at the kernel's occupancy, walking the tiles
+        const int per_cu = std::max(1, 2048 / std::max(64, b->tune.block));
+        grid = (unsigned)std::min<int64_t>(b->ntiles, (int64_t)b->cus * per_cu);
+    }
+    for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
+        const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        if (b->pdim == 2) {
+            if (b->tune.persistent) batch_launch_p<2, true>(b, a, coll, (int32_t)s0, n, grid);
+            else batch_launch_p<2, false>(b, a, coll, (int32_t)s0, n, grid);
+        } else {
+            if (b->tune.persistent) batch_launch_p<3, true>(b, a, coll, (int32_t)s0, n, grid);
+            else batch_launch_p<3, false>(b, a, coll, (int32_t)s0, n, grid);
+        }
+    }
+    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->d_step, nsteps);
+}
+
+int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps) {
+    int rc = batch_layout(b, b->tune.block);
+    if (rc) return rc;
+    if (b->ntiles == 0 || nsteps == 0) return LMH_OK;
+    const BatchArgs a = batch_args(b, p);
+    LMB_HIP(b, hipMemcpyAsync(b->d_step, &step_begin, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    const int64_t S = b->tune.graph_steps;
+    int64_t done = 0;
+    if (S > 0 && nsteps >= S) {
+        // the key: parameters, tuning and steps per replay; the graph holds the kernel arguments
+        std::vector<unsigned char> key(sizeof(lmh_params) + sizeof(lmh_batch_tuning));
+        memcpy(key.data(), p, sizeof(lmh_params));
+        memcpy(key.data() + sizeof(lmh_params), &b->tune, sizeof(lmh_batch_tuning));
+        if (!b->gexec || key != b->gkey) {
+            batch_drop_graph(b);
+            LMB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            batch_enqueue(b, a, p->collision, S);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess) return b->fail(LMH_E_HIP, std::string("batch: graph capture: ") + hipGetErrorString(e));
+            b->graph = g;
+            e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                batch_drop_graph(b);
+                return b->fail(LMH_E_HIP, std::string("batch: graph instantiation: ") + hipGetErrorString(e));
+            }
+            b->gkey = key;
+        }
+        for (; done + S <= nsteps; done += S) LMB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+    }
+    if (done < nsteps) batch_enqueue(b, a, p->collision, nsteps - done);
+    LMB_HIP(b, hipGetLastError());
+    if (p->check_finite) {
+        unsigned long long bad = ULLONG_MAX;
+        LMB_HIP(b, hipMemcpyAsync(&bad, b->d_bad, sizeof(bad), hipMemcpyDeviceToHost, b->stream));
+        LMB_HIP(b, hipStreamSynchronize(b->stream));
+        if (bad != ULLONG_MAX)
+            return b->fail(LMH_E_NONFINITE, "Breaking simulation due to NaN/Inf at step " + std::to_string((long long)bad));
+    }
+    return LMH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmh_batch_create(lmh_batch** out, int32_t device) {
+    if (!out) return LMH_E_INVALID;
+    *out = nullptr;
+    lmh_batch* b = new (std::nothrow) lmh_batch();
+    if (!b) return LMH_E_NOMEM;
+    b->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return LMH_E_HIP;
+    }
+    b->cus = prop.multiProcessorCount;
+    unsigned long long none = ULLONG_MAX;
+    if (hipMalloc((void**)&b->d_bad, sizeof(none)) != hipSuccess || hipMalloc((void**)&b->d_step, sizeof(int64_t)) != hipSuccess ||
+        hipMemcpy(b->d_bad, &none, sizeof(none), hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(b->d_bad);
+        dfree(b->d_step);
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return LMH_E_HIP;
+    }
+    *out = b;
+    return LMH_OK;
+}
+
+void lmh_batch_destroy(lmh_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (auto& m : b->media) {
+        dfree(m.nodes);
+        dfree(m.tg);
+    }
+    for (auto& t : b->tabs) dfree(t.d);
+    for (auto& t : b->jtabs) dfree(t.d);
+    dfree(b->d_state);
+    dfree(b->d_bad);
+    dfree(b->d_step);
+    dfree(b->d_begin);
+    dfree(b->d_runs);
+    dfree(b->d_tile_run);
+    dfree(b->d_obs);
+    dfree(b->d_save);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* lmh_batch_last_error(const lmh_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+void* lmh_batch_stream(lmh_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+int lmh_batch_add_medium_grid2p1(lmh_batch* b, double x0, double dx, int32_t nx, double y0, double dy,
+                                 int32_t ny, const double* tgrid, int32_t nt, const double* T,
+                                 const double* vx, const double* vy, int32_t* index) {
+    if (!b) return LMH_E_INVALID;
+    if (!index || !tgrid || !T || !vx || !vy || nx < 2 || ny < 2 || nt < 2 || !(dx > 0.0) || !(dy > 0.0) ||
+        !std::isfinite(x0) || !std::isfinite(y0))
+        return b->fail(LMH_E_INVALID, "batch_add_medium_grid2p1: need nx, ny, nt >= 2, dx, dy > 0 and all fields");
+    if (!sorted(tgrid, nt)) return b->fail(LMH_E_INVALID, "batch_add_medium_grid2p1: tgrid must be sorted");
+    LMB_HIP(b, hipSetDevice(b->device));
+    const size_t n = (size_t)nx * (size_t)ny * (size_t)nt;
+    std::vector<double4> nodes;
+    try {
+        nodes.resize(n);
+    } catch (...) {
+        return b->fail(LMH_E_NOMEM, "batch_add_medium_grid2p1: host allocation failed");
+    }
+    for (size_t k = 0; k < n; ++k) nodes[k] = make_double4(T[k], vx[k], vy[k], 0.0);
+    lmh_batch::Med m;
+    if (hipMalloc((void**)&m.nodes, n * sizeof(double4)) != hipSuccess ||
+        hipMalloc((void**)&m.tg, (size_t)nt * sizeof(double)) != hipSuccess) {
+        dfree(m.nodes);
+        dfree(m.tg);
+        return b->fail(LMH_E_NOMEM, "batch_add_medium_grid2p1: device allocation failed");
+    }
+    b->media.push_back(m);     // owned from here on, freed by lmh_batch_destroy
+    LMB_HIP(b, hipMemcpy(m.nodes, nodes.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    LMB_HIP(b, hipMemcpy(m.tg, tgrid, (size_t)nt * sizeof(double), hipMemcpyHostToDevice));
+    lmh_batch::Med& mm = b->media.back();
+    mm.nx = nx;
+    mm.ny = ny;
+    mm.nt = nt;
+    mm.x0 = x0;
+    mm.dx = dx;
+    mm.y0 = y0;
+    mm.dy = dy;
+    *index = (int32_t)b->media.size() - 1;
+    return LMH_OK;
+}
+
+int lmh_batch_add_time_table(lmh_batch* b, double Tmin, double invdT, const double* vals, int32_t n,
+                             int32_t* index) {
+    if (!b) return LMH_E_INVALID;
+    if (!index || !vals || n < 1) return b->fail(LMH_E_INVALID, "batch_add_time_table: need values");
+    LMB_HIP(b, hipSetDevice(b->device));
+    lmh_batch::Tab t;
+    if (hipMalloc((void**)&t.d, (size_t)n * sizeof(double)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch_add_time_table: device allocation failed");
+    b->tabs.push_back(t);
+    LMB_HIP(b, hipMemcpy(t.d, vals, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    b->tabs.back().Tmin = Tmin;
+    b->tabs.back().invdT = invdT;
+    b->tabs.back().n = n;
+    *index = (int32_t)b->tabs.size() - 1;
+    return LMH_OK;
+}
+
+int lmh_batch_add_juttner_table(lmh_batch* b, const double* invcdf, int32_t nU, int32_t nT, double Tmin,
+                                double invdT, int32_t* index) {
+    if (!b) return LMH_E_INVALID;
+    if (!index || !invcdf || nU < 2 || nT < 2) return b->fail(LMH_E_INVALID, "batch_add_juttner_table: bad size");
+    LMB_HIP(b, hipSetDevice(b->device));
+    lmh_batch::JTab t;
+    const size_t n = (size_t)nU * (size_t)nT;
+    if (hipMalloc((void**)&t.d, n * sizeof(double)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch_add_juttner_table: device allocation failed");
+    b->jtabs.push_back(t);
+    LMB_HIP(b, hipMemcpy(t.d, invcdf, n * sizeof(double), hipMemcpyHostToDevice));
+    lmh_batch::JTab& tt = b->jtabs.back();
+    tt.Tmin = Tmin;
+    tt.invdT = invdT;
+    tt.nU = nU;
+    tt.nT = nT;
+    *index = (int32_t)b->jtabs.size() - 1;
+    return LMH_OK;
+}
+
+int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, const double* pos,
+                       const double* mom, int32_t pdim) {
+    if (!b) return LMH_E_INVALID;
+    if (!runs || nruns < 1 || !pos || !mom || (pdim != 2 && pdim != 3))
+        return b->fail(LMH_E_INVALID, "batch_set_runs: need runs, positions and momenta (pdim 2 or 3)");
+    int64_t N = 0;
+    for (int32_t r = 0; r < nruns; ++r) {
+        const lmh_run_desc& d = runs[r];
+        if (d.count < 0 || !(d.mass > 0.0) || d.medium < 0 || d.medium >= (int32_t)b->media.size() ||
+            d.drag_table >= (int32_t)b->tabs.size() || d.rta_table >= (int32_t)b->tabs.size() ||
+            d.juttner_table >= (int32_t)b->jtabs.size())
+            return b->fail(LMH_E_INVALID, "batch_set_runs: run " + std::to_string(r) +
+                                              " has a negative count, a non-positive mass or an unknown medium or table");
+        N += d.count;
+    }
+    if (N < 1 || N > ((int64_t)1 << 31) * 64) return b->fail(LMH_E_INVALID, "batch_set_runs: bad total particle count");
+    LMB_HIP(b, hipSetDevice(b->device));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    const size_t rows = (size_t)(2 + pdim);
+    std::vector<double> soa;
+    try {
+        soa.resize(rows * (size_t)N);
+    } catch (...) {
+        return b->fail(LMH_E_NOMEM, "batch_set_runs: host allocation failed");
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < 2; ++d) soa[(size_t)d * N + i] = pos[d + 2 * i];
+        for (int d = 0; d < pdim; ++d) soa[(size_t)(2 + d) * N + i] = mom[d + (int64_t)pdim * i];
+    }
+    batch_drop_graph(b);
+    dfree(b->d_state);
+    dfree(b->d_begin);
+    dfree(b->d_runs);
+    dfree(b->d_tile_run);
+    b->tiles_block = 0;
+    b->runs.assign(runs, runs + nruns);
+    b->begin.assign(nruns + 1, 0);
+    for (int32_t r = 0; r < nruns; ++r) b->begin[r + 1] = b->begin[r] + runs[r].count;
+    b->N = N;
+    b->pdim = pdim;
+    if (hipMalloc((void**)&b->d_state, rows * (size_t)N * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&b->d_begin, (size_t)(nruns + 1) * sizeof(int64_t)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch_set_runs: device allocation failed");
+    LMB_HIP(b, hipMemcpy(b->d_state, soa.data(), rows * (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+    LMB_HIP(b, hipMemcpy(b->d_begin, b->begin.data(), (size_t)(nruns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    unsigned long long none = ULLONG_MAX;
+    LMB_HIP(b, hipMemcpy(b->d_bad, &none, sizeof(none), hipMemcpyHostToDevice));
+    return batch_layout(b, b->tune.block);
+}
+
+int lmh_batch_get_particles(lmh_batch* b, double* pos, double* mom) {
+    if (!b) return LMH_E_INVALID;
+    if (!b->d_state) return b->fail(LMH_E_STATE, "batch_get_particles: no runs set");
+    if (!pos || !mom) return b->fail(LMH_E_INVALID, "batch_get_particles: null output");
+    LMB_HIP(b, hipSetDevice(b->device));
+    const size_t rows = (size_t)(2 + b->pdim);
+    const int64_t N = b->N;
+    std::vector<double> soa;
+    try {
+        soa.resize(rows * (size_t)N);
+    } catch (...) {
+        return b->fail(LMH_E_NOMEM, "batch_get_particles: host allocation failed");
+    }
+    LMB_HIP(b, hipMemcpyAsync(soa.data(), b->d_state, rows * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < 2; ++d) pos[d + 2 * i] = soa[(size_t)d * N + i];
+        for (int d = 0; d < b->pdim; ++d) mom[d + (int64_t)b->pdim * i] = soa[(size_t)(2 + d) * N + i];
+    }
+    return LMH_OK;
+}
+
+int lmh_batch_set_tuning(lmh_batch* b, const lmh_batch_tuning* t) {
+    if (!b) return LMH_E_INVALID;
+    if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
+        (t->persistent != 0 && t->persistent != 1) || t->graph_steps < 0)
+        return b->fail(LMH_E_INVALID, "batch_set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, "
+                                      "persistent in {0, 1}, graph_steps >= 0");
+    b->tune = *t;
+    return LMH_OK;
+}
+
+int lmh_batch_get_tuning(lmh_batch* b, lmh_batch_tuning* t) {
+    if (!b || !t) return LMH_E_INVALID;
+    *t = b->tune;
+    return LMH_OK;
+}
+
+int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step) {
+    if (!b) return LMH_E_INVALID;
+    int rc = batch_check(b, p);
+    if (rc) return rc;
+    LMB_HIP(b, hipSetDevice(b->device));
+    if ((rc = batch_layout(b, b->tune.block))) return rc;
+    if (b->ntiles == 0) return LMH_OK;
+    const BatchArgs a = batch_args(b, p);
+    const dim3 G((unsigned)b->ntiles), B(b->tune.block);
+    if (b->pdim == 2) hipLaunchKernelGGL((k_batch_boost_lab<2>), G, B, 0, b->stream, a, step);
+    else hipLaunchKernelGGL((k_batch_boost_lab<3>), G, B, 0, b->stream, a, step);
+    LMB_HIP(b, hipGetLastError());
+    return LMH_OK;
+}
+
+int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps) {
+    if (!b) return LMH_E_INVALID;
+    int rc = batch_check(b, p);
+    if (rc) return rc;
+    if (nsteps < 0) return b->fail(LMH_E_INVALID, "batch_run: nsteps >= 0");
+    LMB_HIP(b, hipSetDevice(b->device));
+    return batch_run_impl(b, p, step_begin, nsteps);
+}
+
+int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, lmh_batch_tuning* best,
+                       double* best_ms_per_step) {
+    if (!b) return LMH_E_INVALID;
+    int rc = batch_check(b, p);
+    if (rc) return rc;
+    if (trial_steps < 1) return b->fail(LMH_E_INVALID, "batch_autotune: trial_steps >= 1");
+    LMB_HIP(b, hipSetDevice(b->device));
+    const size_t n = (size_t)(2 + b->pdim) * (size_t)b->N;
+    if ((rc = batch_dev(b, &b->d_save, &b->save_cap, n, "batch_autotune"))) return rc;
+    LMB_HIP(b, hipMemcpyAsync(b->d_save, b->d_state, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    lmh_params q = *p;
+    q.check_finite = 0;
+    const lmh_batch_tuning keep = b->tune;
+    lmh_batch_tuning pick = keep;
+    double pick_ms = 0.0;
+    hipEvent_t e0, e1;
+    LMB_HIP(b, hipEventCreate(&e0));
+    LMB_HIP(b, hipEventCreate(&e1));
+    const int blocks[3] = {64, 128, 256};
+    const int spls[3] = {1, 2, 4};
+    for (int bi = 0; bi < 3 && rc == LMH_OK; ++bi)
+        for (int pe = 0; pe < 2 && rc == LMH_OK; ++pe)
+            for (int si = 0; si < 3 && rc == LMH_OK; ++si) {
+                b->tune = lmh_batch_tuning{blocks[bi], spls[si], pe, 0};
+                // one untimed launch for the code object and caches, then the timed steps
+                if ((rc = batch_run_impl(b, &q, 1, spls[si]))) break;
+                if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = b->fail(LMH_E_HIP, "batch_autotune: event"); break; }
+                if ((rc = batch_run_impl(b, &q, 1 + spls[si], trial_steps))) break;
+                if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = b->fail(LMH_E_HIP, "batch_autotune: event");
+                    break;
+                }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double per = ms / trial_steps;
+                if (pick_ms == 0.0 || per < pick_ms) {
+                    pick_ms = per;
+                    pick = b->tune;
+                }
+                if (hipMemcpyAsync(b->d_state, b->d_save, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+                    rc = b->fail(LMH_E_HIP, "batch_autotune: restore");
+                    break;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the state as before the trials, the first non-finite step cleared
+    LMB_HIP(b, hipMemcpyAsync(b->d_state, b->d_save, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    unsigned long long none = ULLONG_MAX;
+    LMB_HIP(b, hipMemcpyAsync(b->d_bad, &none, sizeof(none), hipMemcpyHostToDevice, b->stream));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    if (rc) {
+        b->tune = keep;
+        return rc;
+    }
+    pick.graph_steps = keep.graph_steps;
+    b->tune = pick;
+    if (best) *best = pick;
+    if (best_ms_per_step) *best_ms_per_step = pick_ms;
+    return LMH_OK;
+}
+
+int lmh_batch_observables(lmh_batch* b, const lmh_obs_spec* s, double* out) {
+    if (!b) return LMH_E_INVALID;
+    if (!s || !out) return b->fail(LMH_E_INVALID, "batch_observables: null spec or output");
+    if (!b->d_state) return b->fail(LMH_E_STATE, "batch_observables: no runs set");
+    if (s->nbins < 0 || s->nbins > kObsMaxBins || (s->nbins > 0 && !(s->pt_max > s->pt_min)))
+        return b->fail(LMH_E_INVALID, "batch_observables: 0 <= nbins <= 120 and pt_max > pt_min");
+    LMB_HIP(b, hipSetDevice(b->device));
+    const int K = 5 + s->nbins;
+    const int R = (int)b->runs.size();
+    const int nblk = 16;
+    int rc = batch_dev(b, &b->d_obs, &b->obs_cap, (size_t)K * R * (nblk + 1), "batch_observables");
+    if (rc) return rc;
+    const double* px = b->d_state + 2 * b->N;
+    const size_t shmem = (size_t)K * kObsBlock * sizeof(double);
+    hipLaunchKernelGGL(k_batch_obs_partial, dim3(nblk, R), dim3(kObsBlock), shmem, b->stream, px, px + b->N,
+                       b->d_begin, *s, b->d_obs);
+    double* fin = b->d_obs + (size_t)K * R * nblk;
+    hipLaunchKernelGGL(k_batch_obs_final, dim3(R), dim3(kObsBlock), 0, b->stream, b->d_obs, nblk, K, fin);
+    LMB_HIP(b, hipGetLastError());
+    LMB_HIP(b, hipMemcpyAsync(out, fin, (size_t)K * R * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    return LMH_OK;
+}
+
+int64_t lmh_batch_nonfinite_step(lmh_batch* b) {
+    if (!b || !b->d_bad) return -1;
+    unsigned long long bad = ULLONG_MAX;
+    if (hipSetDevice(b->device) != hipSuccess) return -1;
+    if (hipMemcpyAsync(&bad, b->d_bad, sizeof(bad), hipMemcpyDeviceToHost, b->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return -1;
+    return bad == ULLONG_MAX ? -1 : (int64_t)bad;
 }
 
 }  // extern "C"

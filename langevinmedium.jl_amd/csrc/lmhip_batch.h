@@ -1,0 +1,194 @@
+// lmhip_batch.h -- many independent 2+1D runs in one launch (included by lmhip.hip inside its
+// anonymous namespace, after the forward kernels it reuses).
+//
+// A batch holds runs r = 0..R-1, each with its own particle segment (contiguous in one SoA
+// state), medium, tables, mass and D_s T.  The launch is cut into tiles of blockDim particles,
+// every tile inside one run; tile_run[t] names the run.  Run r's noise is Philox with key
+// (seed, stream + r) and counter (local particle index, step, block), so a run inside the batch
+// follows the trajectories the same run gives alone with stream + r.  The first step of a launch
+// is read from device memory (*step_dev + k_off), so a captured graph of launches advances by
+// itself when a final one-thread kernel moves *step_dev on.
+
+struct BatchRun {
+    int64_t begin, count, tile0;
+    GridMedium g;
+    SpeciesArgs sp;
+    double DsT;
+};
+
+struct BatchArgs {
+    StepArgs base;              // state rows over all runs, the shared parameters
+    const BatchRun* runs;
+    const int32_t* tile_run;
+    int64_t ntiles;
+    const int64_t* step_dev;
+};
+
+// this tile's run as a single-run view: state rows and Philox stream of the run
+__device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& run, int r, StepArgs* a) {
+    *a = b.base;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (a->pos[d]) a->pos[d] += run.begin;
+        if (a->mom[d]) a->mom[d] += run.begin;
+    }
+    a->N = run.count;
+    a->p.DsT = run.DsT;
+    a->p.stream = b.base.p.stream + (uint64_t)r;
+}
+
+// the fused step of k_cart_grid without deferral (Philox noise, the medium interpolated in tau
+// per corner), the same device functions in the same order
+template <int PDIM, int COLL>
+__device__ __forceinline__ void fused_grid_step(const StepArgs& a, const GridMedium& g,
+                                                const SpeciesArgs& sp, int64_t i, int64_t step,
+                                                double* x0, double* x1, double* q) {
+    constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
+    constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
+    const lmh_params& p = a.p;
+    const double m = sp.m;
+    const double t_now = (double)step * p.dt + p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    double Tm, vx, vy;
+    grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    const FlowBoost fb = flow_boost(vx, vy);
+    apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
+    if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
+    if (COLL == LMH_FLUID) {
+        glue_to_flow<PDIM>(q, m, vx, vy, p.relativistic);
+    } else {
+        const double T = jl_max(Tm, 0.0);
+        double etaD = 0.0, kT = 0.0;
+        if (HAS_KICK) drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+        double xi[PDIM];
+        double ucol = 1.0;
+        noise_kick<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, xi, &ucol);
+        if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+        if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+            double usamp, gg[PDIM];
+            noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
+            rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+        }
+        apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
+    }
+    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, g, i, step, 0, j, td, m, x0, x1, q);
+}
+
+// PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
+// otherwise one block per tile.
+template <int PDIM, int COLL, bool PERSIST>
+__global__ __launch_bounds__(kBlock) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
+    const int64_t step0 = *b.step_dev + k_off;
+    const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
+    for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += stride) {
+        const int r = b.tile_run[tile];
+        const BatchRun& run = b.runs[r];
+        const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
+        if (i >= run.count) continue;
+        StepArgs a;
+        batch_view(b, run, r, &a);
+        const SpeciesArgs sp = run.sp;
+        double x0 = a.pos[0][i], x1 = a.pos[1][i];
+        double q[PDIM];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+        int64_t bad = -1;
+        for (int s = 0; s < nsteps; ++s) {
+            fused_grid_step<PDIM, COLL>(a, run.g, sp, i, step0 + s, &x0, &x1, q);
+            if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
+        }
+        a.pos[0][i] = x0;
+        a.pos[1][i] = x1;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+        if (bad >= 0) atomicMin(b.base.bad, (unsigned long long)bad);
+    }
+}
+
+__global__ void k_batch_advance(int64_t* step_dev, int64_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_dev += n;
+}
+
+template <int PDIM>
+__global__ __launch_bounds__(kBlock) void k_batch_boost_lab(BatchArgs b, int64_t step) {
+    for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
+        const int r = b.tile_run[tile];
+        const BatchRun& run = b.runs[r];
+        const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
+        if (i >= run.count) continue;
+        StepArgs a;
+        batch_view(b, run, r, &a);
+        double q[PDIM];
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+        const double t_now = (double)step * a.p.dt + a.p.t0;
+        double td;
+        const int j = axis_cell(run.g.tg, run.g.nt, t_now, &td);
+        double Tm, vx, vy;
+        grid_sample<false>(run.g, a.pos[0][i], a.pos[1][i], j, td, &Tm, &vx, &vy);
+        apply_boost<PDIM>(q, run.sp.m, flow_boost(vx, vy), +1, a.p.relativistic);
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    }
+}
+
+// per-run observables: blockIdx.y is the run, k_obs_partial's sums over its segment
+__global__ __launch_bounds__(kObsBlock) void k_batch_obs_partial(const double* __restrict__ px,
+                                                                  const double* __restrict__ py,
+                                                                  const int64_t* __restrict__ begin,
+                                                                  lmh_obs_spec s,
+                                                                  double* __restrict__ partial) {
+    extern __shared__ double acc[];           // [5 + nbins][kObsBlock]
+    const int t = threadIdx.x;
+    const int K = 5 + s.nbins;
+    const int r = blockIdx.y;
+    const int64_t i0 = begin[r], i1 = begin[r + 1];
+    for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] = 0.0;
+    const double width = (s.pt_max - s.pt_min) / (double)(s.nbins > 0 ? s.nbins : 1);
+    const double inv_w = (double)s.nbins / (s.pt_max - s.pt_min);
+    const double inv_s = s.sigma > 0.0 ? 1.0 / (s.sigma * sqrt(2.0)) : 0.0;
+    for (int64_t i = i0 + (int64_t)blockIdx.x * kObsBlock + t; i < i1; i += (int64_t)gridDim.x * kObsBlock) {
+        double a = px[i], b = py[i];
+        double pt2 = a * a + b * b;
+        double pt = sqrt(pt2);
+        double c2 = pt2 > 0.0 ? (a * a - b * b) / pt2 : 0.0;
+        acc[0 * kObsBlock + t] += 1.0;
+        acc[1 * kObsBlock + t] += pt;
+        acc[2 * kObsBlock + t] += pt2;
+        acc[3 * kObsBlock + t] += c2;
+        acc[4 * kObsBlock + t] += pt * c2;
+        if (s.nbins == 0) continue;
+        if (s.sigma > 0.0) {
+            for (int k = 0; k < s.nbins; ++k) {
+                double lo = s.pt_min + (double)k * width;
+                double hi = lo + width;
+                acc[(5 + k) * kObsBlock + t] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
+            }
+        } else {
+            double f = (pt - s.pt_min) * inv_w;
+            double fl = floor(f);
+            if (fl >= 0.0 && fl < (double)s.nbins) acc[(5 + (int)fl) * kObsBlock + t] += 1.0;
+        }
+    }
+    __syncthreads();
+    for (int stride = kObsBlock / 2; stride > 0; stride >>= 1) {
+        if (t < stride)
+            for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] += acc[k * kObsBlock + t + stride];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < K; ++k)
+            partial[((int64_t)r * gridDim.x + blockIdx.x) * K + k] = acc[k * kObsBlock];
+}
+
+// one block per run: out[r*K + k] = sum over the run's partials, in block order
+__global__ __launch_bounds__(kObsBlock) void k_batch_obs_final(const double* __restrict__ partial, int nblk,
+                                                                int K, double* __restrict__ out) {
+    const int r = blockIdx.x;
+    for (int k = threadIdx.x; k < K; k += kObsBlock) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partial[((int64_t)r * nblk + b) * K + k];
+        out[(int64_t)r * K + k] = s;
+    }
+}

@@ -66,6 +66,16 @@ class lmh_adjoint_spec(ctypes.Structure):
     _fields_ = [("checkpoint_every", c_int32), ("_pad", c_int32), ("dtau_dDsT", c_double)]
 
 
+class lmh_run_desc(ctypes.Structure):
+    _fields_ = [("count", c_int64), ("mass", c_double), ("DsT", c_double), ("medium", c_int32),
+                ("drag_table", c_int32), ("rta_table", c_int32), ("juttner_table", c_int32)]
+
+
+class lmh_batch_tuning(ctypes.Structure):
+    _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("persistent", c_int32),
+                ("graph_steps", c_int32)]
+
+
 def make_params(*, dt, t0=0.0, DsT=0.0, psi2=0.0, gamma0=0.0, Lambda=1.0, seed=0, stream=0,
                 coords=LMH_CARTESIAN, collision=LMH_LANGEVIN, noise=LMH_NOISE_PHILOX,
                 integrator=LMH_PREPOINT, drag_model=LMH_DRAG_TABLE, relativistic=True,
@@ -109,6 +119,25 @@ SIGNATURES = {
     "lmh_adjoint_forward": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64,
                                     POINTER(lmh_adjoint_spec)]),
     "lmh_adjoint_backward": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD, _PD, _PD, _PD]),
+    "lmh_batch_create": (c_int, [POINTER(c_void_p), c_int32]),
+    "lmh_batch_destroy": (None, [c_void_p]),
+    "lmh_batch_last_error": (c_char_p, [c_void_p]),
+    "lmh_batch_stream": (c_void_p, [c_void_p]),
+    "lmh_batch_add_medium_grid2p1": (c_int, [c_void_p, c_double, c_double, c_int32, c_double, c_double,
+                                             c_int32, _PD, c_int32, _PD, _PD, _PD, POINTER(c_int32)]),
+    "lmh_batch_add_time_table": (c_int, [c_void_p, c_double, c_double, _PD, c_int32, POINTER(c_int32)]),
+    "lmh_batch_add_juttner_table": (c_int, [c_void_p, _PD, c_int32, c_int32, c_double, c_double,
+                                            POINTER(c_int32)]),
+    "lmh_batch_set_runs": (c_int, [c_void_p, POINTER(lmh_run_desc), c_int32, _PD, _PD, c_int32]),
+    "lmh_batch_get_particles": (c_int, [c_void_p, _PD, _PD]),
+    "lmh_batch_set_tuning": (c_int, [c_void_p, POINTER(lmh_batch_tuning)]),
+    "lmh_batch_get_tuning": (c_int, [c_void_p, POINTER(lmh_batch_tuning)]),
+    "lmh_batch_boost_to_lab": (c_int, [c_void_p, POINTER(lmh_params), c_int64]),
+    "lmh_batch_run": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64]),
+    "lmh_batch_autotune": (c_int, [c_void_p, POINTER(lmh_params), c_int32, POINTER(lmh_batch_tuning),
+                                   POINTER(c_double)]),
+    "lmh_batch_observables": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD]),
+    "lmh_batch_nonfinite_step": (c_int64, [c_void_p]),
 }
 
 _LIB = None

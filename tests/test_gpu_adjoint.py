@@ -276,7 +276,7 @@ def test_torch_autograd_function(gpu_ok):
         loss = obs[1] / obs[0] + 0.5 * obs[3] / obs[0]
         loss.backward()
         w = np.zeros(5 + OBS["nbins"])
-        n = float(obs[0])
+        n = float(obs[0].detach())
         w[1], w[3] = 1.0 / n, 0.5 / n
         ref = ens.pullback(w)
         assert np.allclose(theta.grad.numpy(), ref, rtol=1e-14, atol=0)
