@@ -111,9 +111,204 @@ def cpu_baseline(g, masses, tabs, DsT, budget_s=12.0, n=20000):
                       f"(C restatement of kernels_cpu.jl, one pass per kernel), 1 thread, {dt:.1f} s"}
 
 
+def bench_batch(args):
+    """256 independent runs in one launch: each run draws D_s(2 pi T) ~ U[1.5, 7], T0 ~ U[0.3, 0.5]
+    GeV, a charm count 10^U[3, 6] and a profile eccentricity ~ U[0, 0.4] (seed 4), on its own
+    200 x 200 x 120 grid; Langevin + RTA as in the headline workload.  The drag tables are per run,
+    the RTA and Juttner tables are shared over the common temperature range.  The launch shape is
+    chosen by the library's autotuner, then `--steps` steps run as replays of a captured graph."""
+    import numpy as np
+    import torch
+    from lmhip import _ffi, media, transport
+    from lmhip.batch import Batch
+
+    rng = np.random.default_rng(4)
+    R = args.runs
+    Ds = rng.uniform(1.5, 7.0, R)
+    T0s = rng.uniform(0.3, 0.5, R)
+    counts = np.round(10.0 ** rng.uniform(3.0, 6.0, R)).astype(np.int64)
+    eccs = rng.uniform(0.0, 0.4, R)
+    m = 1.5
+    t_setup = time.perf_counter()
+    b = Batch(0)
+    Tmin, Tmax = None, None
+    meds = []
+    for r in range(R):
+        g = media.hydro_grid(nx=200, ny=200, nt=120, L=15.0, tau0=0.6, tau1=12.0, T0=float(T0s[r]),
+                             ecc=float(eccs[r]))
+        lo, hi = max(float(np.min(g.T)), 0.0), float(np.max(g.T))
+        Tmin = lo if Tmin is None else min(Tmin, lo)
+        Tmax = hi if Tmax is None else max(Tmax, hi)
+        meds.append(b.add_medium(g))
+        del g
+    rt = b.add_time_table(*transport.build_relaxation_spline(media.tau_R_law(0.16), Tmin=Tmin, Tmax=Tmax, nT=1024))
+    jt = b.add_juttner_table(*transport.build_juttner_invcdf(m, 3, Tmin=Tmin, Tmax=Tmax, nT=256, nU=2048, np_=8192))
+    desc, pos, mom = [], [], []
+    for r in range(R):
+        DsT = media.DsT_from_2piT(float(Ds[r]))
+        drag = b.add_time_table(*transport.build_tau_drag_spline(m, DsT, Tmin=Tmin, Tmax=Tmax, nT=1024))
+        desc.append(dict(count=int(counts[r]), mass=m, DsT=DsT, medium=meds[r], drag_table=drag,
+                         rta_table=rt, juttner_table=jt))
+        pos.append(media.hydro_positions(int(counts[r]), rng, ecc=float(eccs[r])))
+        mom.append(media.pt_rapidity_momenta(int(counts[r]), m, rng))
+    b.set_runs(desc, np.hstack(pos), np.hstack(mom))
+    del pos, mom
+    N = int(counts.sum())
+    params = make_params(0.0, False, "langevin_rta")
+    b.boost_to_lab(params, 0)
+    tune, tuned_ms = b.autotune(params, trial_steps=3)
+    b.set_tuning(block=tune["block"], steps_per_launch=tune["steps_per_launch"],
+                 persistent=tune["persistent"], graph_steps=10)
+    setup = time.perf_counter() - t_setup
+    print(f"batch setup {setup:.1f} s; {R} runs, {N} quarks, tuning {tune} ({tuned_ms:.3f} ms/step)",
+          file=sys.stderr, flush=True)
+    steps = max(10, (args.steps // 10) * 10)
+    b.run(params, 1, args.warmup)
+    b.nonfinite_step()
+    stream = torch.cuda.ExternalStream(b.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    b.run(params, 1 + args.warmup, steps)
+    ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    bad = b.nonfinite_step()
+    if bad >= 0:
+        raise RuntimeError(f"non-finite state at step {bad}")
+    obs = b.observables(nbins=40, pt_min=0.0, pt_max=10.0)
+    b.close()
+    out = {
+        "metric": "particle-steps/s, 256 batched runs (one launch per step, graph-captured)",
+        "value": round(N * steps / (ms / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{R} runs: D_s(2piT)~U[1.5,7], T0~U[0.3,0.5] GeV, N~10^U[3,6] charm, "
+                               "ecc~U[0,0.4], each on its own 200x200x120 grid; Langevin + RTA; "
+                               "dt=0.01; Philox; FP64",
+                   "runs": R, "quarks": N, "tuning": tune, "autotune_ms_per_step": round(tuned_ms, 4),
+                   "graph_steps": 10, "setup_s": round(setup, 1),
+                   "mean_pT_run0": round(float(obs[0, 1] / max(obs[0, 0], 1.0)), 6)},
+        "roofline": {"bound": "hbm", "achieved": round(BYTES_PER_PARTICLE_STEP * N / (ms / steps / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(BYTES_PER_PARTICLE_STEP * N / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_batch_grid (whole step, all runs)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """The pathwise adjoint on the headline medium: 10^6 charm quarks x 1000 steps, checkpoints
+    every 50 steps, Langevin with D_s(2 pi T) = 4 (drag table; d/dDsT) and with the GAMMA_P drag
+    gamma0 = 5.3, Lambda = 5 GeV (d/dgamma0, d/dLambda).  Losses: <p_T>, a Gaussian-smoothed
+    (sigma = 0.25 GeV) 40-bin p_T spectrum on [0, 10] GeV weighted by fixed random weights, and
+    <cos 2 phi>.  Times: the plain forward run (lmh_run), the checkpointed forward, one pullback per
+    loss; gradients checked against central differences of the plain run at relative step 1e-4."""
+    import numpy as np
+    import torch
+    import lmhip
+    from lmhip import _ffi, media
+    from lmhip.adjoint import GridEnsemble
+
+    N, steps, K = args.adj_n, args.adj_steps, args.adj_k
+    rng = np.random.default_rng(5)
+    g = media.hydro_grid(nx=200, ny=200, nt=120, L=15.0, tau0=0.6, tau1=12.0, T0=0.45)
+    pos = media.hydro_positions(N, rng)
+    mom = media.pt_rapidity_momenta(N, 1.5, rng)
+    obs_kw = dict(nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.25)
+    w = {}
+    w["mean_pT"] = np.zeros(45); w["mean_pT"][1] = 1.0 / N
+    w["cos2phi"] = np.zeros(45); w["cos2phi"][3] = 1.0 / N
+    w["spectrum"] = np.zeros(45); w["spectrum"][5:] = np.random.default_rng(6).normal(size=40) / N
+    ctx = lmhip.Context(0)
+    ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=1)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def timed(fn):
+        ctx.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        r = fn()
+        e1.record(stream)
+        e1.synchronize()
+        ctx.synchronize()
+        return r, e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3
+
+    results = {}
+    for name, drag_model, theta, ks in (
+            ("table", _ffi.LMH_DRAG_TABLE, np.array([media.DsT_from_2piT(4.0), 0.0, 5.0]), (0,)),
+            ("gamma_p", _ffi.LMH_DRAG_GAMMA_P, np.array([media.DsT_from_2piT(4.0), 5.3, 5.0]), (1, 2))):
+        ens = GridEnsemble(ctx, g, [1.5], [N], pos, mom, dt=0.01, t0=0.6, nsteps=steps,
+                           drag_model=drag_model, collision=_ffi.LMH_LANGEVIN, seed=3,
+                           checkpoint_every=K, **obs_kw)
+        ens.reset(theta)
+        ctx.run(ens.params(theta), 1, 20, None, 0)                  # warm-up (code objects)
+        p = ens.reset(theta)
+        _, t_fwd, _ = timed(lambda: ctx.run(p, 1, steps, None, 0))
+        obs_plain = ctx.observables(**obs_kw)
+        p = ens.reset(theta)
+        _, t_cfwd, _ = timed(lambda: ctx.adjoint_forward(p, 1, steps, K))
+        obs_ck = ctx.observables(**obs_kw)
+        ens._theta = tuple(theta)
+        grads, t_bwd = {}, {}
+        for lname, wl in w.items():
+            gr, t_b, _ = timed(lambda: ens.pullback(wl))
+            grads[lname], t_bwd[lname] = gr, t_b
+        # central differences of the plain run at relative steps 1e-4 (the stated check), 1e-5
+        # and 1e-6: trajectories that cross a grid cell or a clamp between the two runs make the
+        # difference quotient differ from the pathwise derivative by O(h) per crossing
+        fd = {}
+        for k in ks:
+            for hr in (1e-4, 1e-5, 1e-6):
+                h = hr * theta[k]
+                tp, tm = theta.copy(), theta.copy()
+                tp[k] += h
+                tm[k] -= h
+                op, om = ens.run_plain(tp), ens.run_plain(tm)
+                for lname, wl in w.items():
+                    fd[(lname, k, hr)] = float((op - om) @ wl / (2 * h))
+        checks = {}
+        for lname in w:
+            for k in ks:
+                a = float(grads[lname][k])
+                checks[f"{lname}/d{['DsT', 'gamma0', 'Lambda'][k]}"] = {
+                    "adjoint": a,
+                    **{f"fd_h{hr:g}": fd[(lname, k, hr)] for hr in (1e-4, 1e-5, 1e-6)},
+                    **{f"rel_err_h{hr:g}": abs(a - fd[(lname, k, hr)]) / max(abs(fd[(lname, k, hr)]), 1e-300)
+                       for hr in (1e-4, 1e-5, 1e-6)}}
+        results[name] = {"forward_ms": round(t_fwd, 2), "checkpointed_forward_ms": round(t_cfwd, 2),
+                         "backward_ms": {k: round(v, 2) for k, v in t_bwd.items()},
+                         "ratio_one_loss": round((t_cfwd + t_bwd["mean_pT"]) / t_fwd, 3),
+                         "ratio_three_losses": round((t_cfwd + sum(t_bwd.values())) / t_fwd, 3),
+                         "same_final_state": bool(np.array_equal(obs_plain, obs_ck)),
+                         "checks": checks}
+        print(f"{name}: {json.dumps(results[name])}", file=sys.stderr, flush=True)
+    ctx.close()
+    ratio = max(r["ratio_one_loss"] for r in results.values())
+    out = {"metric": "adjoint/forward time ratio (gradient of one loss), 1e6 quarks x 1000 steps",
+           "value": ratio, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": 20,
+           "ms_per_step": round(results["gamma_p"]["forward_ms"] / steps, 5), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"pathwise adjoint, 2+1D grid 200x200x120, {N} charm x {steps} steps, "
+                                  f"checkpoint every {K}; Langevin; Philox; FP64", "results": results},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "adjoint"],
+                    help="grid: the headline 10^7-quark step (default); batch: 256 independent runs "
+                         "in one launch; adjoint: gradient / forward time of the pathwise adjoint")
+    ap.add_argument("--runs", type=int, default=256, help="batch: number of runs")
+    ap.add_argument("--adj-n", type=int, default=1_000_000, help="adjoint: quarks")
+    ap.add_argument("--adj-steps", type=int, default=1000, help="adjoint: steps")
+    ap.add_argument("--adj-k", type=int, default=50, help="adjoint: checkpoint cadence")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=10_000_000)
@@ -137,6 +332,11 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(particles shard with no per-step exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import numpy as np
     import torch
