@@ -55,8 +55,11 @@ extern "C" {
 #define LMH_LANGEVIN_RTA       3
 
 /* noise source: FIXED reads a host- or device-supplied record per step; PHILOX generates the
- * same record in registers from Philox4x64-10 keyed by (seed, stream), counter
- * (particle, step, block, 0).                                                               */
+ * same record in registers.  Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at
+ * step s is four 64-bit words from two Philox4x32-10 evaluations (half h gives words 2h, 2h+1,
+ * each lo | hi << 32 of consecutive outputs) of counter
+ * (i, s, stream mod 2^32, (stream >> 32) mod 2^29 | (2b + h) << 29) under key
+ * (seed mod 2^32, seed >> 32); i and s must stay below 2^32 and stream below 2^61.           */
 #define LMH_NOISE_FIXED        0
 #define LMH_NOISE_PHILOX       1
 

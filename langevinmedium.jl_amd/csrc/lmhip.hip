@@ -8,7 +8,7 @@
 //     medium sample -> boost to the LRF -> [Bjorken redshift] -> Langevin kick | BGK redraw |
 //     glue to the flow -> boost back -> position push -> non-finite check.
 // Noise is either a fixed record read from HBM (for per-particle parity with the CPU kernels)
-// or Philox4x64-10 generated in registers.
+// or Philox4x32-10 generated in registers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,7 +108,7 @@ __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t
     } else {
         if (a.oxi < 0 && a.orta < 0) return;
         uint64_t w[4];
-        philox4x64((uint64_t)i, (uint64_t)step, 0, 0, a.p.seed, a.p.stream, w);
+        noise_block(a.p.seed, a.p.stream, i, step, 0, w);
         if (a.oxi >= 0) {
             double n0, n1;
             box_muller(w[0], w[1], &n0, &n1);
@@ -134,7 +134,7 @@ __device__ __forceinline__ void noise_redraw(const StepArgs& a, int64_t i, int64
         for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
     } else {
         uint64_t w[4];
-        philox4x64((uint64_t)i, (uint64_t)step, 1, 0, a.p.seed, a.p.stream, w);
+        noise_block(a.p.seed, a.p.stream, i, step, 1, w);
         double n0, n1;
         box_muller(w[0], w[1], &n0, &n1);
         g[0] = n0;
@@ -157,7 +157,7 @@ __device__ __forceinline__ void noise_diffusion(const StepArgs& a, int64_t i, in
         for (int d = 0; d < DIMS; ++d) xp[d] = rec[(int64_t)(a.opos + d) * a.N];
     } else {
         uint64_t w[4];
-        philox4x64((uint64_t)i, (uint64_t)step, 2, 0, a.p.seed, a.p.stream, w);
+        noise_block(a.p.seed, a.p.stream, i, step, 2, w);
         double n0, n1;
         box_muller(w[0], w[1], &n0, &n1);
         xp[0] = n0;
@@ -494,8 +494,9 @@ __device__ __forceinline__ FlowBoost flow_boost(double vx, double vy) {
     double vm2 = vx * vx + vy * vy;
     b.vmag = sqrt(vm2);
     b.on = !(b.vmag < kEps);
-    b.nx = vx / b.vmag;
-    b.ny = vy / b.vmag;
+    const double inv = 1.0 / b.vmag;      // one division for both components (an ulp from vx/|v|)
+    b.nx = vx * inv;
+    b.ny = vy * inv;
     double vmax = vmax_();
     b.v = b.vmag > vmax ? vmax : b.vmag;
     b.gam = 1.0 / sqrt(1.0 - b.v * b.v + 1e-10);
@@ -558,8 +559,10 @@ __device__ __forceinline__ void grid_push(const StepArgs& a, const GridMedium& g
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
     const double E = p.relativistic ? sqrt(E2) : m;
-    *x0 = *x0 + p.dt * q[0] / E;
-    *x1 = *x1 + p.dt * q[1] / E;
+    // dt*p/E as p*(dt/E): one division for both rows, within an ulp of the reference form
+    const double dtE = p.dt / E;
+    *x0 = *x0 + q[0] * dtE;
+    *x1 = *x1 + q[1] * dtE;
     if (p.position_diffusion) {
         double Td, ux, uy;
         grid_sample<SLICE>(g, *x0, *x1, j, td, &Td, &ux, &uy);
@@ -605,8 +608,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* sl
 // The step on the 2+1D grid.  With DEFER (one step per launch), particles that collide are
 // left in the LRF after the kick and queued; k_redraw_grid finishes them (redraw, boost back,
 // push), so that the redraw's noise and table lookups run only in colliding lanes.
+// three waves per SIMD: the register budget the step needs with a few spilled values, against
+// two waves at its unconstrained register count
+constexpr int kStepWaves = 3;
+
 template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER>
-__global__ __launch_bounds__(kBlock) void k_cart_grid(StepArgs a, GridMedium g) {
+__global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
     int seg;
     const int64_t i_thread = particle_of_thread(a, &seg);
     // with DEFER every thread reaches the block-wide ranking; threads past the segment's end
@@ -790,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_gc(StepArgs a, RadialMedium med) {
             xi = a.noise[(int64_t)s_off * a.N + i];
         } else {
             uint64_t w[4];
-            philox4x64((uint64_t)i, (uint64_t)step, 0, 0, p.seed, p.stream, w);
+            noise_block<1>(p.seed, p.stream, i, step, 0, w);
             double n1;
             box_muller(w[0], w[1], &xi, &n1);
         }
@@ -1360,6 +1367,7 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
     ++c->gen;
     if (!pos || !mom || N < 1 || dims < 1 || dims > 3 || pdim < 1 || pdim > 3)
         return c->fail(LMH_E_INVALID, "set_particles: need N >= 1 and 1 <= dims, pdim <= 3");
+    if (N > INT32_MAX) return c->fail(LMH_E_INVALID, "set_particles: at most 2^31 - 1 particles per context");
     if (nspecies < 1 || nspecies > LMH_MAX_SPECIES || !counts || !masses)
         return c->fail(LMH_E_INVALID, "set_particles: need 1..4 species with counts and masses");
     int64_t sum = 0;
@@ -1478,6 +1486,15 @@ int check_run(lmh_ctx* c, const lmh_params* p) {
         if (uses_rta && (!c->rta[k].d || !c->jt[k].d))
             return c->fail(LMH_E_STATE, "RTA needs a relaxation-time table and a Juttner table for species " + std::to_string(k));
     }
+    return LMH_OK;
+}
+
+// the Philox counter holds the step and the particle index in 32 bits and the stream in 61
+int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps) {
+    if (step_begin < 0 || nsteps < 0 || step_begin + nsteps > ((int64_t)1 << 32))
+        return c->fail(LMH_E_INVALID, "steps must lie in [0, 2^32)");
+    if (p->noise == LMH_NOISE_PHILOX && p->stream >= ((uint64_t)1 << 61))
+        return c->fail(LMH_E_INVALID, "the Philox stream must be below 2^61");
     return LMH_OK;
 }
 
@@ -1669,6 +1686,7 @@ int lmh_run(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
     int rc = check_run(c, p);
     if (rc) return rc;
     if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
+    if ((rc = check_steps(c, p, step_begin, nsteps))) return rc;
     if (nsteps == 0) return LMH_OK;
     LMH_HIP(c, hipSetDevice(c->device));
     const double* d_noise = nullptr;
@@ -1704,6 +1722,7 @@ int lmh_run_device_noise(lmh_ctx* c, const lmh_params* p, int64_t step_begin, in
     if (rc) return rc;
     if (nsteps < 0 || steps_per_launch < 0) return c->fail(LMH_E_INVALID, "nsteps and steps_per_launch must be >= 0");
     if (p->noise == LMH_NOISE_FIXED && !d_noise) return c->fail(LMH_E_INVALID, "fixed-noise mode needs a noise record");
+    if ((rc = check_steps(c, p, step_begin, nsteps))) return rc;
     if (nsteps == 0) return LMH_OK;
     LMH_HIP(c, hipSetDevice(c->device));
     return run_impl(c, p, step_begin, nsteps, d_noise, steps_per_launch);
@@ -1814,6 +1833,7 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     if (!s || s->checkpoint_every < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: checkpoint_every >= 1");
     if (nsteps < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: nsteps >= 1");
     if ((rc = check_adjoint(c, p))) return rc;
+    if ((rc = check_steps(c, p, step_begin, nsteps))) return rc;
     LMH_HIP(c, hipSetDevice(c->device));
     const int64_t K = s->checkpoint_every;
     const int64_t ncp = (nsteps + K - 1) / K;
@@ -2306,7 +2326,7 @@ int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, co
     int64_t N = 0;
     for (int32_t r = 0; r < nruns; ++r) {
         const lmh_run_desc& d = runs[r];
-        if (d.count < 0 || !(d.mass > 0.0) || d.medium < 0 || d.medium >= (int32_t)b->media.size() ||
+        if (d.count < 0 || d.count > INT32_MAX || !(d.mass > 0.0) || d.medium < 0 || d.medium >= (int32_t)b->media.size() ||
             d.drag_table >= (int32_t)b->tabs.size() || d.rta_table >= (int32_t)b->tabs.size() ||
             d.juttner_table >= (int32_t)b->jtabs.size())
             return b->fail(LMH_E_INVALID, "batch_set_runs: run " + std::to_string(r) +
@@ -2405,7 +2425,10 @@ int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t
     if (!b) return LMH_E_INVALID;
     int rc = batch_check(b, p);
     if (rc) return rc;
-    if (nsteps < 0) return b->fail(LMH_E_INVALID, "batch_run: nsteps >= 0");
+    if (nsteps < 0 || step_begin < 0 || step_begin + nsteps > ((int64_t)1 << 32))
+        return b->fail(LMH_E_INVALID, "batch_run: steps must lie in [0, 2^32)");
+    if (p->stream >= ((uint64_t)1 << 61) - b->runs.size())
+        return b->fail(LMH_E_INVALID, "batch_run: the Philox stream plus the run count must stay below 2^61");
     LMB_HIP(b, hipSetDevice(b->device));
     return batch_run_impl(b, p, step_begin, nsteps);
 }

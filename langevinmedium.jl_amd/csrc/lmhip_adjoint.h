@@ -7,8 +7,9 @@
 // theta = (DsT, gamma0, Lambda): DsT scales the drag-time table, (gamma0, Lambda) are the
 // GAMMA_P drag eta = gamma0 T^2 / (1 + p^2/Lambda^2).
 //
-// Common random numbers: the noise of particle i at step s is Philox4x64-10 of counter
-// (i, s, block), so the reverse sweep regenerates it instead of storing it per step.  The forward
+// Common random numbers: the noise of particle i at step s is Philox4x32-10 of a counter made of
+// (i, s, stream, block), so each segment's recomputation draws the forward pass's noise again and
+// no noise is kept between the passes.  The forward
 // pass keeps the state every K steps (checkpoints); the backward pass walks the segments from the
 // last to the first, re-runs each segment forward from its checkpoint while writing the step
 // inputs (state and kick noise) to a segment tape, then runs the reverse steps over the tape.

@@ -78,7 +78,7 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const GridMed
 // PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
 // otherwise one block per tile.
 template <int PDIM, int COLL, bool PERSIST>
-__global__ __launch_bounds__(kBlock) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
+__global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
     const int64_t step0 = *b.step_dev + k_off;
     const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
     for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += stride) {

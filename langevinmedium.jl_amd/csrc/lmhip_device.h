@@ -117,21 +117,36 @@ __device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Philox4x64-10 (Salmon et al., SC'11), counter (particle, step, block, 0), key (seed, stream)
-__device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
-                                           uint64_t k0, uint64_t k1, uint64_t out[4]) {
+// Philox4x32-10 (Salmon et al., SC'11).  Block b of particle i at step s: four 64-bit words from
+// two evaluations (half h = 0 -> w0, w1; h = 1 -> w2, w3; a word is lo | hi << 32 of consecutive
+// outputs) of counter (i, s, stream_lo, stream_hi mod 2^29 | (2b + h) << 29) under key
+// (seed_lo, seed_hi).  The host keeps i, s < 2^32 and stream < 2^61.
+__device__ __forceinline__ void philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                           uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        if (r > 0) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
-        uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c0;
-        uint64_t hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c0);
-        uint64_t lo1 = 0xCA5A826395121157ULL * c2;
-        uint64_t hi1 = __umul64hi(0xCA5A826395121157ULL, c2);
-        uint64_t n0 = hi1 ^ c1 ^ k0;
-        uint64_t n2 = hi0 ^ c3 ^ k1;
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// HALVES = 1 fills w0, w1 only
+template <int HALVES = 2>
+__device__ __forceinline__ void noise_block(uint64_t seed, uint64_t stream, int64_t i, int64_t step,
+                                            int block, uint64_t w[4]) {
+#pragma unroll
+    for (int h = 0; h < HALVES; ++h) {
+        uint32_t x[4];
+        philox4x32((uint32_t)(uint64_t)i, (uint32_t)(uint64_t)step, (uint32_t)stream,
+                   ((uint32_t)(stream >> 32) & 0x1FFFFFFFu) | ((uint32_t)(2 * block + h) << 29),
+                   (uint32_t)seed, (uint32_t)(seed >> 32), x);
+        w[2 * h] = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+        w[2 * h + 1] = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+    }
 }
 
 __device__ __forceinline__ double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
@@ -322,8 +337,10 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
 __device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTable& taun, double T,
                                              double ucol) {
     double tau_n = (p.DsT > 0.0 && T > 0.0) ? eval_time_table(taun, T) : 0.0;
-    double Pcol = (tau_n > 0.0 && isfinite(tau_n)) ? jl_clamp(p.dt / tau_n, 0.0, 1.0) : 1.0;
-    return ucol < Pcol;
+    // ucol < clamp(dt/tau_n, 0, 1) with ucol in [0, 1) is ucol * tau_n < dt for tau_n > 0 (no
+    // division; the two forms differ only when ucol * tau_n is within an ulp of dt)
+    if (!(tau_n > 0.0 && isfinite(tau_n))) return ucol < 1.0;
+    return ucol * tau_n < p.dt;
 }
 
 template <int PDIM, bool RADIAL>

@@ -337,30 +337,40 @@ int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int
     return LMO_OK;
 }
 
-/* ============================== Philox4x64-10 and the noise record ==========================
- * Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11): multipliers 0xD2E7470EE14C6C93 and
- * 0xCA5A826395121157, Weyl key increments 0x9E3779B97F4A7C15 and 0xBB67AE8584CAA73B, ten
- * rounds.  numpy.random.Philox implements the same bijection (its first block uses
- * counter + 1).                                                                              */
-static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
-    unsigned __int128 p = (unsigned __int128)a * (unsigned __int128)b;
-    *hi = (uint64_t)(p >> 64);
-    *lo = (uint64_t)p;
-}
-
-void lmo_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
-    uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
-    uint64_t k0 = key[0], k1 = key[1];
+/* ============================== Philox4x32-10 and the noise record ==========================
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, the Random123 reference generator):
+ * multipliers 0xD2511F53 and 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85, ten
+ * rounds of the round function of the paper's Table 2.                                        */
+void lmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
     for (int r = 0; r < 10; ++r) {
-        if (r > 0) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
-        uint64_t hi0, lo0, hi1, lo1;
-        mulhilo64(0xD2E7470EE14C6C93ULL, c0, &hi0, &lo0);
-        mulhilo64(0xCA5A826395121157ULL, c2, &hi1, &lo1);
-        uint64_t n0 = hi1 ^ c1 ^ k0;
-        uint64_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at step s: four 64-bit words
+ * from two Philox4x32-10 evaluations (half h = 0 gives w0, w1; h = 1 gives w2, w3; each word is
+ * lo | hi << 32 of consecutive outputs) of counter
+ *     (i mod 2^32, s mod 2^32, stream mod 2^32, (stream >> 32) mod 2^29 | (2 b + h) << 29)
+ * under key (seed mod 2^32, seed >> 32).  The host keeps i, s < 2^32 and stream < 2^61.       */
+void lmo_noise_block(uint64_t seed, uint64_t stream, int64_t i, int64_t step, int32_t block,
+                     uint64_t w[4]) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t ctr[4] = {(uint32_t)(uint64_t)i, (uint32_t)(uint64_t)step, (uint32_t)stream,
+                                 ((uint32_t)(stream >> 32) & 0x1FFFFFFFu) | ((uint32_t)(2 * block + h) << 29)};
+        uint32_t x[4];
+        lmo_philox4x32_10(ctr, key, x);
+        w[2 * h] = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+        w[2 * h + 1] = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+    }
 }
 
 /* uniforms from a 64-bit word: [0,1) and (0,1], 53 bits each */
@@ -471,7 +481,7 @@ static inline double u_low44(const uint64_t w[4]) {
 }
 
 /* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
- * Philox4x64-10 of counter (i, s, b, 0) under key (seed, stream).
+ * lmo_noise_block(seed, stream, i, s, b).
  *   block 0: Box-Muller pairs (w0,w1), (w2,w3) -> xi[0..pdim) (GC: xi = n0);
  *            u_col = the low-bit uniform of the block
  *   block 1: (g0, g1) = BoxMuller(w0, w1), g2 = first of BoxMuller(w2, w3);
@@ -481,13 +491,11 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
                       double* rec) {
     int32_t oxi, orta, opos;
     noise_layout(p, dims, pdim, &oxi, &orta, &opos);
-    uint64_t key[2] = {p->seed, p->stream};
     for (int64_t i = 0; i < N; ++i) {
         uint64_t w[4];
         double n[4];
         if (oxi >= 0 || orta >= 0) {
-            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 0, 0};
-            lmo_philox4x64_10(ctr, key, w);
+            lmo_noise_block(p->seed, p->stream, i, step, 0, w);
             box_muller(w[0], w[1], &n[0], &n[1]);
             box_muller(w[2], w[3], &n[2], &n[3]);
             if (oxi >= 0) {
@@ -497,16 +505,14 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
             if (orta >= 0) rec[(int64_t)orta * N + i] = u_low44(w);
         }
         if (orta >= 0) {
-            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 1, 0};
-            lmo_philox4x64_10(ctr, key, w);
+            lmo_noise_block(p->seed, p->stream, i, step, 1, w);
             box_muller(w[0], w[1], &n[0], &n[1]);
             box_muller(w[2], w[3], &n[2], &n[3]);
             rec[(int64_t)(orta + 1) * N + i] = u_low44(w);
             for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = n[d];
         }
         if (opos >= 0) {
-            uint64_t ctr[4] = {(uint64_t)i, (uint64_t)step, 2, 0};
-            lmo_philox4x64_10(ctr, key, w);
+            lmo_noise_block(p->seed, p->stream, i, step, 2, w);
             box_muller(w[0], w[1], &n[0], &n[1]);
             box_muller(w[2], w[3], &n[2], &n[3]);
             for (int32_t d = 0; d < dims; ++d) rec[(int64_t)(opos + d) * N + i] = n[d];

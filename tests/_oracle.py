@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_uint64
+from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_uint32, c_uint64
 
 import numpy as np
 
@@ -70,7 +70,8 @@ def lib():
                                                  c_int32, c_double, PD, PD, PD]),
             "lmo_juttner_invcdf_lookup": (c_double, [PD, c_int32, c_int32, c_double, c_double, c_double, c_double]),
             "lmo_draw_juttner_rejection": (c_int, [c_double, c_double, c_int32, c_uint64, c_int64, PD]),
-            "lmo_philox4x64_10": (None, [POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
+            "lmo_philox4x32_10": (None, [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]),
+            "lmo_noise_block": (None, [c_uint64, c_uint64, c_int64, c_int64, c_int32, POINTER(c_uint64)]),
             "lmo_noise_channels": (c_int32, [P, c_int32, c_int32]),
             "lmo_noise_record": (None, [P, c_int32, c_int32, c_int64, c_int64, PD]),
             "lmo_kernel_boost_to_rest": (None, [PD, PD, POINTER(RadialMedium), c_double, c_int64, c_int64,

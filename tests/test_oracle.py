@@ -29,29 +29,51 @@ def interp(x, y, V, xi, yi, gc=False):
     return f(O.dp(x), x.size, O.dp(y), y.size, O.dp(V), xi, yi)
 
 
-# ---------------------------------------------------------------- Philox4x64-10 vs numpy ------
-@pytest.mark.parametrize("key,ctr", [((0, 0), (0, 0, 0, 0)), ((1, 2), (5, 0, 0, 0)),
-                                     ((0xDEADBEEF, 77), (123456789, 42, 2, 0)),
-                                     ((2**64 - 1, 2**63), (2**64 - 2, 2**64 - 1, 7, 3))])
-def test_philox_matches_numpy(key, ctr):
-    # numpy's Philox advances the counter before its first block
-    bg = np.random.Philox(key=np.array(key, dtype=np.uint64), counter=np.array(ctr, dtype=np.uint64))
-    want = bg.random_raw(4)
-    c = list(ctr)
-    c[0] = (c[0] + 1) % 2**64
-    if c[0] == 0:
-        c[1] = (c[1] + 1) % 2**64
-    out = (ctypes.c_uint64 * 4)()
-    L.lmo_philox4x64_10((ctypes.c_uint64 * 4)(*c), (ctypes.c_uint64 * 2)(*key), out)
-    assert list(out) == [int(w) for w in want]
+# ---------------------------------------------------------------- Philox4x32-10 ----------------
+# Random123's known-answer vectors for philox4x32-10 (counter, key -> output)
+PHILOX32_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
 
 
-def test_philox_known_answer_zero():
-    # the Random123 known-answer vector for philox4x64-10, counter 0, key 0
-    out = (ctypes.c_uint64 * 4)()
-    L.lmo_philox4x64_10((ctypes.c_uint64 * 4)(0, 0, 0, 0), (ctypes.c_uint64 * 2)(0, 0), out)
-    assert [hex(v) for v in out] == ["0x16554d9eca36314c", "0xdb20fe9d672d0fdc",
-                                     "0xd7e772cee186176b", "0x7e68b68aec7ba23b"]
+@pytest.mark.parametrize("ctr,key,want", PHILOX32_KAT)
+def test_philox4x32_known_answers(ctr, key, want):
+    out = (ctypes.c_uint32 * 4)()
+    L.lmo_philox4x32_10((ctypes.c_uint32 * 4)(*ctr), (ctypes.c_uint32 * 2)(*key), out)
+    assert tuple(out) == want
+
+
+def _philox_words(seed, stream, i, step, block):
+    """the four words of a noise block from the KAT-pinned generator and the documented counter
+    layout: (i, step, stream_lo, stream_hi mod 2^29 | (2 block + half) << 29), key seed"""
+    w = []
+    for h in range(2):
+        ctr = (i & 0xFFFFFFFF, step & 0xFFFFFFFF, stream & 0xFFFFFFFF,
+               ((stream >> 32) & 0x1FFFFFFF) | ((2 * block + h) << 29))
+        out = (ctypes.c_uint32 * 4)()
+        L.lmo_philox4x32_10((ctypes.c_uint32 * 4)(*ctr),
+                            (ctypes.c_uint32 * 2)(seed & 0xFFFFFFFF, seed >> 32), out)
+        w += [out[0] | (out[1] << 32), out[2] | (out[3] << 32)]
+    return w
+
+
+def test_noise_block_layout():
+    for seed, stream, i, step, block in ((0, 0, 0, 0, 0), (99, 3, 7, 17, 1), (2**64 - 1, 2**61 - 1, 2**32 - 1, 2**32 - 1, 2)):
+        w = (ctypes.c_uint64 * 4)()
+        L.lmo_noise_block(seed, stream, i, step, block, w)
+        assert list(w) == _philox_words(seed, stream, i, step, block)
+    # distinct blocks, halves, streams and steps give distinct words
+    seen = set()
+    for block in range(3):
+        for stream in (0, 1, 2**32, 2**60):
+            for step in (1, 2):
+                w = (ctypes.c_uint64 * 4)()
+                L.lmo_noise_block(5, stream, 11, step, block, w)
+                seen.update(w)
+    assert len(seen) == 3 * 4 * 2 * 4
 
 
 def test_noise_record_box_muller():
@@ -59,26 +81,26 @@ def test_noise_record_box_muller():
                          collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
     rec = O.noise_record(p, 2, 3, 5, 17, 1)[0]
     assert rec.shape == (3 + 5 + 2, 5)
+    u = lambda v: mp.mpf(v >> 11) * mp.mpf(2) ** -53
+    u1 = lambda v: mp.mpf((v >> 11) + 1) * mp.mpf(2) ** -53
+    low = lambda w: (((w[0] & 0x7FF) << 33) | ((w[1] & 0x7FF) << 22) | ((w[2] & 0x7FF) << 11) | (w[3] & 0x7FF)) * 2.0 ** -44
+    twopi = mp.mpf(6.283185307179586)
     for i in range(5):
-        bg = np.random.Philox(key=np.array([99, 3], dtype=np.uint64),
-                              counter=np.array([i - 1 if i else 2**64 - 1, 17 if i else 16, 0, 0], dtype=np.uint64))
-        w = [int(v) for v in bg.random_raw(4)]
-        u = lambda v: mp.mpf(v >> 11) * mp.mpf(2) ** -53
-        u1 = lambda v: mp.mpf((v >> 11) + 1) * mp.mpf(2) ** -53
-        th = mp.mpf(6.283185307179586) * u(w[1])
-        n0 = mp.sqrt(-2 * mp.log(u1(w[0]))) * mp.cos(th)
-        n1 = mp.sqrt(-2 * mp.log(u1(w[0]))) * mp.sin(th)
-        assert abs(rec[0, i] - float(n0)) <= 4e-15 * max(1, abs(float(n0)))
-        assert abs(rec[1, i] - float(n1)) <= 4e-15 * max(1, abs(float(n1)))
+        w = _philox_words(99, 3, i, 17, 0)
+        r0 = mp.sqrt(-2 * mp.log(u1(w[0])))
+        r2 = mp.sqrt(-2 * mp.log(u1(w[2])))
+        for val, ref in ((rec[0, i], r0 * mp.cos(twopi * u(w[1]))), (rec[1, i], r0 * mp.sin(twopi * u(w[1]))),
+                         (rec[2, i], r2 * mp.cos(twopi * u(w[3])))):
+            assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
         # the collision uniform is the 44 low bits of block 0, the quantile those of block 1
-        low = lambda w: (((w[0] & 0x7FF) << 33) | ((w[1] & 0x7FF) << 22) | ((w[2] & 0x7FF) << 11) | (w[3] & 0x7FF)) * 2.0 ** -44
         assert rec[3, i] == low(w)
-        bg1 = np.random.Philox(key=np.array([99, 3], dtype=np.uint64),
-                               counter=np.array([i - 1 if i else 2**64 - 1, 17 if i else 16, 1, 0], dtype=np.uint64))
-        w1 = [int(v) for v in bg1.random_raw(4)]
+        w1 = _philox_words(99, 3, i, 17, 1)
         assert rec[4, i] == low(w1)
-        g0 = mp.sqrt(-2 * mp.log(u1(w1[0]))) * mp.cos(mp.mpf(6.283185307179586) * u(w1[1]))
+        g0 = mp.sqrt(-2 * mp.log(u1(w1[0]))) * mp.cos(twopi * u(w1[1]))
         assert abs(rec[5, i] - float(g0)) <= 4e-15 * max(1, abs(float(g0)))
+        w2 = _philox_words(99, 3, i, 17, 2)
+        x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(twopi * u(w2[1]))
+        assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
 
 
 def test_noise_transforms_accuracy():
