@@ -143,6 +143,9 @@ typedef struct lmh_tuning {
     int32_t defer;             /* 2+1D grid, one-step launches with RTA: 1 queues the colliding
                                   particles and finishes them in a second kernel, so the
                                   redraw runs in colliding lanes only (default 1)              */
+    int32_t specialize;        /* 1: the common configuration (relativistic, pre-point, drag
+                                  table, no Bjorken redshift, no position diffusion) runs
+                                  kernels compiled for it, with identical results (default 1) */
 } lmh_tuning;
 
 /* device time per launch class, recorded with HIP events on the context's stream between

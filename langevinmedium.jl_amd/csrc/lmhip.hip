@@ -60,6 +60,22 @@ struct StepArgs {
 // |v| <= sqrt(1 - 1e-12), kernels_cpu.jl:102
 __device__ __forceinline__ double vmax_() { return sqrt(1.0 - 1e-12); }
 
+// The common configuration -- relativistic, pre-point kick, drag table, no Bjorken redshift, no
+// position diffusion -- as compile-time constants: with SPEC the option branches it does not
+// take drop out of a kernel, which then needs far fewer registers.  The values are the same.
+template <bool SPEC>
+__device__ __forceinline__ lmh_params step_params(const lmh_params& in) {
+    lmh_params p = in;
+    if (SPEC) {
+        p.relativistic = 1;
+        p.integrator = LMH_PREPOINT;
+        p.drag_model = LMH_DRAG_TABLE;
+        p.bjorken_redshift = 0;
+        p.position_diffusion = 0;
+    }
+    return p;
+}
+
 // this block's species segment and this thread's particle (-1 past the segment's end)
 __device__ __forceinline__ int64_t particle_of_thread(const StepArgs& a, int* seg) {
     int k = 0;
@@ -224,14 +240,15 @@ __device__ __forceinline__ bool finite_all(const double* a, int n) {
 
 // ---------------------------------------------------------------------------------------------
 // Cartesian step on the radial (r, t) medium: the reference's own medium (simulate_cpu.jl).
-template <int DIMS, int PDIM, int NOISE, int COLL>
+template <int DIMS, int PDIM, int NOISE, int COLL, bool SPEC>
 __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium med) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
     const SpeciesArgs sp = a.species[seg];
     constexpr bool RADIAL = DIMS == 1;
-    const lmh_params& p = a.p;
+    const lmh_params pp = step_params<SPEC>(a.p);
+    const lmh_params& p = pp;
     const double m = sp.m;
     double x[DIMS], q[PDIM];
 #pragma unroll
@@ -551,10 +568,9 @@ __device__ __forceinline__ void glue_to_flow(double* q, double m, double vx, dou
 
 // position push (kernels_cpu.jl:537-541, 586-607) with the optional position diffusion
 template <int PDIM, int NOISE, bool SLICE>
-__device__ __forceinline__ void grid_push(const StepArgs& a, const GridMedium& g, int64_t i,
-                                          int64_t step, int s_off, int j, double td, double m,
-                                          double* x0, double* x1, const double* q) {
-    const lmh_params& p = a.p;
+__device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p, const GridMedium& g,
+                                          int64_t i, int64_t step, int s_off, int j, double td,
+                                          double m, double* x0, double* x1, const double* q) {
     double E2 = m * m;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
@@ -612,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* sl
 // two waves at its unconstrained register count
 constexpr int kStepWaves = 3;
 
-template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER>
+template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER, bool SPEC>
 __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
     int seg;
     const int64_t i_thread = particle_of_thread(a, &seg);
@@ -624,7 +640,8 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
     const SpeciesArgs sp = a.species[seg];
     constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
     constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
-    const lmh_params& p = a.p;
+    const lmh_params pp = step_params<SPEC>(a.p);
+    const lmh_params& p = pp;
     const double m = sp.m;
     double x0 = a.pos[0][i], x1 = a.pos[1][i];
     double q[PDIM];
@@ -671,7 +688,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
             }
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
         }
-        grid_push<PDIM, NOISE, SLICE>(a, g, i, step, s_off, j, td, m, &x0, &x1, q);
+        grid_push<PDIM, NOISE, SLICE>(a, p, g, i, step, s_off, j, td, m, &x0, &x1, q);
         if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
     }
     a.pos[0][i] = x0;
@@ -688,10 +705,11 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
 // packs their queued particles into its lanes.
 constexpr int kRedrawSpan = 8;
 
-template <int PDIM, int NOISE, bool SLICE>
+template <int PDIM, int NOISE, bool SLICE, bool SPEC>
 __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g, int64_t nblocks) {
     const int64_t step = a.step_begin;
-    const lmh_params& p = a.p;
+    const lmh_params pp = step_params<SPEC>(a.p);
+    const lmh_params& p = pp;
     const double t_now = (double)step * p.dt + p.t0;
     double td;
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
@@ -744,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
             noise_redraw<PDIM, NOISE>(a, i, step, 0, &usamp, gg);
             rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
-            grid_push<PDIM, NOISE, SLICE>(a, g, i, step, 0, j, td, m, &x0, &x1, q);
+            grid_push<PDIM, NOISE, SLICE>(a, p, g, i, step, 0, j, td, m, &x0, &x1, q);
             a.pos[0][i] = x0;
             a.pos[1][i] = x1;
 #pragma unroll
@@ -943,7 +961,7 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 0, 1};
+    lmh_tuning tune = {256, 1, 0, 1, 1};
 
     // tables
     struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
@@ -983,6 +1001,11 @@ struct lmh_ctx {
     size_t tape_cap = 0;
     double* d_adjw = nullptr;         // lam, per-particle gradients, partial sums, cotangent
     size_t adjw_cap = 0;
+    // the backward pass re-runs segment c-1 on a second stream while segment c is reversed;
+    // the two tape buffers alternate, ordered by events
+    hipStream_t tape_stream = nullptr;
+    hipEvent_t ev_tape[2] = {nullptr, nullptr};
+    hipEvent_t ev_rev[2] = {nullptr, nullptr};
 
     // scratch
     double* d_noise = nullptr;
@@ -1236,6 +1259,14 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_queue);
     dfree(c->d_qcount);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->tape_stream) {
+        (void)hipStreamSynchronize(c->tape_stream);
+        (void)hipStreamDestroy(c->tape_stream);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (c->ev_tape[k]) (void)hipEventDestroy(c->ev_tape[k]);
+        if (c->ev_rev[k]) (void)hipEventDestroy(c->ev_rev[k]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1498,38 +1529,49 @@ int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nst
     return LMH_OK;
 }
 
-template <int DIMS, int PDIM, int NOISE>
+// the parameters are those the SPEC kernels fix at compile time
+bool spec_config(const lmh_ctx* c, const lmh_params* p) {
+    return c->tune.specialize && p->relativistic && p->integrator == LMH_PREPOINT &&
+           p->drag_model == LMH_DRAG_TABLE && !p->bjorken_redshift && !p->position_diffusion;
+}
+
+template <int DIMS, int PDIM, int NOISE, bool SPEC>
 void launch_radial_n(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int coll) {
     const dim3 G(nb), B(c->tune.block);
     switch (coll) {
-        case LMH_LANGEVIN: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN>), G, B, 0, c->stream, a, med); break;
-        case LMH_RTA: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_RTA>), G, B, 0, c->stream, a, med); break;
-        case LMH_FLUID: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_FLUID>), G, B, 0, c->stream, a, med); break;
-        default: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN_RTA>), G, B, 0, c->stream, a, med); break;
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN, SPEC>), G, B, 0, c->stream, a, med); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_RTA, SPEC>), G, B, 0, c->stream, a, med); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_FLUID, SPEC>), G, B, 0, c->stream, a, med); break;
+        default: hipLaunchKernelGGL((k_cart_radial<DIMS, PDIM, NOISE, LMH_LANGEVIN_RTA, SPEC>), G, B, 0, c->stream, a, med); break;
     }
 }
 
 template <int DIMS, int PDIM>
-void launch_radial(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int noise, int coll) {
-    if (noise == LMH_NOISE_FIXED) launch_radial_n<DIMS, PDIM, LMH_NOISE_FIXED>(c, a, nb, med, coll);
-    else launch_radial_n<DIMS, PDIM, LMH_NOISE_PHILOX>(c, a, nb, med, coll);
+void launch_radial(lmh_ctx* c, const StepArgs& a, unsigned nb, const RadialMedium& med, int noise, int coll, bool spec) {
+    if (noise == LMH_NOISE_FIXED) {
+        if (spec) launch_radial_n<DIMS, PDIM, LMH_NOISE_FIXED, true>(c, a, nb, med, coll);
+        else launch_radial_n<DIMS, PDIM, LMH_NOISE_FIXED, false>(c, a, nb, med, coll);
+    } else {
+        if (spec) launch_radial_n<DIMS, PDIM, LMH_NOISE_PHILOX, true>(c, a, nb, med, coll);
+        else launch_radial_n<DIMS, PDIM, LMH_NOISE_PHILOX, false>(c, a, nb, med, coll);
+    }
 }
 
-template <int PDIM, int NOISE, bool SLICE>
+template <int PDIM, int NOISE, bool SLICE, bool SPEC>
 void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll, bool defer) {
     const dim3 G(nb), B(c->tune.block);
     {
     LaunchMark mk(c, LMH_PROF_STEP);
     if (coll == LMH_LANGEVIN) {
-        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN, false>), G, B, 0, c->stream, a, g);
+        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN, false, SPEC>), G, B, 0, c->stream, a, g);
     } else if (coll == LMH_FLUID) {
-        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_FLUID, false>), G, B, 0, c->stream, a, g);
+        hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_FLUID, false, SPEC>), G, B, 0, c->stream, a, g);
     } else if (coll == LMH_RTA) {
-        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, true>), G, B, 0, c->stream, a, g);
-        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, false>), G, B, 0, c->stream, a, g);
+        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, true, SPEC>), G, B, 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_RTA, false, SPEC>), G, B, 0, c->stream, a, g);
     } else {
-        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, true>), G, B, 0, c->stream, a, g);
-        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, false>), G, B, 0, c->stream, a, g);
+        if (defer) hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, true, SPEC>), G, B, 0, c->stream, a, g);
+        else hipLaunchKernelGGL((k_cart_grid<PDIM, NOISE, SLICE, LMH_LANGEVIN_RTA, false, SPEC>), G, B, 0, c->stream, a, g);
     }
     }
     if (defer) {
@@ -1537,20 +1579,27 @@ void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium&
         // one wave per kRedrawSpan step-kernel blocks
         const int64_t nwaves = ((int64_t)nb + kRedrawSpan - 1) / kRedrawSpan;
         const unsigned nr = (unsigned)std::max<int64_t>(1, (nwaves + (kBlock / 64) - 1) / (kBlock / 64));
-        hipLaunchKernelGGL((k_redraw_grid<PDIM, NOISE, SLICE>), dim3(nr), dim3(kBlock), 0, c->stream, a, g, (int64_t)nb);
+        hipLaunchKernelGGL((k_redraw_grid<PDIM, NOISE, SLICE, SPEC>), dim3(nr), dim3(kBlock), 0, c->stream, a, g, (int64_t)nb);
+    }
+}
+
+template <int PDIM, bool SPEC>
+void launch_grid_n(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice,
+                   int coll, bool defer) {
+    if (noise == LMH_NOISE_FIXED) {
+        if (slice) launch_grid_s<PDIM, LMH_NOISE_FIXED, true, SPEC>(c, a, nb, g, coll, defer);
+        else launch_grid_s<PDIM, LMH_NOISE_FIXED, false, SPEC>(c, a, nb, g, coll, defer);
+    } else {
+        if (slice) launch_grid_s<PDIM, LMH_NOISE_PHILOX, true, SPEC>(c, a, nb, g, coll, defer);
+        else launch_grid_s<PDIM, LMH_NOISE_PHILOX, false, SPEC>(c, a, nb, g, coll, defer);
     }
 }
 
 template <int PDIM>
 void launch_grid(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice,
-                 int coll, bool defer) {
-    if (noise == LMH_NOISE_FIXED) {
-        if (slice) launch_grid_s<PDIM, LMH_NOISE_FIXED, true>(c, a, nb, g, coll, defer);
-        else launch_grid_s<PDIM, LMH_NOISE_FIXED, false>(c, a, nb, g, coll, defer);
-    } else {
-        if (slice) launch_grid_s<PDIM, LMH_NOISE_PHILOX, true>(c, a, nb, g, coll, defer);
-        else launch_grid_s<PDIM, LMH_NOISE_PHILOX, false>(c, a, nb, g, coll, defer);
-    }
+                 int coll, bool defer, bool spec) {
+    if (spec) launch_grid_n<PDIM, true>(c, a, nb, g, noise, slice, coll, defer);
+    else launch_grid_n<PDIM, false>(c, a, nb, g, noise, slice, coll, defer);
 }
 
 int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
@@ -1565,6 +1614,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     if (nb == 0) return LMH_OK;
     const bool has_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
     const bool defer = grid && k == 1 && has_rta && c->tune.defer;
+    const bool spec = spec_config(c, p);
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
         const bool slice = grid && n == 1 && c->tune.slice;
@@ -1587,14 +1637,14 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             else
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_PHILOX>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
         } else if (grid) {
-            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1);
-            else launch_grid<3>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1);
+            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1, spec);
+            else launch_grid<3>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1, spec);
         } else {
             LaunchMark mk(c, LMH_PROF_STEP);
-            if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise, p->collision);
-            else if (c->dims == 2 && c->pdim == 2) launch_radial<2, 2>(c, a, nb, rmed, p->noise, p->collision);
-            else if (c->dims == 2 && c->pdim == 3) launch_radial<2, 3>(c, a, nb, rmed, p->noise, p->collision);
-            else launch_radial<3, 3>(c, a, nb, rmed, p->noise, p->collision);
+            if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise, p->collision, spec);
+            else if (c->dims == 2 && c->pdim == 2) launch_radial<2, 2>(c, a, nb, rmed, p->noise, p->collision, spec);
+            else if (c->dims == 2 && c->pdim == 3) launch_radial<2, 3>(c, a, nb, rmed, p->noise, p->collision, spec);
+            else launch_radial<3, 3>(c, a, nb, rmed, p->noise, p->collision, spec);
         }
         LMH_HIP(c, hipGetLastError());
     }
@@ -1615,8 +1665,10 @@ extern "C" {
 int lmh_set_tuning(lmh_ctx* c, const lmh_tuning* t) {
     if (!c) return LMH_E_INVALID;
     if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
-        (t->slice != 0 && t->slice != 1) || (t->defer != 0 && t->defer != 1))
-        return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice and defer in {0, 1}");
+        (t->slice != 0 && t->slice != 1) || (t->defer != 0 && t->defer != 1) ||
+        (t->specialize != 0 && t->specialize != 1))
+        return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice, defer "
+                                      "and specialize in {0, 1}");
     c->tune = *t;
     return LMH_OK;
 }
@@ -1809,15 +1861,25 @@ int check_adjoint(lmh_ctx* c, const lmh_params* p) {
     return LMH_OK;
 }
 
-template <int PDIM>
-void launch_adj_segment(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll,
-                        const double* ckpt, int64_t step0, int32_t nk, double* lam, double* gacc,
-                        double dtau) {
+template <int PDIM, int COLL, int DRAG>
+void launch_adj_t(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g,
+                  const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam, double* gacc,
+                  double dtau) {
     const dim3 G(nb), B(c->tune.block);
-    if (coll == LMH_LANGEVIN_RTA)
-        hipLaunchKernelGGL((k_adj_segment<PDIM, LMH_LANGEVIN_RTA>), G, B, 0, c->stream, a, g, ckpt, step0, nk, c->d_tape, lam, gacc, dtau);
-    else
-        hipLaunchKernelGGL((k_adj_segment<PDIM, LMH_LANGEVIN>), G, B, 0, c->stream, a, g, ckpt, step0, nk, c->d_tape, lam, gacc, dtau);
+    if (tape) hipLaunchKernelGGL((k_adj_tape<PDIM, COLL, DRAG>), G, B, 0, c->tape_stream, a, g, ckpt, step0, nk, tp);
+    else hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, a, g, step0, nk, tp, lam, gacc, dtau);
+}
+
+template <int PDIM>
+void launch_adj(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, int coll, int drag,
+                const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam, double* gacc, double dtau) {
+    if (coll == LMH_LANGEVIN_RTA) {
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+        else launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_TABLE>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+    } else {
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+        else launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_TABLE>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+    }
 }
 
 }  // namespace
@@ -1878,7 +1940,15 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     const int64_t ncp = (c->adj.nsteps + K - 1) / K;
     const int nw = 5 + os->nbins;
     int rc;
-    if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, (size_t)K * TR * (size_t)N, "adjoint_backward tape"))) return rc;
+    const size_t tape_n = (size_t)K * TR * (size_t)N;
+    if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, 2 * tape_n, "adjoint_backward tape"))) return rc;
+    if (!c->tape_stream) {
+        LMH_HIP(c, hipStreamCreateWithFlags(&c->tape_stream, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) {
+            LMH_HIP(c, hipEventCreateWithFlags(&c->ev_tape[k], hipEventDisableTiming));
+            LMH_HIP(c, hipEventCreateWithFlags(&c->ev_rev[k], hipEventDisableTiming));
+        }
+    }
     const size_t o_lam = 0, o_g = (size_t)R * N, o_part = o_g + (size_t)kAdjThetas * N;
     const size_t o_out = o_part + (size_t)kAdjThetas * kSumBlocks, o_w = o_out + kAdjThetas;
     if ((rc = ensure_dev(c, &c->d_adjw, &c->adjw_cap, o_w + nw, "adjoint_backward workspace"))) return rc;
@@ -1898,12 +1968,29 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     StepArgs a = make_args(c, &p, c->tune.block, &nb);
     GridMedium g = make_grid(c);
     const size_t per = (size_t)R * N;
+    // the tape stream starts after the checkpoints are final (the forward pass and the seed
+    // were enqueued on the main stream)
+    LMH_HIP(c, hipEventRecord(c->ev_rev[0], c->stream));
+    LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[0], 0));
     for (int64_t cp = ncp - 1; cp >= 0; --cp) {
         const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
         const int64_t step0 = c->adj.step_begin + cp * K;
-        LaunchMark mk(c, LMH_PROF_STEP);
-        if (c->pdim == 2) launch_adj_segment<2>(c, a, nb, g, p.collision, c->d_ckpt + per * cp, step0, nk, lam, gacc, c->adj.dtau_dDsT);
-        else launch_adj_segment<3>(c, a, nb, g, p.collision, c->d_ckpt + per * cp, step0, nk, lam, gacc, c->adj.dtau_dDsT);
+        const int buf = (int)(cp & 1);
+        double* tp = c->d_tape + tape_n * buf;
+        // re-run segment cp into tape buffer buf, once the reverse of segment cp + 2 (the
+        // buffer's previous reader) is done
+        if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
+        if (c->pdim == 2) launch_adj<2>(c, true, a, nb, g, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+        else launch_adj<3>(c, true, a, nb, g, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+        LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
+        // reverse segment cp on the main stream
+        LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
+        {
+            LaunchMark mk(c, LMH_PROF_STEP);
+            if (c->pdim == 2) launch_adj<2>(c, false, a, nb, g, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+            else launch_adj<3>(c, false, a, nb, g, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+        }
+        LMH_HIP(c, hipEventRecord(c->ev_rev[buf], c->stream));
         LMH_HIP(c, hipGetLastError());
     }
     hipLaunchKernelGGL(k_rows_partial, dim3(kSumBlocks), dim3(kSumBlock), 0, c->stream, gacc, kAdjThetas, N, c->d_adjw + o_part);
@@ -2109,15 +2196,24 @@ int batch_check(lmh_batch* b, const lmh_params* p) {
     return LMH_OK;
 }
 
-template <int PDIM, bool PERSIST>
-void batch_launch_p(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, int32_t n, unsigned grid) {
+template <int PDIM, bool PERSIST, bool SPEC>
+void batch_launch_s(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, int32_t n, unsigned grid) {
     const dim3 G(grid), B(b->tune.block);
     switch (coll) {
-        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
-        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_RTA, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
-        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_FLUID, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
-        default: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN_RTA, PERSIST>), G, B, 0, b->stream, a, k_off, n); break;
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN, PERSIST, SPEC>), G, B, 0, b->stream, a, k_off, n); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_RTA, PERSIST, SPEC>), G, B, 0, b->stream, a, k_off, n); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_FLUID, PERSIST, SPEC>), G, B, 0, b->stream, a, k_off, n); break;
+        default: hipLaunchKernelGGL((k_batch_grid<PDIM, LMH_LANGEVIN_RTA, PERSIST, SPEC>), G, B, 0, b->stream, a, k_off, n); break;
     }
+}
+
+template <int PDIM, bool PERSIST>
+void batch_launch_p(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, int32_t n, unsigned grid) {
+    const lmh_params& p = a.base.p;
+    const bool spec = p.relativistic && p.integrator == LMH_PREPOINT && p.drag_model == LMH_DRAG_TABLE &&
+                      !p.bjorken_redshift && !p.position_diffusion;
+    if (spec) batch_launch_s<PDIM, PERSIST, true>(b, a, coll, k_off, n, grid);
+    else batch_launch_s<PDIM, PERSIST, false>(b, a, coll, k_off, n, grid);
 }
 
 // steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps

@@ -98,44 +98,14 @@ __device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, 
     s->dvyy = (e1 - e0) * gy;
 }
 
-// reverse of apply_boost (relativistic) for input momentum q: qb holds the adjoint of the output
-// on entry and of the input on return; the adjoints of the flow factors are accumulated.
-template <int PDIM>
-__device__ __forceinline__ void boost_reverse(const double* q, double m, const FlowBoost& b, int sign,
-                                              double* qb, double* nxb, double* nyb, double* vb,
-                                              double* gamb) {
-    double p2 = 0.0;
-#pragma unroll
-    for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
-    const double E = sqrt(p2 + m * m);
-    const double s = sign > 0 ? 1.0 : -1.0;
-    const double ppar = q[0] * b.nx + q[1] * b.ny;
-    const double inner = ppar + s * b.v * E;
-    const double delta = b.gam * inner - ppar;          // q' = q + delta * n on rows 0, 1
-    const double deltab = qb[0] * b.nx + qb[1] * b.ny;
-    *nxb += qb[0] * delta;
-    *nyb += qb[1] * delta;
-    *gamb += deltab * inner;
-    *vb += deltab * b.gam * s * E;
-    const double Eb = deltab * b.gam * s * b.v;
-    const double pparb = deltab * (b.gam - 1.0);
-    *nxb += pparb * q[0];
-    *nyb += pparb * q[1];
-    qb[0] += pparb * b.nx;
-    qb[1] += pparb * b.ny;
-#pragma unroll
-    for (int d = 0; d < PDIM; ++d) qb[d] += Eb * q[d] / E;
-}
-
 // One forward step, exactly the fused k_cart_grid step (non-deferred, Philox, no position
 // diffusion), for the tape recomputation.  xi and ucol are the step's kick noise.
 template <int PDIM, int COLL>
-__device__ __forceinline__ void adj_forward_step(const StepArgs& a, const GridMedium& g,
-                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
-                                                 double* x0, double* x1, double* q,
-                                                 const double* xi, double ucol) {
+__device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_params& p,
+                                                 const GridMedium& g, const SpeciesArgs& sp,
+                                                 int64_t i, int64_t step, double* x0, double* x1,
+                                                 double* q, const double* xi, double ucol) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
-    const lmh_params& p = a.p;
     const double m = sp.m;
     const double t_now = (double)step * p.dt + p.t0;
     double td;
@@ -155,21 +125,36 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const GridMe
         rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
     }
     apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
-    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, g, i, step, 0, j, td, m, x0, x1, q);
+    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+}
+
+// Reciprocal and reciprocal square root for values that only scale derivatives, where an ulp or
+// two from 1/x and 1/sqrt(x) does not matter: the hardware estimates refined by one Newton step
+// (three instructions instead of a correctly rounded division's ten).
+__device__ __forceinline__ double drv_rcp(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+__device__ __forceinline__ double drv_rsq(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    return fma(r, fma(-(0.5 * x * r), r, 0.5), r);
 }
 
 // One reverse step.  (x0, x1, qin) is the step's input state and (xi, ucol) its kick noise;
 // lam holds the adjoint of the step's output state (x0, x1, q) on entry and of its input state
-// on return; th accumulates dL/dtheta.  The pre-point exact-OU kick, relativistic boosts.
+// on return; th accumulates dL/dtheta.  The pre-point exact-OU kick, relativistic boosts.  The
+// forward values are recomputed from the input state; the collision decision sees the forward
+// pass's temperature (the same grid interpolation), everything else here only scales the
+// derivatives.
 template <int PDIM, int COLL>
-__device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMedium& g,
-                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
-                                                 double x0, double x1, const double* qin,
-                                                 const double* xi, double ucol,
+__device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_params& p,
+                                                 const GridMedium& g, const SpeciesArgs& sp,
+                                                 int64_t i, int64_t step, double x0, double x1,
+                                                 const double* qin, const double* xi, double ucol,
                                                  double dtau_dDsT, double* lam, double* th) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
-    const lmh_params& p = a.p;
-    const double m = sp.m;
+    const double m = sp.m, m2 = sp.m * sp.m;
     const double dt = p.dt;
     const double t_now = (double)step * dt + p.t0;
     double td;
@@ -178,54 +163,79 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
     // ---- forward values
     SampleGrad s;
     grid_sample_grad(g, x0, x1, j, td, &s);
-    const FlowBoost fb = flow_boost(s.vx, s.vy);
+    const double vmag = sqrt(s.vx * s.vx + s.vy * s.vy);
+    const bool on = !(vmag < kEps);
+    const double ivm = on ? drv_rcp(vmag) : 0.0;
+    const double nx = s.vx * ivm, ny = s.vy * ivm;
+    const double vmax = vmax_();
+    const double v = vmag > vmax ? vmax : vmag;
+    const double gam = drv_rsq(1.0 - v * v + 1e-10);
+    // boost to the LRF: rows 0, 1 move by d1 along n
+    double p2in = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) p2in += qin[d] * qin[d];
+    const double iE1 = drv_rsq(p2in + m2);
+    const double E1 = (p2in + m2) * iE1;
+    const double ppar1 = qin[0] * nx + qin[1] * ny;
+    const double in1 = ppar1 - v * E1;
+    const double d1 = on ? gam * in1 - ppar1 : 0.0;
     double qB[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) qB[d] = qin[d];
-    apply_boost<PDIM>(qB, m, fb, -1, 1);
+    qB[0] += d1 * nx;
+    qB[1] += d1 * ny;
     double fbj = 1.0;
     if (PDIM == 3 && p.bjorken_redshift) {
         const double tau_a = p.t0 + (double)(step - 1) * dt;
         if (tau_a > 0.0) fbj = tau_a / (tau_a + dt);
-        bjorken(p, step, &qB[2 % PDIM]);
+        qB[2 % PDIM] *= fbj;
     }
     const double T = jl_max(s.T, 0.0);
-    double etaD = 0.0, kT = 0.0, tau = 0.0, dtau_dT = 0.0;
+    double etaD = 0.0, kT = 0.0, kappa = 0.0, tau = 0.0, itau = 0.0, dtau_dT = 0.0;
     bool tab_ok = false;
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
         etaD = p.gamma0 * T * T;
-        kT = sqrt(2.0 * m * T * etaD);
+        kappa = 2.0 * m * T * etaD;
+        kT = sqrt(kappa);
     } else {
         const double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
         tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(sp.drag, T);
         tab_ok = tau > 0.0 && isfinite(tau);
-        etaD = tab_ok ? (1.0 / tau) : 0.0;
-        const double kappa = tab_ok ? ((2.0 * m * T) / tau) : 0.0;
-        kT = sqrt(kappa);
-        if (tab_ok) dtau_dT = eval_time_table_slope(sp.drag, T);
+        if (tab_ok) {
+            itau = drv_rcp(tau);
+            etaD = itau;
+            kappa = 2.0 * m * T * itau;
+            kT = sqrt(kappa);
+            dtau_dT = eval_time_table_slope(sp.drag, T);
+        }
     }
+    const double ikT = kT > 0.0 ? drv_rcp(kT) : 0.0;
     // kick (ou_kick, pre-point branch)
-    double qK[PDIM];
-    double eta = 0.0, aa = 1.0, rr = 0.0, npref = 0.0, E2 = 0.0, den = 1.0, q2B = 0.0;
-    const bool kick = etaD > 0.0;
+    double q2B = 0.0;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q2B += qB[d] * qB[d];
+    const bool kick = etaD > 0.0;
+    double eta = 0.0, aa = 1.0, rr = 0.0, srr = 0.0, isrr = 0.0, ieta = 0.0, iE2 = 0.0, iden = 1.0, iL = 0.0;
+    double qK[PDIM];
     if (kick) {
         if (p.drag_model == LMH_DRAG_GAMMA_P) {
-            den = 1.0 + q2B / (p.Lambda * p.Lambda);
-            eta = etaD / den;
+            iL = drv_rcp(p.Lambda);
+            iden = drv_rcp(1.0 + q2B * iL * iL);
+            eta = etaD * iden;
         } else {
-            E2 = sqrt(q2B + m * m);
-            eta = p.relativistic ? etaD * m / E2 : etaD;
+            iE2 = drv_rsq(q2B + m2);
+            eta = etaD * m * iE2;
         }
         aa = exp(-eta * dt);
-        rr = (1.0 - aa * aa) / (2.0 * eta * dt);
-        npref = eta > 0.0 ? (kT * sqrt(rr)) : 0.0;
+        ieta = drv_rcp(eta);
+        rr = (1.0 - aa * aa) * (0.5 * ieta / dt);
+        isrr = rr > 0.0 ? drv_rsq(rr) : 0.0;
+        srr = rr * isrr;
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) qK[d] = qB[d] + ((aa - 1.0) * qB[d] + a.sqrt_dt * (npref * xi[d]));
+        for (int d = 0; d < PDIM; ++d) qK[d] = aa * qB[d] + a.sqrt_dt * (kT * srr) * xi[d];
     } else {
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) qK[d] = qB[d] + (0.0 + a.sqrt_dt * (kT * xi[d]));
+        for (int d = 0; d < PDIM; ++d) qK[d] = qB[d] + a.sqrt_dt * kT * xi[d];
     }
     // collision (decision fixed)
     bool col = false;
@@ -236,23 +246,36 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
         col = true;
         double usamp, gg[PDIM];
         noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
-        rta_redraw<PDIM, false>(sp.jt, T, qC, usamp, gg);
         double nrm2 = 0.0;
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) nrm2 += gg[d] * gg[d];
-        const double inv = 1.0 / sqrt(jl_max(nrm2, 1e-300));
+        const double inv = drv_rsq(jl_max(nrm2, 1e-300));
+        const double pstar = juttner_lookup(sp.jt, usamp, T);
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) ghat[d] = gg[d] * inv;
+        for (int d = 0; d < PDIM; ++d) {
+            ghat[d] = gg[d] * inv;
+            qC[d] = pstar * ghat[d];
+        }
         dpstar = juttner_lookup_dT(sp.jt, usamp, T);
     }
+    // boost back to the lab: rows 0, 1 move by d3 along n
+    double p2C = 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) p2C += qC[d] * qC[d];
+    const double iE3 = drv_rsq(p2C + m2);
+    const double E3 = (p2C + m2) * iE3;
+    const double ppar3 = qC[0] * nx + qC[1] * ny;
+    const double in3 = ppar3 + v * E3;
+    const double d3 = on ? gam * in3 - ppar3 : 0.0;
     double qF[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) qF[d] = qC[d];
-    apply_boost<PDIM>(qF, m, fb, +1, 1);
-    double E4sq = m * m;
+    qF[0] += d3 * nx;
+    qF[1] += d3 * ny;
+    double p2F = 0.0;
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) E4sq += qF[d] * qF[d];
-    const double E4 = sqrt(E4sq);
+    for (int d = 0; d < PDIM; ++d) p2F += qF[d] * qF[d];
+    const double iE4 = drv_rsq(p2F + m2);
 
     // ---- reverse sweep
     double qb[PDIM];
@@ -260,15 +283,31 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
     for (int d = 0; d < PDIM; ++d) qb[d] = lam[2 + d];
     double x0b = lam[0], x1b = lam[1];
     // push: x += dt * q / E
-    qb[0] += lam[0] * dt / E4;
-    qb[1] += lam[1] * dt / E4;
     {
-        const double E4b = -(lam[0] * qF[0] + lam[1] * qF[1]) * dt / (E4 * E4);
+        const double w = dt * iE4;
+        qb[0] += lam[0] * w;
+        qb[1] += lam[1] * w;
+        const double Eb = -(lam[0] * qF[0] + lam[1] * qF[1]) * w * iE4;      // d/dE of dt*q/E
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) qb[d] += E4b * qF[d] / E4;
+        for (int d = 0; d < PDIM; ++d) qb[d] += Eb * qF[d] * iE4;
     }
     double nxb = 0.0, nyb = 0.0, vb = 0.0, gamb = 0.0, Tb = 0.0;
-    if (fb.on) boost_reverse<PDIM>(qC, m, fb, +1, qb, &nxb, &nyb, &vb, &gamb);
+    // boost back (q' = q + (gam (ppar + v E) - ppar) n), input qC
+    if (on) {
+        const double deltab = qb[0] * nx + qb[1] * ny;
+        nxb += qb[0] * d3;
+        nyb += qb[1] * d3;
+        gamb += deltab * in3;
+        vb += deltab * gam * E3;
+        const double Eb = deltab * gam * v;
+        const double pparb = deltab * (gam - 1.0);
+        nxb += pparb * qC[0];
+        nyb += pparb * qC[1];
+        qb[0] += pparb * nx;
+        qb[1] += pparb * ny;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qb[d] += Eb * iE3 * qC[d];
+    }
     if (col) {
         double s_ = 0.0;
 #pragma unroll
@@ -277,7 +316,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) qb[d] = 0.0;
     }
-    // kick
+    // kick: qK = a qB + sqrt(dt) kT sqrt(r) xi, a = exp(-eta dt), r = (1 - a^2)/(2 eta dt)
     double kTb = 0.0, etaDb = 0.0;
     {
         double xq = 0.0;
@@ -289,30 +328,27 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
             for (int d = 0; d < PDIM; ++d) ab += qb[d] * qB[d];
             const double npb = a.sqrt_dt * xq;
             double etab = 0.0;
-            if (eta > 0.0 && rr > 0.0) {
-                const double srr = sqrt(rr);
+            if (rr > 0.0) {
                 kTb += npb * srr;
-                const double rrb = npb * kT / (2.0 * srr);
-                ab += -rrb * aa / (eta * dt);
-                etab += -rrb * rr / eta;
+                const double rrb = npb * kT * 0.5 * isrr;
+                ab += -rrb * aa * ieta / dt;
+                etab += -rrb * rr * ieta;
             }
             etab += -ab * dt * aa;
 #pragma unroll
             for (int d = 0; d < PDIM; ++d) qb[d] *= aa;
             if (p.drag_model == LMH_DRAG_GAMMA_P) {
-                etaDb += etab / den;
-                const double denb = -etab * eta / den;
-                const double L2 = p.Lambda * p.Lambda;
+                etaDb += etab * iden;
+                const double denb = -etab * eta * iden;          // eta = etaD / den
+                const double iL2 = iL * iL;
 #pragma unroll
-                for (int d = 0; d < PDIM; ++d) qb[d] += denb * 2.0 * qB[d] / L2;
-                th[2] += denb * (-2.0 * q2B / (L2 * p.Lambda));
-            } else if (p.relativistic) {
-                etaDb += etab * m / E2;
-                const double E2b = -etab * eta / E2;
-#pragma unroll
-                for (int d = 0; d < PDIM; ++d) qb[d] += E2b * qB[d] / E2;
+                for (int d = 0; d < PDIM; ++d) qb[d] += denb * 2.0 * qB[d] * iL2;
+                th[2] += denb * (-2.0 * q2B * iL2 * iL);
             } else {
-                etaDb += etab;
+                etaDb += etab * m * iE2;                        // eta = etaD m / E
+                const double E2b = -etab * eta * iE2;
+#pragma unroll
+                for (int d = 0; d < PDIM; ++d) qb[d] += E2b * qB[d] * iE2;
             }
         } else {
             kTb += a.sqrt_dt * xq;
@@ -320,35 +356,41 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const GridMe
     }
     // drag coefficients
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
-        if (kT > 0.0) {
-            const double kappab = kTb / (2.0 * kT);      // kappa = 2 m T etaD
-            Tb += kappab * 2.0 * m * etaD;
-            etaDb += kappab * 2.0 * m * T;
-        }
+        const double kappab = kTb * 0.5 * ikT;                 // kappa = 2 m T etaD
+        Tb += kappab * 2.0 * m * etaD;
+        etaDb += kappab * 2.0 * m * T;
         Tb += etaDb * 2.0 * p.gamma0 * T;
         th[1] += etaDb * T * T;
     } else if (tab_ok) {
-        double taub = -etaDb / (tau * tau);
-        if (kT > 0.0) {
-            const double kappa = kT * kT;
-            const double kappab = kTb / (2.0 * kT);      // kappa = 2 m T / tau
-            Tb += kappab * 2.0 * m / tau;
-            taub += -kappab * kappa / tau;
-        }
+        const double kappab = kTb * 0.5 * ikT;                 // kappa = 2 m T / tau
+        Tb += kappab * 2.0 * m * itau;
+        const double taub = -etaDb * itau * itau - kappab * kappa * itau;
         Tb += taub * dtau_dT;
         th[0] += taub * tau * dtau_dDsT;
     }
     const double Tmb = s.T > 0.0 ? Tb : 0.0;
-    // Bjorken redshift, then the boost to the LRF from the input momentum
+    // Bjorken redshift, then the boost to the LRF (ppar - v E), input qin
     if (PDIM == 3) qb[2 % PDIM] *= fbj;
-    if (fb.on) {
-        boost_reverse<PDIM>(qin, m, fb, -1, qb, &nxb, &nyb, &vb, &gamb);
+    if (on) {
+        const double deltab = qb[0] * nx + qb[1] * ny;
+        nxb += qb[0] * d1;
+        nyb += qb[1] * d1;
+        gamb += deltab * in1;
+        vb -= deltab * gam * E1;
+        const double Eb = -deltab * gam * v;
+        const double pparb = deltab * (gam - 1.0);
+        nxb += pparb * qin[0];
+        nyb += pparb * qin[1];
+        qb[0] += pparb * nx;
+        qb[1] += pparb * ny;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) qb[d] += Eb * iE1 * qin[d];
         // flow factors: gam(v), v = min(|v|, vmax), n = v_vec / |v|
-        vb += gamb * fb.gam * fb.gam * fb.gam * fb.v;
-        double vmagb = fb.vmag > vmax_() ? 0.0 : vb;
-        vmagb += -(nxb * s.vx + nyb * s.vy) / (fb.vmag * fb.vmag);
-        const double vxb = nxb / fb.vmag + vmagb * s.vx / fb.vmag;
-        const double vyb = nyb / fb.vmag + vmagb * s.vy / fb.vmag;
+        vb += gamb * gam * gam * gam * v;
+        double vmagb = vmag > vmax ? 0.0 : vb;
+        vmagb -= (nxb * s.vx + nyb * s.vy) * ivm * ivm;
+        const double vxb = nxb * ivm + vmagb * s.vx * ivm;
+        const double vyb = nyb * ivm + vmagb * s.vy * ivm;
         x0b += Tmb * s.dTx + vxb * s.dvxx + vyb * s.dvyx;
         x1b += Tmb * s.dTy + vxb * s.dvxy + vyb * s.dvyy;
     } else {
@@ -404,25 +446,34 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const double* __restrict__ 
     for (int r = 4; r < R; ++r) lam[(int64_t)r * N + i] = 0.0;
 }
 
-// One backward segment: re-run the segment's nk steps from its checkpoint while writing each
-// step's input state and kick noise to the tape (rows x0, x1, q..., xi..., ucol), then the
-// reverse steps from the last to the first.  lam carries the state adjoint between segments
-// (rows as the state), gacc the per-particle dL/dtheta (3 rows).
-template <int PDIM, int COLL>
-__global__ __launch_bounds__(kBlock) void k_adj_segment(StepArgs a, GridMedium g,
-                                                         const double* __restrict__ ckpt,
-                                                         int64_t seg_step0, int32_t nk,
-                                                         double* __restrict__ tape,
-                                                         double* __restrict__ lam,
-                                                         double* __restrict__ gacc,
-                                                         double dtau_dDsT) {
+// The options the adjoint supports are fixed at compile time (relativistic pre-point kick, no
+// position diffusion, the drag model DRAG); the Bjorken redshift stays a run-time choice.
+template <int DRAG>
+__device__ __forceinline__ lmh_params adjoint_params(const lmh_params& in) {
+    lmh_params p = in;
+    p.relativistic = 1;
+    p.integrator = LMH_PREPOINT;
+    p.position_diffusion = 0;
+    p.drag_model = DRAG;
+    return p;
+}
+
+// A backward segment, first half: re-run the segment's nk steps from its checkpoint, writing
+// each step's input state and kick noise to the tape (rows x0, x1, q..., xi...[, ucol with
+// collisions]).
+template <int PDIM, int COLL, int DRAG>
+__global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, GridMedium g,
+                                                                  const double* __restrict__ ckpt,
+                                                                  int64_t seg_step0, int32_t nk,
+                                                                  double* __restrict__ tape) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
     const SpeciesArgs sp = a.species[seg];
+    const lmh_params p = adjoint_params<DRAG>(a.p);
     const int64_t N = a.N;
     constexpr int R = 2 + PDIM;
-    constexpr int TR = R + PDIM + 1;
+    constexpr int TR = R + PDIM + (COLL == LMH_LANGEVIN_RTA ? 1 : 0);
     double x0 = ckpt[i], x1 = ckpt[N + i];
     double q[PDIM];
 #pragma unroll
@@ -439,24 +490,51 @@ __global__ __launch_bounds__(kBlock) void k_adj_segment(StepArgs a, GridMedium g
         noise_kick<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, xi, &ucol);
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) tk[(int64_t)(R + d) * N] = xi[d];
-        tk[(int64_t)(R + PDIM) * N] = ucol;
-        adj_forward_step<PDIM, COLL>(a, g, sp, i, step, &x0, &x1, q, xi, ucol);
+        if (COLL == LMH_LANGEVIN_RTA) tk[(int64_t)(R + PDIM) * N] = ucol;
+        // the last step's output is the next checkpoint: only its inputs go to the tape
+        if (k + 1 < nk) adj_forward_step<PDIM, COLL>(a, p, g, sp, i, step, &x0, &x1, q, xi, ucol);
     }
+}
+
+// Second half: the reverse steps over the tape from the last to the first.  lam carries the
+// state adjoint between segments (rows as the state), gacc the per-particle dL/dtheta (3 rows).
+template <int PDIM, int COLL, int DRAG>
+__global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g, int64_t seg_step0,
+                                                        int32_t nk, const double* __restrict__ tape,
+                                                        double* __restrict__ lam,
+                                                        double* __restrict__ gacc, double dtau_dDsT) {
+    int seg;
+    const int64_t i = particle_of_thread(a, &seg);
+    if (i < 0) return;
+    const SpeciesArgs sp = a.species[seg];
+    const lmh_params p = adjoint_params<DRAG>(a.p);
+    const int64_t N = a.N;
+    constexpr int R = 2 + PDIM;
+    constexpr int TR = R + PDIM + (COLL == LMH_LANGEVIN_RTA ? 1 : 0);
     double lm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) lm[r] = lam[(int64_t)r * N + i];
     double th[kAdjThetas] = {0.0, 0.0, 0.0};
+    // the tape row of step k - 1 is loaded while step k is reversed
+    double row[TR];
+    {
+        const double* tk = tape + (int64_t)(nk - 1) * TR * N + i;
+#pragma unroll
+        for (int r = 0; r < TR; ++r) row[r] = tk[(int64_t)r * N];
+    }
     for (int k = nk - 1; k >= 0; --k) {
         const int64_t step = seg_step0 + k;
-        const double* tk = tape + (int64_t)k * TR * N + i;
-        double qin[PDIM], xi[PDIM];
+        double cur[TR];
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) {
-            qin[d] = tk[(int64_t)(2 + d) * N];
-            xi[d] = tk[(int64_t)(R + d) * N];
+        for (int r = 0; r < TR; ++r) cur[r] = row[r];
+        if (k > 0) {
+            const double* tn = tape + (int64_t)(k - 1) * TR * N + i;
+#pragma unroll
+            for (int r = 0; r < TR; ++r) row[r] = tn[(int64_t)r * N];
         }
-        adj_reverse_step<PDIM, COLL>(a, g, sp, i, step, tk[0], tk[N], qin, xi,
-                                     tk[(int64_t)(R + PDIM) * N], dtau_dDsT, lm, th);
+        const double ucol = COLL == LMH_LANGEVIN_RTA ? cur[R + PDIM] : 1.0;
+        adj_reverse_step<PDIM, COLL>(a, p, g, sp, i, step, cur[0], cur[1], cur + 2, cur + R, ucol,
+                                     dtau_dDsT, lm, th);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) lam[(int64_t)r * N + i] = lm[r];

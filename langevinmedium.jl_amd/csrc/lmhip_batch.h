@@ -40,12 +40,11 @@ __device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& r
 // the fused step of k_cart_grid without deferral (Philox noise, the medium interpolated in tau
 // per corner), the same device functions in the same order
 template <int PDIM, int COLL>
-__device__ __forceinline__ void fused_grid_step(const StepArgs& a, const GridMedium& g,
+__device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
                                                 double* x0, double* x1, double* q) {
     constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
     constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
-    const lmh_params& p = a.p;
     const double m = sp.m;
     const double t_now = (double)step * p.dt + p.t0;
     double td;
@@ -72,12 +71,12 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const GridMed
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }
-    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, g, i, step, 0, j, td, m, x0, x1, q);
+    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
 }
 
 // PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
-// otherwise one block per tile.
-template <int PDIM, int COLL, bool PERSIST>
+// otherwise one block per tile.  SPEC as in k_cart_grid.
+template <int PDIM, int COLL, bool PERSIST, bool SPEC>
 __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
     const int64_t step0 = *b.step_dev + k_off;
     const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
@@ -88,6 +87,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, 
         if (i >= run.count) continue;
         StepArgs a;
         batch_view(b, run, r, &a);
+        const lmh_params p = step_params<SPEC>(a.p);
         const SpeciesArgs sp = run.sp;
         double x0 = a.pos[0][i], x1 = a.pos[1][i];
         double q[PDIM];
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, 
         for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
         int64_t bad = -1;
         for (int s = 0; s < nsteps; ++s) {
-            fused_grid_step<PDIM, COLL>(a, run.g, sp, i, step0 + s, &x0, &x1, q);
+            fused_grid_step<PDIM, COLL>(a, p, run.g, sp, i, step0 + s, &x0, &x1, q);
             if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
         }
         a.pos[0][i] = x0;

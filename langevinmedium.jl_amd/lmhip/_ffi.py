@@ -50,7 +50,8 @@ class lmh_params(ctypes.Structure):
 
 
 class lmh_tuning(ctypes.Structure):
-    _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("slice", c_int32), ("defer", c_int32)]
+    _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("slice", c_int32), ("defer", c_int32),
+                ("specialize", c_int32)]
 
 
 class lmh_profile(ctypes.Structure):

@@ -69,15 +69,15 @@ class Context:
         self._check(self._lib.lmh_synchronize(self._h))
 
     # ---- launch shape ----
-    def set_tuning(self, block=256, steps_per_launch=1, slice=0, defer=1):
-        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer))
+    def set_tuning(self, block=256, steps_per_launch=1, slice=0, defer=1, specialize=1):
+        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer), int(specialize))
         self._check(self._lib.lmh_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _ffi.lmh_tuning()
         self._check(self._lib.lmh_get_tuning(self._h, ctypes.byref(t)))
         return {"block": t.block, "steps_per_launch": t.steps_per_launch, "slice": t.slice,
-                "defer": t.defer}
+                "defer": t.defer, "specialize": t.specialize}
 
     # ---- medium ----
     def set_medium_radial(self, rgrid, tgrid, T, v, v2=None):

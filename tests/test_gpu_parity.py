@@ -287,3 +287,37 @@ def test_general_coords_driver_shapes(gpu_ok):
     assert len(tp) == len(mh) == len(ph) == 5
     assert mh[0].shape == (5000,) and ph[0].shape == (1, 5000)
     assert np.all(np.isfinite(mh[-1])) and np.all(mh[-1] >= 1.5 * (1 - 1e-6))
+
+
+@pytest.mark.parametrize("medium", ["grid", "radial"])
+def test_specialized_kernels_identical_to_generic(gpu_ok, medium):
+    """the kernels compiled for the common configuration give the generic kernels' bits"""
+    N = 5000
+    res = []
+    for spec in (0, 1):
+        with lmhip.Context(0) as ctx:
+            ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=1, specialize=spec)
+            if medium == "grid":
+                g_ = small_grid()
+                pos = media.hydro_positions(N, np.random.default_rng(77)) * 2.0
+                mom = media.pt_rapidity_momenta(N, 1.5, np.random.default_rng(78))
+                from helpers import tables, load_tables
+                drag, rt, jt = tables([1.5], g_.T, 0.6366, 3, True, media.tau_R_law(0.16))
+                ctx.set_medium_grid2p1(g_.x0, g_.dx, g_.nx, g_.y0, g_.dy, g_.ny, g_.tgrid, g_.T, g_.vx, g_.vy)
+                load_tables(ctx, drag, rt, jt)
+                coll = _ffi.LMH_LANGEVIN_RTA
+            else:
+                xg, tg, Tf, Vf, V2 = flowing_medium()
+                pos = np.asfortranarray(np.random.default_rng(79).normal(0, 3, (3, N)))
+                mom = np.asfortranarray(np.random.default_rng(80).normal(0, 2, (3, N)))
+                ctx.set_medium_radial(xg, tg, Tf, Vf, V2)
+                from helpers import tables, load_tables
+                drag, rt, jt = tables([1.5], Tf, 0.6366, 3)
+                load_tables(ctx, drag, rt, jt)
+                coll = _ffi.LMH_LANGEVIN
+            p = _ffi.make_params(dt=0.02, t0=0.6, DsT=0.6366, noise=_ffi.LMH_NOISE_PHILOX, seed=5,
+                                 collision=coll)
+            ctx.set_particles(pos, mom, [1.5])
+            ctx.run(p, 1, 20, None, 0)
+            res.append(ctx.get_particles())
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
