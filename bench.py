@@ -75,40 +75,52 @@ def make_params(DsT, noise_fixed=False, collision="langevin_rta"):
                             check_finite=False)
 
 
-def cpu_baseline(g, masses, tabs, DsT, budget_s=12.0, n=20000):
-    """The oracle (C restatement of the reference CPU kernels, one pass per kernel, 1 thread) on
-    a bounded sample of the same workload: n quarks of the same mix, the Philox noise record it
-    reads generated inside the timed loop, as the reference draws randn every step."""
+def cpu_baseline(g, masses, tabs, DsT, budget_s=12.0, n1=20000, nmt=200000):
+    """The oracle (C restatement of the reference CPU kernels, one pass per kernel) on a bounded
+    sample of the same workload: n quarks of the same mix, the Philox noise record it reads
+    generated inside the timed loop, as the reference draws randn every step.  Timed on one
+    thread (n1 quarks) and on all the host threads OpenMP is given (nmt quarks, the per-particle
+    loops split between threads), half the budget each; `value` is the all-thread figure."""
     import ctypes
     import numpy as np
     import _oracle as O
-    from lmhip import _ffi
-
-    rng = np.random.default_rng(99)
-    nc = int(round(0.7 * n))
-    counts = [nc, n - nc]
     from lmhip import media
-    pos = media.hydro_positions(n, rng)
-    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], masses[0], rng),
-                                       media.pt_rapidity_momenta(counts[1], masses[1], rng)]))
-    keep = O.Keep()
-    gm = O.grid_medium(keep, g)
-    sp = O.species(keep, masses, counts, drag=[t[0] for t in tabs], rta=[t[1] for t in tabs],
-                   jt=[(t[2][0], t[2][1], t[2][2], t[2][3], t[2][4]) for t in tabs])
-    p = make_params(DsT, noise_fixed=True)
-    C = O.lib().lmo_noise_channels(ctypes.byref(p), 2, 3)
-    rec = np.zeros((1, C, n))
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        O.lib().lmo_noise_record(ctypes.byref(p), 2, 3, n, 1 + steps, O.dp(rec[0]))
-        rc, _ = O.run_grid(p, gm, sp, pos, mom, 1 + steps, 1, rec)
-        assert rc == 0
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} quarks (70% c / 30% b) x {steps} steps of the same workload, oracle "
-                      f"(C restatement of kernels_cpu.jl, one pass per kernel), 1 thread, {dt:.1f} s"}
+
+    L = O.lib()
+
+    def timed(n, threads, budget):
+        rng = np.random.default_rng(99)
+        nc = int(round(0.7 * n))
+        counts = [nc, n - nc]
+        pos = media.hydro_positions(n, rng)
+        mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], masses[0], rng),
+                                           media.pt_rapidity_momenta(counts[1], masses[1], rng)]))
+        keep = O.Keep()
+        gm = O.grid_medium(keep, g)
+        sp = O.species(keep, masses, counts, drag=[t[0] for t in tabs], rta=[t[1] for t in tabs],
+                       jt=[(t[2][0], t[2][1], t[2][2], t[2][3], t[2][4]) for t in tabs])
+        p = make_params(DsT, noise_fixed=True)
+        C = L.lmo_noise_channels(ctypes.byref(p), 2, 3)
+        rec = np.zeros((1, C, n))
+        L.lmo_set_threads(threads)
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            L.lmo_noise_record(ctypes.byref(p), 2, 3, n, 1 + steps, O.dp(rec[0]))
+            rc, _ = O.run_grid(p, gm, sp, pos, mom, 1 + steps, 1, rec)
+            assert rc == 0
+            steps += 1
+        return n * steps / (time.perf_counter() - t0), steps
+
+    threads = max(1, int(L.lmo_get_threads()))
+    v1, s1 = timed(n1, 1, budget_s / 2)
+    vm, sm = timed(nmt, threads, budget_s / 2)
+    L.lmo_set_threads(threads)
+    return {"value": vm, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "one_core_value": v1,
+            "sample": f"oracle (C restatement of kernels_cpu.jl, one pass per kernel) on the same workload: "
+                      f"{nmt} quarks x {sm} steps on {threads} threads (OpenMP over particles) and "
+                      f"{n1} quarks x {s1} steps on 1 thread, {budget_s / 2:.0f} s each"}
 
 
 def bench_batch(args):

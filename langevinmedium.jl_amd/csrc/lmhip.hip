@@ -916,13 +916,22 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_partial(const double* __restr
         for (int k = 0; k < K; ++k) partial[(int64_t)blockIdx.x * K + k] = acc[k * kObsBlock];
 }
 
-__global__ __launch_bounds__(kObsBlock) void k_obs_final(const double* __restrict__ partial, int nblk,
-                                                          int K, double* __restrict__ out) {
-    for (int k = threadIdx.x; k < K; k += kObsBlock) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * K + k];
-        out[k] = s;
+// one block per observable: the block's threads sum strided partials, then a fixed tree
+constexpr int kObsFinalBlock = 256;
+
+__global__ __launch_bounds__(kObsFinalBlock) void k_obs_final(const double* __restrict__ partial, int nblk,
+                                                               int K, double* __restrict__ out) {
+    __shared__ double acc[kObsFinalBlock];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kObsFinalBlock) s += partial[(int64_t)b * K + k];
+    acc[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kObsFinalBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) acc[threadIdx.x] += acc[threadIdx.x + st];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[k] = acc[0];
 }
 
 #include "lmhip_adjoint.h"
@@ -1820,7 +1829,7 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     const double* py = px + c->N;
     size_t shmem = (size_t)K * kObsBlock * sizeof(double);
     hipLaunchKernelGGL(k_obs_partial, dim3(nblk), dim3(kObsBlock), shmem, c->stream, px, py, i0, i1, *s, c->d_obs);
-    hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(kObsBlock), 0, c->stream, c->d_obs, nblk, K, c->d_obs + (size_t)K * nblk);
+    hipLaunchKernelGGL(k_obs_final, dim3(K), dim3(kObsFinalBlock), 0, c->stream, c->d_obs, nblk, K, c->d_obs + (size_t)K * nblk);
     LMH_HIP(c, hipGetLastError());
     LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "liblmoracle.so")
 SOURCES = [os.path.join(HERE, "lm_oracle.c")]
 DEPS = SOURCES + [os.path.join(HERE, "lm_oracle.h"),
                   os.path.join(os.path.dirname(HERE), "include", "lmhip.h")]
-CFLAGS = ["-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math",
+CFLAGS = ["-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
           "-Wall", "-Wextra"]
 
 

@@ -19,6 +19,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -337,6 +338,11 @@ int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int
     return LMO_OK;
 }
 
+/* threads of the per-particle loops (OpenMP): the passes over particles are independent per
+ * particle, so the results do not depend on the thread count */
+void lmo_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t lmo_get_threads(void) { return (int32_t)omp_get_max_threads(); }
+
 /* ============================== Philox4x32-10 and the noise record ==========================
  * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, the Random123 reference generator):
  * multipliers 0xD2511F53 and 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85, ten
@@ -491,6 +497,7 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
                       double* rec) {
     int32_t oxi, orta, opos;
     noise_layout(p, dims, pdim, &oxi, &orta, &opos);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
         uint64_t w[4];
         double n[4];
@@ -632,6 +639,7 @@ static void kernel_bjorken(double* mom, int32_t pdim, int64_t step, double dt, d
     double tau_a = t0 + (double)(step - 1) * dt;
     if (!(tau_a > 0.0)) return;
     double f = tau_a / (tau_a + dt);
+    #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) mom[2 + pdim * i] *= f;
 }
 
@@ -710,6 +718,7 @@ static void forces_and_update(const lmh_params* p, const lmo_table* tab, double 
                               int32_t pdim, int32_t radial_mode, int64_t N,
                               double* det /* (pdim,N) */, double* sto) {
     double dt = p->dt;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
         double T = Tloc[i];
         double etaD, kT;
@@ -749,6 +758,7 @@ static void forces_and_update(const lmh_params* p, const lmo_table* tab, double 
         }
     }
     double sqrt_dt = sqrt(dt);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i)
         for (int32_t d = 0; d < pdim; ++d) {
             int64_t k = d + (int64_t)pdim * i;
@@ -764,6 +774,7 @@ static void rta_collision(const lmh_params* p, const lmo_table* taun, const lmo_
                           int64_t N, const double* ucol, const double* usamp,
                           const double* g /* (pdim,N) */) {
     double dt = p->dt;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
         double T = Tloc[i];
         double tau_n = (p->DsT > 0.0 && T > 0.0)
@@ -926,15 +937,17 @@ static void work_free(work_t* w) {
  * the RTA and diffusion kernels, read from the fixed-noise record instead. */
 static void gather_channels(const double* rec, int64_t Ntot, int64_t i0, int64_t n, int32_t c0,
                             int32_t rows, double* out) {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i)
         for (int32_t d = 0; d < rows; ++d)
             out[d + (int64_t)rows * i] = rec[(int64_t)(c0 + d) * Ntot + i0 + i];
 }
 
 static int all_finite(const double* a, int64_t n) {
-    for (int64_t k = 0; k < n; ++k)
-        if (!isfinite(a[k])) return 0;
-    return 1;
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (int64_t k = 0; k < n; ++k) ok = ok && isfinite(a[k]);
+    return ok;
 }
 
 static int64_t species_begin(const lmo_species* sp, int32_t k) {
@@ -1245,17 +1258,20 @@ static void step_grid_segment(const lmh_params* p, const lmo_grid_medium* g,
     }
     if (opos >= 0) gather_channels(rec, Ntot, i0, n, opos, dims, w->xpos);
     /* medium at each particle: aux = (T, vx, vy) */
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         double s3[3];
         lmo_grid_sample(g, pos[0 + dims * i], pos[1 + dims * i], t_now, s3);
         w->aux[3 * i] = s3[0]; w->aux[3 * i + 1] = s3[1]; w->aux[3 * i + 2] = s3[2];
     }
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i)
         grid_boost(mom + (int64_t)pdim * i, pdim, m, w->aux[3 * i + 1], w->aux[3 * i + 2], -1,
                    p->relativistic);
     if (p->bjorken_redshift) kernel_bjorken(mom, pdim, step, dt, t0, n);
     if (p->collision == LMH_FLUID) {
         /* kernels_cpu.jl:713-731 with r_hat -> v_hat; a fluid at rest sets p_perp to 0 */
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
             double vx = w->aux[3 * i + 1], vy = w->aux[3 * i + 2];
             double vmag = sqrt(vx * vx + vy * vy);
@@ -1270,17 +1286,20 @@ static void step_grid_segment(const lmh_params* p, const lmo_grid_medium* g,
             q[1] = m * gam * v * ny;
         }
     } else {
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) w->T[i] = jl_max(w->aux[3 * i], 0.0);
         if (p->collision == LMH_LANGEVIN || p->collision == LMH_LANGEVIN_RTA)
             forces_and_update(p, &sp->drag[k], m, w->T, mom, w->xi, pdim, 0, n, w->det, w->sto);
         if (p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA)
             rta_collision(p, &sp->rta[k], &sp->jt[k], w->T, mom, pdim, 0, n, w->ucol, w->usamp,
                           w->g);
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i)
             grid_boost(mom + (int64_t)pdim * i, pdim, m, w->aux[3 * i + 1], w->aux[3 * i + 2], +1,
                        p->relativistic);
     }
     /* push (kernels_cpu.jl:537-541, 586-607) */
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         double E2 = m * m;
         for (int32_t d = 0; d < pdim; ++d) E2 += mom[d + pdim * i] * mom[d + pdim * i];

@@ -63,6 +63,10 @@ typedef struct lmo_species {
     lmo_jtable jt[LMH_MAX_SPECIES];
 } lmo_species;
 
+/* ---- threads of the per-particle loops ------------------------------------------------------ */
+void lmo_set_threads(int32_t n);
+int32_t lmo_get_threads(void);
+
 /* ---- scalar building blocks ---------------------------------------------------------------- */
 double lmo_fmGeV(void);
 int64_t lmo_searchsortedlast(const double* v, int64_t n, double x);

@@ -72,6 +72,8 @@ def lib():
             "lmo_draw_juttner_rejection": (c_int, [c_double, c_double, c_int32, c_uint64, c_int64, PD]),
             "lmo_philox4x32_10": (None, [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]),
             "lmo_noise_block": (None, [c_uint64, c_uint64, c_int64, c_int64, c_int32, POINTER(c_uint64)]),
+            "lmo_set_threads": (None, [c_int32]),
+            "lmo_get_threads": (c_int32, []),
             "lmo_noise_channels": (c_int32, [P, c_int32, c_int32]),
             "lmo_noise_record": (None, [P, c_int32, c_int32, c_int64, c_int64, PD]),
             "lmo_kernel_boost_to_rest": (None, [PD, PD, POINTER(RadialMedium), c_double, c_int64, c_int64,
