@@ -103,6 +103,31 @@ def test_noise_record_box_muller():
         assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
 
 
+def test_noise_record_statistics():
+    """the record's normals are standard normal and its uniforms uniform (seeded, 4e5 draws per
+    channel; 5 sigma on the moments, Kolmogorov-Smirnov p > 1e-4), and distinct channels are
+    uncorrelated: a wrong word, shift or counter in the generator shows up here"""
+    from scipy import stats
+    p = _ffi.make_params(dt=0.01, seed=7, stream=2, noise=_ffi.LMH_NOISE_PHILOX,
+                         collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
+    rec = O.noise_record(p, 2, 3, 200000, 3, 2)
+    normals, uniforms = (0, 1, 2, 5, 6, 7, 8, 9), (3, 4)
+    for c in normals:
+        x = rec[:, c, :].ravel()
+        n = x.size
+        assert abs(x.mean()) < 5 * np.sqrt(1 / n)
+        assert abs(x.var() - 1) < 5 * np.sqrt(2 / n)
+        assert stats.kstest(x, "norm").pvalue > 1e-4
+    for c in uniforms:
+        x = rec[:, c, :].ravel()
+        assert 0.0 <= x.min() and x.max() < 1.0
+        assert stats.kstest(x, "uniform").pvalue > 1e-4
+    flat = rec.transpose(1, 0, 2).reshape(rec.shape[1], -1)
+    cc = np.corrcoef(flat)
+    off = cc[~np.eye(cc.shape[0], dtype=bool)]
+    assert np.max(np.abs(off)) < 5 / np.sqrt(flat.shape[1])
+
+
 def test_noise_transforms_accuracy():
     # the Box-Muller log and sincospi are built from correctly rounded operations only
     f = L.lmo_noise_transforms
