@@ -266,6 +266,21 @@ int lmh_profile_stop(lmh_ctx* ctx, lmh_profile* out);     /* synchronises */
 /* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */
 int lmh_observables(lmh_ctx* ctx, const lmh_obs_spec* spec, double* out);
 
+/* ---- on-the-fly freeze-out capture (kernels_gpu.jl:583-635, simulate_gpu.jl:53-75,256-265,
+ * 375-402; the GPU driver's freezeout_capture/freezeout_interp) ---------------------------- */
+/* Tfo > 0 enables the capture for the particles set (and for every later lmh_set_particles),
+ * with nothing latched; Tfo <= 0 disables it.  After each step of lmh_run, a particle not yet
+ * latched whose medium temperature at its new position and the step's time t is below Tfo is
+ * latched: its state interpolated linearly in T between the step's input and output state (the
+ * last state above Tfo; interp = 0 or no earlier T above Tfo books the output), the time
+ * interpolated the same way between t - dt and t, flag 1.  T is the step's medium
+ * interpolation (interpolate_2d_cpu semantics on the radial medium, the 2+1D grid's otherwise).
+ * Cartesian coordinates; the capture runs in the generic (not the specialised or deferred)
+ * kernels.                                                                                   */
+int lmh_freezeout_enable(lmh_ctx* ctx, double Tfo, int32_t interp);
+/* pos (dims, N), mom (pdim, N), tau (N), flag (N: 1 latched, 0 not yet)                       */
+int lmh_freezeout_get(lmh_ctx* ctx, double* pos, double* mom, double* tau, double* flag);
+
 /* ---- pathwise adjoint (extension; the reference has no differentiation, Project.toml:6-27) -- */
 /* The forward half of a ChainRulesCore.rrule / torch.autograd.Function: runs steps
  * step_begin .. step_begin+nsteps-1 exactly as lmh_run does (same final state, same noise) and

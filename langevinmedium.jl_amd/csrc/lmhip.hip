@@ -55,10 +55,48 @@ struct StepArgs {
     int32_t qblock;             // the step kernel's block size
     lmh_params p;
     double sqrt_dt;
+    // on-the-fly freeze-out capture (kernels_gpu.jl:583-635): the latched state of each particle
+    // (rows as the state), its time, a flag, and T after the particle's previous step
+    struct {
+        int32_t on, interp;
+        double Tfo;
+        double* pos;
+        double* mom;
+        double* tau;
+        double* flag;
+        double* lastT;
+    } fo;
 };
 
 // |v| <= sqrt(1 - 1e-12), kernels_cpu.jl:102
 __device__ __forceinline__ double vmax_() { return sqrt(1.0 - 1e-12); }
+
+// The freeze-out latch after a step (kernels_gpu.jl:583-635): T is the medium at the pushed
+// position and the step's time t.  Above Tfo, T is remembered; the first step below it books the
+// state interpolated linearly in T between the step's input state (the last state above Tfo)
+// and its output (frac = 1, the raw output, without interpolation or without a previous T above
+// Tfo).  tprev is the previous step's time.
+template <int DIMS, int PDIM>
+__device__ __forceinline__ void freeze_latch(const StepArgs& a, int64_t i, double T, double t,
+                                             double tprev, const double* xin, const double* qin,
+                                             const double* x, const double* q, bool* frozen,
+                                             double* lastT) {
+    const double Tfo = a.fo.Tfo;
+    if (!(T < Tfo)) {
+        *lastT = T;
+        return;
+    }
+    const double Tl = *lastT;
+    const double frac = (a.fo.interp && Tl > Tfo && Tl > T) ? (Tl - Tfo) / (Tl - T) : 1.0;
+    const int64_t N = a.N;
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) a.fo.pos[(int64_t)d * N + i] = xin[d] + frac * (x[d] - xin[d]);
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.fo.mom[(int64_t)d * N + i] = qin[d] + frac * (q[d] - qin[d]);
+    a.fo.tau[i] = tprev + frac * (t - tprev);
+    a.fo.flag[i] = 1.0;
+    *frozen = true;
+}
 
 // The common configuration -- relativistic, pre-point kick, drag table, no Bjorken redshift, no
 // position diffusion -- as compile-time constants: with SPEC the option branches it does not
@@ -255,12 +293,27 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
     for (int d = 0; d < DIMS; ++d) x[d] = a.pos[d][i];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    // freeze-out capture runs in the generic kernels only
+    const bool fo = !SPEC && a.fo.on;
+    bool frozen = false;
+    double lastT = 0.0;
+    if (fo) {
+        frozen = a.fo.flag[i] != 0.0;
+        lastT = a.fo.lastT[i];
+    }
     int64_t bad = -1;
     for (int s_off = 0; s_off < a.nsteps; ++s_off) {
         const int64_t step = a.step_begin + s_off;
         const double t_now = (double)step * p.dt + p.t0;
         double yd;
         const int j = axis_cell(med.tg, med.nt, t_now, &yd);
+        double xin[DIMS], qin[PDIM];
+        if (fo && !frozen) {
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) xin[d] = x[d];
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) qin[d] = q[d];
+        }
         double xi[PDIM];
         double ucol = 1.0;
         noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
@@ -398,12 +451,25 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                 }
             }
         }
+        if (fo && !frozen) {
+            // T at the pushed position: r from all position rows (|x| in radial mode)
+            double rr = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) rr += x[d] * x[d];
+            const double rf = RADIAL ? fabs(x[0]) : sqrt(rr);
+            double xdf;
+            const int irf = axis_cell(med.xg, med.nx, rf, &xdf);
+            const double Tf = bilinear(med.T, med.nx, irf, j, xdf, yd);
+            freeze_latch<DIMS, PDIM>(a, i, Tf, t_now, (double)(step - 1) * p.dt + p.t0, xin, qin, x, q,
+                                     &frozen, &lastT);
+        }
         if (bad < 0 && !(finite_all(x, DIMS) && finite_all(q, PDIM))) bad = step;
     }
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) a.pos[d][i] = x[d];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    if (fo) a.fo.lastT[i] = lastT;
     if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
 }
 
@@ -647,12 +713,27 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
     double q[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    // freeze-out capture runs in the generic, non-deferred kernels only
+    const bool fo = !SPEC && !DEFER && a.fo.on;
+    bool frozen = false;
+    double lastT = 0.0;
+    if (fo) {
+        frozen = a.fo.flag[i] != 0.0;
+        lastT = a.fo.lastT[i];
+    }
     int64_t bad = -1;
     for (int s_off = 0; s_off < a.nsteps; ++s_off) {
         const int64_t step = a.step_begin + s_off;
         const double t_now = (double)step * p.dt + p.t0;
         double td;
         const int j = axis_cell(g.tg, g.nt, t_now, &td);
+        double xin[2], qin[PDIM];
+        if (fo && !frozen) {
+            xin[0] = x0;
+            xin[1] = x1;
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d) qin[d] = q[d];
+        }
         double Tm, vx, vy;
         grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
         const FlowBoost fb = flow_boost(vx, vy);
@@ -689,12 +770,20 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
         }
         grid_push<PDIM, NOISE, SLICE>(a, p, g, i, step, s_off, j, td, m, &x0, &x1, q);
+        if (fo && !frozen) {
+            double Tf, ux, uy;
+            grid_sample<SLICE>(g, x0, x1, j, td, &Tf, &ux, &uy);
+            const double xo[2] = {x0, x1};
+            freeze_latch<2, PDIM>(a, i, Tf, t_now, (double)(step - 1) * p.dt + p.t0, xin, qin, xo, q,
+                                  &frozen, &lastT);
+        }
         if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
     }
     a.pos[0][i] = x0;
     a.pos[1][i] = x1;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    if (fo) a.fo.lastT[i] = lastT;
     if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
 }
 
@@ -992,6 +1081,12 @@ struct lmh_ctx {
     struct Mark { int cls; hipEvent_t a, b; };
     std::vector<Mark> marks;
 
+    // on-the-fly freeze-out capture: rows fo_pos (dims), fo_mom (pdim), tau, flag, last T
+    bool fo_on = false;
+    int32_t fo_interp = 1;
+    double fo_Tfo = 0.0;
+    double* d_fo = nullptr;
+
     // bumped by every entry point that changes the particles, the medium or the tables; an
     // adjoint backward pass needs the state its forward pass left
     uint64_t gen = 0;
@@ -1126,6 +1221,17 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     a.qblock = block;
     a.sqrt_dt = std::sqrt(p->dt);
     a.C = noise_layout(p, c->dims, c->pdim, &a.oxi, &a.orta, &a.opos);
+    if (c->fo_on && c->d_fo) {
+        const int64_t N = c->N;
+        a.fo.on = 1;
+        a.fo.interp = c->fo_interp;
+        a.fo.Tfo = c->fo_Tfo;
+        a.fo.pos = c->d_fo;
+        a.fo.mom = c->d_fo + (int64_t)c->dims * N;
+        a.fo.tau = c->d_fo + (int64_t)(c->dims + c->pdim) * N;
+        a.fo.flag = a.fo.tau + N;
+        a.fo.lastT = a.fo.flag + N;
+    }
     *nblocks = (unsigned)blk;
     return a;
 }
@@ -1178,6 +1284,21 @@ void host_time_cell(const std::vector<double>& tg, double t, int* j, double* td)
 }
 
 unsigned grid_blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// freeze-out arrays for the current particles, all zero (nothing latched, no previous T)
+int fo_reset(lmh_ctx* c) {
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_fo);
+    const size_t n = (size_t)(c->dims + c->pdim + 3) * (size_t)c->N;
+    if (hipMalloc((void**)&c->d_fo, n * sizeof(double)) != hipSuccess) {
+        c->d_fo = nullptr;
+        c->fo_on = false;
+        return c->fail(LMH_E_NOMEM, "freeze-out: device allocation failed");
+    }
+    LMH_HIP(c, hipMemsetAsync(c->d_fo, 0, n * sizeof(double), c->stream));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
 
 hipEvent_t next_event(lmh_ctx* c) {
     if (c->ev_used == c->ev_pool.size()) {
@@ -1257,6 +1378,7 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_ckpt);
     dfree(c->d_tape);
     dfree(c->d_adjw);
+    dfree(c->d_fo);
     for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
         dfree(c->drag[k].d);
         dfree(c->rta[k].d);
@@ -1456,6 +1578,10 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         hipMalloc((void**)&c->d_qcount, max_blocks * sizeof(int32_t)) != hipSuccess) {
         return c->fail(LMH_E_NOMEM, "set_particles: queue allocation failed");
     }
+    if (c->fo_on) {
+        int rc2 = fo_reset(c);
+        if (rc2) return rc2;
+    }
     return upload_species(c);
 }
 
@@ -1501,6 +1627,7 @@ int check_run(lmh_ctx* c, const lmh_params* p) {
         return c->fail(LMH_E_INVALID, "unknown drag model");
     if (p->coords == LMH_GENERAL_COORDS) {
         if (c->medium != 0) return c->fail(LMH_E_STATE, "general coordinates need the radial medium");
+        if (c->fo_on) return c->fail(LMH_E_INVALID, "freeze-out capture runs in Cartesian coordinates");
         if (c->dims != 2 || c->pdim != 2)
             return c->fail(LMH_E_INVALID, "general coordinates need dims == pdim == 2 (tau, r)");
         if (p->collision != LMH_LANGEVIN || p->drag_model != LMH_DRAG_TABLE ||
@@ -1540,7 +1667,7 @@ int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nst
 
 // the parameters are those the SPEC kernels fix at compile time
 bool spec_config(const lmh_ctx* c, const lmh_params* p) {
-    return c->tune.specialize && p->relativistic && p->integrator == LMH_PREPOINT &&
+    return c->tune.specialize && !c->fo_on && p->relativistic && p->integrator == LMH_PREPOINT &&
            p->drag_model == LMH_DRAG_TABLE && !p->bjorken_redshift && !p->position_diffusion;
 }
 
@@ -1622,7 +1749,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     StepArgs a = make_args(c, p, c->tune.block, &nb);
     if (nb == 0) return LMH_OK;
     const bool has_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
-    const bool defer = grid && k == 1 && has_rta && c->tune.defer;
+    const bool defer = grid && k == 1 && has_rta && c->tune.defer && !c->fo_on;
     const bool spec = spec_config(c, p);
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
@@ -1833,6 +1960,53 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     LMH_HIP(c, hipGetLastError());
     LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// on-the-fly freeze-out capture (kernels_gpu.jl:583-635, simulate_gpu.jl:53-75,375-402)
+extern "C" {
+
+int lmh_freezeout_enable(lmh_ctx* c, double Tfo, int32_t interp) {
+    if (!c) return LMH_E_INVALID;
+    if (!c->d_state) return c->fail(LMH_E_STATE, "freezeout_enable: set the particles first");
+    if (!std::isfinite(Tfo)) return c->fail(LMH_E_INVALID, "freezeout_enable: Tfo must be finite");
+    LMH_HIP(c, hipSetDevice(c->device));
+    if (!(Tfo > 0.0)) {
+        c->fo_on = false;
+        LMH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c->d_fo);
+        return LMH_OK;
+    }
+    c->fo_on = true;
+    c->fo_Tfo = Tfo;
+    c->fo_interp = interp ? 1 : 0;
+    return fo_reset(c);
+}
+
+int lmh_freezeout_get(lmh_ctx* c, double* pos, double* mom, double* tau, double* flag) {
+    if (!c) return LMH_E_INVALID;
+    if (!c->fo_on || !c->d_fo) return c->fail(LMH_E_STATE, "freezeout_get: freeze-out capture is not enabled");
+    if (!pos || !mom || !tau || !flag) return c->fail(LMH_E_INVALID, "freezeout_get: null output");
+    LMH_HIP(c, hipSetDevice(c->device));
+    const int64_t N = c->N;
+    const size_t rows = (size_t)(c->dims + c->pdim + 2);
+    std::vector<double> h;
+    try {
+        h.resize(rows * (size_t)N);
+    } catch (...) {
+        return c->fail(LMH_E_NOMEM, "freezeout_get: host allocation failed");
+    }
+    LMH_HIP(c, hipMemcpyAsync(h.data(), c->d_fo, rows * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < c->dims; ++d) pos[d + (int64_t)c->dims * i] = h[(size_t)d * N + i];
+        for (int d = 0; d < c->pdim; ++d) mom[d + (int64_t)c->pdim * i] = h[(size_t)(c->dims + d) * N + i];
+        tau[i] = h[(size_t)(c->dims + c->pdim) * N + i];
+        flag[i] = h[(size_t)(c->dims + c->pdim + 1) * N + i];
+    }
     return LMH_OK;
 }
 

@@ -188,6 +188,19 @@ class Context:
         self._check(self._lib.lmh_observables(self._h, ctypes.byref(spec), dptr(out)))
         return out
 
+    # ---- on-the-fly freeze-out capture ----
+    def freezeout_enable(self, Tfo: float, interp: bool = True):
+        self._check(self._lib.lmh_freezeout_enable(self._h, float(Tfo), int(bool(interp))))
+
+    def freezeout_get(self):
+        """(pos (dims, N), mom (pdim, N), tau (N), flag (N)) of the latched states"""
+        pos = np.empty((self.dims, self.N), dtype=np.float64, order="F")
+        mom = np.empty((self.pdim, self.N), dtype=np.float64, order="F")
+        tau = np.empty(self.N, dtype=np.float64)
+        flag = np.empty(self.N, dtype=np.float64)
+        self._check(self._lib.lmh_freezeout_get(self._h, dptr(pos), dptr(mom), dptr(tau), dptr(flag)))
+        return pos, mom, tau, flag
+
     # ---- pathwise adjoint ----
     def adjoint_forward(self, params, step_begin: int, nsteps: int, checkpoint_every: int = 50,
                         dtau_dDsT: float | None = None):

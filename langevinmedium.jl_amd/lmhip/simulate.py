@@ -173,9 +173,10 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
                         noise="philox", seed: int = 0, steps_per_launch: int = 0,
                         species_counts=None, drag_model="table", rta_time=None):
     coll, cm = _collision_code(collision_mode, momentum_langevin, DsT)
-    if freezeout_capture:
-        raise NotImplementedError("freezeout_capture is not available on the HIP backend yet")
     steps, save_every, num_saves = step_counts(initial_time, final_time, Δt, save_interval)
+    if freezeout_capture:
+        # simulate_gpu.jl:68-75: no trajectory history, one chunk of all the steps
+        num_saves, save_every = 1, max(steps, 1)
     xgrid, tgrid = (np.asarray(a, dtype=np.float64) for a in SpaceTimeGrid)
     Tf = np.asfortranarray(TemperatureEvolutionn, dtype=np.float64)
     Vf = np.asfortranarray(VelocityEvolutionn, dtype=np.float64)
@@ -240,6 +241,12 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
         C = ctx.noise_channels(params)
         params.noise, rec = _noise_mode(noise, steps, C, N)
         ctx.boost_to_lab(params, 0)                     # simulate_cpu.jl:160-163
+        if freezeout_capture:
+            # simulate_gpu.jl:375-402: the latched states are the result
+            ctx.freezeout_enable(Tfo, freezeout_interp)
+            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, lambda: None)
+            fpos, fmom, ftau, fflag = ctx.freezeout_get()
+            return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
         pos0, mom0 = ctx.get_particles()
         mom_hist, pos_hist = [mom0], [pos0]
 
@@ -318,13 +325,16 @@ def _simulate_grid(backend, medium: Grid2p1Medium, *, Δt: float = 0.01, initial
                    collision_mode="langevin", integrator_mode: int = 0, relativistic: bool = True,
                    bjorken_redshift: bool = False, position_diffusion: bool = False,
                    noise="philox", seed: int = 0, steps_per_launch: int = 0,
-                   drag_model="table", rta_time=None):
+                   drag_model="table", rta_time=None, freezeout_capture: bool = False,
+                   freezeout_interp: bool = True):
     """The 2+1D grid medium (extension): positions (2, N), momenta (2 or 3, N) in the LRF."""
     coll, cm = _collision_code(collision_mode, momentum_langevin, DsT)
     tg = np.asarray(medium.tgrid, dtype=np.float64)
     t0 = float(tg[0]) if initial_time is None else float(initial_time)
     t1 = float(tg[-1]) if final_time is None else float(final_time)
     steps, save_every, num_saves = step_counts(t0, t1, Δt, save_interval)
+    if freezeout_capture:
+        num_saves, save_every = 1, max(steps, 1)
     positions = np.array(x_init, dtype=np.float64, order="F", ndmin=2)
     momenta = np.array(p_init, dtype=np.float64, order="F", ndmin=2)
     N = positions.shape[1]
@@ -352,6 +362,11 @@ def _simulate_grid(backend, medium: Grid2p1Medium, *, Δt: float = 0.01, initial
         C = ctx.noise_channels(params)
         params.noise, rec = _noise_mode(noise, steps, C, N)
         ctx.boost_to_lab(params, 0)
+        if freezeout_capture:
+            ctx.freezeout_enable(Tfo, freezeout_interp)
+            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, lambda: None)
+            fpos, fmom, ftau, fflag = ctx.freezeout_get()
+            return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
         pos0, mom0 = ctx.get_particles()
         mom_hist, pos_hist = [mom0], [pos0]
 

@@ -1345,6 +1345,61 @@ int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_specie
     return rc;
 }
 
+/* ======================== on-the-fly freeze-out capture ====================================
+ * kernel_capture_freezeout_gpu! (kernels_gpu.jl:583-635), applied after a step: for each
+ * particle not yet latched, T at its position and tau = step*dt + t0; above Tfo the state, tau
+ * and T are cached in last_*; the first time below, the state and tau are interpolated linearly
+ * in T between the cache and the current state (frac = 1 without interp or without a cached T
+ * above Tfo) and the flag is set.  T comes from the step's interpolation: interpolate_2d_cpu on
+ * the radial medium (r = |x| in radial mode, else from all position rows), the grid sample on
+ * the 2+1D grid (exactly one of rmed, gmed is given).                                          */
+void lmo_freezeout_capture(const lmo_radial_medium* rmed, const lmo_grid_medium* gmed,
+                           const double* pos, const double* mom, int32_t dims, int32_t pdim,
+                           int64_t N, int64_t step, double dt, double t0, double Tfo,
+                           int32_t interp, double* fo_pos, double* fo_mom, double* fo_tau,
+                           double* fo_flag, double* last_pos, double* last_mom,
+                           double* last_tau, double* last_T) {
+    const double tau = (double)step * dt + t0;
+    for (int64_t i = 0; i < N; ++i) {
+        if (fo_flag[i] != 0.0) continue;
+        double T;
+        if (gmed) {
+            double s3[3];
+            lmo_grid_sample(gmed, pos[0 + dims * i], pos[1 + dims * i], tau, s3);
+            T = s3[0];
+        } else {
+            double r;
+            if (dims == 1) {
+                r = fabs(pos[i]);
+            } else {
+                double r2 = 0.0;
+                for (int32_t d = 0; d < dims; ++d) r2 += pos[d + dims * i] * pos[d + dims * i];
+                r = sqrt(r2);
+            }
+            T = lmo_interpolate_2d(rmed->xgrid, rmed->nx, rmed->tgrid, rmed->nt, rmed->T, r, tau);
+        }
+        if (T < Tfo) {
+            double Tl = last_T[i];
+            double frac = (interp && Tl > Tfo && Tl > T) ? (Tl - Tfo) / (Tl - T) : 1.0;
+            for (int32_t d = 0; d < dims; ++d) {
+                int64_t k = d + (int64_t)dims * i;
+                fo_pos[k] = last_pos[k] + frac * (pos[k] - last_pos[k]);
+            }
+            for (int32_t d = 0; d < pdim; ++d) {
+                int64_t k = d + (int64_t)pdim * i;
+                fo_mom[k] = last_mom[k] + frac * (mom[k] - last_mom[k]);
+            }
+            fo_tau[i] = last_tau[i] + frac * (tau - last_tau[i]);
+            fo_flag[i] = 1.0;
+        } else {
+            for (int32_t d = 0; d < dims; ++d) last_pos[d + (int64_t)dims * i] = pos[d + (int64_t)dims * i];
+            for (int32_t d = 0; d < pdim; ++d) last_mom[d + (int64_t)pdim * i] = mom[d + (int64_t)pdim * i];
+            last_tau[i] = tau;
+            last_T[i] = T;
+        }
+    }
+}
+
 /* ================================ observables (extension) ===================================
  * over particles [i0, i1) of a (pdim, N) momentum matrix, see lmh_obs_spec in lmhip.h      */
 int lmo_observables(const double* mom, int32_t pdim, int64_t i0, int64_t i1,

@@ -129,6 +129,14 @@ int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_specie
                  int32_t pdim, double* pos, double* mom, int64_t N, int64_t step_begin,
                  int64_t nsteps, const double* noise, int64_t* bad_step);
 
+/* ---- on-the-fly freeze-out capture ------------------------------------------------------------ */
+void lmo_freezeout_capture(const lmo_radial_medium* rmed, const lmo_grid_medium* gmed,
+                           const double* pos, const double* mom, int32_t dims, int32_t pdim,
+                           int64_t N, int64_t step, double dt, double t0, double Tfo,
+                           int32_t interp, double* fo_pos, double* fo_mom, double* fo_tau,
+                           double* fo_flag, double* last_pos, double* last_mom,
+                           double* last_tau, double* last_T);
+
 /* ---- observables ----------------------------------------------------------------------------- */
 int lmo_observables(const double* mom, int32_t pdim, int64_t i0, int64_t i1,
                     const lmh_obs_spec* spec, double* out);

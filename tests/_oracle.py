@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_uint32, c_uint64
+from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -73,6 +73,9 @@ def lib():
             "lmo_philox4x32_10": (None, [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]),
             "lmo_noise_block": (None, [c_uint64, c_uint64, c_int64, c_int64, c_int32, POINTER(c_uint64)]),
             "lmo_set_threads": (None, [c_int32]),
+            "lmo_freezeout_capture": (None, [c_void_p, c_void_p, PD, PD, c_int32, c_int32, c_int64, c_int64,
+                                             c_double, c_double, c_double, c_int32, PD, PD, PD, PD,
+                                             PD, PD, PD, PD]),
             "lmo_get_threads": (c_int32, []),
             "lmo_noise_channels": (c_int32, [P, c_int32, c_int32]),
             "lmo_noise_record": (None, [P, c_int32, c_int32, c_int64, c_int64, PD]),
@@ -203,3 +206,23 @@ def observables(mom, i0, i1, nbins, pt_min, pt_max, sigma):
     rc = lib().lmo_observables(dp(m), m.shape[0], i0, i1, ctypes.byref(spec), dp(out))
     assert rc == 0
     return out
+
+
+class Freezeout:
+    """the capture arrays of lmo_freezeout_capture, all zero (the GPU driver's CUDA.zeros)"""
+
+    def __init__(self, dims, pdim, N):
+        self.dims, self.pdim, self.N = dims, pdim, N
+        z = lambda r: np.zeros((r, N), order="F")
+        self.fo_pos, self.fo_mom = z(dims), z(pdim)
+        self.fo_tau, self.fo_flag = np.zeros(N), np.zeros(N)
+        self.last_pos, self.last_mom = z(dims), z(pdim)
+        self.last_tau, self.last_T = np.zeros(N), np.zeros(N)
+
+    def capture(self, rmed, gmed, pos, mom, step, dt, t0, Tfo, interp):
+        lib().lmo_freezeout_capture(ctypes.byref(rmed) if rmed is not None else None,
+                                    ctypes.byref(gmed) if gmed is not None else None,
+                                    dp(pos), dp(mom), self.dims, self.pdim, self.N, step, dt, t0, Tfo,
+                                    int(bool(interp)), dp(self.fo_pos), dp(self.fo_mom), dp(self.fo_tau),
+                                    dp(self.fo_flag), dp(self.last_pos), dp(self.last_mom),
+                                    dp(self.last_tau), dp(self.last_T))

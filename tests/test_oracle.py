@@ -14,7 +14,8 @@ import pytest
 from scipy.special import kve
 
 import _oracle as O
-from lmhip import _ffi
+import lmhip
+from lmhip import _ffi, media
 
 mp.mp.dps = 50
 L = O.lib()
@@ -665,3 +666,43 @@ def test_observables_vs_numpy():
     assert sm[5:].sum() == pytest.approx(np.sum(norm.cdf((8 - pt) / 0.3) - norm.cdf(-pt / 0.3)), rel=1e-10)
     # and each bin is the Gaussian mass of the bin around every particle
     assert sm[5 + 7] == pytest.approx(np.sum(norm.cdf((1.6 - pt) / 0.3) - norm.cdf((1.4 - pt) / 0.3)), rel=1e-9)
+
+
+# ---------------------------------------------------------------- freeze-out capture ----------
+def _bjorken_capture(interp, Tfo=0.2, N=300, dt=0.01, T0=0.4, tau0=0.6):
+    xg, tg, Tf, Vf, _ = media.bjorken_bath(T0=T0, tau0=tau0, Tstop=0.15, ntau=4096)
+    keep = O.Keep()
+    med = O.radial_medium(keep, xg, tg, Tf, Vf)
+    drag = lmhip.build_tau_drag_spline(1.5, 0.6366, Tmin=float(Tf.min()), Tmax=float(Tf.max()), nT=1024)
+    sp = O.species(keep, [1.5], [N], drag=[drag])
+    p = _ffi.make_params(dt=dt, t0=tau0, DsT=0.6366, noise=_ffi.LMH_NOISE_FIXED)
+    rng = np.random.default_rng(4)
+    pos = np.asfortranarray(rng.normal(0, 2, (3, N)))
+    mom = np.asfortranarray(rng.normal(0, 1, (3, N)))
+    fo = O.Freezeout(3, 3, N)
+    steps = int((tau0 * (T0 / Tfo) ** 3 - tau0) / dt) + 20
+    hist = []
+    for s in range(1, steps + 1):
+        xi = np.ascontiguousarray(rng.normal(size=(1, 3, N)))
+        hist.append((pos.copy(), mom.copy()))
+        rc, _ = O.run_cartesian(p, med, sp, pos, mom, s, 1, xi)
+        assert rc == 0
+        fo.capture(med, None, pos, mom, s, dt, tau0, Tfo, interp)
+    hist.append((pos.copy(), mom.copy()))
+    return fo, hist, tau0 * (T0 / Tfo) ** 3, dt, tau0
+
+
+def test_freezeout_bjorken_crossing_time():
+    """T(tau) = T0 (tau0/tau)^(1/3) at every r: every quark latches at tau* = tau0 (T0/Tfo)^3,
+    to O(dt^2) with the interpolation in T and within one step after tau* without it"""
+    fo, hist, tstar, dt, tau0 = _bjorken_capture(True)
+    assert np.all(fo.fo_flag == 1.0)
+    assert np.max(np.abs(fo.fo_tau - tstar)) < 1e-5
+    fo0, _, _, _, _ = _bjorken_capture(False)
+    lag = fo0.fo_tau - tstar
+    assert np.all(lag > 0.0) and np.all(lag <= dt + 1e-12)
+    # the latched momentum lies between the states before and after the crossing step
+    s = int(round((fo0.fo_tau[0] - tau0) / dt))
+    lo, hi = hist[s - 1][1], hist[s][1]
+    assert np.all((fo.fo_mom - np.minimum(lo, hi)) >= -1e-12) and np.all((np.maximum(lo, hi) - fo.fo_mom) >= -1e-12)
+    assert np.allclose(fo0.fo_mom, hi, rtol=1e-15, atol=1e-15)   # last + 1 * (current - last)
