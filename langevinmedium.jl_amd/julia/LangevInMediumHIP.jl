@@ -16,8 +16,10 @@ import LangevInMedium.Simulate: simulate_ensemble_bulk
 using LangevInMedium.Backends: AbstractBackend
 using LangevInMedium.Transport: build_tau_drag_spline, build_taun_current_spline, build_juttner_invcdf
 using LangevInMedium.Utils: check_momentum_dims, append_thermal_pz, sample_initial_particles_milne!
+import ChainRulesCore
+using ChainRulesCore: NoTangent
 
-export HIPBackend, HIP_GCBackend
+export HIPBackend, HIP_GCBackend, GridEnsemble, final_observables
 
 const liblmhip = get(ENV, "LMHIP_LIB", joinpath(@__DIR__, "..", "liblmhip.so"))
 
@@ -100,7 +102,6 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
         seed::Integer = 0, device::Integer = 0)
     (collision_mode == :langevin || collision_mode == :rta) ||
         error("simulate_ensemble_bulk(::HIPBackend): collision_mode=$(collision_mode) is not supported on GPU (only :langevin and :rta).")
-    freezeout_capture && error("freezeout_capture is not available on the HIP backend")
     (x_init === nothing || p_init === nothing) && error("pass x_init and p_init (host samplers run before the backend)")
     dimensions == 1 && error("radial mode: reduce x_init/p_init to (1, N) as simulate_cpu.jl:102-136 does")
     steps = floor(Int, (final_time - initial_time) / Δt)
@@ -138,6 +139,15 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
                   LMH_NOISE_PHILOX, Int32(integrator_mode), 0, relativistic, bjorken_redshift,
                   position_diffusion, reflecting_boundary, 1)
     boost_to_lab!(c, p, 0)
+    if freezeout_capture
+        # simulate_gpu.jl:68-75,375-402: no history; the latched states are the result
+        check(c, ccall((:lmh_freezeout_enable, liblmhip), Cint, (Ptr{Cvoid}, Float64, Int32), c.h, Tfo, freezeout_interp))
+        run!(c, p, 1, steps)
+        fpos = zeros(dimensions, N); fmom = zeros(pdim, N); ftau = zeros(N); fflag = zeros(N)
+        check(c, ccall((:lmh_freezeout_get, liblmhip), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), c.h, fpos, fmom, ftau, fflag))
+        return (pos = fpos, mom = fmom, tau = ftau, flag = fflag)
+    end
     pos0, mom0 = get_particles(c, dimensions, pdim, N)
     pos_hist = [pos0]; mom_hist = [mom0]
     step = 1
@@ -192,6 +202,79 @@ function simulate_ensemble_bulk(::HIP_GCBackend, T_profile_MIS, ur_profile_MIS, 
         end
     end
     return range(initial_time, final_time, length = num_saves + 1), mom_hist, pos_hist
+end
+
+# ---- differentiable final-state observables: the pathwise adjoint behind a ChainRulesCore.rrule --
+# theta = (DsT, gamma0, Lambda).  The ensemble fixes the 2+1D medium, the initial state, the step
+# count and the options; final_observables returns lmh_observables of the final lab momenta.
+struct LmhObsSpec
+    nbins::Int32; species::Int32; pt_min::Float64; pt_max::Float64; sigma::Float64
+end
+struct LmhAdjointSpec
+    checkpoint_every::Int32; _pad::Int32; dtau_dDsT::Float64
+end
+
+struct GridEnsemble
+    c::Ctx
+    x0::Float64; dx::Float64; nx::Int; y0::Float64; dy::Float64; ny::Int
+    tgrid::Vector{Float64}; T::Array{Float64,3}; vx::Array{Float64,3}; vy::Array{Float64,3}
+    pos::Matrix{Float64}; mom::Matrix{Float64}; m::Float64
+    dt::Float64; t0::Float64; nsteps::Int; drag_model::Int32; seed::UInt64
+    checkpoint_every::Int; obs::LmhObsSpec
+end
+
+function GridEnsemble(x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, pos, mom, m; dt, t0, nsteps,
+                      drag_model = 0, seed = 0, checkpoint_every = 50, nbins = 40, pt_min = 0.0,
+                      pt_max = 10.0, sigma = 0.0, device = 0)
+    c = Ctx(device)
+    GC.@preserve tgrid T vx vy check(c, ccall((:lmh_set_medium_grid2p1, liblmhip), Cint,
+        (Ptr{Cvoid}, Float64, Float64, Int32, Float64, Float64, Int32, Ptr{Float64}, Int32,
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        c.h, x0, dx, nx, y0, dy, ny, tgrid, length(tgrid), T, vx, vy))
+    return GridEnsemble(c, x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, Matrix{Float64}(pos),
+                        Matrix{Float64}(mom), m, dt, t0, nsteps, Int32(drag_model), UInt64(seed),
+                        checkpoint_every, LmhObsSpec(nbins, -1, pt_min, pt_max, sigma))
+end
+
+function _reset!(e::GridEnsemble, theta)
+    DsT, g0, Lam = theta
+    Tmin = max(minimum(e.T), 0.0); Tmax = max(maximum(e.T), Tmin + eps(Float64))
+    e.drag_model == 0 && set_table!(e.c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(e.m, DsT; Tmin = Tmin, Tmax = Tmax, nT = 1024)...)
+    set_particles!(e.c, e.pos, e.mom, e.m)
+    p = LmhParams(e.dt, e.t0, DsT, 0.0, g0, Lam, e.seed, 0, LMH_CARTESIAN, LMH_LANGEVIN,
+                  LMH_NOISE_PHILOX, 0, e.drag_model, 1, 0, 0, 0, 0)
+    boost_to_lab!(e.c, p, 0)
+    return p
+end
+
+function _observables(e::GridEnsemble)
+    out = zeros(5 + e.obs.nbins)
+    check(e.c, ccall((:lmh_observables, liblmhip), Cint, (Ptr{Cvoid}, Ref{LmhObsSpec}, Ptr{Float64}),
+                     e.c.h, e.obs, out))
+    return out
+end
+
+function final_observables(theta::AbstractVector{<:Real}, e::GridEnsemble)
+    p = _reset!(e, theta)
+    run!(e.c, p, 1, e.nsteps)
+    return _observables(e)
+end
+
+function ChainRulesCore.rrule(::typeof(final_observables), theta::AbstractVector{<:Real}, e::GridEnsemble)
+    p = _reset!(e, theta)
+    spec = LmhAdjointSpec(e.checkpoint_every, 0, theta[1] > 0 ? 1.0 / theta[1] : 0.0)
+    check(e.c, ccall((:lmh_adjoint_forward, liblmhip), Cint,
+        (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ref{LmhAdjointSpec}), e.c.h, p, 1, e.nsteps, spec))
+    obs = _observables(e)
+    function final_observables_pullback(dobs)
+        w = collect(Float64, ChainRulesCore.unthunk(dobs))
+        g = zeros(3)
+        check(e.c, ccall((:lmh_adjoint_backward, liblmhip), Cint,
+            (Ptr{Cvoid}, Ref{LmhObsSpec}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            e.c.h, e.obs, w, g, C_NULL, C_NULL))
+        return NoTangent(), g, NoTangent()
+    end
+    return obs, final_observables_pullback
 end
 
 end # module
