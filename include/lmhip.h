@@ -266,6 +266,14 @@ int lmh_profile_stop(lmh_ctx* ctx, lmh_profile* out);     /* synchronises */
 /* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */
 int lmh_observables(lmh_ctx* ctx, const lmh_obs_spec* spec, double* out);
 
+/* ---- snapshots (the saved states of simulate_cpu.jl:305-316, simulate_gpu.jl:385-410) ------ */
+/* Takes the current state in the order of the context's stream (after every step enqueued so
+ * far) and copies it to pinned host memory on a second stream while later steps run; returns
+ * immediately with the snapshot's index.  lmh_snapshot_get waits for that copy, writes pos
+ * (dims, N) and mom (pdim, N) and frees the index for reuse.                                  */
+int lmh_snapshot(lmh_ctx* ctx, int32_t* index);
+int lmh_snapshot_get(lmh_ctx* ctx, int32_t index, double* pos, double* mom);
+
 /* ---- on-the-fly freeze-out capture (kernels_gpu.jl:583-635, simulate_gpu.jl:53-75,256-265,
  * 375-402; the GPU driver's freezeout_capture/freezeout_interp) ---------------------------- */
 /* Tfo > 0 enables the capture for the particles set (and for every later lmh_set_particles),

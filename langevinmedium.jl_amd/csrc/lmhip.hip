@@ -1081,6 +1081,19 @@ struct lmh_ctx {
     struct Mark { int cls; hipEvent_t a, b; };
     std::vector<Mark> marks;
 
+    // snapshots: a device copy of the state taken on the step stream, then copied to pinned
+    // host memory on the copy stream while the next steps run
+    struct Snap {
+        double* d = nullptr;          // device copy of the state
+        double* h = nullptr;          // pinned host copy
+        size_t n = 0;                 // doubles
+        hipEvent_t taken = nullptr;   // the device copy is complete (step stream)
+        hipEvent_t done = nullptr;    // the host copy is complete (copy stream)
+        bool busy = false;
+    };
+    std::vector<Snap> snaps;
+    hipStream_t copy_stream = nullptr;
+
     // on-the-fly freeze-out capture: rows fo_pos (dims), fo_mom (pdim), tau, flag, last T
     bool fo_on = false;
     int32_t fo_interp = 1;
@@ -1393,6 +1406,16 @@ void lmh_destroy(lmh_ctx* c) {
     if (c->tape_stream) {
         (void)hipStreamSynchronize(c->tape_stream);
         (void)hipStreamDestroy(c->tape_stream);
+    }
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    for (auto& sn : c->snaps) {
+        dfree(sn.d);
+        if (sn.h) (void)hipHostFree(sn.h);
+        if (sn.taken) (void)hipEventDestroy(sn.taken);
+        if (sn.done) (void)hipEventDestroy(sn.done);
     }
     for (int k = 0; k < 2; ++k) {
         if (c->ev_tape[k]) (void)hipEventDestroy(c->ev_tape[k]);
@@ -1960,6 +1983,70 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     LMH_HIP(c, hipGetLastError());
     LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
+    return LMH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// snapshots (simulate_cpu.jl:305-316, simulate_gpu.jl:385-410: the saved states of the history)
+extern "C" {
+
+int lmh_snapshot(lmh_ctx* c, int32_t* index) {
+    if (!c) return LMH_E_INVALID;
+    if (!index) return c->fail(LMH_E_INVALID, "snapshot: null index");
+    if (!c->d_state) return c->fail(LMH_E_STATE, "snapshot: no particles set");
+    LMH_HIP(c, hipSetDevice(c->device));
+    if (!c->copy_stream) LMH_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    const size_t n = (size_t)(c->dims + c->pdim) * (size_t)c->N;
+    int k = -1;
+    for (size_t j = 0; j < c->snaps.size(); ++j)
+        if (!c->snaps[j].busy && c->snaps[j].n == n) { k = (int)j; break; }
+    if (k < 0) {
+        lmh_ctx::Snap sn;
+        sn.n = n;
+        if (hipMalloc((void**)&sn.d, n * sizeof(double)) != hipSuccess) return c->fail(LMH_E_NOMEM, "snapshot: device allocation failed");
+        if (hipHostMalloc((void**)&sn.h, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            dfree(sn.d);
+            return c->fail(LMH_E_NOMEM, "snapshot: pinned host allocation failed");
+        }
+        if (hipEventCreateWithFlags(&sn.taken, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sn.done, hipEventDisableTiming) != hipSuccess) {
+            dfree(sn.d);
+            (void)hipHostFree(sn.h);
+            return c->fail(LMH_E_HIP, "snapshot: event creation failed");
+        }
+        c->snaps.push_back(sn);
+        k = (int)c->snaps.size() - 1;
+    }
+    lmh_ctx::Snap& sn = c->snaps[k];
+    LMH_HIP(c, hipMemcpyAsync(sn.d, c->d_state, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    LMH_HIP(c, hipEventRecord(sn.taken, c->stream));
+    LMH_HIP(c, hipStreamWaitEvent(c->copy_stream, sn.taken, 0));
+    LMH_HIP(c, hipMemcpyAsync(sn.h, sn.d, n * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+    LMH_HIP(c, hipEventRecord(sn.done, c->copy_stream));
+    sn.busy = true;
+    *index = k;
+    return LMH_OK;
+}
+
+int lmh_snapshot_get(lmh_ctx* c, int32_t index, double* pos, double* mom) {
+    if (!c) return LMH_E_INVALID;
+    if (index < 0 || index >= (int32_t)c->snaps.size() || !c->snaps[index].busy)
+        return c->fail(LMH_E_INVALID, "snapshot_get: no such snapshot");
+    if (!pos || !mom) return c->fail(LMH_E_INVALID, "snapshot_get: null output");
+    lmh_ctx::Snap& sn = c->snaps[index];
+    const size_t rows = (size_t)(c->dims + c->pdim);
+    if (sn.n != rows * (size_t)c->N) return c->fail(LMH_E_STATE, "snapshot_get: the particles changed since the snapshot");
+    LMH_HIP(c, hipSetDevice(c->device));
+    LMH_HIP(c, hipEventSynchronize(sn.done));
+    const int64_t N = c->N;
+    const double* h = sn.h;
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < c->dims; ++d) pos[d + (int64_t)c->dims * i] = h[(size_t)d * N + i];
+        for (int d = 0; d < c->pdim; ++d) mom[d + (int64_t)c->pdim * i] = h[(size_t)(c->dims + d) * N + i];
+    }
+    sn.busy = false;
     return LMH_OK;
 }
 

@@ -117,6 +117,8 @@ SIGNATURES = {
     "lmh_observables": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD]),
     "lmh_profile_start": (c_int, [c_void_p]),
     "lmh_profile_stop": (c_int, [c_void_p, POINTER(lmh_profile)]),
+    "lmh_snapshot": (c_int, [c_void_p, POINTER(c_int32)]),
+    "lmh_snapshot_get": (c_int, [c_void_p, c_int32, _PD, _PD]),
     "lmh_freezeout_enable": (c_int, [c_void_p, c_double, c_int32]),
     "lmh_freezeout_get": (c_int, [c_void_p, _PD, _PD, _PD, _PD]),
     "lmh_adjoint_forward": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64,

@@ -188,6 +188,19 @@ class Context:
         self._check(self._lib.lmh_observables(self._h, ctypes.byref(spec), dptr(out)))
         return out
 
+    # ---- snapshots ----
+    def snapshot(self) -> int:
+        """take the state after every step enqueued so far; the host copy overlaps later steps"""
+        k = ctypes.c_int32(-1)
+        self._check(self._lib.lmh_snapshot(self._h, ctypes.byref(k)))
+        return k.value
+
+    def snapshot_get(self, index: int):
+        pos = np.empty((self.dims, self.N), dtype=np.float64, order="F")
+        mom = np.empty((self.pdim, self.N), dtype=np.float64, order="F")
+        self._check(self._lib.lmh_snapshot_get(self._h, int(index), dptr(pos), dptr(mom)))
+        return pos, mom
+
     # ---- on-the-fly freeze-out capture ----
     def freezeout_enable(self, Tfo: float, interp: bool = True):
         self._check(self._lib.lmh_freezeout_enable(self._h, float(Tfo), int(bool(interp))))

@@ -124,10 +124,13 @@ def _load_tables(ctx, masses, Tfield, *, DsT, momentum_langevin, cm, pdim, dst_k
                 ctx.set_juttner_table(k, np.zeros(4), 2, 2, 0.0, 1.0)
 
 
-def _run_chunks(ctx, params, steps, save_every, num_saves, noise, steps_per_launch, snapshot):
-    """simulate_cpu.jl:215-316: run the steps, snapshot at step % save_every == 0."""
+def _run_chunks(ctx, params, steps, save_every, num_saves, noise, steps_per_launch, on_snapshot):
+    """simulate_cpu.jl:215-316: run the steps, snapshot at step % save_every == 0.  A snapshot is
+    taken on the device after its chunk and copied to the host while the next chunks run; the
+    host side (on_snapshot(pos, mom), in order) trails by up to two snapshots."""
     fixed = params.noise == _ffi.LMH_NOISE_FIXED
-    hist = []
+    pending = []
+    taken = 0
     step = 1
     while step <= steps:
         n = save_every if step + save_every - 1 <= steps else steps - step + 1
@@ -139,9 +142,13 @@ def _run_chunks(ctx, params, steps, save_every, num_saves, noise, steps_per_laun
                 raise RuntimeError(str(e)) from None
             raise
         step += n
-        if n == save_every and (step - 1) % save_every == 0 and len(hist) < num_saves:
-            hist.append(snapshot())
-    return hist
+        if on_snapshot is not None and n == save_every and (step - 1) % save_every == 0 and taken < num_saves:
+            pending.append(ctx.snapshot())
+            taken += 1
+            if len(pending) > 2:
+                on_snapshot(*ctx.snapshot_get(pending.pop(0)))
+    for k in pending:
+        on_snapshot(*ctx.snapshot_get(k))
 
 
 def _noise_mode(noise, steps, C, N):
@@ -244,17 +251,15 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
         if freezeout_capture:
             # simulate_gpu.jl:375-402: the latched states are the result
             ctx.freezeout_enable(Tfo, freezeout_interp)
-            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, lambda: None)
+            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, None)
             fpos, fmom, ftau, fflag = ctx.freezeout_get()
             return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
         pos0, mom0 = ctx.get_particles()
         mom_hist, pos_hist = [mom0], [pos0]
 
-        def snap():
-            p_, m_ = ctx.get_particles()
+        def snap(p_, m_):
             pos_hist.append(p_)
             mom_hist.append(m_)
-            return None
 
         _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, snap)
     finally:
@@ -303,11 +308,9 @@ def _simulate_gc(backend, T_profile_MIS, ur_profile_MIS, mu_profile_MIS, Tempera
         mom_hist = [np.abs(momenta[1, :]).copy()]          # simulate_cpu_general_coords.jl:53
         pos_hist = [positions[1:2, :].copy()]
 
-        def snap():
-            p_, m_ = ctx.get_particles()
+        def snap(p_, m_):
             mom_hist.append(np.sqrt(m_[0] ** 2 + m_[1] ** 2))   # :120-122
             pos_hist.append(p_[1:2, :].copy())
-            return None
 
         _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, snap)
     finally:
@@ -364,17 +367,15 @@ def _simulate_grid(backend, medium: Grid2p1Medium, *, Δt: float = 0.01, initial
         ctx.boost_to_lab(params, 0)
         if freezeout_capture:
             ctx.freezeout_enable(Tfo, freezeout_interp)
-            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, lambda: None)
+            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, None)
             fpos, fmom, ftau, fflag = ctx.freezeout_get()
             return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
         pos0, mom0 = ctx.get_particles()
         mom_hist, pos_hist = [mom0], [pos0]
 
-        def snap():
-            p_, m_ = ctx.get_particles()
+        def snap(p_, m_):
             pos_hist.append(p_)
             mom_hist.append(m_)
-            return None
 
         _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, snap)
     finally:

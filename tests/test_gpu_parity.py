@@ -321,3 +321,41 @@ def test_specialized_kernels_identical_to_generic(gpu_ok, medium):
             ctx.run(p, 1, 20, None, 0)
             res.append(ctx.get_particles())
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_snapshots_equal_synchronous_downloads(gpu_ok):
+    """lmh_snapshot copies the state in stream order while later steps run: every snapshot equals
+    the state lmh_get_particles returns at that point of a second, synchronous run"""
+    g_ = small_grid()
+    N = 20000
+    pos = media.hydro_positions(N, np.random.default_rng(3)) * 2.0
+    mom = media.pt_rapidity_momenta(N, 1.5, np.random.default_rng(4))
+    from helpers import tables, load_tables
+    drag, rt, jt = tables([1.5], g_.T, 0.6366, 3, True, media.tau_R_law(0.16))
+    p = _ffi.make_params(dt=0.02, t0=0.6, DsT=0.6366, noise=_ffi.LMH_NOISE_PHILOX, seed=9,
+                         collision=_ffi.LMH_LANGEVIN_RTA, check_finite=False)
+    res = []
+    for async_ in (False, True):
+        with lmhip.Context(0) as ctx:
+            ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=1)
+            ctx.set_medium_grid2p1(g_.x0, g_.dx, g_.nx, g_.y0, g_.dy, g_.ny, g_.tgrid, g_.T, g_.vx, g_.vy)
+            load_tables(ctx, drag, rt, jt)
+            ctx.set_particles(pos, mom, [1.5])
+            out, idx = [], []
+            for c in range(5):
+                ctx.run(p, 1 + 7 * c, 7)
+                if async_:
+                    idx.append(ctx.snapshot())
+                else:
+                    out.append(ctx.get_particles())
+            if async_:
+                out = [ctx.snapshot_get(k) for k in idx]
+                # freed indices are reused, and a snapshot is fetched once
+                k = ctx.snapshot()
+                assert k in idx
+                ctx.snapshot_get(k)
+                with pytest.raises(lmhip.LMHError):
+                    ctx.snapshot_get(k)
+            res.append(out)
+    for (sp, sm), (gp, gm) in zip(res[0], res[1]):
+        assert np.array_equal(sp, gp) and np.array_equal(sm, gm)
