@@ -740,7 +740,10 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
         lastT = a.fo.lastT[i];
     }
     int64_t bad = -1;
-    for (int s_off = 0; s_off < a.nsteps; ++s_off) {
+    // the sliced and the deferred launches are one step each (the slice holds one time); a
+    // loop-free body keeps the per-step constants out of the registers of a loop
+    const int nsteps = (SLICE || DEFER) ? 1 : a.nsteps;
+    for (int s_off = 0; s_off < nsteps; ++s_off) {
         const int64_t step = a.step_begin + s_off;
         const double t_now = (double)step * p.dt + p.t0;
         double td;
