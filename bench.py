@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 BYTES_PER_PARTICLE_STEP = 80   # read + write x, y, px, py, pz (FP64)
 # HBM bytes per launch of the step kernel on the default workload, from rocprofv3 FETCH_SIZE
 # (x2.00) + WRITE_SIZE (x1.00), factors from a 1 GiB device copy: profiles/k_cart_grid_1e7_grid2p1_pmc.md
-MEASURED_TRAFFIC_PER_QUARK = 80.79
+MEASURED_TRAFFIC_PER_QUARK = 80.84
 
 
 def build_workload(N, seed=3):

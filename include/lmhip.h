@@ -139,7 +139,8 @@ typedef struct lmh_tuning {
     int32_t block;             /* threads per block: 64, 128 or 256 (default 256)               */
     int32_t steps_per_launch;  /* steps a particle stays in registers when lmh_run gets 0 (1)  */
     int32_t slice;             /* 2+1D grid, one-step launches: 1 interpolates the medium in tau
-                                  once per step into a slice the particles read (default 0)    */
+                                  once per step into a slice the particles read (default 1;
+                                  lmh_adjoint_forward always runs this way)                    */
     int32_t defer;             /* 2+1D grid, one-step launches with RTA: 1 queues the colliding
                                   particles and finishes them in a second kernel, so the
                                   redraw runs in colliding lanes only (default 1)              */
@@ -291,8 +292,9 @@ int lmh_freezeout_get(lmh_ctx* ctx, double* pos, double* mom, double* tau, doubl
 
 /* ---- pathwise adjoint (extension; the reference has no differentiation, Project.toml:6-27) -- */
 /* The forward half of a ChainRulesCore.rrule / torch.autograd.Function: runs steps
- * step_begin .. step_begin+nsteps-1 exactly as lmh_run does (same final state, same noise) and
- * keeps checkpoints.  The 2+1D grid in Cartesian coordinates, Philox noise, the relativistic
+ * step_begin .. step_begin+nsteps-1 exactly as lmh_run does with one step per launch and the
+ * sliced medium (the default tuning: same final state, same noise) and keeps checkpoints; the
+ * pullback re-runs segments from them through the same per-step medium slices.  The 2+1D grid in Cartesian coordinates, Philox noise, the relativistic
  * pre-point Langevin kick with or without RTA collisions, drag TABLE or GAMMA_P.            */
 int lmh_adjoint_forward(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, int64_t nsteps,
                         const lmh_adjoint_spec* spec);

@@ -686,12 +686,10 @@ __device__ __forceinline__ int block_rank(bool want, int32_t* block_count) {
     return want ? before + __popcll(below) : -1;
 }
 
-// the medium at the step's time, per cell: f = a + b xd + (c + d xd) yd with a = f00, b = f10 - f00,
-// c = f01 - f00, d = f11 - f10 - f01 + f00 of the tau-interpolated corners
-__global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t ncx = g.nx - 1, ncell = ncx * (g.ny - 1);
-    if (k >= ncell) return;
+// the medium at one time (tau cell j, fraction td), per cell: f = a + b xd + (c + d xd) yd with
+// a = f00, b = f10 - f00, c = f01 - f00, d = f11 - f10 - f01 + f00 of the tau-interpolated corners
+__device__ __forceinline__ void slice_cell(const GridMedium& g, int j, double td, int64_t k, double4* out) {
+    const int64_t ncx = g.nx - 1;
     const int64_t ix = k % ncx, iy = k / ncx;
     const int64_t nxy = (int64_t)g.nx * g.ny;
     const int64_t n00 = ix + (int64_t)g.nx * iy;
@@ -699,15 +697,31 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* sl
     const double4* s1 = s0 + nxy;
     const double4 c00 = lerp4(s0[0], s1[0], td), c10 = lerp4(s0[1], s1[1], td);
     const double4 c01 = lerp4(s0[g.nx], s1[g.nx], td), c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
-    double4* out = slice + 3 * k;
     out[0] = make_double4(c00.x, c10.x - c00.x, c01.x - c00.x, (c11.x - c10.x) - (c01.x - c00.x));
     out[1] = make_double4(c00.y, c10.y - c00.y, c01.y - c00.y, (c11.y - c10.y) - (c01.y - c00.y));
     out[2] = make_double4(c00.z, c10.z - c00.z, c01.z - c00.z, (c11.z - c10.z) - (c01.z - c00.z));
 }
 
-// The step on the 2+1D grid.  With DEFER (one step per launch), particles that collide are
-// left in the LRF after the kick and queued; k_redraw_grid finishes them (redraw, boost back,
-// push), so that the redraw's noise and table lookups run only in colliding lanes.
+__global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t ncell = (int64_t)(g.nx - 1) * (g.ny - 1);
+    if (k >= ncell) return;
+    slice_cell(g, j, td, k, slice + 3 * k);
+}
+
+// the slices of steps step0 .. step0 + nk - 1 (time cell of each step as the step kernels find it)
+__global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* slices, int64_t step0,
+                                                        int32_t nk, double dt, double t0) {
+    const int64_t ncell = (int64_t)(g.nx - 1) * (g.ny - 1);
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ncell * nk) return;
+    const int64_t s = k / ncell, cell = k - s * ncell;
+    const double t_now = (double)(step0 + s) * dt + t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    slice_cell(g, j, td, cell, slices + 3 * k);
+}
+
 // three waves per SIMD: the register budget the step needs with a few spilled values, against
 // two waves at its unconstrained register count
 constexpr int kStepWaves = 3;
@@ -1080,7 +1094,7 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 0, 1, 1};
+    lmh_tuning tune = {256, 1, 1, 1, 1};
 
     // tables
     struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
@@ -1137,6 +1151,8 @@ struct lmh_ctx {
     size_t ckpt_cap = 0;
     double* d_tape = nullptr;
     size_t tape_cap = 0;
+    double* d_adjslice = nullptr;     // the medium slices of a segment's steps, two buffers
+    size_t adjslice_cap = 0;
     double* d_adjw = nullptr;         // lam, per-particle gradients, partial sums, cotangent
     size_t adjw_cap = 0;
     // the backward pass re-runs segment c-1 on a second stream while segment c is reversed;
@@ -1411,6 +1427,7 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_slice);
     dfree(c->d_ckpt);
     dfree(c->d_tape);
+    dfree(c->d_adjslice);
     dfree(c->d_adjw);
     dfree(c->d_fo);
     for (int k = 0; k < LMH_MAX_SPECIES; ++k) {
@@ -2153,25 +2170,38 @@ int check_adjoint(lmh_ctx* c, const lmh_params* p) {
 }
 
 template <int PDIM, int COLL, int DRAG>
-void launch_adj_t(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g,
+void launch_adj_t(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, const double4* sl,
                   const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam, double* gacc,
                   double dtau) {
     const dim3 G(nb), B(c->tune.block);
-    if (tape) hipLaunchKernelGGL((k_adj_tape<PDIM, COLL, DRAG>), G, B, 0, c->tape_stream, a, g, ckpt, step0, nk, tp);
-    else hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, a, g, step0, nk, tp, lam, gacc, dtau);
+    if (tape) hipLaunchKernelGGL((k_adj_tape<PDIM, COLL, DRAG>), G, B, 0, c->tape_stream, a, g, sl, ckpt, step0, nk, tp);
+    else hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, a, g, sl, step0, nk, tp, lam, gacc, dtau);
 }
 
 template <int PDIM>
-void launch_adj(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, int coll, int drag,
-                const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam, double* gacc, double dtau) {
+void launch_adj(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, const double4* sl,
+                int coll, int drag, const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam,
+                double* gacc, double dtau) {
     if (coll == LMH_LANGEVIN_RTA) {
-        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
-        else launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_TABLE>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
+        else launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_TABLE>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
     } else {
-        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
-        else launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_TABLE>(c, tape, a, nb, g, ckpt, step0, nk, tp, lam, gacc, dtau);
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
+        else launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_TABLE>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
     }
 }
+
+// the forward pass of the adjoint steps through the per-step medium slices, one step per launch,
+// which the backward pass rebuilds for its recomputation: both see the same medium, bit for bit
+struct SliceTuning {
+    lmh_ctx* c;
+    lmh_tuning saved;
+    explicit SliceTuning(lmh_ctx* c_) : c(c_), saved(c_->tune) {
+        c->tune.slice = 1;
+        c->tune.steps_per_launch = 1;
+    }
+    ~SliceTuning() { c->tune = saved; }
+};
 
 }  // namespace
 
@@ -2194,11 +2224,12 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     const size_t per = R * (size_t)c->N;
     if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
     ++c->gen;
+    SliceTuning sliced(c);
     for (int64_t cp = 0; cp < ncp; ++cp) {
         LMH_HIP(c, hipMemcpyAsync(c->d_ckpt + per * (size_t)cp, c->d_state, per * sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
         const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
-        if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 0))) return rc;
+        if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 1))) return rc;
     }
     c->adj.p = *p;
     c->adj.step_begin = step_begin;
@@ -2233,6 +2264,10 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     int rc;
     const size_t tape_n = (size_t)K * TR * (size_t)N;
     if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, 2 * tape_n, "adjoint_backward tape"))) return rc;
+    const int64_t ncell = (int64_t)(c->gx - 1) * (c->gy - 1);
+    const size_t slices_n = (size_t)K * 12 * (size_t)ncell;     // K slices of ncell x 3 double4
+    if ((rc = ensure_dev(c, &c->d_adjslice, &c->adjslice_cap, 2 * slices_n, "adjoint_backward medium slices")))
+        return rc;
     if (!c->tape_stream) {
         LMH_HIP(c, hipStreamCreateWithFlags(&c->tape_stream, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k) {
@@ -2268,18 +2303,21 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
         const int64_t step0 = c->adj.step_begin + cp * K;
         const int buf = (int)(cp & 1);
         double* tp = c->d_tape + tape_n * buf;
-        // re-run segment cp into tape buffer buf, once the reverse of segment cp + 2 (the
-        // buffer's previous reader) is done
+        const double4* sl = reinterpret_cast<const double4*>(c->d_adjslice + slices_n * buf);
+        // re-run segment cp into tape and slice buffers buf, once the reverse of segment cp + 2
+        // (their previous reader) is done
         if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
-        if (c->pdim == 2) launch_adj<2>(c, true, a, nb, g, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
-        else launch_adj<3>(c, true, a, nb, g, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+        hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(ncell * nk)), dim3(kBlock), 0, c->tape_stream, g,
+                           const_cast<double4*>(sl), step0, nk, p.dt, p.t0);
+        if (c->pdim == 2) launch_adj<2>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+        else launch_adj<3>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
         LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
         // reverse segment cp on the main stream
         LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
         {
             LaunchMark mk(c, LMH_PROF_STEP);
-            if (c->pdim == 2) launch_adj<2>(c, false, a, nb, g, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
-            else launch_adj<3>(c, false, a, nb, g, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+            if (c->pdim == 2) launch_adj<2>(c, false, a, nb, g, sl, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+            else launch_adj<3>(c, false, a, nb, g, sl, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
         }
         LMH_HIP(c, hipEventRecord(c->ev_rev[buf], c->stream));
         LMH_HIP(c, hipGetLastError());

@@ -9,10 +9,12 @@
 //
 // Common random numbers: the noise of particle i at step s is Philox4x32-10 of a counter made of
 // (i, s, stream, block), so each segment's recomputation draws the forward pass's noise again and
-// no noise is kept between the passes.  The forward
-// pass keeps the state every K steps (checkpoints); the backward pass walks the segments from the
-// last to the first, re-runs each segment forward from its checkpoint while writing the step
-// inputs (state and kick noise) to a segment tape, then runs the reverse steps over the tape.
+// no noise is kept between the passes.  The forward pass keeps the state every K steps
+// (checkpoints); the backward pass walks the segments from the last to the first: it prepares the
+// medium slices of the segment's steps (the per-cell coefficients the forward step kernels read),
+// re-runs the segment forward from its checkpoint while writing the step inputs (state and kick
+// noise) to a segment tape -- the forward pass's trajectory, bit for bit -- then runs the reverse
+// steps over the tape.
 // One thread owns one particle throughout, so no adjoint is ever exchanged between threads.
 //
 // The RTA collision decision is a discontinuous function of the state; the pathwise derivative
@@ -56,50 +58,39 @@ __device__ __forceinline__ double juttner_lookup_dT(const JuttnerTable& jt, doub
     return (c1 - c0) * jt.invdT;
 }
 
-// The medium at (x, y) as grid_sample<false> computes it (same operations, same rounding) and
-// its derivatives with respect to x and y (0 along an axis where the position is clamped).
+// The medium at (x, y) from the step's slice (g.slice), as grid_sample<true> computes it (same
+// operations, same rounding), and its derivatives with respect to x and y (0 along an axis where
+// the position is clamped): d/dxd of a + b xd + (c + d xd) yd is b + d yd, d/dyd is c + d xd.
 struct SampleGrad {
     double T, vx, vy;
     double dTx, dTy, dvxx, dvxy, dvyx, dvyy;
 };
 
-__device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, int j,
-                                                 double td, SampleGrad* s) {
+__device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, SampleGrad* s) {
     double xc = jl_clamp(x, g.x0, g.xlast);
     double yc = jl_clamp(y, g.y0, g.ylast);
     double fx = (xc - g.x0) * g.inv_dx;
     double fy = (yc - g.y0) * g.inv_dy;
     int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
     double xd = fx - (double)ix, yd = fy - (double)iy;
-    int64_t n00 = ix + (int64_t)g.nx * iy;
-    const int64_t nxy = (int64_t)g.nx * g.ny;
-    const double4* s0 = g.nodes + nxy * j + n00;
-    const double4* s1 = s0 + nxy;
-    double4 c00 = lerp4(s0[0], s1[0], td);
-    double4 c10 = lerp4(s0[1], s1[1], td);
-    double4 c01 = lerp4(s0[g.nx], s1[g.nx], td);
-    double4 c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
-    double a0 = c00.x * (1.0 - xd) + c10.x * xd;
-    double a1 = c01.x * (1.0 - xd) + c11.x * xd;
-    s->T = a0 * (1.0 - yd) + a1 * yd;
-    double b0 = c00.y * (1.0 - xd) + c10.y * xd;
-    double b1 = c01.y * (1.0 - xd) + c11.y * xd;
-    s->vx = b0 * (1.0 - yd) + b1 * yd;
-    double e0 = c00.z * (1.0 - xd) + c10.z * xd;
-    double e1 = c01.z * (1.0 - xd) + c11.z * xd;
-    s->vy = e0 * (1.0 - yd) + e1 * yd;
+    const double4* cell = g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
+    const double4 cT = cell[0], cx = cell[1], cy = cell[2];
+    s->T = fma(fma(cT.w, xd, cT.z), yd, fma(cT.y, xd, cT.x));
+    s->vx = fma(fma(cx.w, xd, cx.z), yd, fma(cx.y, xd, cx.x));
+    s->vy = fma(fma(cy.w, xd, cy.z), yd, fma(cy.y, xd, cy.x));
     const double gx = (x >= g.x0 && x <= g.xlast) ? g.inv_dx : 0.0;
     const double gy = (y >= g.y0 && y <= g.ylast) ? g.inv_dy : 0.0;
-    s->dTx = ((c10.x - c00.x) * (1.0 - yd) + (c11.x - c01.x) * yd) * gx;
-    s->dvxx = ((c10.y - c00.y) * (1.0 - yd) + (c11.y - c01.y) * yd) * gx;
-    s->dvyx = ((c10.z - c00.z) * (1.0 - yd) + (c11.z - c01.z) * yd) * gx;
-    s->dTy = (a1 - a0) * gy;
-    s->dvxy = (b1 - b0) * gy;
-    s->dvyy = (e1 - e0) * gy;
+    s->dTx = fma(cT.w, yd, cT.y) * gx;
+    s->dvxx = fma(cx.w, yd, cx.y) * gx;
+    s->dvyx = fma(cy.w, yd, cy.y) * gx;
+    s->dTy = fma(cT.w, xd, cT.z) * gy;
+    s->dvxy = fma(cx.w, xd, cx.z) * gy;
+    s->dvyy = fma(cy.w, xd, cy.z) * gy;
 }
 
 // One forward step, exactly the fused k_cart_grid step (non-deferred, Philox, no position
-// diffusion), for the tape recomputation.  xi and ucol are the step's kick noise.
+// diffusion, the medium from the step's slice g.slice), for the tape recomputation.  xi and ucol
+// are the step's kick noise.
 template <int PDIM, int COLL>
 __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_params& p,
                                                  const GridMedium& g, const SpeciesArgs& sp,
@@ -107,11 +98,11 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_pa
                                                  double* q, const double* xi, double ucol) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m;
-    const double t_now = (double)step * p.dt + p.t0;
-    double td;
-    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    // the slice holds the step's time; no time cell is needed (no position diffusion here)
+    const int j = 0;
+    const double td = 0.0;
     double Tm, vx, vy;
-    grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    grid_sample<true>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
     const FlowBoost fb = flow_boost(vx, vy);
     apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
     if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
@@ -156,13 +147,10 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m, m2 = sp.m * sp.m;
     const double dt = p.dt;
-    const double t_now = (double)step * dt + p.t0;
-    double td;
-    const int j = axis_cell(g.tg, g.nt, t_now, &td);
 
     // ---- forward values
     SampleGrad s;
-    grid_sample_grad(g, x0, x1, j, td, &s);
+    grid_sample_grad(g, x0, x1, &s);
     const double vmag = sqrt(s.vx * s.vx + s.vy * s.vy);
     const bool on = !(vmag < kEps);
     const double ivm = on ? drv_rcp(vmag) : 0.0;
@@ -463,6 +451,7 @@ __device__ __forceinline__ lmh_params adjoint_params(const lmh_params& in) {
 // collisions]).
 template <int PDIM, int COLL, int DRAG>
 __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, GridMedium g,
+                                                                  const double4* __restrict__ slices,
                                                                   const double* __restrict__ ckpt,
                                                                   int64_t seg_step0, int32_t nk,
                                                                   double* __restrict__ tape) {
@@ -492,14 +481,20 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
         for (int d = 0; d < PDIM; ++d) tk[(int64_t)(R + d) * N] = xi[d];
         if (COLL == LMH_LANGEVIN_RTA) tk[(int64_t)(R + PDIM) * N] = ucol;
         // the last step's output is the next checkpoint: only its inputs go to the tape
-        if (k + 1 < nk) adj_forward_step<PDIM, COLL>(a, p, g, sp, i, step, &x0, &x1, q, xi, ucol);
+        if (k + 1 < nk) {
+            GridMedium gk = g;
+            gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
+            adj_forward_step<PDIM, COLL>(a, p, gk, sp, i, step, &x0, &x1, q, xi, ucol);
+        }
     }
 }
 
 // Second half: the reverse steps over the tape from the last to the first.  lam carries the
 // state adjoint between segments (rows as the state), gacc the per-particle dL/dtheta (3 rows).
 template <int PDIM, int COLL, int DRAG>
-__global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g, int64_t seg_step0,
+__global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g,
+                                                        const double4* __restrict__ slices,
+                                                        int64_t seg_step0,
                                                         int32_t nk, const double* __restrict__ tape,
                                                         double* __restrict__ lam,
                                                         double* __restrict__ gacc, double dtau_dDsT) {
@@ -533,7 +528,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
             for (int r = 0; r < TR; ++r) row[r] = tn[(int64_t)r * N];
         }
         const double ucol = COLL == LMH_LANGEVIN_RTA ? cur[R + PDIM] : 1.0;
-        adj_reverse_step<PDIM, COLL>(a, p, g, sp, i, step, cur[0], cur[1], cur + 2, cur + R, ucol,
+        GridMedium gk = g;
+        gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
+        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, cur + R, ucol,
                                      dtau_dDsT, lm, th);
     }
 #pragma unroll

@@ -153,21 +153,26 @@ def test_adjoint_matches_oracle_finite_differences(gpu_ok, case):
 
 
 def test_adjoint_forward_state_is_the_run_state(gpu_ok):
-    """lmh_adjoint_forward leaves the state lmh_run leaves, bit for bit, for any checkpoint cadence"""
+    """lmh_adjoint_forward leaves the state lmh_run leaves with the default tuning (one step per
+    launch, sliced medium), bit for bit, for any checkpoint cadence and whatever tuning is set"""
     g, masses, counts, pos, mom = _setup(N=3000)
     rt, jt = _rta_tables(g, masses)
     theta = (0.6366, 5.3, 5.0)
     res = []
     with lmhip.Context(0) as ctx:
-        for K in (0, 1, 7, 40):
+        for K in (0, 1, 7, 40, -7):
+            if K < 0:
+                ctx.set_tuning(block=256, steps_per_launch=4, slice=0)
             ens = GridEnsemble(ctx, g, masses, counts, pos, mom, dt=DT, t0=T0, nsteps=40,
-                               collision=_ffi.LMH_LANGEVIN_RTA, seed=3, checkpoint_every=max(K, 1),
+                               collision=_ffi.LMH_LANGEVIN_RTA, seed=3, checkpoint_every=max(abs(K), 1),
                                rta_tables=rt, juttner_tables=jt, **OBS)
             if K == 0:
                 ens.run_plain(theta)
             else:
                 ens.forward(theta)
             res.append(ctx.get_particles())
+        # the tuning set by hand is kept
+        assert ctx.get_tuning()["steps_per_launch"] == 4 and ctx.get_tuning()["slice"] == 0
     for r in res[1:]:
         assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
 
