@@ -312,7 +312,8 @@ int lmh_adjoint_backward(lmh_ctx* ctx, const lmh_obs_spec* spec, const double* c
 /* Runs r = 0..nruns-1 on the 2+1D grid, each with its own medium, tables, mass, D_s T and
  * particle count, advanced together by one launch per step (or per steps_per_launch steps).
  * Run r draws Philox noise with key (seed, stream + r) and counter (local index, step, block):
- * it follows the trajectories lmh_run gives for the same particles alone with stream + r.   */
+ * it follows the trajectories lmh_run gives for the same particles alone with stream + r and
+ * tuning slice = 0 (the batch interpolates each medium corner in tau).                       */
 int         lmh_batch_create(lmh_batch** out, int32_t device);
 void        lmh_batch_destroy(lmh_batch* b);
 const char* lmh_batch_last_error(const lmh_batch* b);

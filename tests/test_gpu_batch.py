@@ -70,6 +70,8 @@ def _alone(runs, collision, steps, boost=True, **kw):
         p = _params(collision, DsT=r["DsT"], stream=STREAM0 + k, **kw)
         g = r["g"]
         with lmhip.Context(0) as ctx:
+            # the batch kernels interpolate each corner in tau (the unsliced form)
+            ctx.set_tuning(slice=0)
             ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
             ctx.set_time_table(_ffi.LMH_TABLE_DRAG, 0, *r["drag"])
             if r["rt"] is not None:
