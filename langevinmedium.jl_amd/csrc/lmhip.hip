@@ -581,14 +581,16 @@ struct FlowBoost {
 __device__ __forceinline__ FlowBoost flow_boost(double vx, double vy) {
     FlowBoost b;
     double vm2 = vx * vx + vy * vy;
-    b.vmag = sqrt(vm2);
+    // |v| and 1/|v| from one reciprocal square root (an ulp from sqrt and vx/|v|); v = 0 gives
+    // |v| = 0 and the boost is off
+    const double inv = rsqrt_pos(fmax(vm2, 1e-300));
+    b.vmag = vm2 * inv;
     b.on = !(b.vmag < kEps);
-    const double inv = 1.0 / b.vmag;      // one division for both components (an ulp from vx/|v|)
     b.nx = vx * inv;
     b.ny = vy * inv;
     double vmax = vmax_();
     b.v = b.vmag > vmax ? vmax : b.vmag;
-    b.gam = 1.0 / sqrt(1.0 - b.v * b.v + 1e-10);
+    b.gam = rsqrt_pos(1.0 - b.v * b.v + 1e-10);
     return b;
 }
 
@@ -605,7 +607,7 @@ __device__ __forceinline__ void apply_boost(double* q, double m, const FlowBoost
     double p2 = 0.0;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
-    double E = sqrt(p2 + m * m);
+    double E = sqrt_pos(p2 + m * m);
     double px = q[0], py = q[1];
     double p_par = px * b.nx + py * b.ny;
     double p_perp_x = px - p_par * b.nx;
@@ -646,9 +648,9 @@ __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p
     double E2 = m * m;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
-    const double E = p.relativistic ? sqrt(E2) : m;
-    // dt*p/E as p*(dt/E): one division for both rows, within an ulp of the reference form
-    const double dtE = p.dt / E;
+    // dt*p/E as p*(dt/E) with dt/E = dt/sqrt(E^2) from one reciprocal square root, within an
+    // ulp or two of the reference form
+    const double dtE = p.relativistic ? p.dt * rsqrt_pos(E2) : p.dt / m;
     *x0 = *x0 + q[0] * dtE;
     *x1 = *x1 + q[1] * dtE;
     if (p.position_diffusion) {
@@ -687,39 +689,41 @@ __device__ __forceinline__ int block_rank(bool want, int32_t* block_count) {
 }
 
 // the medium at one time (tau cell j, fraction td), per cell: f = a + b xd + (c + d xd) yd with
-// a = f00, b = f10 - f00, c = f01 - f00, d = f11 - f10 - f01 + f00 of the tau-interpolated corners
-__device__ __forceinline__ void slice_cell(const GridMedium& g, int j, double td, int64_t k, double4* out) {
+// a = f00, b = f10 - f00, c = f01 - f00, d = f11 - f10 - f01 + f00 of the tau-interpolated corners.
+// One thread per (cell, field) record: field fl (0 T, 1 vx, 2 vy) of cell k is slice[3 k + fl], so
+// consecutive lanes write consecutive 32-byte records.
+__device__ __forceinline__ double4 slice_coeffs(const GridMedium& g, int j, double td, int64_t k, int fl) {
     const int64_t ncx = g.nx - 1;
     const int64_t ix = k % ncx, iy = k / ncx;
     const int64_t nxy = (int64_t)g.nx * g.ny;
-    const int64_t n00 = ix + (int64_t)g.nx * iy;
-    const double4* s0 = g.nodes + nxy * j + n00;
-    const double4* s1 = s0 + nxy;
-    const double4 c00 = lerp4(s0[0], s1[0], td), c10 = lerp4(s0[1], s1[1], td);
-    const double4 c01 = lerp4(s0[g.nx], s1[g.nx], td), c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
-    out[0] = make_double4(c00.x, c10.x - c00.x, c01.x - c00.x, (c11.x - c10.x) - (c01.x - c00.x));
-    out[1] = make_double4(c00.y, c10.y - c00.y, c01.y - c00.y, (c11.y - c10.y) - (c01.y - c00.y));
-    out[2] = make_double4(c00.z, c10.z - c00.z, c01.z - c00.z, (c11.z - c10.z) - (c01.z - c00.z));
+    const double* s0 = reinterpret_cast<const double*>(g.nodes + nxy * j + ix + (int64_t)g.nx * iy) + fl;
+    const double* s1 = s0 + 4 * nxy;
+    const int64_t up = 4 * (int64_t)g.nx;
+    const double c00 = s0[0] * (1.0 - td) + s1[0] * td;
+    const double c10 = s0[4] * (1.0 - td) + s1[4] * td;
+    const double c01 = s0[up] * (1.0 - td) + s1[up] * td;
+    const double c11 = s0[up + 4] * (1.0 - td) + s1[up + 4] * td;
+    return make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t ncell = (int64_t)(g.nx - 1) * (g.ny - 1);
-    if (k >= ncell) return;
-    slice_cell(g, j, td, k, slice + 3 * k);
+    const int64_t nrec = 3 * (int64_t)(g.nx - 1) * (g.ny - 1);
+    if (k >= nrec) return;
+    slice[k] = slice_coeffs(g, j, td, k / 3, (int)(k % 3));
 }
 
 // the slices of steps step0 .. step0 + nk - 1 (time cell of each step as the step kernels find it)
 __global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* slices, int64_t step0,
                                                         int32_t nk, double dt, double t0) {
-    const int64_t ncell = (int64_t)(g.nx - 1) * (g.ny - 1);
+    const int64_t nrec = 3 * (int64_t)(g.nx - 1) * (g.ny - 1);
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= ncell * nk) return;
-    const int64_t s = k / ncell, cell = k - s * ncell;
+    if (k >= nrec * nk) return;
+    const int64_t s = k / nrec, r = k - s * nrec;
     const double t_now = (double)(step0 + s) * dt + t0;
     double td;
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
-    slice_cell(g, j, td, cell, slices + 3 * k);
+    slices[k] = slice_coeffs(g, j, td, r / 3, (int)(r % 3));
 }
 
 // three waves per SIMD: the register budget the step needs with a few spilled values, against
@@ -1822,7 +1826,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             host_time_cell(tg, t_now, &j, &td);
             const int64_t nxy = (int64_t)c->gx * c->gy;
             LaunchMark mk(c, LMH_PROF_SLICE);
-            hipLaunchKernelGGL(k_grid_slice, dim3(grid_blocks((int64_t)(c->gx - 1) * (c->gy - 1))), dim3(kBlock), 0, c->stream, gmed, c->d_slice, j, td);
+            hipLaunchKernelGGL(k_grid_slice, dim3(grid_blocks(3 * (int64_t)(c->gx - 1) * (c->gy - 1))), dim3(kBlock), 0, c->stream, gmed, c->d_slice, j, td);
         }
         a.step_begin = step_begin + s0;
         a.nsteps = n;
@@ -2307,7 +2311,7 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
         // re-run segment cp into tape and slice buffers buf, once the reverse of segment cp + 2
         // (their previous reader) is done
         if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
-        hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(ncell * nk)), dim3(kBlock), 0, c->tape_stream, g,
+        hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(3 * ncell * nk)), dim3(kBlock), 0, c->tape_stream, g,
                            const_cast<double4*>(sl), step0, nk, p.dt, p.t0);
         if (c->pdim == 2) launch_adj<2>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
         else launch_adj<3>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);

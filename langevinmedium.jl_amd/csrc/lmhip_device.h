@@ -126,9 +126,12 @@ __device__ __forceinline__ void philox4x32(uint32_t c0, uint32_t c1, uint32_t c2
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        // one 32x32 -> 64-bit multiply per product, one three-input XOR (bitop3 0x96) per word
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
@@ -160,9 +163,32 @@ __device__ __forceinline__ double u_low44(const uint64_t* w) {
     return (double)b * 0x1.0p-44;
 }
 
+// 1/sqrt(x) for x > 0 (normal): the hardware estimate refined by two Newton steps, within an ulp
+// of the correctly rounded 1/sqrt(x); half the instructions of sqrt followed by a division
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    return y;
+}
+
+// sqrt(x) for x >= 0: g = x/sqrt(x) from the once-refined estimate, then one residual step
+// g + (x - g^2)/(2 sqrt(x)); 0 and NaN pass through.  No rescaling of tiny or huge arguments
+// (the callers' arguments are energies, rates and variances far from the exponent limits).
+__device__ __forceinline__ double sqrt_pos(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    const double g = x * y;
+    const double r = fma(fma(-g, g, x), 0.5 * y, g);
+    return x > 0.0 ? r : x;
+}
+
 // The two transcendental functions of the Box-Muller transform, written with correctly rounded
-// operations only (+, -, *, /, sqrt, fma, floor, exact bit manipulation), so that the same
-// definition gives the same bits on the host and on the device.
+// operations only (+, -, *, /, fma, floor, exact bit manipulation), so that the same definition
+// gives the same bits on the host and on the device; the radius's square root is sqrt_pos here
+// (within an ulp of the host's correctly rounded sqrt).
 //
 // noise_log(u), u in [2^-53, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with
 // s = (m - 1)/(m + 1), |s| <= 0.1716, by the atanh series through s^19 (truncation < 3e-17).
@@ -223,7 +249,7 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
 
 // Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb
 __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
-    double r = sqrt(-2.0 * noise_log(u_oc(wa)));
+    double r = sqrt_pos(-2.0 * noise_log(u_oc(wa)));
     double s, c;
     noise_sincospi(2.0 * u_co(wb), &s, &c);
     *n0 = r * c;
@@ -246,7 +272,7 @@ __device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable
     bool ok = tau > 0.0 && isfinite(tau);
     *etaD = ok ? (1.0 / tau) : 0.0;
     double kappa = ok ? ((2.0 * m * T) / tau) : 0.0;
-    *kT = sqrt(kappa);
+    *kT = sqrt_pos(kappa);
 }
 
 // effective drag at momentum q: eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
@@ -257,8 +283,7 @@ __device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, doubl
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q2 += q[d] * q[d];
     if (p.drag_model == LMH_DRAG_GAMMA_P) return etaD / (1.0 + q2 / (p.Lambda * p.Lambda));
-    double E = sqrt(q2 + m * m);
-    return p.relativistic ? etaD * m / E : etaD;
+    return p.relativistic ? (etaD * m) * rsqrt_pos(q2 + m * m) : etaD;
 }
 
 // The LRF momentum kick: kernel_compute_all_forces_cpu! (kernels_cpu.jl:293-352) followed by
@@ -315,7 +340,7 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
             eta_eff = eta_of<PDIM>(p, etaD, m, qm);
         }
         double a = exp(-eta_eff * dt);
-        double noise_pref = (dt > 0.0 && eta_eff > 0.0) ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+        double noise_pref = (dt > 0.0 && eta_eff > 0.0) ? (kT * sqrt_pos((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) {
             double det = (a - 1.0) * q[d];
