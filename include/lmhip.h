@@ -164,8 +164,12 @@ typedef struct lmh_profile {
  * is proportional to DsT (effective_DsT without modifiers or with the quadratic law,
  * transport.jl:139-156), 0 for the linear law, which does not depend on DsT.                */
 typedef struct lmh_adjoint_spec {
-    int32_t checkpoint_every;
-    int32_t _pad;
+    int32_t checkpoint_every;  /* segment length K: a checkpoint every K steps                  */
+    int32_t keep_tape;         /* 1: keep every step's tape row and medium slice in device
+                                  memory (nsteps x (2 + 2 pdim [+1 with RTA]) x N doubles plus
+                                  nsteps x 96 B per grid cell), so that the pullback only runs
+                                  the reverse sweep; 0: checkpoints only, the pullback re-runs
+                                  each segment                                                 */
     double  dtau_dDsT;
 } lmh_adjoint_spec;
 
@@ -294,7 +298,8 @@ int lmh_freezeout_get(lmh_ctx* ctx, double* pos, double* mom, double* tau, doubl
 /* The forward half of a ChainRulesCore.rrule / torch.autograd.Function: runs steps
  * step_begin .. step_begin+nsteps-1 exactly as lmh_run does with one step per launch and the
  * sliced medium (the default tuning: same final state, same noise) and keeps checkpoints; the
- * pullback re-runs segments from them through the same per-step medium slices.  The 2+1D grid in Cartesian coordinates, Philox noise, the relativistic
+ * pullback re-runs segments from them through the same per-step medium slices.  With
+ * spec->keep_tape the forward pass writes the whole tape instead (no freeze-out capture).  The 2+1D grid in Cartesian coordinates, Philox noise, the relativistic
  * pre-point Langevin kick with or without RTA collisions, drag TABLE or GAMMA_P.            */
 int lmh_adjoint_forward(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, int64_t nsteps,
                         const lmh_adjoint_spec* spec);
@@ -302,7 +307,7 @@ int lmh_adjoint_forward(lmh_ctx* ctx, const lmh_params* p, int64_t step_begin, i
  * lmh_observables(spec) of the final state, grad receives dL/d(DsT, gamma0, Lambda) along the
  * forward pass's noise path (collision decisions held fixed); dpos (dims, N) and dmom
  * (pdim, N), both or neither, receive dL/d(state before the forward pass).  Re-runs each
- * checkpoint segment; may be called again with another cotangent.  Fails with LMH_E_STATE
+ * checkpoint segment (not with keep_tape); may be called again with another cotangent.  Fails with LMH_E_STATE
  * when the particles, medium or tables changed since lmh_adjoint_forward.  Synchronises.   */
 int lmh_adjoint_backward(lmh_ctx* ctx, const lmh_obs_spec* spec, const double* cotangent,
                          double* grad, double* dpos, double* dmom);

@@ -1148,6 +1148,7 @@ struct lmh_ctx {
         lmh_params p;
         int64_t step_begin = 0, nsteps = 0;
         int32_t K = 1;
+        bool keep = false;            // the forward pass kept the whole tape and every slice
         double dtau_dDsT = 0.0;
         uint64_t gen = 0;
     } adj;
@@ -2173,26 +2174,49 @@ int check_adjoint(lmh_ctx* c, const lmh_params* p) {
     return LMH_OK;
 }
 
+// one adjoint segment launch: the tape kernel (tape = true; on stream st, with the final state
+// to state_out when it is the forward pass) or the reverse kernel (on the context's stream)
+struct AdjLaunch {
+    const StepArgs* a;
+    unsigned nb;
+    const GridMedium* g;
+    const double4* sl;
+    int64_t step0;
+    int32_t nk;
+    double* tp;
+    const double* ckpt = nullptr;
+    double* state_out = nullptr;
+    hipStream_t st = nullptr;
+    double* lam = nullptr;
+    double* gacc = nullptr;
+    double dtau = 0.0;
+};
+
 template <int PDIM, int COLL, int DRAG>
-void launch_adj_t(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, const double4* sl,
-                  const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam, double* gacc,
-                  double dtau) {
-    const dim3 G(nb), B(c->tune.block);
-    if (tape) hipLaunchKernelGGL((k_adj_tape<PDIM, COLL, DRAG>), G, B, 0, c->tape_stream, a, g, sl, ckpt, step0, nk, tp);
-    else hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, a, g, sl, step0, nk, tp, lam, gacc, dtau);
+void launch_adj_t(lmh_ctx* c, bool tape, const AdjLaunch& l) {
+    const dim3 G(l.nb), B(c->tune.block);
+    if (tape)
+        hipLaunchKernelGGL((k_adj_tape<PDIM, COLL, DRAG>), G, B, 0, l.st, *l.a, *l.g, l.sl, l.ckpt, l.step0, l.nk,
+                           l.tp, l.state_out);
+    else
+        hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, *l.a, *l.g, l.sl, l.step0, l.nk,
+                           l.tp, l.lam, l.gacc, l.dtau);
 }
 
 template <int PDIM>
-void launch_adj(lmh_ctx* c, bool tape, const StepArgs& a, unsigned nb, const GridMedium& g, const double4* sl,
-                int coll, int drag, const double* ckpt, int64_t step0, int32_t nk, double* tp, double* lam,
-                double* gacc, double dtau) {
+void launch_adj_p(lmh_ctx* c, bool tape, int coll, int drag, const AdjLaunch& l) {
     if (coll == LMH_LANGEVIN_RTA) {
-        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
-        else launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_TABLE>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_GAMMA_P>(c, tape, l);
+        else launch_adj_t<PDIM, LMH_LANGEVIN_RTA, LMH_DRAG_TABLE>(c, tape, l);
     } else {
-        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_GAMMA_P>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
-        else launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_TABLE>(c, tape, a, nb, g, sl, ckpt, step0, nk, tp, lam, gacc, dtau);
+        if (drag == LMH_DRAG_GAMMA_P) launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_GAMMA_P>(c, tape, l);
+        else launch_adj_t<PDIM, LMH_LANGEVIN, LMH_DRAG_TABLE>(c, tape, l);
     }
+}
+
+void launch_adj(lmh_ctx* c, bool tape, const lmh_params& p, const AdjLaunch& l) {
+    if (c->pdim == 2) launch_adj_p<2>(c, tape, p.collision, p.drag_model, l);
+    else launch_adj_p<3>(c, tape, p.collision, p.drag_model, l);
 }
 
 // the forward pass of the adjoint steps through the per-step medium slices, one step per launch,
@@ -2218,27 +2242,70 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     int rc = check_run(c, p);
     if (rc) return rc;
     if (!s || s->checkpoint_every < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: checkpoint_every >= 1");
+    if (s->keep_tape != 0 && s->keep_tape != 1) return c->fail(LMH_E_INVALID, "adjoint_forward: keep_tape is 0 or 1");
     if (nsteps < 1) return c->fail(LMH_E_INVALID, "adjoint_forward: nsteps >= 1");
     if ((rc = check_adjoint(c, p))) return rc;
     if ((rc = check_steps(c, p, step_begin, nsteps))) return rc;
+    if (s->keep_tape && c->fo_on)
+        return c->fail(LMH_E_INVALID, "adjoint_forward: keep_tape runs without freeze-out capture");
     LMH_HIP(c, hipSetDevice(c->device));
     const int64_t K = s->checkpoint_every;
     const int64_t ncp = (nsteps + K - 1) / K;
     const size_t R = (size_t)(c->dims + c->pdim);
     const size_t per = R * (size_t)c->N;
-    if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
     ++c->gen;
-    SliceTuning sliced(c);
-    for (int64_t cp = 0; cp < ncp; ++cp) {
-        LMH_HIP(c, hipMemcpyAsync(c->d_ckpt + per * (size_t)cp, c->d_state, per * sizeof(double),
-                                  hipMemcpyDeviceToDevice, c->stream));
-        const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
-        if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 1))) return rc;
+    if (s->keep_tape) {
+        // the whole tape and every step's medium slice stay in device memory: the pullback only
+        // runs the reverse sweep.  The forward steps are the tape kernel's, segment by segment.
+        const size_t TR = R + (size_t)c->pdim + (p->collision == LMH_LANGEVIN_RTA ? 1 : 0);
+        const int64_t ncell = (int64_t)(c->gx - 1) * (c->gy - 1);
+        if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, (size_t)nsteps * TR * (size_t)c->N,
+                             "adjoint_forward whole tape (keep_tape)")))
+            return rc;
+        if ((rc = ensure_dev(c, &c->d_adjslice, &c->adjslice_cap, (size_t)nsteps * 12 * (size_t)ncell,
+                             "adjoint_forward medium slices (keep_tape)")))
+            return rc;
+        unsigned nb = 0;
+        StepArgs a = make_args(c, p, c->tune.block, &nb);
+        GridMedium g = make_grid(c);
+        for (int64_t cp = 0; cp < ncp; ++cp) {
+            const int32_t nk = (int32_t)std::min<int64_t>(K, nsteps - cp * K);
+            const int64_t step0 = step_begin + cp * K;
+            double4* sl = reinterpret_cast<double4*>(c->d_adjslice) + (size_t)cp * K * 3 * (size_t)ncell;
+            hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(3 * ncell * nk)), dim3(kBlock), 0, c->stream, g, sl,
+                               step0, nk, p->dt, p->t0);
+            if (nb > 0) {
+                AdjLaunch l{&a, nb, &g, sl, step0, nk, c->d_tape + (size_t)cp * K * TR * (size_t)c->N};
+                l.ckpt = c->d_state;
+                l.state_out = c->d_state;
+                l.st = c->stream;
+                LaunchMark mk(c, LMH_PROF_STEP);
+                launch_adj(c, true, *p, l);
+            }
+            LMH_HIP(c, hipGetLastError());
+        }
+        if (p->check_finite) {
+            unsigned long long bad = ULLONG_MAX;
+            LMH_HIP(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+            LMH_HIP(c, hipStreamSynchronize(c->stream));
+            if (bad != ULLONG_MAX)
+                return c->fail(LMH_E_NONFINITE, "Breaking simulation due to NaN/Inf at step " + std::to_string((long long)bad));
+        }
+    } else {
+        if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
+        SliceTuning sliced(c);
+        for (int64_t cp = 0; cp < ncp; ++cp) {
+            LMH_HIP(c, hipMemcpyAsync(c->d_ckpt + per * (size_t)cp, c->d_state, per * sizeof(double),
+                                      hipMemcpyDeviceToDevice, c->stream));
+            const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
+            if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 1))) return rc;
+        }
     }
     c->adj.p = *p;
     c->adj.step_begin = step_begin;
     c->adj.nsteps = nsteps;
     c->adj.K = (int32_t)K;
+    c->adj.keep = s->keep_tape != 0;
     c->adj.dtau_dDsT = s->dtau_dDsT;
     c->adj.gen = c->gen;
     c->adj.valid = true;
@@ -2261,18 +2328,21 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     const lmh_params& p = c->adj.p;
     const int64_t N = c->N;
     const int R = c->dims + c->pdim;
-    const int TR = R + c->pdim + 1;
+    const int TR = R + c->pdim + (p.collision == LMH_LANGEVIN_RTA ? 1 : 0);     // the tape kernels' rows
     const int64_t K = c->adj.K;
     const int64_t ncp = (c->adj.nsteps + K - 1) / K;
     const int nw = 5 + os->nbins;
     int rc;
+    const bool keep = c->adj.keep;
     const size_t tape_n = (size_t)K * TR * (size_t)N;
-    if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, 2 * tape_n, "adjoint_backward tape"))) return rc;
     const int64_t ncell = (int64_t)(c->gx - 1) * (c->gy - 1);
     const size_t slices_n = (size_t)K * 12 * (size_t)ncell;     // K slices of ncell x 3 double4
-    if ((rc = ensure_dev(c, &c->d_adjslice, &c->adjslice_cap, 2 * slices_n, "adjoint_backward medium slices")))
-        return rc;
-    if (!c->tape_stream) {
+    if (!keep) {
+        if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, 2 * tape_n, "adjoint_backward tape"))) return rc;
+        if ((rc = ensure_dev(c, &c->d_adjslice, &c->adjslice_cap, 2 * slices_n, "adjoint_backward medium slices")))
+            return rc;
+    }
+    if (!keep && !c->tape_stream) {
         LMH_HIP(c, hipStreamCreateWithFlags(&c->tape_stream, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k) {
             LMH_HIP(c, hipEventCreateWithFlags(&c->ev_tape[k], hipEventDisableTiming));
@@ -2298,33 +2368,52 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     StepArgs a = make_args(c, &p, c->tune.block, &nb);
     GridMedium g = make_grid(c);
     const size_t per = (size_t)R * N;
-    // the tape stream starts after the checkpoints are final (the forward pass and the seed
-    // were enqueued on the main stream)
-    LMH_HIP(c, hipEventRecord(c->ev_rev[0], c->stream));
-    LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[0], 0));
-    for (int64_t cp = ncp - 1; cp >= 0; --cp) {
-        const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
-        const int64_t step0 = c->adj.step_begin + cp * K;
-        const int buf = (int)(cp & 1);
-        double* tp = c->d_tape + tape_n * buf;
-        const double4* sl = reinterpret_cast<const double4*>(c->d_adjslice + slices_n * buf);
-        // re-run segment cp into tape and slice buffers buf, once the reverse of segment cp + 2
-        // (their previous reader) is done
-        if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
-        hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(3 * ncell * nk)), dim3(kBlock), 0, c->tape_stream, g,
-                           const_cast<double4*>(sl), step0, nk, p.dt, p.t0);
-        if (c->pdim == 2) launch_adj<2>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
-        else launch_adj<3>(c, true, a, nb, g, sl, p.collision, p.drag_model, c->d_ckpt + per * cp, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
-        LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
-        // reverse segment cp on the main stream
-        LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
-        {
+    if (keep) {
+        // the forward pass kept every step's tape row and medium slice: reverse sweeps only
+        for (int64_t cp = ncp - 1; cp >= 0; --cp) {
+            const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
+            const int64_t step0 = c->adj.step_begin + cp * K;
+            AdjLaunch l{&a, nb, &g, reinterpret_cast<const double4*>(c->d_adjslice) + (size_t)cp * K * 3 * (size_t)ncell,
+                        step0, nk, c->d_tape + (size_t)cp * tape_n};
+            l.lam = lam;
+            l.gacc = gacc;
+            l.dtau = c->adj.dtau_dDsT;
             LaunchMark mk(c, LMH_PROF_STEP);
-            if (c->pdim == 2) launch_adj<2>(c, false, a, nb, g, sl, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
-            else launch_adj<3>(c, false, a, nb, g, sl, p.collision, p.drag_model, nullptr, step0, nk, tp, lam, gacc, c->adj.dtau_dDsT);
+            launch_adj(c, false, p, l);
+            LMH_HIP(c, hipGetLastError());
         }
-        LMH_HIP(c, hipEventRecord(c->ev_rev[buf], c->stream));
-        LMH_HIP(c, hipGetLastError());
+    } else {
+        // the tape stream starts after the checkpoints are final (the forward pass and the seed
+        // were enqueued on the main stream)
+        LMH_HIP(c, hipEventRecord(c->ev_rev[0], c->stream));
+        LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[0], 0));
+        for (int64_t cp = ncp - 1; cp >= 0; --cp) {
+            const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
+            const int64_t step0 = c->adj.step_begin + cp * K;
+            const int buf = (int)(cp & 1);
+            double4* sl = reinterpret_cast<double4*>(c->d_adjslice + slices_n * buf);
+            AdjLaunch l{&a, nb, &g, sl, step0, nk, c->d_tape + tape_n * buf};
+            l.ckpt = c->d_ckpt + per * cp;
+            l.st = c->tape_stream;
+            l.lam = lam;
+            l.gacc = gacc;
+            l.dtau = c->adj.dtau_dDsT;
+            // re-run segment cp into tape and slice buffers buf, once the reverse of segment
+            // cp + 2 (their previous reader) is done
+            if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
+            hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(3 * ncell * nk)), dim3(kBlock), 0, c->tape_stream,
+                               g, sl, step0, nk, p.dt, p.t0);
+            launch_adj(c, true, p, l);
+            LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
+            // reverse segment cp on the main stream
+            LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
+            {
+                LaunchMark mk(c, LMH_PROF_STEP);
+                launch_adj(c, false, p, l);
+            }
+            LMH_HIP(c, hipEventRecord(c->ev_rev[buf], c->stream));
+            LMH_HIP(c, hipGetLastError());
+        }
     }
     hipLaunchKernelGGL(k_rows_partial, dim3(kSumBlocks), dim3(kSumBlock), 0, c->stream, gacc, kAdjThetas, N, c->d_adjw + o_part);
     hipLaunchKernelGGL(k_rows_final, dim3(kAdjThetas), dim3(kSumBlock), 0, c->stream, c->d_adjw + o_part, kSumBlocks, c->d_adjw + o_out);

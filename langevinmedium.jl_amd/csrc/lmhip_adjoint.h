@@ -151,9 +151,11 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
     // ---- forward values
     SampleGrad s;
     grid_sample_grad(g, x0, x1, &s);
-    const double vmag = sqrt(s.vx * s.vx + s.vy * s.vy);
+    const double vm2 = s.vx * s.vx + s.vy * s.vy;
+    const double ivm0 = rsqrt_pos(fmax(vm2, 1e-300));
+    const double vmag = vm2 * ivm0;
     const bool on = !(vmag < kEps);
-    const double ivm = on ? drv_rcp(vmag) : 0.0;
+    const double ivm = on ? ivm0 : 0.0;
     const double nx = s.vx * ivm, ny = s.vy * ivm;
     const double vmax = vmax_();
     const double v = vmag > vmax ? vmax : vmag;
@@ -184,7 +186,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
         etaD = p.gamma0 * T * T;
         kappa = 2.0 * m * T * etaD;
-        kT = sqrt(kappa);
+        kT = sqrt_pos(kappa);
     } else {
         const double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
         tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(sp.drag, T);
@@ -193,7 +195,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
             itau = drv_rcp(tau);
             etaD = itau;
             kappa = 2.0 * m * T * itau;
-            kT = sqrt(kappa);
+            kT = sqrt_pos(kappa);
             dtau_dT = eval_time_table_slope(sp.drag, T);
         }
     }
@@ -448,13 +450,16 @@ __device__ __forceinline__ lmh_params adjoint_params(const lmh_params& in) {
 
 // A backward segment, first half: re-run the segment's nk steps from its checkpoint, writing
 // each step's input state and kick noise to the tape (rows x0, x1, q..., xi...[, ucol with
-// collisions]).
+// collisions]).  With state_out (the forward pass that keeps the whole tape) the kernel is the
+// forward pass itself: it also takes the last step, writes the final state there (in place over
+// ckpt is fine: each thread reads its particle before writing it) and flags non-finite states.
 template <int PDIM, int COLL, int DRAG>
 __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, GridMedium g,
                                                                   const double4* __restrict__ slices,
-                                                                  const double* __restrict__ ckpt,
+                                                                  const double* ckpt,
                                                                   int64_t seg_step0, int32_t nk,
-                                                                  double* __restrict__ tape) {
+                                                                  double* __restrict__ tape,
+                                                                  double* state_out) {
     int seg;
     const int64_t i = particle_of_thread(a, &seg);
     if (i < 0) return;
@@ -467,6 +472,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
     double q[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = ckpt[(int64_t)(2 + d) * N + i];
+    int64_t bad = -1;
     for (int k = 0; k < nk; ++k) {
         const int64_t step = seg_step0 + k;
         double* tk = tape + (int64_t)k * TR * N + i;
@@ -481,11 +487,19 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
         for (int d = 0; d < PDIM; ++d) tk[(int64_t)(R + d) * N] = xi[d];
         if (COLL == LMH_LANGEVIN_RTA) tk[(int64_t)(R + PDIM) * N] = ucol;
         // the last step's output is the next checkpoint: only its inputs go to the tape
-        if (k + 1 < nk) {
+        if (k + 1 < nk || state_out) {
             GridMedium gk = g;
             gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
             adj_forward_step<PDIM, COLL>(a, p, gk, sp, i, step, &x0, &x1, q, xi, ucol);
+            if (state_out && bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
         }
+    }
+    if (state_out) {
+        state_out[i] = x0;
+        state_out[N + i] = x1;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) state_out[(int64_t)(2 + d) * N + i] = q[d];
+        if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
     }
 }
 

@@ -211,7 +211,7 @@ struct LmhObsSpec
     nbins::Int32; species::Int32; pt_min::Float64; pt_max::Float64; sigma::Float64
 end
 struct LmhAdjointSpec
-    checkpoint_every::Int32; _pad::Int32; dtau_dDsT::Float64
+    checkpoint_every::Int32; keep_tape::Int32; dtau_dDsT::Float64
 end
 
 struct GridEnsemble
@@ -220,12 +220,12 @@ struct GridEnsemble
     tgrid::Vector{Float64}; T::Array{Float64,3}; vx::Array{Float64,3}; vy::Array{Float64,3}
     pos::Matrix{Float64}; mom::Matrix{Float64}; m::Float64
     dt::Float64; t0::Float64; nsteps::Int; drag_model::Int32; seed::UInt64
-    checkpoint_every::Int; obs::LmhObsSpec
+    checkpoint_every::Int; keep_tape::Bool; obs::LmhObsSpec
 end
 
 function GridEnsemble(x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, pos, mom, m; dt, t0, nsteps,
-                      drag_model = 0, seed = 0, checkpoint_every = 50, nbins = 40, pt_min = 0.0,
-                      pt_max = 10.0, sigma = 0.0, device = 0)
+                      drag_model = 0, seed = 0, checkpoint_every = 50, keep_tape = false, nbins = 40,
+                      pt_min = 0.0, pt_max = 10.0, sigma = 0.0, device = 0)
     c = Ctx(device)
     GC.@preserve tgrid T vx vy check(c, ccall((:lmh_set_medium_grid2p1, liblmhip), Cint,
         (Ptr{Cvoid}, Float64, Float64, Int32, Float64, Float64, Int32, Ptr{Float64}, Int32,
@@ -233,7 +233,7 @@ function GridEnsemble(x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, pos, mom, m; dt,
         c.h, x0, dx, nx, y0, dy, ny, tgrid, length(tgrid), T, vx, vy))
     return GridEnsemble(c, x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, Matrix{Float64}(pos),
                         Matrix{Float64}(mom), m, dt, t0, nsteps, Int32(drag_model), UInt64(seed),
-                        checkpoint_every, LmhObsSpec(nbins, -1, pt_min, pt_max, sigma))
+                        checkpoint_every, keep_tape, LmhObsSpec(nbins, -1, pt_min, pt_max, sigma))
 end
 
 function _reset!(e::GridEnsemble, theta)
@@ -262,7 +262,7 @@ end
 
 function ChainRulesCore.rrule(::typeof(final_observables), theta::AbstractVector{<:Real}, e::GridEnsemble)
     p = _reset!(e, theta)
-    spec = LmhAdjointSpec(e.checkpoint_every, 0, theta[1] > 0 ? 1.0 / theta[1] : 0.0)
+    spec = LmhAdjointSpec(e.checkpoint_every, Int32(e.keep_tape), theta[1] > 0 ? 1.0 / theta[1] : 0.0)
     check(e.c, ccall((:lmh_adjoint_forward, liblmhip), Cint,
         (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ref{LmhAdjointSpec}), e.c.h, p, 1, e.nsteps, spec))
     obs = _observables(e)

@@ -64,7 +64,7 @@ class lmh_obs_spec(ctypes.Structure):
 
 
 class lmh_adjoint_spec(ctypes.Structure):
-    _fields_ = [("checkpoint_every", c_int32), ("_pad", c_int32), ("dtau_dDsT", c_double)]
+    _fields_ = [("checkpoint_every", c_int32), ("keep_tape", c_int32), ("dtau_dDsT", c_double)]
 
 
 class lmh_run_desc(ctypes.Structure):

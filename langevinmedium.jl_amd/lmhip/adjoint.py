@@ -6,7 +6,9 @@ theta = (DsT, gamma0, Lambda) and returns the observables of the final lab momen
 `lmh_observables` vector), keeping checkpoints; `GridEnsemble.pullback(cotangent)` returns
 dL/dtheta for L = cotangent . observables from the pathwise adjoint (liblmhip's
 `lmh_adjoint_backward`).  `observables(theta, ensemble)` wraps the pair as a
-torch.autograd.Function, the PyTorch counterpart of a ChainRulesCore.rrule.
+torch.autograd.Function, the PyTorch counterpart of a ChainRulesCore.rrule.  With
+keep_tape=True the forward pass keeps every step's tape row in device memory (nsteps x
+(2 + 2 pdim [+1]) x N doubles) and the pullback runs the reverse sweeps only.
 
 DsT enters through the drag-time table (rebuilt for each DsT, `transport.jl:165-209`);
 gamma0 and Lambda through the GAMMA_P drag eta = gamma0 T^2/(1 + p^2/Lambda^2).  The noise is
@@ -26,7 +28,7 @@ class GridEnsemble:
                  drag_model=_ffi.LMH_DRAG_TABLE, collision=_ffi.LMH_LANGEVIN, seed=0, stream=0,
                  checkpoint_every=50, nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.0, species=-1,
                  rta_tables=None, juttner_tables=None, nT=1024, boost_initial=True,
-                 bjorken_redshift=False):
+                 bjorken_redshift=False, keep_tape=False):
         self.ctx = ctx
         self.medium = medium
         self.masses = [float(m) for m in masses]
@@ -37,6 +39,7 @@ class GridEnsemble:
         self.drag_model, self.collision = int(drag_model), int(collision)
         self.seed, self.stream = int(seed), int(stream)
         self.K = int(checkpoint_every)
+        self.keep_tape = bool(keep_tape)
         self.obs = dict(nbins=int(nbins), pt_min=float(pt_min), pt_max=float(pt_max),
                         sigma=float(sigma), species=int(species))
         self.nT = int(nT)
@@ -88,7 +91,7 @@ class GridEnsemble:
 
     def forward(self, theta):
         p = self.reset(theta)
-        self.ctx.adjoint_forward(p, 1, self.nsteps, self.K)
+        self.ctx.adjoint_forward(p, 1, self.nsteps, self.K, keep_tape=self.keep_tape)
         self._theta = tuple(float(v) for v in theta)
         return self.ctx.observables(**self.obs)
 

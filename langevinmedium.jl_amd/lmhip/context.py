@@ -216,13 +216,14 @@ class Context:
 
     # ---- pathwise adjoint ----
     def adjoint_forward(self, params, step_begin: int, nsteps: int, checkpoint_every: int = 50,
-                        dtau_dDsT: float | None = None):
-        """Steps as run() does, keeping a checkpoint every `checkpoint_every` steps.  dtau_dDsT is
-        (d tau_drag/d DsT)/tau_drag of the drag table: 1/DsT (the default) for tables
-        proportional to DsT, 0 for the linear effective-D_s law."""
+                        dtau_dDsT: float | None = None, keep_tape: bool = False):
+        """Steps as run() does, keeping a checkpoint every `checkpoint_every` steps (with
+        keep_tape, the whole tape instead, so that the pullback only runs reverse sweeps).
+        dtau_dDsT is (d tau_drag/d DsT)/tau_drag of the drag table: 1/DsT (the default) for
+        tables proportional to DsT, 0 for the linear effective-D_s law."""
         if dtau_dDsT is None:
             dtau_dDsT = 1.0 / params.DsT if params.DsT > 0.0 else 0.0
-        spec = _ffi.lmh_adjoint_spec(int(checkpoint_every), 0, float(dtau_dDsT))
+        spec = _ffi.lmh_adjoint_spec(int(checkpoint_every), int(bool(keep_tape)), float(dtau_dDsT))
         self._check(self._lib.lmh_adjoint_forward(self._h, ctypes.byref(params), int(step_begin),
                                                   int(nsteps), ctypes.byref(spec)))
 

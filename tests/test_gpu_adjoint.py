@@ -269,6 +269,37 @@ def test_adjoint_backward_needs_its_forward_state(gpu_ok):
             ctx.adjoint_forward(bad, 1, 5, 2)
 
 
+@pytest.mark.parametrize("drag_model,rta", [(_ffi.LMH_DRAG_TABLE, True), (_ffi.LMH_DRAG_GAMMA_P, False)])
+def test_adjoint_keep_tape_equals_checkpointed(gpu_ok, drag_model, rta):
+    """keep_tape: the forward pass writes the whole tape; its final state is lmh_run's and its
+    gradients (parameters and initial state) are the checkpointed pullback's, bit for bit"""
+    N, steps = 3000, 45
+    g, masses, counts, pos, mom = _setup(N=N)
+    rt, jt = _rta_tables(g, masses) if rta else (None, None)
+    coll = _ffi.LMH_LANGEVIN_RTA if rta else _ffi.LMH_LANGEVIN
+    w = _cotangents(N, OBS["nbins"])["mixed"]
+    theta = (0.6366, 5.3, 5.0)
+    out = {}
+    with lmhip.Context(0) as ctx:
+        for keep in (False, True):
+            ens = GridEnsemble(ctx, g, masses, counts, pos, mom, dt=DT, t0=T0, nsteps=steps,
+                               drag_model=drag_model, collision=coll, seed=8, checkpoint_every=10,
+                               rta_tables=rt, juttner_tables=jt, keep_tape=keep, **OBS)
+            obs = ens.forward(theta)
+            state = ctx.get_particles()
+            grad, dpos, dmom = ens.pullback(w, state_grad=True)
+            out[keep] = (obs, state[0], state[1], grad, dpos, dmom, ens.pullback(w))
+        ens.run_plain(theta)
+        plain = ctx.get_particles()
+        # the whole tape is not kept under freeze-out capture
+        ctx.freezeout_enable(0.2)
+        with pytest.raises(lmhip.LMHError, match="freeze-out"):
+            ens.forward(theta)
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[True][1], plain[0]) and np.array_equal(out[True][2], plain[1])
+
+
 def test_torch_autograd_function(gpu_ok):
     import torch
     from lmhip.adjoint import observables
