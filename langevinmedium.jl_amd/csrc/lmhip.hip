@@ -49,7 +49,9 @@ struct StepArgs {
     int64_t block0[LMH_MAX_SPECIES + 1];
     const double* noise;
     unsigned long long* bad;    // first non-finite step (atomicMin), ULLONG_MAX if none
-    const SpeciesArgs* species; // per-species mass and tables, device memory
+    SpeciesArgs species[LMH_MAX_SPECIES];  // per-species mass and tables, read with scalar
+                                           // loads from the kernel arguments (block-uniform index)
+    const SpeciesArgs* species_dev;        // the same in device memory, for per-lane indices
     int32_t* queue;             // deferred collisions: block b's particles at [b*qblock, ...)
     int32_t* qcount;            // deferred collisions per block of the step kernel
     int32_t qblock;             // the step kernel's block size
@@ -367,8 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             const double T = jl_max(bilinear(med.T, med.nx, ir, j, xd, yd), 0.0);
             if (COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA) {
                 double etaD, kT;
-                drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
-                ou_kick<PDIM, RADIAL>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+                drag_coeffs<RADIAL>(p, sp.drag, m, T, &etaD, &kT);
+                ou_kick<PDIM, RADIAL>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
             }
             if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) && rta_collides(p, sp.rta, T, ucol)) {
                 double usamp, g[PDIM];
@@ -515,10 +517,8 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
 // one 96-byte record per particle instead of four scattered corners; the same bilinear function,
 // rounded within a few ulps of the corner form.
 __device__ __forceinline__ int grid_cell(double f, int n) {
-    double fl = floor(f);
-    if (!(fl >= 0.0)) return 0;
-    if (fl > (double)(n - 2)) return n - 2;
-    return (int)fl;
+    // clamp(floor(f), 0, n - 2) with min/max (a NaN f takes cell 0)
+    return (int)fmin(fmax(floor(f), 0.0), (double)(n - 2));
 }
 
 __device__ __forceinline__ double4 lerp4(const double4& a, const double4& b, double td) {
@@ -533,8 +533,10 @@ __device__ __forceinline__ double4 lerp4(const double4& a, const double4& b, dou
 template <bool SLICE>
 __device__ __forceinline__ void grid_sample(const GridMedium& g, double x, double y, int j,
                                             double td, double* T, double* vx, double* vy) {
-    double xc = jl_clamp(x, g.x0, g.xlast);
-    double yc = jl_clamp(y, g.y0, g.ylast);
+    // the query clamped to the grid as interpolate_2d_cpu does (kernels_cpu.jl:27-28); min/max
+    // take a NaN coordinate to the lower edge (its state is non-finite already)
+    double xc = fmin(fmax(x, g.x0), g.xlast);
+    double yc = fmin(fmax(y, g.y0), g.ylast);
     double fx = (xc - g.x0) * g.inv_dx;
     double fy = (yc - g.y0) * g.inv_dy;
     int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
@@ -580,7 +582,7 @@ struct FlowBoost {
 
 __device__ __forceinline__ FlowBoost flow_boost(double vx, double vy) {
     FlowBoost b;
-    double vm2 = vx * vx + vy * vy;
+    double vm2 = fma(vy, vy, vx * vx);
     // |v| and 1/|v| from one reciprocal square root (an ulp from sqrt and vx/|v|); v = 0 gives
     // |v| = 0 and the boost is off
     const double inv = rsqrt_pos(fmax(vm2, 1e-300));
@@ -604,17 +606,18 @@ __device__ __forceinline__ void apply_boost(double* q, double m, const FlowBoost
         q[1] += s * (m * b.vmag * b.ny);
         return;
     }
-    double p2 = 0.0;
+    // the reference's expressions with each product-sum as one fused multiply-add
+    double p2 = q[0] * q[0];
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) p2 += q[d] * q[d];
-    double E = sqrt_pos(p2 + m * m);
-    double px = q[0], py = q[1];
-    double p_par = px * b.nx + py * b.ny;
-    double p_perp_x = px - p_par * b.nx;
-    double p_perp_y = py - p_par * b.ny;
-    double p_par_new = sign > 0 ? b.gam * (p_par + b.v * E) : b.gam * (p_par - b.v * E);
-    q[0] = p_par_new * b.nx + p_perp_x;
-    q[1] = p_par_new * b.ny + p_perp_y;
+    for (int d = 1; d < PDIM; ++d) p2 = fma(q[d], q[d], p2);
+    const double E = sqrt_pos(fma(m, m, p2));
+    const double px = q[0], py = q[1];
+    const double p_par = fma(py, b.ny, px * b.nx);
+    const double p_perp_x = fma(-p_par, b.nx, px);
+    const double p_perp_y = fma(-p_par, b.ny, py);
+    const double p_par_new = b.gam * (sign > 0 ? fma(b.v, E, p_par) : fma(-b.v, E, p_par));
+    q[0] = fma(p_par_new, b.nx, p_perp_x);
+    q[1] = fma(p_par_new, b.ny, p_perp_y);
 }
 
 // kernel_set_to_fluid_velocity_cpu! (kernels_cpu.jl:713-731) with r_hat -> v_hat; a fluid at
@@ -647,12 +650,12 @@ __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p
                                           double m, double* x0, double* x1, const double* q) {
     double E2 = m * m;
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
+    for (int d = 0; d < PDIM; ++d) E2 = fma(q[d], q[d], E2);
     // dt*p/E as p*(dt/E) with dt/E = dt/sqrt(E^2) from one reciprocal square root, within an
     // ulp or two of the reference form
     const double dtE = p.relativistic ? p.dt * rsqrt_pos(E2) : p.dt / m;
-    *x0 = *x0 + q[0] * dtE;
-    *x1 = *x1 + q[1] * dtE;
+    *x0 = fma(q[0], dtE, *x0);
+    *x1 = fma(q[1], dtE, *x1);
     if (p.position_diffusion) {
         double Td, ux, uy;
         grid_sample<SLICE>(g, *x0, *x1, j, td, &Td, &ux, &uy);
@@ -749,6 +752,8 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
     double q[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    // every state load is issued before the medium sample's address arithmetic waits for x, y
+    __builtin_amdgcn_sched_barrier(0);
     // freeze-out capture runs in the generic, non-deferred kernels only
     const bool fo = !SPEC && !DEFER && a.fo.on;
     bool frozen = false;
@@ -773,6 +778,11 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
 #pragma unroll
             for (int d = 0; d < PDIM; ++d) qin[d] = q[d];
         }
+        // the kick noise first: its arithmetic needs no memory and runs while the state and the
+        // medium sample are in flight
+        double xi[PDIM];
+        double ucol = 1.0;
+        noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
         double Tm, vx, vy;
         grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
         const FlowBoost fb = flow_boost(vx, vy);
@@ -782,15 +792,13 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
             glue_to_flow<PDIM>(q, m, vx, vy, p.relativistic);
         } else {
             const double T = jl_max(Tm, 0.0);
+            // both time-table reads are issued together, before the kick needs the first
+            const double tn = HAS_RTA ? rta_lookup(sp.rta, T) : 0.0;
             double etaD = 0.0, kT = 0.0;
-            if (HAS_KICK) drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
-            // the noise is drawn where it is used, to keep it out of the live range of the rest
-            double xi[PDIM];
-            double ucol = 1.0;
-            noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
-            if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+            if (HAS_KICK) drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
+            if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
             if (HAS_RTA) {
-                const bool col = rta_collides(p, sp.rta, T, ucol);
+                const bool col = rta_decide(p, tn, T, ucol);
                 if (DEFER) {
                     const int rank = block_rank(valid && col, &a.qcount[blockIdx.x]);
                     if (!valid) return;
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
 #pragma unroll
             for (int s2 = 1; s2 < LMH_MAX_SPECIES; ++s2)
                 if (s2 < a.nseg && i >= a.begin[s2]) seg = s2;
-            const SpeciesArgs sp = a.species[seg];
+            const SpeciesArgs sp = a.species_dev[seg];
             const double m = sp.m;
             double x0 = a.pos[0][i], x1 = a.pos[1][i];
             double q[PDIM];
@@ -1270,7 +1278,8 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     a.begin[LMH_MAX_SPECIES] = b;
     a.block0[LMH_MAX_SPECIES] = blk;
     // segments past nseg are empty and never selected
-    a.species = c->d_species;
+    memcpy(a.species, c->h_species, sizeof(a.species));
+    a.species_dev = c->d_species;
     a.queue = c->d_queue;
     a.qcount = c->d_qcount;
     a.qblock = block;

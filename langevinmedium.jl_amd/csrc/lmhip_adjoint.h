@@ -67,8 +67,8 @@ struct SampleGrad {
 };
 
 __device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, SampleGrad* s) {
-    double xc = jl_clamp(x, g.x0, g.xlast);
-    double yc = jl_clamp(y, g.y0, g.ylast);
+    double xc = fmin(fmax(x, g.x0), g.xlast);
+    double yc = fmin(fmax(y, g.y0), g.ylast);
     double fx = (xc - g.x0) * g.inv_dx;
     double fy = (yc - g.y0) * g.inv_dy;
     int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
@@ -108,8 +108,8 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_pa
     if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
     const double T = jl_max(Tm, 0.0);
     double etaD = 0.0, kT = 0.0;
-    drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
-    ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+    drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
+    ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
     if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
         double usamp, gg[PDIM];
         noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
@@ -541,7 +541,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
 #pragma unroll
             for (int r = 0; r < TR; ++r) row[r] = tn[(int64_t)r * N];
         }
-        const double ucol = COLL == LMH_LANGEVIN_RTA ? cur[R + PDIM] : 1.0;
+        // the tape row holds the collision uniform only with collisions (index kept in range
+        // for the row without it)
+        const double ucol = COLL == LMH_LANGEVIN_RTA ? cur[COLL == LMH_LANGEVIN_RTA ? R + PDIM : 0] : 1.0;
         GridMedium gk = g;
         gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
         adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, cur + R, ucol,

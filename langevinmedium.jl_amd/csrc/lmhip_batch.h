@@ -59,11 +59,11 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
     } else {
         const double T = jl_max(Tm, 0.0);
         double etaD = 0.0, kT = 0.0;
-        if (HAS_KICK) drag_coeffs(p, sp.drag, m, T, &etaD, &kT);
+        if (HAS_KICK) drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
         double xi[PDIM];
         double ucol = 1.0;
         noise_kick<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, xi, &ucol);
-        if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, m, a.sqrt_dt, q, xi);
+        if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
         if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
             double usamp, gg[PDIM];
             noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);

@@ -64,6 +64,13 @@ __device__ __forceinline__ double bilinear(const double* __restrict__ f, int nx,
     return c0 * (1.0 - yd) + c1 * yd;
 }
 
+// A load of read-only table data through the global address space: a global rather than a
+// flat access (the tables' pointers come from memory, where the compiler cannot tell their
+// address space), so that waiting for it does not also wait for LDS and scalar loads.
+__device__ __forceinline__ double ld_global(const double* p) {
+    return *(const __attribute__((address_space(1))) double*)p;
+}
+
 // eval_tau_n_spline, transport.jl:231-253 (index clamped, fraction not clamped)
 struct TimeTable {
     const double* vals;
@@ -72,18 +79,19 @@ struct TimeTable {
 };
 
 __device__ __forceinline__ double eval_time_table(const TimeTable& t, double T) {
-    if (t.n < 2) return t.vals[0];
-    if (!isfinite(T)) return 0.0;
     double u = (T - t.Tmin) * t.invdT;
-    double fu = floor(u);
-    int i;
-    if (!(fu >= 0.0)) i = 1;
-    else if (fu > (double)(t.n - 2)) i = t.n - 1;
-    else i = (int)fu + 1;
-    double w = u - (double)(i - 1);
-    double y0 = t.vals[i - 1];
-    double y1 = t.vals[i];
-    return (1.0 - w) * y0 + w * y1;
+    // the 1-based segment i = clamp(floor(u) + 1, 1, n - 1) as a clamp of floor(u) in double
+    // (min/max instead of branches; a NaN u takes the first segment), node 0 for a one-node
+    // table.  The loads are in bounds for any T, so they are issued unconditionally and the
+    // special cases select afterwards.
+    const double fc = fmin(fmax(floor(u), 0.0), (double)(t.n - 2));
+    const bool one = t.n < 2;
+    const int i = one ? 0 : (int)fc + 1;
+    double w = u - fc;
+    double y0 = ld_global(t.vals + (one ? 0 : i - 1));
+    double y1 = ld_global(t.vals + i);
+    const double r = (1.0 - w) * y0 + w * y1;
+    return one ? y0 : (isfinite(T) ? r : 0.0);
 }
 
 // inverse-CDF Jüttner table and its bilinear lookup, kernels_gpu.jl:88-117
@@ -109,8 +117,8 @@ __device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double 
     tT = tT < 0.0 ? 0.0 : (tT > 1.0 ? 1.0 : tT);
     const double* b0 = jt.vals + (int64_t)jT * jt.nU + iu;
     const double* b1 = b0 + jt.nU;
-    double v00 = b0[0], v10 = b0[1];
-    double v01 = b1[0], v11 = b1[1];
+    double v00 = ld_global(b0), v10 = ld_global(b0 + 1);
+    double v01 = ld_global(b1), v11 = ld_global(b1 + 1);
     double c0 = v00 * (1.0 - tu) + v10 * tu;
     double c1 = v01 * (1.0 - tu) + v11 * tu;
     return c0 * (1.0 - tT) + c1 * tT;
@@ -173,6 +181,15 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
     return y;
 }
 
+// 1/x for finite x > 0 (normal): the hardware estimate refined by two Newton steps, within an
+// ulp of the correctly rounded 1/x at half the instructions of a division
+__device__ __forceinline__ double rcp_pos(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+
 // sqrt(x) for x >= 0: g = x/sqrt(x) from the once-refined estimate, then one residual step
 // g + (x - g^2)/(2 sqrt(x)); 0 and NaN pass through.  No rescaling of tiny or huge arguments
 // (the callers' arguments are energies, rates and variances far from the exponent limits).
@@ -185,10 +202,11 @@ __device__ __forceinline__ double sqrt_pos(double x) {
     return x > 0.0 ? r : x;
 }
 
-// The two transcendental functions of the Box-Muller transform, written with correctly rounded
-// operations only (+, -, *, /, fma, floor, exact bit manipulation), so that the same definition
-// gives the same bits on the host and on the device; the radius's square root is sqrt_pos here
-// (within an ulp of the host's correctly rounded sqrt).
+// The two transcendental functions of the Box-Muller transform, the same definition as the host
+// restatement of the noise record (series, splitting and quadrant logic, fused multiply-adds in
+// the same places), from the refined reciprocal and square-root estimates (within an ulp of the
+// correctly rounded division and square root the host uses), so the two records agree within
+// an ulp or two.
 //
 // noise_log(u), u in [2^-53, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with
 // s = (m - 1)/(m + 1), |s| <= 0.1716, by the atanh series through s^19 (truncation < 3e-17).
@@ -203,7 +221,7 @@ __device__ __forceinline__ double noise_log(double u) {
         m = 0.5 * m;
         e += 1;
     }
-    double s = (m - 1.0) / (m + 1.0);
+    double s = (m - 1.0) * rcp_pos(m + 1.0);
     double z = s * s;
     double p = 1.0 / 19.0;
     p = fma(p, z, 1.0 / 17.0);
@@ -220,25 +238,24 @@ __device__ __forceinline__ double noise_log(double u) {
 }
 
 // noise_sincospi(x), x in [0, 2): x = k/2 + r with |r| <= 1/4 (exact), Taylor polynomials of
-// sin(pi r) through r^19 and cos(pi r) through r^20 (truncation < 1e-17), then the quadrant.
+// sin(pi r) through r^15 and cos(pi r) through r^16 (truncation below 7e-17 and 2e-18 relative),
+// then the quadrant.
 __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs) {
-    constexpr double SP[10] = {3.141592653589793, -5.16771278004997, 2.5501640398773455,
-                               -0.5992645293207921, 0.08214588661112823, -0.0073704309457143504,
-                               0.00046630280576761255, -2.1915353447830217e-05,
-                               7.952054001475513e-07, -2.2948428997269873e-08};
-    constexpr double CP[11] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895,
-                               0.2353306303588932, -0.02580689139001406, 0.0019295743094039231,
-                               -0.0001046381049248457, 4.303069587032947e-06,
-                               -1.3878952462213771e-07, 3.604730797462501e-09};
+    constexpr double SP[8] = {3.141592653589793, -5.16771278004997, 2.5501640398773455,
+                              -0.5992645293207921, 0.08214588661112823, -0.0073704309457143504,
+                              0.00046630280576761255, -2.1915353447830217e-05};
+    constexpr double CP[9] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895,
+                              0.2353306303588932, -0.02580689139001406, 0.0019295743094039231,
+                              -0.0001046381049248457, 4.303069587032947e-06};
     double k = floor(2.0 * x + 0.5);
     double r = x - 0.5 * k;
     double z = r * r;
-    double ps = SP[9];
+    double ps = SP[7];
 #pragma unroll
-    for (int j = 8; j >= 0; --j) ps = fma(ps, z, SP[j]);
-    double pc = CP[10];
+    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[8];
 #pragma unroll
-    for (int j = 9; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
     double sr = r * ps, cr = pc;
     int q = (int)k & 3;
     double s1 = (q & 1) ? cr : sr;
@@ -257,8 +274,30 @@ __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0,
 }
 
 // ---------------------------------------------------------------------------------------------
-// drag/noise coefficients eta_D and kT from the local (clamped) T: kernels_cpu.jl:270-290, or
-// the GAMMA_P model eta_D = gamma0*T^2, kappa = 2 m T eta_D.
+// exp(-x) for x >= 0.  Up to x = 1/16 (every drag attenuation of a step of a few hundredths of
+// tau) the Taylor polynomial through x^9, whose remainder (< 3e-19) is far below the rounding
+// of the result; larger or non-finite arguments take exp.
+__device__ __forceinline__ double exp_neg(double x) {
+    if (!(x <= 0.0625)) return exp(-x);
+    const double y = -x;
+    double r = 1.0 / 362880.0;
+    r = fma(r, y, 1.0 / 40320.0);
+    r = fma(r, y, 1.0 / 5040.0);
+    r = fma(r, y, 1.0 / 720.0);
+    r = fma(r, y, 1.0 / 120.0);
+    r = fma(r, y, 1.0 / 24.0);
+    r = fma(r, y, 1.0 / 6.0);
+    r = fma(r, y, 0.5);
+    r = fma(r, y, 1.0);
+    return fma(r, y, 1.0);
+}
+
+// drag/noise coefficients from the local (clamped) T: kernels_cpu.jl:270-290, eta_D = 1/tau and
+// kT = sqrt(kappa), kappa = 2 m T / tau; or the GAMMA_P model eta_D = gamma0*T^2,
+// kappa = 2 m T eta_D.  eta_D = 0 gives kappa = 0 in both models.  EXACT keeps the reference's
+// divisions (the radial-mode Euler kick uses kT); otherwise 1/tau is the refined reciprocal and
+// kappa = 2 m T eta_D (the exact-OU kick below needs kT only in its radial form).
+template <bool EXACT>
 __device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable& tab, double m,
                                             double T, double* etaD, double* kT) {
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
@@ -270,20 +309,39 @@ __device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable
     double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
     double tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(tab, T);
     bool ok = tau > 0.0 && isfinite(tau);
-    *etaD = ok ? (1.0 / tau) : 0.0;
-    double kappa = ok ? ((2.0 * m * T) / tau) : 0.0;
-    *kT = sqrt_pos(kappa);
+    if (EXACT) {
+        *etaD = ok ? (1.0 / tau) : 0.0;
+        double kappa = ok ? ((2.0 * m * T) / tau) : 0.0;
+        *kT = sqrt_pos(kappa);
+        return;
+    }
+    const double e = ok ? rcp_pos(tau) : 0.0;
+    *etaD = e;
+    *kT = sqrt_pos((2.0 * m * T) * e);
 }
 
-// effective drag at momentum q: eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
-// or eta_D/(1+|q|^2/Lambda^2) for GAMMA_P
+// effective drag at momentum q -- eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
+// or eta_D/(1+|q|^2/Lambda^2) for GAMMA_P -- and c = kappa/(2 eta_eff) = m T eta_D/eta_eff:
+// T E (relativistic), m T, or m T (1+|q|^2/Lambda^2) (GAMMA_P)
 template <int PDIM>
-__device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, double m, const double* q) {
-    double q2 = 0.0;
+__device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, double m, double T,
+                                         const double* q, double* c) {
+    double q2 = q[0] * q[0];
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) q2 += q[d] * q[d];
-    if (p.drag_model == LMH_DRAG_GAMMA_P) return etaD / (1.0 + q2 / (p.Lambda * p.Lambda));
-    return p.relativistic ? (etaD * m) * rsqrt_pos(q2 + m * m) : etaD;
+    for (int d = 1; d < PDIM; ++d) q2 = fma(q[d], q[d], q2);
+    if (p.drag_model == LMH_DRAG_GAMMA_P) {
+        const double den = 1.0 + q2 / (p.Lambda * p.Lambda);
+        *c = (m * T) * den;
+        return etaD / den;
+    }
+    if (p.relativistic) {
+        const double E2 = fma(m, m, q2);
+        const double iE = rsqrt_pos(E2);
+        *c = T * (E2 * iE);
+        return (etaD * m) * iE;
+    }
+    *c = m * T;
+    return etaD;
 }
 
 // The LRF momentum kick: kernel_compute_all_forces_cpu! (kernels_cpu.jl:293-352) followed by
@@ -291,9 +349,13 @@ __device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, doubl
 // form of radial mode (:293-323, sto = kT*xi since kL == kT).  MIDPOINT follows
 // kernels_gpu.jl:368-381; POSTPOINT re-evaluates the drag at the updated momentum by three
 // fixed-point sweeps.
+//
+// The exact-OU noise sqrt(dt)*sto = sqrt(dt) kT sqrt((1 - a^2)/(2 eta_eff dt)) xi (:340-349) is
+// written as sqrt((1 - a^2) kappa/(2 eta_eff)) xi = sqrt((1 - a^2) c) xi with c from eta_of: the
+// same quantity without the divisions (within a few ulps of the reference's rounding).
 template <int PDIM, bool RADIAL>
-__device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double kT, double m,
-                                        double sqrt_dt, double* q, const double* xi) {
+__device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double kT, double T,
+                                        double m, double sqrt_dt, double* q, const double* xi) {
     const double dt = p.dt;
     if (RADIAL) {
         double s = 0.0;
@@ -310,62 +372,63 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
         }
         return;
     }
-    if (p.integrator == LMH_POSTPOINT && etaD > 0.0 && dt > 0.0) {
-        double qk[PDIM], dk[PDIM], sk[PDIM];
+    if (!(etaD > 0.0)) {
+        // eta_D = 0: kappa = 0, the kick is sqrt(dt) * (0 * xi)
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = q[d] + (0.0 + sqrt_dt * (0.0 * xi[d]));
+        return;
+    }
+    double c;
+    if (p.integrator == LMH_POSTPOINT && dt > 0.0) {
+        double qk[PDIM];
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) qk[d] = q[d];
         for (int it = 0; it < 3; ++it) {
-            double eta_eff = eta_of<PDIM>(p, etaD, m, qk);
-            double a = exp(-eta_eff * dt);
-            double noise_pref = eta_eff > 0.0 ? (kT * sqrt((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
+            const double eta_eff = eta_of<PDIM>(p, etaD, m, T, qk, &c);
+            const double a = exp_neg(eta_eff * dt);
+            const double s = sqrt_pos(fma(-a, a, 1.0) * c);
 #pragma unroll
-            for (int d = 0; d < PDIM; ++d) {
-                dk[d] = (a - 1.0) * q[d];
-                sk[d] = noise_pref * xi[d];
-            }
-#pragma unroll
-            for (int d = 0; d < PDIM; ++d) qk[d] = q[d] + (dk[d] + sqrt_dt * sk[d]);
+            for (int d = 0; d < PDIM; ++d) qk[d] = q[d] + fma(s, xi[d], (a - 1.0) * q[d]);
         }
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) q[d] = qk[d];
         return;
     }
-    if (etaD > 0.0) {
-        double eta_eff = eta_of<PDIM>(p, etaD, m, q);
-        if (p.integrator == LMH_MIDPOINT && dt > 0.0) {
-            double a_half = exp(-eta_eff * (0.5 * dt));
-            double qm[PDIM];
+    double eta_eff = eta_of<PDIM>(p, etaD, m, T, q, &c);
+    if (p.integrator == LMH_MIDPOINT && dt > 0.0) {
+        const double a_half = exp_neg(eta_eff * (0.5 * dt));
+        double qm[PDIM];
 #pragma unroll
-            for (int d = 0; d < PDIM; ++d) qm[d] = a_half * q[d];
-            eta_eff = eta_of<PDIM>(p, etaD, m, qm);
-        }
-        double a = exp(-eta_eff * dt);
-        double noise_pref = (dt > 0.0 && eta_eff > 0.0) ? (kT * sqrt_pos((1.0 - a * a) / (2.0 * eta_eff * dt))) : 0.0;
-#pragma unroll
-        for (int d = 0; d < PDIM; ++d) {
-            double det = (a - 1.0) * q[d];
-            double sto = noise_pref * xi[d];
-            q[d] = q[d] + (det + sqrt_dt * sto);
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < PDIM; ++d) {
-            double sto = kT * xi[d];
-            q[d] = q[d] + (0.0 + sqrt_dt * sto);
-        }
+        for (int d = 0; d < PDIM; ++d) qm[d] = a_half * q[d];
+        eta_eff = eta_of<PDIM>(p, etaD, m, T, qm, &c);
     }
+    const double a = exp_neg(eta_eff * dt);
+    const double s = dt > 0.0 ? sqrt_pos(fma(-a, a, 1.0) * c) : 0.0;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = q[d] + fma(s, xi[d], (a - 1.0) * q[d]);
 }
 
 // BGK collision in the LRF: kernel_rta_collision_cpu! (kernels_cpu.jl:485-505) with the
 // inverse-CDF draw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701).  The decision and the
 // redraw are split so that the redraw's noise is generated only for colliding particles.
-__device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTable& taun, double T,
-                                             double ucol) {
-    double tau_n = (p.DsT > 0.0 && T > 0.0) ? eval_time_table(taun, T) : 0.0;
+// The collision test, split so that the table read can be issued early and its value used late:
+// rta_lookup reads tau_n(T) (the table must exist), rta_decide applies the reference's cases
+// (no DsT or T <= 0: tau_n = 0, a certain collision).
+__device__ __forceinline__ double rta_lookup(const TimeTable& taun, double T) {
+    return eval_time_table(taun, T);
+}
+
+__device__ __forceinline__ bool rta_decide(const lmh_params& p, double tn, double T, double ucol) {
+    const double tau_n = (p.DsT > 0.0 && T > 0.0) ? tn : 0.0;
     // ucol < clamp(dt/tau_n, 0, 1) with ucol in [0, 1) is ucol * tau_n < dt for tau_n > 0 (no
     // division; the two forms differ only when ucol * tau_n is within an ulp of dt)
     if (!(tau_n > 0.0 && isfinite(tau_n))) return ucol < 1.0;
     return ucol * tau_n < p.dt;
+}
+
+__device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTable& taun, double T,
+                                             double ucol) {
+    return rta_decide(p, rta_lookup(taun, T), T, ucol);
 }
 
 template <int PDIM, bool RADIAL>

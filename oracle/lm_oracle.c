@@ -418,25 +418,25 @@ static inline double noise_log(double u) {
 }
 
 /* noise_sincospi(x): sin(pi x) and cos(pi x) for x in [0, 2): x = k/2 + r with k the nearest
- * half-integer count and |r| <= 1/4 (exact), the Taylor polynomials of sin(pi r) through r^19
- * and cos(pi r) through r^20 (truncation < 1e-17), then the quadrant k mod 4.                 */
-static const double SP[10] = {  /* pi^(2j+1) (-1)^j / (2j+1)!, rounded to double */
+ * half-integer count and |r| <= 1/4 (exact), the Taylor polynomials of sin(pi r) through r^15
+ * and cos(pi r) through r^16 (truncation below 7e-17 and 2e-18 relative), then the quadrant
+ * k mod 4.                                                                                     */
+static const double SP[8] = {  /* pi^(2j+1) (-1)^j / (2j+1)!, rounded to double */
     3.141592653589793, -5.16771278004997, 2.5501640398773455, -0.5992645293207921,
     0.08214588661112823, -0.0073704309457143504, 0.00046630280576761255,
-    -2.1915353447830217e-05, 7.952054001475513e-07, -2.2948428997269873e-08};
-static const double CP[11] = {  /* pi^(2j) (-1)^j / (2j)!, rounded to double */
+    -2.1915353447830217e-05};
+static const double CP[9] = {  /* pi^(2j) (-1)^j / (2j)!, rounded to double */
     1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895, 0.2353306303588932,
-    -0.02580689139001406, 0.0019295743094039231, -0.0001046381049248457, 4.303069587032947e-06,
-    -1.3878952462213771e-07, 3.604730797462501e-09};
+    -0.02580689139001406, 0.0019295743094039231, -0.0001046381049248457, 4.303069587032947e-06};
 
 static inline void noise_sincospi(double x, double* sn, double* cs) {
     double k = floor(2.0 * x + 0.5);
     double r = x - 0.5 * k;
     double z = r * r;
-    double ps = SP[9];
-    for (int j = 8; j >= 0; --j) ps = fma(ps, z, SP[j]);
-    double pc = CP[10];
-    for (int j = 9; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    double ps = SP[7];
+    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[8];
+    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
     double sr = r * ps, cr = pc;
     int q = (int)k & 3;
     double s1 = (q & 1) ? cr : sr;
