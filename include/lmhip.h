@@ -147,6 +147,14 @@ typedef struct lmh_tuning {
     int32_t specialize;        /* 1: the common configuration (relativistic, pre-point, drag
                                   table, no Bjorken redshift, no position diffusion) runs
                                   kernels compiled for it, with identical results (default 1) */
+    int32_t sort_every;        /* 2+1D grid, Philox noise, no freeze-out capture: lmh_run keeps
+                                  the particles ordered by grid cell (species segments kept),
+                                  sorting them on the device before its first step and again
+                                  every sort_every steps, so that neighbouring threads read
+                                  neighbouring medium cells.  Each particle keeps its identity
+                                  (its noise counters), so results do not depend on it; every
+                                  other entry point first restores the caller's order.  0: never
+                                  (default 100)                                                 */
 } lmh_tuning;
 
 /* device time per launch class, recorded with HIP events on the context's stream between
@@ -154,9 +162,10 @@ typedef struct lmh_tuning {
 #define LMH_PROF_STEP      0   /* the step kernels                                         */
 #define LMH_PROF_REDRAW    1   /* the deferred-collision kernels                           */
 #define LMH_PROF_SLICE     2   /* the medium-slice kernels                                 */
+#define LMH_PROF_SORT      3   /* the cell sort (keys, radix sort, permutation)            */
 typedef struct lmh_profile {
-    int64_t launches[3];
-    double  ms[3];
+    int64_t launches[4];
+    double  ms[4];
 } lmh_profile;
 
 /* pathwise adjoint (extension): the forward pass keeps the state every checkpoint_every steps.

@@ -10,6 +10,7 @@
 // Noise is either a fixed record read from HBM (for per-particle parity with the CPU kernels)
 // or Philox4x32-10 generated in registers.
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -52,6 +53,8 @@ struct StepArgs {
     SpeciesArgs species[LMH_MAX_SPECIES];  // per-species mass and tables, read with scalar
                                            // loads from the kernel arguments (block-uniform index)
     const SpeciesArgs* species_dev;        // the same in device memory, for per-lane indices
+    const uint32_t* id;         // a particle's identity (its noise counters) when the particles
+                                // are ordered by grid cell; null: the slot is the identity
     int32_t* queue;             // deferred collisions: block b's particles at [b*qblock, ...)
     int32_t* qcount;            // deferred collisions per block of the step kernel
     int32_t qblock;             // the step kernel's block size
@@ -643,7 +646,8 @@ __device__ __forceinline__ void glue_to_flow(double* q, double m, double vx, dou
     q[1] = m * gam * v * ny;
 }
 
-// position push (kernels_cpu.jl:537-541, 586-607) with the optional position diffusion
+// position push (kernels_cpu.jl:537-541, 586-607) with the optional position diffusion; i is
+// the particle's identity (its noise counters, the record column in fixed-noise mode)
 template <int PDIM, int NOISE, bool SLICE>
 __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                           int64_t i, int64_t step, int s_off, int j, double td,
@@ -729,6 +733,58 @@ __global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* s
     slices[k] = slice_coeffs(g, j, td, r / 3, (int)(r % 3));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Cell sorting (extension).  The quarks are kept ordered by the grid cell of their position
+// within each species segment, so that the lanes of a wave read the same few medium records and
+// the slice gathers coalesce.  A sort is a stable radix sort of (segment, cell) keys with the
+// slot as value, then one gather of every state row and of the identity array through the
+// permutation.  The identity array carries each quark's original slot, which keys its Philox
+// noise, so a sorted run draws the same noise for every quark as the unsorted run.  Restoring
+// the caller's order is the same gather after a radix sort by identity.
+struct SortKeyArgs {
+    const double* x;
+    const double* y;
+    int64_t begin[LMH_MAX_SPECIES + 1];
+    int32_t nseg;
+    int32_t cell_bits;
+    int64_t N;
+};
+
+// key = segment << cell_bits | cell (the cell grid_sample reads), value = slot
+__global__ __launch_bounds__(kBlock) void k_sort_keys(SortKeyArgs a, GridMedium g, uint32_t* keys, uint32_t* vals) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= a.N) return;
+    int seg = 0;
+#pragma unroll
+    for (int k = 1; k < LMH_MAX_SPECIES; ++k)
+        if (k < a.nseg && s >= a.begin[k]) seg = k;
+    const double xc = fmin(fmax(a.x[s], g.x0), g.xlast);
+    const double yc = fmin(fmax(a.y[s], g.y0), g.ylast);
+    const int ix = grid_cell((xc - g.x0) * g.inv_dx, g.nx);
+    const int iy = grid_cell((yc - g.y0) * g.inv_dy, g.ny);
+    keys[s] = ((uint32_t)seg << a.cell_bits) | (uint32_t)(ix + (g.nx - 1) * iy);
+    vals[s] = (uint32_t)s;
+}
+
+// key = identity, value = slot (the sort that restores the caller's order)
+__global__ __launch_bounds__(kBlock) void k_unsort_keys(const uint32_t* id, int64_t N, uint32_t* keys, uint32_t* vals) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= N) return;
+    keys[s] = id[s];
+    vals[s] = (uint32_t)s;
+}
+
+// dst row r, slot s = src row r, slot perm[s]; the identity likewise (the slot itself when src_id
+// is null, i.e. the particles are in the caller's order)
+__global__ __launch_bounds__(kBlock) void k_permute(const double* src, double* dst, int32_t rows, int64_t N,
+                                                    const uint32_t* perm, const uint32_t* src_id, uint32_t* dst_id) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= N) return;
+    const int64_t from = perm[s];
+    for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + s] = src[(int64_t)r * N + from];
+    dst_id[s] = src_id ? src_id[from] : (uint32_t)from;
+}
+
 // three waves per SIMD: the register budget the step needs with a few spilled values, against
 // two waves at its unconstrained register count
 constexpr int kStepWaves = 3;
@@ -752,6 +808,8 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
     double q[PDIM];
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    // the particle's identity keys its noise (the slot unless the particles are cell-sorted)
+    const int64_t pid = a.id ? (int64_t)a.id[i] : i;
     // every state load is issued before the medium sample's address arithmetic waits for x, y
     __builtin_amdgcn_sched_barrier(0);
     // freeze-out capture runs in the generic, non-deferred kernels only
@@ -782,7 +840,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
         // medium sample are in flight
         double xi[PDIM];
         double ucol = 1.0;
-        noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
+        noise_kick<PDIM, NOISE>(a, pid, step, s_off, xi, &ucol);
         double Tm, vx, vy;
         grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
         const FlowBoost fb = flow_boost(vx, vy);
@@ -810,13 +868,13 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
                     }
                 } else if (col) {
                     double usamp, gg[PDIM];
-                    noise_redraw<PDIM, NOISE>(a, i, step, s_off, &usamp, gg);
+                    noise_redraw<PDIM, NOISE>(a, pid, step, s_off, &usamp, gg);
                     rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
                 }
             }
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
         }
-        grid_push<PDIM, NOISE, SLICE>(a, p, g, i, step, s_off, j, td, m, &x0, &x1, q);
+        grid_push<PDIM, NOISE, SLICE>(a, p, g, pid, step, s_off, j, td, m, &x0, &x1, q);
         if (fo && !frozen) {
             double Tf, ux, uy;
             grid_sample<SLICE>(g, x0, x1, j, td, &Tf, &ux, &uy);
@@ -895,10 +953,11 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
             const FlowBoost fb = flow_boost(vx, vy);
             const double T = jl_max(Tm, 0.0);
             double usamp, gg[PDIM];
-            noise_redraw<PDIM, NOISE>(a, i, step, 0, &usamp, gg);
+            const int64_t pid = a.id ? (int64_t)a.id[i] : i;
+            noise_redraw<PDIM, NOISE>(a, pid, step, 0, &usamp, gg);
             rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
-            grid_push<PDIM, NOISE, SLICE>(a, p, g, i, step, 0, j, td, m, &x0, &x1, q);
+            grid_push<PDIM, NOISE, SLICE>(a, p, g, pid, step, 0, j, td, m, &x0, &x1, q);
             a.pos[0][i] = x0;
             a.pos[1][i] = x1;
 #pragma unroll
@@ -1106,7 +1165,20 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 1, 1, 1};
+    lmh_tuning tune = {256, 1, 1, 1, 1, 100};
+
+    // cell sorting: the alternate state and identity buffers of the permutation, the radix
+    // sort's keys, values and workspace; is_sorted says the state is in cell order with d_id
+    // holding each slot's identity
+    bool is_sorted = false;
+    int64_t steps_since_sort = 0;
+    double* d_state_alt = nullptr;
+    uint32_t* d_id = nullptr;
+    uint32_t* d_id_alt = nullptr;
+    uint32_t* d_keys = nullptr;       // keys in, keys out, values in, values out: 4 N words
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int64_t sort_n = 0;               // N the sort buffers were allocated for
 
     // tables
     struct Tab { double* d = nullptr; double Tmin = 0, invdT = 1; int n = 0; };
@@ -1280,6 +1352,7 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     // segments past nseg are empty and never selected
     memcpy(a.species, c->h_species, sizeof(a.species));
     a.species_dev = c->d_species;
+    a.id = c->is_sorted ? c->d_id : nullptr;
     a.queue = c->d_queue;
     a.qcount = c->d_qcount;
     a.qblock = block;
@@ -1390,6 +1463,97 @@ struct LaunchMark {
     }
 };
 
+// ---- cell sorting (host side) ----
+int bits_for(uint64_t n) {   // bits that hold the values 0 .. n - 1
+    int b = 0;
+    while (b < 32 && ((uint64_t)1 << b) < n) ++b;
+    return b;
+}
+
+int sort_alloc(lmh_ctx* c) {
+    if (c->sort_n == c->N && c->d_state_alt) return LMH_OK;
+    const size_t N = (size_t)c->N, rows = (size_t)(c->dims + c->pdim);
+    dfree(c->d_state_alt);
+    dfree(c->d_id);
+    dfree(c->d_id_alt);
+    dfree(c->d_keys);
+    dfree(c->d_sort_tmp);
+    c->sort_tmp_bytes = 0;
+    c->sort_n = 0;
+    if (hipMalloc((void**)&c->d_state_alt, rows * N * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->d_id, N * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->d_id_alt, N * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->d_keys, 4 * N * sizeof(uint32_t)) != hipSuccess)
+        return c->fail(LMH_E_NOMEM, "cell sort: buffer allocation failed");
+    uint32_t* k = c->d_keys;
+    LMH_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, c->sort_tmp_bytes, k, k + N, k + 2 * N, k + 3 * N,
+                                                   (int)N, 0, 32, c->stream));
+    if (hipMalloc(&c->d_sort_tmp, std::max<size_t>(c->sort_tmp_bytes, 1)) != hipSuccess)
+        return c->fail(LMH_E_NOMEM, "cell sort: workspace allocation failed");
+    c->sort_n = c->N;
+    return LMH_OK;
+}
+
+// radix-sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit), then
+// permute the state and the identities through the sorted values and swap the buffers
+int sort_apply(lmh_ctx* c, int end_bit) {
+    const int64_t N = c->N;
+    uint32_t* k = c->d_keys;
+    size_t bytes = c->sort_tmp_bytes;
+    LMH_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, bytes, k, k + N, k + 2 * N, k + 3 * N,
+                                                   (int)N, 0, end_bit, c->stream));
+    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, c->d_state_alt,
+                       (int32_t)(c->dims + c->pdim), N, k + 3 * N, c->is_sorted ? c->d_id : nullptr, c->d_id_alt);
+    LMH_HIP(c, hipGetLastError());
+    std::swap(c->d_state, c->d_state_alt);
+    std::swap(c->d_id, c->d_id_alt);
+    return LMH_OK;
+}
+
+// order the particles by (species segment, grid cell); the 2+1D grid medium must be set
+int sort_by_cell(lmh_ctx* c) {
+    int rc = sort_alloc(c);
+    if (rc) return rc;
+    LaunchMark mk(c, LMH_PROF_SORT);
+    const int64_t N = c->N;
+    SortKeyArgs ka;
+    memset(&ka, 0, sizeof(ka));
+    ka.x = c->d_state;
+    ka.y = c->d_state + N;
+    int64_t b = 0;
+    for (int k = 0; k <= LMH_MAX_SPECIES; ++k) {
+        ka.begin[k] = b;
+        if (k < c->nspecies) b += c->counts[k];
+    }
+    ka.nseg = c->nspecies;
+    ka.cell_bits = bits_for((uint64_t)(c->gx - 1) * (uint64_t)(c->gy - 1));
+    ka.N = N;
+    const int end_bit = ka.cell_bits + bits_for((uint64_t)c->nspecies);
+    if (end_bit > 32) return c->fail(LMH_E_INVALID, "cell sort: too many grid cells");
+    hipLaunchKernelGGL(k_sort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, ka, make_grid(c),
+                       c->d_keys, c->d_keys + 2 * N);
+    LMH_HIP(c, hipGetLastError());
+    if ((rc = sort_apply(c, end_bit))) return rc;
+    c->is_sorted = true;
+    c->steps_since_sort = 0;
+    return LMH_OK;
+}
+
+// restore the caller's order (a no-op unless the particles are cell-sorted)
+int ensure_unsorted(lmh_ctx* c) {
+    if (!c->is_sorted) return LMH_OK;
+    LMH_HIP(c, hipSetDevice(c->device));
+    LaunchMark mk(c, LMH_PROF_SORT);
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_unsort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_id, N, c->d_keys,
+                       c->d_keys + 2 * N);
+    LMH_HIP(c, hipGetLastError());
+    int rc = sort_apply(c, bits_for((uint64_t)N));
+    if (rc) return rc;
+    c->is_sorted = false;
+    return LMH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1454,6 +1618,11 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_species);
     dfree(c->d_queue);
     dfree(c->d_qcount);
+    dfree(c->d_state_alt);
+    dfree(c->d_id);
+    dfree(c->d_id_alt);
+    dfree(c->d_keys);
+    dfree(c->d_sort_tmp);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->tape_stream) {
         (void)hipStreamSynchronize(c->tape_stream);
@@ -1629,6 +1798,17 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         for (int d = 0; d < pdim; ++d) soa[(size_t)(dims + d) * N + i] = mom[d + (int64_t)pdim * i];
     }
     dfree(c->d_state);
+    c->is_sorted = false;
+    c->steps_since_sort = 0;
+    if (c->sort_n != N || (c->d_state_alt && c->dims + c->pdim != dims + pdim)) {
+        dfree(c->d_state_alt);
+        dfree(c->d_id);
+        dfree(c->d_id_alt);
+        dfree(c->d_keys);
+        dfree(c->d_sort_tmp);
+        c->sort_tmp_bytes = 0;
+        c->sort_n = 0;
+    }
     if (hipMalloc((void**)&c->d_state, rows * (size_t)N * sizeof(double)) != hipSuccess) {
         c->d_state = nullptr;
         c->N = 0;
@@ -1665,6 +1845,7 @@ int lmh_get_particles(lmh_ctx* c, double* pos, double* mom) {
     if (!c->d_state) return c->fail(LMH_E_STATE, "get_particles: no particles set");
     if (!pos || !mom) return c->fail(LMH_E_INVALID, "get_particles: null output");
     LMH_HIP(c, hipSetDevice(c->device));
+    if (int rc = ensure_unsorted(c)) return rc;
     size_t rows = (size_t)(c->dims + c->pdim);
     std::vector<double> soa;
     try {
@@ -1820,6 +2001,14 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     const std::vector<double>& tg = c->h_tg;
     RadialMedium rmed = make_radial(c);
     GridMedium gmed = make_grid(c);
+    // cell sorting applies to the 2+1D grid with Philox noise (the noise follows the identity)
+    // and without freeze-out capture (its records are per slot); anything else runs in the
+    // caller's order
+    const bool sortable = grid && p->noise == LMH_NOISE_PHILOX && !c->fo_on && c->tune.sort_every > 0;
+    if (!sortable) {
+        const int rc = ensure_unsorted(c);
+        if (rc) return rc;
+    }
     unsigned nb = 0;
     StepArgs a = make_args(c, p, c->tune.block, &nb);
     if (nb == 0) return LMH_OK;
@@ -1828,6 +2017,11 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     const bool spec = spec_config(c, p);
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every)) {
+            const int rc = sort_by_cell(c);
+            if (rc) return rc;
+            a = make_args(c, p, c->tune.block, &nb);
+        }
         const bool slice = grid && n == 1 && c->tune.slice;
         if (slice) {
             int j;
@@ -1858,6 +2052,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             else launch_radial<3, 3>(c, a, nb, rmed, p->noise, p->collision, spec);
         }
         LMH_HIP(c, hipGetLastError());
+        c->steps_since_sort += n;
     }
     if (p->check_finite) {
         unsigned long long bad = ULLONG_MAX;
@@ -1877,9 +2072,9 @@ int lmh_set_tuning(lmh_ctx* c, const lmh_tuning* t) {
     if (!c) return LMH_E_INVALID;
     if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
         (t->slice != 0 && t->slice != 1) || (t->defer != 0 && t->defer != 1) ||
-        (t->specialize != 0 && t->specialize != 1))
+        (t->specialize != 0 && t->specialize != 1) || t->sort_every < 0)
         return c->fail(LMH_E_INVALID, "set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, slice, defer "
-                                      "and specialize in {0, 1}");
+                                      "and specialize in {0, 1}, sort_every >= 0");
     c->tune = *t;
     return LMH_OK;
 }
@@ -1923,6 +2118,7 @@ int lmh_boost_to_lab(lmh_ctx* c, const lmh_params* p, int64_t step) {
     if (rc) return rc;
     if (p->coords != LMH_CARTESIAN) return c->fail(LMH_E_INVALID, "boost_to_lab: Cartesian only");
     LMH_HIP(c, hipSetDevice(c->device));
+    if ((rc = ensure_unsorted(c))) return rc;
     RadialMedium rmed = make_radial(c);
     GridMedium gmed = make_grid(c);
     unsigned nb = 0;
@@ -2009,6 +2205,7 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
         return c->fail(LMH_E_INVALID, "observables: 0 <= nbins <= 120 and pt_max > pt_min");
     if (s->species >= c->nspecies) return c->fail(LMH_E_INVALID, "observables: species out of range");
     LMH_HIP(c, hipSetDevice(c->device));
+    if (int rc = ensure_unsorted(c)) return rc;
     int64_t i0 = 0, i1 = c->N;
     if (s->species >= 0) {
         i0 = species_begin(c, s->species);
@@ -2049,6 +2246,7 @@ int lmh_snapshot(lmh_ctx* c, int32_t* index) {
     if (!index) return c->fail(LMH_E_INVALID, "snapshot: null index");
     if (!c->d_state) return c->fail(LMH_E_STATE, "snapshot: no particles set");
     LMH_HIP(c, hipSetDevice(c->device));
+    if (int rc = ensure_unsorted(c)) return rc;
     if (!c->copy_stream) LMH_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     const size_t n = (size_t)(c->dims + c->pdim) * (size_t)c->N;
     int k = -1;
@@ -2113,6 +2311,7 @@ int lmh_freezeout_enable(lmh_ctx* c, double Tfo, int32_t interp) {
     if (!c->d_state) return c->fail(LMH_E_STATE, "freezeout_enable: set the particles first");
     if (!std::isfinite(Tfo)) return c->fail(LMH_E_INVALID, "freezeout_enable: Tfo must be finite");
     LMH_HIP(c, hipSetDevice(c->device));
+    if (int rc = ensure_unsorted(c)) return rc;
     if (!(Tfo > 0.0)) {
         c->fo_on = false;
         LMH_HIP(c, hipStreamSynchronize(c->stream));
@@ -2236,6 +2435,7 @@ struct SliceTuning {
     explicit SliceTuning(lmh_ctx* c_) : c(c_), saved(c_->tune) {
         c->tune.slice = 1;
         c->tune.steps_per_launch = 1;
+        c->tune.sort_every = 0;       // the checkpoints are taken in the caller's order
     }
     ~SliceTuning() { c->tune = saved; }
 };
@@ -2258,6 +2458,7 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     if (s->keep_tape && c->fo_on)
         return c->fail(LMH_E_INVALID, "adjoint_forward: keep_tape runs without freeze-out capture");
     LMH_HIP(c, hipSetDevice(c->device));
+    if ((rc = ensure_unsorted(c))) return rc;
     const int64_t K = s->checkpoint_every;
     const int64_t ncp = (nsteps + K - 1) / K;
     const size_t R = (size_t)(c->dims + c->pdim);
@@ -2334,6 +2535,7 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
         return c->fail(LMH_E_INVALID, "adjoint_backward: 0 <= nbins <= 120 and pt_max > pt_min");
     if (os->species >= c->nspecies) return c->fail(LMH_E_INVALID, "adjoint_backward: species out of range");
     LMH_HIP(c, hipSetDevice(c->device));
+    if (int rc = ensure_unsorted(c)) return rc;
     const lmh_params& p = c->adj.p;
     const int64_t N = c->N;
     const int R = c->dims + c->pdim;

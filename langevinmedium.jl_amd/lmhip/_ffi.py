@@ -51,11 +51,11 @@ class lmh_params(ctypes.Structure):
 
 class lmh_tuning(ctypes.Structure):
     _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("slice", c_int32), ("defer", c_int32),
-                ("specialize", c_int32)]
+                ("specialize", c_int32), ("sort_every", c_int32)]
 
 
 class lmh_profile(ctypes.Structure):
-    _fields_ = [("launches", c_int64 * 3), ("ms", c_double * 3)]
+    _fields_ = [("launches", c_int64 * 4), ("ms", c_double * 4)]
 
 
 class lmh_obs_spec(ctypes.Structure):

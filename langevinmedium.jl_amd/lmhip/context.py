@@ -69,15 +69,16 @@ class Context:
         self._check(self._lib.lmh_synchronize(self._h))
 
     # ---- launch shape ----
-    def set_tuning(self, block=256, steps_per_launch=1, slice=1, defer=1, specialize=1):
-        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer), int(specialize))
+    def set_tuning(self, block=256, steps_per_launch=1, slice=1, defer=1, specialize=1, sort_every=100):
+        t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer), int(specialize),
+                            int(sort_every))
         self._check(self._lib.lmh_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _ffi.lmh_tuning()
         self._check(self._lib.lmh_get_tuning(self._h, ctypes.byref(t)))
         return {"block": t.block, "steps_per_launch": t.steps_per_launch, "slice": t.slice,
-                "defer": t.defer, "specialize": t.specialize}
+                "defer": t.defer, "specialize": t.specialize, "sort_every": t.sort_every}
 
     # ---- medium ----
     def set_medium_radial(self, rgrid, tgrid, T, v, v2=None):
@@ -178,7 +179,7 @@ class Context:
     def profile_stop(self) -> dict:
         p = _ffi.lmh_profile()
         self._check(self._lib.lmh_profile_stop(self._h, ctypes.byref(p)))
-        names = ("step", "redraw", "slice")
+        names = ("step", "redraw", "slice", "sort")
         return {n: {"launches": int(p.launches[k]), "ms": float(p.ms[k])} for k, n in enumerate(names)}
 
     # ---- observables ----

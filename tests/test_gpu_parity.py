@@ -208,6 +208,42 @@ def test_grid_deferred_collisions_identical(gpu_ok, slice_):
     assert np.mean(np.any(res[1][1] != mom, axis=0)) > 0.5
 
 
+def test_cell_sort_identical(gpu_ok):
+    """ordering the quarks by grid cell changes no result: two species with RTA, re-sorts inside
+    a run and the caller's order restored between runs give the unsorted run's bits"""
+    g_ = small_grid()
+    rng = np.random.default_rng(43)
+    counts = [12000, 8000]
+    N = sum(counts)
+    pos = media.hydro_positions(N, rng) * 2.0
+    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], 1.5, rng),
+                                       media.pt_rapidity_momenta(counts[1], 4.75, rng)]))
+    p = _ffi.make_params(dt=0.05, t0=0.6, DsT=0.6366, noise=_ffi.LMH_NOISE_PHILOX, seed=9,
+                         collision=_ffi.LMH_LANGEVIN_RTA)
+    from helpers import tables, load_tables
+    drag, rt, jt = tables([1.5, 4.75], g_.T, 0.6366, 3, True, media.tau_R_law(0.16))
+    res = {}
+    for every in (0, 4, 1):
+        with lmhip.Context(0) as ctx:
+            ctx.set_tuning(sort_every=every)
+            ctx.set_medium_grid2p1(g_.x0, g_.dx, g_.nx, g_.y0, g_.dy, g_.ny, g_.tgrid, g_.T, g_.vx, g_.vy)
+            load_tables(ctx, drag, rt, jt)
+            ctx.set_particles(pos, mom, [1.5, 4.75], counts)
+            ctx.profile_start()
+            ctx.run(p, 1, 10, None, 0)
+            obs = ctx.observables(20, 0.0, 10.0, 0.0, -1)
+            mid = ctx.get_particles()
+            ctx.run(p, 11, 15, None, 0)
+            prof = ctx.profile_stop()
+            res[every] = (obs, mid, ctx.get_particles(), prof["sort"]["launches"])
+    for every in (4, 1):
+        assert res[every][3] > 0, "no sort ran"
+        assert np.array_equal(res[every][0], res[0][0])
+        for k in (1, 2):
+            assert np.array_equal(res[every][k][0], res[0][k][0]) and np.array_equal(res[every][k][1], res[0][k][1])
+    assert res[0][3] == 0
+
+
 def test_nonfinite_reported_at_first_bad_step(gpu_ok):
     rng = np.random.default_rng(0)
     N = 1000
