@@ -351,6 +351,8 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--slice", type=int, default=1)
     ap.add_argument("--defer", type=int, default=1)
+    ap.add_argument("--sort-every", type=int, default=100,
+                    help="steps between device sorts of the quarks by grid cell (0: never)")
     ap.add_argument("--collision", default="langevin_rta", choices=sorted(COLLISIONS),
                     help="diagnostic: another momentum update than the workload's")
     ap.add_argument("--noise", default="philox", choices=["philox", "fixed"],
@@ -393,14 +395,14 @@ def main():
     tabs = tables_for(g, masses, DsT, 3)
     params = make_params(DsT, args.noise == "fixed", args.collision)
     ctx = lmhip.Context(0)
-    ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice, defer=args.defer)
+    ctx.set_tuning(block=args.block, steps_per_launch=max(1, args.spl), slice=args.slice, defer=args.defer,
+                   sort_every=args.sort_every)
     ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
     for k, (drag, rta, jt) in enumerate(tabs):
         ctx.set_time_table(_ffi.LMH_TABLE_DRAG, k, *drag)
         ctx.set_time_table(_ffi.LMH_TABLE_RTA, k, *rta)
         ctx.set_juttner_table(k, jt[0], jt[1], jt[2], jt[3], jt[4])
     ctx.set_particles(pos, mom, masses, counts)
-    del pos, mom
     ctx.boost_to_lab(params, 0)
     ctx.synchronize()
     print(f"setup {time.perf_counter() - t_setup:.1f} s; N={N} medium {g.nbytes() / 1e6:.0f} MB", file=sys.stderr, flush=True)
@@ -408,7 +410,7 @@ def main():
     record = None
     if args.noise == "fixed":
         # one record for every step run below, (steps, C, N) on the device
-        total = args.warmup + 2 * args.steps
+        total = args.warmup + args.steps
         C = ctx.noise_channels(params)
         record = torch.randn((total, C, N), dtype=torch.float64, device="cuda")
         orta = 3 if args.collision == "langevin_rta" else 0
@@ -437,11 +439,20 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     ms = ev0.elapsed_time(ev1)
-    # the same number of steps again with every launch bracketed by events on the library's
-    # stream: the device time of the step kernel alone
+    timed_obs = ctx.observables(40, 0.0, 10.0, 0.0, -1)
+    # the same steps again from the same initial state, every launch bracketed by events on the
+    # library's stream: the device time of each kernel class over exactly the timed steps (the
+    # step kernel's cost changes as the medium cools and expands)
+    ctx.set_particles(pos, mom, masses, counts)
+    del pos, mom
+    ctx.boost_to_lab(params, 0)
+    run(1, args.warmup)
+    ctx.synchronize()
     ctx.profile_start()
-    run(1 + args.warmup + args.steps, args.steps)
+    run(1 + args.warmup, args.steps)
     prof = ctx.profile_stop()
+    if not np.array_equal(ctx.observables(40, 0.0, 10.0, 0.0, -1), timed_obs):
+        raise RuntimeError("the profiled pass did not reproduce the timed pass")
     bad = ctx.nonfinite_step()
     if bad >= 0:
         raise RuntimeError(f"non-finite state at step {bad}")
@@ -453,7 +464,7 @@ def main():
     t_launch = prof["step"]["ms"] / 1e3 / kernel_launches
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
     achieved = bytes_per_launch / t_launch / 1e9
-    default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox"
+    default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox" and args.sort_every == 100
                      and args.defer == 1 and args.slice == 1 and not args.host_sort)
     traffic = args.traffic
     if traffic is None and default_shape:
@@ -482,6 +493,7 @@ def main():
                                "RTA tau_R=5(eta/s)/T; dt=0.01 fm/c; Philox noise; FP64",
                    "quarks": N, "grid": [200, 200, 120], "block": args.block,
                    "steps_per_launch": max(1, args.spl), "slice": args.slice, "defer": args.defer,
+                   "sort_every": args.sort_every,
                    "collision": args.collision, "noise": args.noise, "host_sort": args.host_sort,
                    "wall_s_timed": round(wall, 4), "mean_pT": round(obs[1] / obs[0], 6)},
         "roofline": roof,

@@ -53,9 +53,10 @@ struct StepArgs {
     SpeciesArgs species[LMH_MAX_SPECIES];  // per-species mass and tables, read with scalar
                                            // loads from the kernel arguments (block-uniform index)
     const SpeciesArgs* species_dev;        // the same in device memory, for per-lane indices
-    const uint32_t* id;         // a particle's identity (its noise counters) when the particles
-                                // are ordered by grid cell; null: the slot is the identity
-    int32_t* queue;             // deferred collisions: block b's particles at [b*qblock, ...)
+    const uint32_t* id;         // each slot's particle identity (its noise counters): the slot
+                                // itself unless the particles are ordered by grid cell
+    double4* qrec;              // deferred collisions: block b's records (two double4 each:
+                                // x, y, T, vx | vy, identity, slot, 0) at [b*qblock, ...)
     int32_t* qcount;            // deferred collisions per block of the step kernel
     int32_t qblock;             // the step kernel's block size
     lmh_params p;
@@ -155,21 +156,25 @@ struct GridMedium {
 // and the collision uniform (block 0), the redraw's quantile and direction (block 1, colliding
 // particles only) and the position-diffusion normals (block 2).  FIXED reads the record layout
 // of include/lmhip.h; PHILOX generates the same values.
-template <int PDIM, int NOISE>
+// KICK, RTA: whether the step draws the kick normals and the collision uniform, when the kernel
+// knows it at compile time (-1: from the record layout at run time)
+template <int PDIM, int NOISE, int KICK = -1, int RTA = -1>
 __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t step, int s_off,
                                            double* xi, double* ucol) {
+    const bool kick = KICK < 0 ? a.oxi >= 0 : KICK != 0;
+    const bool rta = RTA < 0 ? a.orta >= 0 : RTA != 0;
     if (NOISE == LMH_NOISE_FIXED) {
         const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
-        if (a.oxi >= 0) {
+        if (kick) {
 #pragma unroll
             for (int d = 0; d < PDIM; ++d) xi[d] = rec[(int64_t)(a.oxi + d) * a.N];
         }
-        if (a.orta >= 0) *ucol = rec[(int64_t)a.orta * a.N];
+        if (rta) *ucol = rec[(int64_t)a.orta * a.N];
     } else {
-        if (a.oxi < 0 && a.orta < 0) return;
+        if (!kick && !rta) return;
         uint64_t w[4];
         noise_block(a.p.seed, a.p.stream, i, step, 0, w);
-        if (a.oxi >= 0) {
+        if (kick) {
             double n0, n1;
             box_muller(w[0], w[1], &n0, &n1);
             xi[0] = n0;
@@ -180,7 +185,7 @@ __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t
                 xi[2 % PDIM] = n2;
             }
         }
-        if (a.orta >= 0) *ucol = u_low44(w);
+        if (rta) *ucol = u_low44(w);
     }
 }
 
@@ -516,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
 // ---------------------------------------------------------------------------------------------
 // 2+1D grid medium (extension): sample at (x, y) with the time cell (j, td) of the step.  Each
 // corner is interpolated in tau first, then the corners in x, then in y.  SLICE reads the cell's
-// bilinear coefficients of the tau-interpolated corners, prepared once per step by k_grid_slice:
+// bilinear coefficients of the tau-interpolated corners, prepared once per step by k_grid_slices:
 // one 96-byte record per particle instead of four scattered corners; the same bilinear function,
 // rounded within a few ulps of the corner form.
 __device__ __forceinline__ int grid_cell(double f, int n) {
@@ -531,6 +536,27 @@ __device__ __forceinline__ double4 lerp4(const double4& a, const double4& b, dou
     c.z = a.z * (1.0 - td) + b.z * td;
     c.w = 0.0;
     return c;
+}
+
+// the slice path in two parts, so that a kernel can place independent work between the gather
+// and its use: the cell's record (and the fractions), then the bilinear combination
+__device__ __forceinline__ const double4* slice_cell(const GridMedium& g, double x, double y, double* xd,
+                                                     double* yd) {
+    const double xc = fmin(fmax(x, g.x0), g.xlast);
+    const double yc = fmin(fmax(y, g.y0), g.ylast);
+    const double fx = (xc - g.x0) * g.inv_dx;
+    const double fy = (yc - g.y0) * g.inv_dy;
+    const int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
+    *xd = fx - (double)ix;
+    *yd = fy - (double)iy;
+    return g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
+}
+
+__device__ __forceinline__ void slice_combine(const double4& cT, const double4& cx, const double4& cy,
+                                              double xd, double yd, double* T, double* vx, double* vy) {
+    *T = fma(fma(cT.w, xd, cT.z), yd, fma(cT.y, xd, cT.x));
+    *vx = fma(fma(cx.w, xd, cx.z), yd, fma(cx.y, xd, cx.x));
+    *vy = fma(fma(cy.w, xd, cy.z), yd, fma(cy.y, xd, cy.x));
 }
 
 template <bool SLICE>
@@ -713,13 +739,6 @@ __device__ __forceinline__ double4 slice_coeffs(const GridMedium& g, int j, doub
     return make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_slice(GridMedium g, double4* slice, int j, double td) {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t nrec = 3 * (int64_t)(g.nx - 1) * (g.ny - 1);
-    if (k >= nrec) return;
-    slice[k] = slice_coeffs(g, j, td, k / 3, (int)(k % 3));
-}
-
 // the slices of steps step0 .. step0 + nk - 1 (time cell of each step as the step kernels find it)
 __global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* slices, int64_t step0,
                                                         int32_t nk, double dt, double t0) {
@@ -766,6 +785,11 @@ __global__ __launch_bounds__(kBlock) void k_sort_keys(SortKeyArgs a, GridMedium 
     vals[s] = (uint32_t)s;
 }
 
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* id, int64_t N) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s < N) id[s] = (uint32_t)s;
+}
+
 // key = identity, value = slot (the sort that restores the caller's order)
 __global__ __launch_bounds__(kBlock) void k_unsort_keys(const uint32_t* id, int64_t N, uint32_t* keys, uint32_t* vals) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -774,16 +798,19 @@ __global__ __launch_bounds__(kBlock) void k_unsort_keys(const uint32_t* id, int6
     vals[s] = (uint32_t)s;
 }
 
-// dst row r, slot s = src row r, slot perm[s]; the identity likewise (the slot itself when src_id
-// is null, i.e. the particles are in the caller's order)
+// dst row r, slot s = src row r, slot perm[s]; the identity likewise
 __global__ __launch_bounds__(kBlock) void k_permute(const double* src, double* dst, int32_t rows, int64_t N,
                                                     const uint32_t* perm, const uint32_t* src_id, uint32_t* dst_id) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= N) return;
     const int64_t from = perm[s];
     for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + s] = src[(int64_t)r * N + from];
-    dst_id[s] = src_id ? src_id[from] : (uint32_t)from;
+    dst_id[s] = src_id[from];
 }
+
+// the one-step launches of lmh_run read the medium slices of up to kSliceRing steps computed by
+// one k_grid_slices launch
+constexpr int kSliceRing = 16;
 
 // three waves per SIMD: the register budget the step needs with a few spilled values, against
 // two waves at its unconstrained register count
@@ -809,7 +836,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
     // the particle's identity keys its noise (the slot unless the particles are cell-sorted)
-    const int64_t pid = a.id ? (int64_t)a.id[i] : i;
+    const int64_t pid = (int64_t)a.id[i];
     // every state load is issued before the medium sample's address arithmetic waits for x, y
     __builtin_amdgcn_sched_barrier(0);
     // freeze-out capture runs in the generic, non-deferred kernels only
@@ -836,13 +863,31 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
 #pragma unroll
             for (int d = 0; d < PDIM; ++d) qin[d] = q[d];
         }
-        // the kick noise first: its arithmetic needs no memory and runs while the state and the
-        // medium sample are in flight
+        double Tm, vx, vy;
+        double4 cT, cx, cy;
+        double xd, yd;
+        if (SLICE) {
+            // the slice record is requested first ...
+            const double4* cell = slice_cell(g, x0, x1, &xd, &yd);
+            cT = cell[0];
+            cx = cell[1];
+            cy = cell[2];
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            grid_sample<false>(g, x0, x1, j, td, &Tm, &vx, &vy);
+        }
+        // ... then the kick noise, which needs no memory, is computed while it is in flight (an
+        // empty asm that reads the noise keeps the compiler from sinking it to its uses)
         double xi[PDIM];
         double ucol = 1.0;
-        noise_kick<PDIM, NOISE>(a, pid, step, s_off, xi, &ucol);
-        double Tm, vx, vy;
-        grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
+        noise_kick<PDIM, NOISE, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, pid, step, s_off, xi, &ucol);
+        if (NOISE == LMH_NOISE_PHILOX) {
+#pragma unroll
+            for (int d = 0; d < PDIM; ++d)
+                if (HAS_KICK) asm volatile("" ::"v"(xi[d]));
+            if (HAS_RTA) asm volatile("" ::"v"(ucol));
+        }
+        if (SLICE) slice_combine(cT, cx, cy, xd, yd, &Tm, &vx, &vy);
         const FlowBoost fb = flow_boost(vx, vy);
         apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
         if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
@@ -861,9 +906,12 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, Gr
                     const int rank = block_rank(valid && col, &a.qcount[blockIdx.x]);
                     if (!valid) return;
                     if (rank >= 0) {
-                        a.queue[(int64_t)blockIdx.x * a.qblock + rank] = (int32_t)i;
-#pragma unroll
-                        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+                        // everything the redraw needs: the position, T and the flow at it, the
+                        // identity and the slot (the LRF momentum is replaced by the redraw)
+                        double4* r = a.qrec + 2 * ((int64_t)blockIdx.x * a.qblock + rank);
+                        r[0] = make_double4(x0, x1, T, vx);
+                        r[1] = make_double4(vy, __longlong_as_double((long long)pid),
+                                            __longlong_as_double((long long)i), 0.0);
                         return;
                     }
                 } else if (col) {
@@ -937,23 +985,21 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
                     start = end_of[t] - cnt_of[t];
                 }
             }
-            const int64_t i = a.queue[(b0 + bsel) * a.qblock + (k - start)];
+            const double4* r = a.qrec + 2 * ((b0 + bsel) * a.qblock + (k - start));
+            const double4 r0 = r[0], r1 = r[1];
+            double x0 = r0.x, x1 = r0.y;
+            const double T = r0.z, vx = r0.w, vy = r1.x;
+            const int64_t pid = (int64_t)__double_as_longlong(r1.y);
+            const int64_t i = (int64_t)__double_as_longlong(r1.z);
             int seg = 0;
 #pragma unroll
             for (int s2 = 1; s2 < LMH_MAX_SPECIES; ++s2)
                 if (s2 < a.nseg && i >= a.begin[s2]) seg = s2;
             const SpeciesArgs sp = a.species_dev[seg];
             const double m = sp.m;
-            double x0 = a.pos[0][i], x1 = a.pos[1][i];
             double q[PDIM];
-#pragma unroll
-            for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
-            double Tm, vx, vy;
-            grid_sample<SLICE>(g, x0, x1, j, td, &Tm, &vx, &vy);
             const FlowBoost fb = flow_boost(vx, vy);
-            const double T = jl_max(Tm, 0.0);
             double usamp, gg[PDIM];
-            const int64_t pid = a.id ? (int64_t)a.id[i] : i;
             noise_redraw<PDIM, NOISE>(a, pid, step, 0, &usamp, gg);
             rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
@@ -1154,7 +1200,6 @@ struct lmh_ctx {
     int medium = -1;                // 0 radial, 1 grid
     double* d_rg = nullptr;
     double* d_tg = nullptr;
-    std::vector<double> h_tg;       // host copy of the time grid (slice path)
     double* d_T = nullptr;
     double* d_V = nullptr;
     double* d_V2 = nullptr;
@@ -1190,7 +1235,7 @@ struct lmh_ctx {
     SpeciesArgs h_species[LMH_MAX_SPECIES];
     SpeciesArgs* d_species = nullptr;
     // deferred collisions: particle indices and the two step-parity counters
-    int32_t* d_queue = nullptr;
+    double4* d_qrec = nullptr;        // two double4 per queued particle
     int32_t* d_qcount = nullptr;      // one count per step-kernel block
 
     // launch timing
@@ -1352,8 +1397,8 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     // segments past nseg are empty and never selected
     memcpy(a.species, c->h_species, sizeof(a.species));
     a.species_dev = c->d_species;
-    a.id = c->is_sorted ? c->d_id : nullptr;
-    a.queue = c->d_queue;
+    a.id = c->d_id;
+    a.qrec = c->d_qrec;
     a.qcount = c->d_qcount;
     a.qblock = block;
     a.sqrt_dt = std::sqrt(p->dt);
@@ -1402,22 +1447,6 @@ GridMedium make_grid(const lmh_ctx* c) {
     g.inv_dy = 1.0 / c->gdy;
     g.ylast = c->gy0 + (double)(c->gy - 1) * c->gdy;
     return g;
-}
-
-// host copy of the medium time cell of interpolate_2d_cpu (kernels_cpu.jl:28-45)
-void host_time_cell(const std::vector<double>& tg, double t, int* j, double* td) {
-    int n = (int)tg.size();
-    double tc = t > tg[n - 1] ? tg[n - 1] : (t < tg[0] ? tg[0] : t);
-    int lo = 0, hi = n + 1;
-    while (lo < hi - 1) {
-        int m = lo + ((hi - lo) >> 1);
-        if (tc < tg[m - 1]) hi = m; else lo = m;
-    }
-    lo = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
-    double a = tg[lo - 1], b = tg[lo];
-    double d = b - a;
-    *td = std::fabs(d) < kEps ? 0.0 : (tc - a) / d;
-    *j = lo - 1;
 }
 
 unsigned grid_blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1474,14 +1503,12 @@ int sort_alloc(lmh_ctx* c) {
     if (c->sort_n == c->N && c->d_state_alt) return LMH_OK;
     const size_t N = (size_t)c->N, rows = (size_t)(c->dims + c->pdim);
     dfree(c->d_state_alt);
-    dfree(c->d_id);
     dfree(c->d_id_alt);
     dfree(c->d_keys);
     dfree(c->d_sort_tmp);
     c->sort_tmp_bytes = 0;
     c->sort_n = 0;
     if (hipMalloc((void**)&c->d_state_alt, rows * N * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->d_id, N * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&c->d_id_alt, N * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&c->d_keys, 4 * N * sizeof(uint32_t)) != hipSuccess)
         return c->fail(LMH_E_NOMEM, "cell sort: buffer allocation failed");
@@ -1503,7 +1530,7 @@ int sort_apply(lmh_ctx* c, int end_bit) {
     LMH_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, bytes, k, k + N, k + 2 * N, k + 3 * N,
                                                    (int)N, 0, end_bit, c->stream));
     hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, c->d_state_alt,
-                       (int32_t)(c->dims + c->pdim), N, k + 3 * N, c->is_sorted ? c->d_id : nullptr, c->d_id_alt);
+                       (int32_t)(c->dims + c->pdim), N, k + 3 * N, c->d_id, c->d_id_alt);
     LMH_HIP(c, hipGetLastError());
     std::swap(c->d_state, c->d_state_alt);
     std::swap(c->d_id, c->d_id_alt);
@@ -1616,7 +1643,7 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_noise);
     dfree(c->d_obs);
     dfree(c->d_species);
-    dfree(c->d_queue);
+    dfree(c->d_qrec);
     dfree(c->d_qcount);
     dfree(c->d_state_alt);
     dfree(c->d_id);
@@ -1679,7 +1706,6 @@ int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const dou
     }
     dfree(c->d_nodes);
     dfree(c->d_slice);
-    c->h_tg.assign(tgrid, tgrid + nt);
     c->nr = nr;
     c->nt = nt;
     c->r_axis_eps = std::max(1e-12, 0.5 * std::fabs(rgrid[1] - rgrid[0]));
@@ -1710,7 +1736,7 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
     dfree(c->d_nodes);
     dfree(c->d_slice);
     if (hipMalloc((void**)&c->d_nodes, n * sizeof(double4)) != hipSuccess ||
-        hipMalloc((void**)&c->d_slice, 3 * (size_t)(nx - 1) * (size_t)(ny - 1) * sizeof(double4)) != hipSuccess)
+        hipMalloc((void**)&c->d_slice, kSliceRing * 3 * (size_t)(nx - 1) * (size_t)(ny - 1) * sizeof(double4)) != hipSuccess)
         return c->fail(LMH_E_NOMEM, "set_medium_grid2p1: device allocation failed");
     LMH_HIP(c, hipMemcpy(c->d_nodes, nodes.data(), n * sizeof(double4), hipMemcpyHostToDevice));
     int rc = upload(c, &c->d_tg, tgrid, nt);
@@ -1719,7 +1745,6 @@ int lmh_set_medium_grid2p1(lmh_ctx* c, double x0, double dx, int32_t nx, double 
     dfree(c->d_T);
     dfree(c->d_V);
     dfree(c->d_V2);
-    c->h_tg.assign(tgrid, tgrid + nt);
     c->gx = nx;
     c->gy = ny;
     c->nt = nt;
@@ -1802,7 +1827,6 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
     c->steps_since_sort = 0;
     if (c->sort_n != N || (c->d_state_alt && c->dims + c->pdim != dims + pdim)) {
         dfree(c->d_state_alt);
-        dfree(c->d_id);
         dfree(c->d_id_alt);
         dfree(c->d_keys);
         dfree(c->d_sort_tmp);
@@ -1815,6 +1839,13 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         return c->fail(LMH_E_NOMEM, "set_particles: device allocation failed");
     }
     LMH_HIP(c, hipMemcpy(c->d_state, soa.data(), rows * (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+    // identities: each slot its own until the particles are sorted by cell
+    dfree(c->d_id);
+    if (hipMalloc((void**)&c->d_id, (size_t)N * sizeof(uint32_t)) != hipSuccess)
+        return c->fail(LMH_E_NOMEM, "set_particles: identity allocation failed");
+    hipLaunchKernelGGL(k_iota, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_id, N);
+    LMH_HIP(c, hipGetLastError());
+    LMH_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long none = ULLONG_MAX;
     LMH_HIP(c, hipMemcpy(c->d_bad, &none, sizeof(none), hipMemcpyHostToDevice));
     c->dims = dims;
@@ -1825,11 +1856,14 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
         c->counts[k] = k < nspecies ? counts[k] : 0;
         c->mass[k] = k < nspecies ? masses[k] : 0.0;
     }
-    // a full block region per step-kernel block: at most N/64 + one partial block per species
-    dfree(c->d_queue);
+    // a full block region per step-kernel block: the step kernel's blocks cover each species'
+    // count rounded up to the block size, so the regions end below N + nspecies * block; at
+    // most N/64 + one partial block per species counts
+    dfree(c->d_qrec);
     dfree(c->d_qcount);
     const size_t max_blocks = (size_t)(N / 64) + LMH_MAX_SPECIES + 1;
-    if (hipMalloc((void**)&c->d_queue, max_blocks * kBlock * sizeof(int32_t)) != hipSuccess ||
+    const size_t max_rec = (size_t)N + (size_t)(LMH_MAX_SPECIES + 1) * kBlock;
+    if (hipMalloc((void**)&c->d_qrec, max_rec * 2 * sizeof(double4)) != hipSuccess ||
         hipMalloc((void**)&c->d_qcount, max_blocks * sizeof(int32_t)) != hipSuccess) {
         return c->fail(LMH_E_NOMEM, "set_particles: queue allocation failed");
     }
@@ -1998,7 +2032,6 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
              const double* d_noise, int32_t spl) {
     const bool grid = c->medium == 1 && p->coords == LMH_CARTESIAN;
     const int32_t k = spl > 0 ? spl : std::max(1, c->tune.steps_per_launch);
-    const std::vector<double>& tg = c->h_tg;
     RadialMedium rmed = make_radial(c);
     GridMedium gmed = make_grid(c);
     // cell sorting applies to the 2+1D grid with Philox noise (the noise follows the identity)
@@ -2015,6 +2048,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     const bool has_rta = p->collision == LMH_RTA || p->collision == LMH_LANGEVIN_RTA;
     const bool defer = grid && k == 1 && has_rta && c->tune.defer && !c->fo_on;
     const bool spec = spec_config(c, p);
+    int64_t ring0 = -1;       // first step of the slices in the ring
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
         if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every)) {
@@ -2023,14 +2057,18 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             a = make_args(c, p, c->tune.block, &nb);
         }
         const bool slice = grid && n == 1 && c->tune.slice;
+        GridMedium gstep = gmed;
         if (slice) {
-            int j;
-            double td;
-            const double t_now = (double)(step_begin + s0) * p->dt + p->t0;
-            host_time_cell(tg, t_now, &j, &td);
-            const int64_t nxy = (int64_t)c->gx * c->gy;
-            LaunchMark mk(c, LMH_PROF_SLICE);
-            hipLaunchKernelGGL(k_grid_slice, dim3(grid_blocks(3 * (int64_t)(c->gx - 1) * (c->gy - 1))), dim3(kBlock), 0, c->stream, gmed, c->d_slice, j, td);
+            // the slices of the next kSliceRing steps in one launch, then one per step from the ring
+            const int64_t rec = 3 * (int64_t)(c->gx - 1) * (c->gy - 1);
+            if (ring0 < 0 || s0 - ring0 >= kSliceRing) {
+                ring0 = s0;
+                const int32_t nk = (int32_t)std::min<int64_t>(kSliceRing, nsteps - s0);
+                LaunchMark mk(c, LMH_PROF_SLICE);
+                hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(rec * nk)), dim3(kBlock), 0, c->stream, gmed,
+                                   c->d_slice, step_begin + s0, nk, p->dt, p->t0);
+            }
+            gstep.slice = c->d_slice + rec * (s0 - ring0);
         }
         a.step_begin = step_begin + s0;
         a.nsteps = n;
@@ -2042,8 +2080,8 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             else
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_PHILOX>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
         } else if (grid) {
-            if (c->pdim == 2) launch_grid<2>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1, spec);
-            else launch_grid<3>(c, a, nb, gmed, p->noise, slice, p->collision, defer && n == 1, spec);
+            if (c->pdim == 2) launch_grid<2>(c, a, nb, gstep, p->noise, slice, p->collision, defer && n == 1, spec);
+            else launch_grid<3>(c, a, nb, gstep, p->noise, slice, p->collision, defer && n == 1, spec);
         } else {
             LaunchMark mk(c, LMH_PROF_STEP);
             if (c->dims == 1) launch_radial<1, 1>(c, a, nb, rmed, p->noise, p->collision, spec);
