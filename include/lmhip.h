@@ -55,11 +55,15 @@ extern "C" {
 #define LMH_LANGEVIN_RTA       3
 
 /* noise source: FIXED reads a host- or device-supplied record per step; PHILOX generates the
- * same record in registers.  Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at
+ * record in registers.  Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at
  * step s is four 64-bit words from two Philox4x32-10 evaluations (half h gives words 2h, 2h+1,
- * each lo | hi << 32 of consecutive outputs) of counter
+ * each lo | hi << 32 of consecutive outputs; the redraw block evaluates half 0 only) of counter
  * (i, s, stream mod 2^32, (stream >> 32) mod 2^29 | (2b + h) << 29) under key
- * (seed mod 2^32, seed >> 32); i and s must stay below 2^32 and stream below 2^61.           */
+ * (seed mod 2^32, seed >> 32); i and s must stay below 2^32 and stream below 2^61.  Normals are
+ * Box-Muller pairs of 53-bit uniforms; the collision uniform takes the 11 low bits of each of
+ * block 0's words; the redraw's quantile is the 53-bit uniform of word 0 of block 1 and its
+ * direction the unit vector of word 1 (z = 2u - 1 from the top 32 bits, the azimuth from the
+ * low 32 bits) in place of the Gaussian g of a fixed record (the redraw normalises either).  */
 #define LMH_NOISE_FIXED        0
 #define LMH_NOISE_PHILOX       1
 

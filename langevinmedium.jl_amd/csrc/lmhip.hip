@@ -198,18 +198,11 @@ __device__ __forceinline__ void noise_redraw(const StepArgs& a, int64_t i, int64
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
     } else {
+        // one Philox evaluation: the quantile from w0, the direction from w1
         uint64_t w[4];
-        noise_block(a.p.seed, a.p.stream, i, step, 1, w);
-        double n0, n1;
-        box_muller(w[0], w[1], &n0, &n1);
-        g[0] = n0;
-        if (PDIM >= 2) g[1 % PDIM] = n1;
-        if (PDIM >= 3) {
-            double n2, n3;
-            box_muller(w[2], w[3], &n2, &n3);
-            g[2 % PDIM] = n2;
-        }
-        *usamp = u_low44(w);
+        noise_block<1>(a.p.seed, a.p.stream, i, step, 1, w);
+        *usamp = u_co(w[0]);
+        unit_direction<PDIM>(w[1], g);
     }
 }
 
@@ -798,14 +791,25 @@ __global__ __launch_bounds__(kBlock) void k_unsort_keys(const uint32_t* id, int6
     vals[s] = (uint32_t)s;
 }
 
-// dst row r, slot s = src row r, slot perm[s]; the identity likewise
-__global__ __launch_bounds__(kBlock) void k_permute(const double* src, double* dst, int32_t rows, int64_t N,
-                                                    const uint32_t* perm, const uint32_t* src_id, uint32_t* dst_id) {
+// dst row r, slot s = src row r, slot perm[s]; the identity likewise.  Every row is read before
+// any is written (the buffers never alias), so a thread's loads are in flight together.
+constexpr int kMaxRows = 6;
+__global__ __launch_bounds__(kBlock) void k_permute(const double* __restrict__ src, double* __restrict__ dst,
+                                                    int32_t rows, int64_t N, const uint32_t* __restrict__ perm,
+                                                    const uint32_t* __restrict__ src_id,
+                                                    uint32_t* __restrict__ dst_id) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= N) return;
     const int64_t from = perm[s];
-    for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + s] = src[(int64_t)r * N + from];
-    dst_id[s] = src_id[from];
+    double v[kMaxRows];
+#pragma unroll
+    for (int r = 0; r < kMaxRows; ++r)
+        if (r < rows) v[r] = src[(int64_t)r * N + from];
+    const uint32_t id = src_id[from];
+#pragma unroll
+    for (int r = 0; r < kMaxRows; ++r)
+        if (r < rows) dst[(int64_t)r * N + s] = v[r];
+    dst_id[s] = id;
 }
 
 // the one-step launches of lmh_run read the medium slices of up to kSliceRing steps computed by

@@ -264,6 +264,30 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
     *cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
+// An isotropic unit vector from one 64-bit word: PDIM 3 from z = 2u - 1 (u from the top 32 bits
+// at the midpoints of 2^-32 cells, so |z| < 1 and z is exact) and the azimuth 2 pi u' (u' from
+// the low 32 bits); PDIM 2 from the angle 2 pi u (u the 53-bit uniform); PDIM 1 the sign bit
+template <int PDIM>
+__device__ __forceinline__ void unit_direction(uint64_t w, double* g) {
+    if (PDIM == 1) {
+        g[0] = (w >> 63) ? -1.0 : 1.0;
+        return;
+    }
+    double s, c;
+    if (PDIM == 2) {
+        noise_sincospi(2.0 * u_co(w), &s, &c);
+        g[0] = c;
+        g[1 % PDIM] = s;
+        return;
+    }
+    const double z = ((double)(uint32_t)(w >> 32) + 0.5) * 0x1.0p-31 - 1.0;
+    const double rho = sqrt_pos((1.0 - z) * (1.0 + z));
+    noise_sincospi((double)(uint32_t)w * 0x1.0p-31, &s, &c);
+    g[0] = rho * c;
+    g[1 % PDIM] = rho * s;
+    g[2 % PDIM] = z;
+}
+
 // Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb
 __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
     double r = sqrt_pos(-2.0 * noise_log(u_oc(wa)));

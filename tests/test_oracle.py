@@ -65,7 +65,9 @@ def test_noise_block_layout():
     for seed, stream, i, step, block in ((0, 0, 0, 0, 0), (99, 3, 7, 17, 1), (2**64 - 1, 2**61 - 1, 2**32 - 1, 2**32 - 1, 2)):
         w = (ctypes.c_uint64 * 4)()
         L.lmo_noise_block(seed, stream, i, step, block, w)
-        assert list(w) == _philox_words(seed, stream, i, step, block)
+        want = _philox_words(seed, stream, i, step, block)
+        # the redraw block evaluates its first half only
+        assert list(w) == (want[:2] + [0, 0] if block == 1 else want)
     # distinct blocks, halves, streams and steps give distinct words
     seen = set()
     for block in range(3):
@@ -73,8 +75,8 @@ def test_noise_block_layout():
             for step in (1, 2):
                 w = (ctypes.c_uint64 * 4)()
                 L.lmo_noise_block(5, stream, 11, step, block, w)
-                seen.update(w)
-    assert len(seen) == 3 * 4 * 2 * 4
+                seen.update(list(w)[:2] if block == 1 else list(w))
+    assert len(seen) == (4 + 2 + 4) * 4 * 2
 
 
 def test_noise_record_box_muller():
@@ -93,26 +95,44 @@ def test_noise_record_box_muller():
         for val, ref in ((rec[0, i], r0 * mp.cos(twopi * u(w[1]))), (rec[1, i], r0 * mp.sin(twopi * u(w[1]))),
                          (rec[2, i], r2 * mp.cos(twopi * u(w[3])))):
             assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
-        # the collision uniform is the 44 low bits of block 0, the quantile those of block 1
+        # the collision uniform is the 44 low bits of block 0
         assert rec[3, i] == low(w)
+        # the quantile is the 53-bit uniform of block 1's first word, the redraw direction the
+        # unit vector of its second: z = 2 u - 1 at the midpoint of the top 32 bits' cell, the
+        # azimuth 2 pi u' from the low 32 bits
         w1 = _philox_words(99, 3, i, 17, 1)
-        assert rec[4, i] == low(w1)
-        g0 = mp.sqrt(-2 * mp.log(u1(w1[0]))) * mp.cos(twopi * u(w1[1]))
-        assert abs(rec[5, i] - float(g0)) <= 4e-15 * max(1, abs(float(g0)))
+        assert rec[4, i] == float(u(w1[0]))
+        z = (mp.mpf(w1[1] >> 32) + mp.mpf(0.5)) * mp.mpf(2) ** -31 - 1
+        phi = 2 * mp.pi * mp.mpf(w1[1] & 0xFFFFFFFF) * mp.mpf(2) ** -32
+        rho = mp.sqrt(1 - z * z)
+        for k, ref in enumerate((rho * mp.cos(phi), rho * mp.sin(phi), z)):
+            assert abs(rec[5 + k, i] - float(ref)) <= 4e-15
         w2 = _philox_words(99, 3, i, 17, 2)
         x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(twopi * u(w2[1]))
         assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
 
 
 def test_noise_record_statistics():
-    """the record's normals are standard normal and its uniforms uniform (seeded, 4e5 draws per
-    channel; 5 sigma on the moments, Kolmogorov-Smirnov p > 1e-4), and distinct channels are
-    uncorrelated: a wrong word, shift or counter in the generator shows up here"""
+    """the record's normals are standard normal, its uniforms uniform and its redraw direction
+    isotropic (seeded, 4e5 draws per channel; 5 sigma on the moments, Kolmogorov-Smirnov
+    p > 1e-4), and distinct channels are uncorrelated: a wrong word, shift or counter in the
+    generator shows up here"""
     from scipy import stats
     p = _ffi.make_params(dt=0.01, seed=7, stream=2, noise=_ffi.LMH_NOISE_PHILOX,
                          collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
     rec = O.noise_record(p, 2, 3, 200000, 3, 2)
-    normals, uniforms = (0, 1, 2, 5, 6, 7, 8, 9), (3, 4)
+    normals, uniforms, direction = (0, 1, 2, 8, 9), (3, 4), (5, 6, 7)
+    # the redraw direction is a unit vector with independent, uniformly distributed z and azimuth
+    g = rec[:, direction, :]
+    assert np.max(np.abs(np.sum(g * g, axis=1) - 1.0)) < 1e-15
+    n = g[:, 0, :].size
+    for k in range(3):
+        x = g[:, k, :].ravel()
+        assert abs(x.mean()) < 5 * np.sqrt(1 / (3 * n))
+        assert abs(x.var() - 1 / 3) < 5 * np.sqrt(4 / (45 * n))
+    assert stats.kstest(g[:, 2, :].ravel(), "uniform", args=(-1.0, 2.0)).pvalue > 1e-4
+    phi = np.arctan2(g[:, 1, :], g[:, 0, :]).ravel()
+    assert stats.kstest(phi, "uniform", args=(-np.pi, 2 * np.pi)).pvalue > 1e-4
     for c in normals:
         x = rec[:, c, :].ravel()
         n = x.size
@@ -150,10 +170,13 @@ def test_noise_record_moments():
     p = _ffi.make_params(dt=0.01, seed=5, noise=_ffi.LMH_NOISE_PHILOX, collision=_ffi.LMH_LANGEVIN_RTA)
     rec = O.noise_record(p, 2, 3, 200_000, 1, 1)[0]
     n = rec.shape[1]
-    for c in (0, 1, 2, 5, 6, 7):
+    for c in (0, 1, 2):                                  # the kick normals
         assert abs(rec[c].mean()) < 5 / math.sqrt(n)
         assert abs(rec[c].var() - 1) < 6 * math.sqrt(2 / n)
-    for c in (3, 4):                                     # the low-bit uniforms
+    for c in (5, 6, 7):                                  # the redraw direction's components
+        assert abs(rec[c].mean()) < 5 / math.sqrt(3 * n)
+        assert abs(rec[c].var() - 1 / 3) < 6 * math.sqrt(4 / 45 / n)
+    for c in (3, 4):                                     # the collision and quantile uniforms
         assert abs(rec[c].mean() - 0.5) < 5 * math.sqrt(1 / 12 / n)
         assert abs(rec[c].var() - 1 / 12) < 0.003
         assert abs(np.corrcoef(rec[c], rec[0])[0, 1]) < 0.01
