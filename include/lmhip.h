@@ -158,7 +158,7 @@ typedef struct lmh_tuning {
                                   neighbouring medium cells.  Each particle keeps its identity
                                   (its noise counters), so results do not depend on it; every
                                   other entry point first restores the caller's order.  0: never
-                                  (default 100)                                                 */
+                                  (default 200)                                                 */
 } lmh_tuning;
 
 /* device time per launch class, recorded with HIP events on the context's stream between

@@ -972,12 +972,13 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
             const int up = __shfl_up(incl, o);
             if (lane >= o) incl += up;
         }
-        // broadcast the span's intervals while every lane of the wave is active
+        // broadcast the span's intervals while every lane of the wave is active (wave-uniform
+        // values, read into scalar registers)
         int end_of[kRedrawSpan], cnt_of[kRedrawSpan];
 #pragma unroll
         for (int t = 0; t < kRedrawSpan; ++t) {
-            end_of[t] = __shfl(incl, t);
-            cnt_of[t] = __shfl(cnt, t);
+            end_of[t] = __builtin_amdgcn_readlane(incl, t);
+            cnt_of[t] = __builtin_amdgcn_readlane(cnt, t);
         }
         const int total = end_of[kRedrawSpan - 1];
         for (int k = lane; k < total; k += 64) {
@@ -1214,7 +1215,7 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 1, 1, 1, 100};
+    lmh_tuning tune = {256, 1, 1, 1, 1, 200};
 
     // cell sorting: the alternate state and identity buffers of the permutation, the radix
     // sort's keys, values and workspace; is_sorted says the state is in cell order with d_id
