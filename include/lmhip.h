@@ -247,6 +247,37 @@ int lmh_set_juttner_table(lmh_ctx* ctx, int32_t species,
                           const double* invcdf, int32_t nU, int32_t nT,
                           double Tmin, double invdT);
 
+/* ---- initial-state samplers on the device (utils.jl:33-172, 499-558) --------------------- */
+/* The inverse-CDF tables of sample_particles_from_FONLL, built on the host as utils.jl:52-71
+ * (p|r) and 130-146 (polar r marginal) or 80-90 (Cartesian spatial density) build them.       */
+typedef struct lmh_fonll_tables {
+    const double* r_grid;      /* nr, ascending                                               */
+    const double* p_grid;      /* np, ascending                                               */
+    const double* cdf_r;       /* nr: CDF of the r marginal (polar sampling; else unused)     */
+    const double* n_r;         /* nr: spatial density on r_grid (Cartesian sampling)          */
+    const double* cdf_p;       /* (np, nr) column-major: the p|r CDF of every r node          */
+    const double* p_ok;        /* nr: 1 where the p|r density is positive (else |p| = 0)      */
+    int32_t nr, np;
+    int32_t cartesian;         /* 0: polar (r, phi) sampling, 1: rejection in the (x, y) square */
+    int32_t pad_;
+    double  rmax, fmax;        /* Cartesian: the square's half side and the envelope          */
+} lmh_fonll_tables;
+/* sample_particles_from_FONLL (utils.jl:33-172) for N quarks, with the antithetic pairing of
+ * simulate_cpu.jl:66-89 when antithetic != 0 (quarks 2k, 2k+1 share a position and have
+ * opposite momenta; an odd last quark is drawn alone).  Quark k's draws are Philox4x32-10 words
+ * of noise block 3 under key (seed, stream); x_out and p_out receive (2, N) column-major
+ * matrices.  Needs no particles or medium; N < 2^32.  Synchronises.                          */
+int lmh_sample_fonll(lmh_ctx* ctx, const lmh_fonll_tables* tables, int64_t N, int32_t antithetic,
+                     uint64_t seed, uint64_t stream, double* x_out, double* p_out);
+/* append_thermal_pz (utils.jl:533-558): out (3, N) receives the (2, N) transverse momenta mom
+ * and a p_z row drawn by sample_pz_conditional_juttner (utils.jl:499-530) at m_T =
+ * sqrt(m^2 + p_T^2) and T = max(T(|x|, t0), 0) of the context's radial medium (pos is (dims,
+ * N)); antithetic != 0 gives quark 2k+1 minus the p_z of quark 2k.  Philox words of block 3
+ * under key (seed, stream).  Synchronises.                                                   */
+int lmh_append_thermal_pz(lmh_ctx* ctx, const double* mom, const double* pos, int32_t dims, int64_t N,
+                          double m, double t0, int32_t antithetic, uint64_t seed, uint64_t stream,
+                          double* out);
+
 /* ---- particles --------------------------------------------------------------------------- */
 /* pos (dims, N) and mom (pdim, N), Julia layout.  Species are contiguous segments:
  * counts[k] particles of mass masses[k], in order; sum(counts) == N.

@@ -1182,6 +1182,7 @@ __global__ __launch_bounds__(kObsFinalBlock) void k_obs_final(const double* __re
 
 #include "lmhip_adjoint.h"
 #include "lmhip_batch.h"
+#include "lmhip_sample.h"
 
 }  // namespace
 
@@ -2276,6 +2277,133 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
     return LMH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// initial-state samplers (utils.jl:33-172, 499-558): tables and inputs copied to one device
+// allocation for the call, one thread per quark, results copied back
+extern "C" {
+
+int lmh_sample_fonll(lmh_ctx* c, const lmh_fonll_tables* tb, int64_t N, int32_t antithetic,
+                     uint64_t seed, uint64_t stream, double* x_out, double* p_out) {
+    if (!c) return LMH_E_INVALID;
+    if (!tb || !x_out || !p_out || !tb->r_grid || !tb->p_grid || !tb->cdf_p || !tb->p_ok)
+        return c->fail(LMH_E_INVALID, "sample_fonll: null tables or output");
+    if (tb->nr < 2 || tb->np < 2 || N < 0 || N >= ((int64_t)1 << 32))
+        return c->fail(LMH_E_INVALID, "sample_fonll: need nr, np >= 2 and 0 <= N < 2^32");
+    if (tb->cartesian ? (!tb->n_r || !(tb->rmax > 0.0) || !(tb->fmax > 0.0)) : !tb->cdf_r)
+        return c->fail(LMH_E_INVALID, "sample_fonll: the polar sampler needs cdf_r, the Cartesian one n_r, "
+                                      "rmax > 0 and fmax > 0");
+    if (!sorted(tb->r_grid, tb->nr) || !sorted(tb->p_grid, tb->np))
+        return c->fail(LMH_E_INVALID, "sample_fonll: the grids must be ascending");
+    if (N == 0) return LMH_OK;
+    LMH_HIP(c, hipSetDevice(c->device));
+    const size_t nr = (size_t)tb->nr, np = (size_t)tb->np, n = (size_t)N;
+    const size_t ntab = 5 * nr + np + np * nr;
+    double* d = nullptr;
+    int32_t* d_fail = nullptr;
+    if (hipMalloc((void**)&d, (ntab + 4 * n) * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_fail, sizeof(int32_t)) != hipSuccess) {
+        dfree(d);
+        return c->fail(LMH_E_NOMEM, "sample_fonll: device allocation failed");
+    }
+    std::vector<double> h;
+    try {
+        h.assign(ntab, 0.0);
+    } catch (...) {
+        dfree(d);
+        dfree(d_fail);
+        return c->fail(LMH_E_NOMEM, "sample_fonll: host allocation failed");
+    }
+    double* rg = h.data();
+    double* cdf_r = rg + nr;
+    double* n_r = cdf_r + nr;
+    double* p_ok = n_r + nr;
+    double* pg = p_ok + nr;
+    double* cdf_p = pg + np;
+    memcpy(rg, tb->r_grid, nr * sizeof(double));
+    if (tb->cdf_r) memcpy(cdf_r, tb->cdf_r, nr * sizeof(double));
+    if (tb->n_r) memcpy(n_r, tb->n_r, nr * sizeof(double));
+    memcpy(p_ok, tb->p_ok, nr * sizeof(double));
+    memcpy(pg, tb->p_grid, np * sizeof(double));
+    memcpy(cdf_p, tb->cdf_p, np * nr * sizeof(double));
+    int rc = LMH_OK;
+    const hipError_t e1 = hipMemcpyAsync(d, h.data(), ntab * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    const hipError_t e2 = hipMemsetAsync(d_fail, 0, sizeof(int32_t), c->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess) rc = c->fail(LMH_E_HIP, "sample_fonll: table upload failed");
+    if (!rc) {
+        FonllTables t;
+        t.rg = d;
+        t.cdf_r = d + nr;
+        t.fsp = d + 2 * nr;
+        t.p_ok = d + 3 * nr;
+        t.pg = d + 4 * nr;
+        t.cdf_p = d + 4 * nr + np;
+        t.nr = tb->nr;
+        t.np = tb->np;
+        t.cartesian = tb->cartesian ? 1 : 0;
+        t.antithetic = antithetic ? 1 : 0;
+        t.rmax = tb->rmax;
+        t.fmax = tb->fmax;
+        t.N = N;
+        t.nhalf = antithetic ? N / 2 : 0;
+        t.seed = seed;
+        t.stream = stream;
+        t.max_attempts = 100000;
+        double* dx = d + ntab;
+        double* dp = dx + 2 * n;
+        hipLaunchKernelGGL(k_sample_fonll, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, t, dx, dp, d_fail);
+        int32_t fail = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(x_out, dx, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(p_out, dp, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(&fail, d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = c->fail(LMH_E_HIP, "sample_fonll: launch or download failed");
+        else if (fail)
+            rc = c->fail(LMH_E_STATE, "sample_fonll: the Cartesian rejection sampler accepted no point within "
+                                      "100000 attempts for some quark");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(d);
+    dfree(d_fail);
+    return rc;
+}
+
+int lmh_append_thermal_pz(lmh_ctx* c, const double* mom, const double* pos, int32_t dims, int64_t N,
+                          double m, double t0, int32_t antithetic, uint64_t seed, uint64_t stream,
+                          double* out) {
+    if (!c) return LMH_E_INVALID;
+    if (!mom || !pos || !out || dims < 1 || dims > 3 || N < 0 || N >= ((int64_t)1 << 32) || !std::isfinite(m))
+        return c->fail(LMH_E_INVALID, "append_thermal_pz: need momenta, positions, 1 <= dims <= 3, 0 <= N < 2^32 "
+                                      "and a finite mass");
+    if (c->medium != 0) return c->fail(LMH_E_STATE, "append_thermal_pz: needs the radial medium (T(r, t0))");
+    if (N == 0) return LMH_OK;
+    LMH_HIP(c, hipSetDevice(c->device));
+    const size_t n = (size_t)N;
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, (2 + (size_t)dims + 3) * n * sizeof(double)) != hipSuccess)
+        return c->fail(LMH_E_NOMEM, "append_thermal_pz: device allocation failed");
+    double* dm = d;
+    double* dx = dm + 2 * n;
+    double* dout = dx + (size_t)dims * n;
+    int rc = LMH_OK;
+    if (hipMemcpyAsync(dm, mom, 2 * n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dx, pos, (size_t)dims * n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        rc = c->fail(LMH_E_HIP, "append_thermal_pz: upload failed");
+    } else {
+        hipLaunchKernelGGL(k_thermal_pz, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, dm, dx, dims, N, m, t0,
+                           antithetic ? 1 : 0, seed, stream, make_radial(c), dout);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = c->fail(LMH_E_HIP, "append_thermal_pz: launch or download failed");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(d);
+    return rc;
 }
 
 }  // extern "C"

@@ -8,14 +8,14 @@
 #
 #   using LangevInMedium, LangevInMediumHIP
 #   t, mom, pos = simulate_ensemble_bulk(HIPBackend(), r, p, dens, Tevo, Vevo, (xgrid, tgrid);
-#                                        x_init = x0, p_init = p0, N_particles = N, ...)
+#                                        N_particles = N, ...)
 module LangevInMediumHIP
 
 using LangevInMedium
 import LangevInMedium.Simulate: simulate_ensemble_bulk
 using LangevInMedium.Backends: AbstractBackend
 using LangevInMedium.Transport: build_tau_drag_spline, build_taun_current_spline, build_juttner_invcdf
-using LangevInMedium.Utils: check_momentum_dims, append_thermal_pz, sample_initial_particles_milne!
+using LangevInMedium.Utils: check_momentum_dims, sample_initial_particles_milne!
 import ChainRulesCore
 using ChainRulesCore: NoTangent
 
@@ -86,6 +86,73 @@ run!(c, p::LmhParams, step0, n) = check(c, ccall((:lmh_run, liblmhip), Cint,
 boost_to_lab!(c, p::LmhParams, step) = check(c, ccall((:lmh_boost_to_lab, liblmhip), Cint,
     (Ptr{Cvoid}, Ref{LmhParams}, Int64), c.h, p, step))
 
+# ---- initial state: utils.jl:33-172 and 499-558 with the draws on the device ---------------------
+# lmh_fonll_tables (include/lmhip.h), field for field
+struct LmhFonllTables
+    r_grid::Ptr{Float64}; p_grid::Ptr{Float64}; cdf_r::Ptr{Float64}; n_r::Ptr{Float64}
+    cdf_p::Ptr{Float64}; p_ok::Ptr{Float64}
+    nr::Int32; np::Int32; cartesian::Int32; pad_::Int32
+    rmax::Float64; fmax::Float64
+end
+
+# utils.jl:60-67,134-141: first node 0, strictly increasing by at least eps, last node exactly 1
+function _monotone_normalised!(cdf)
+    cdf[1] = 0.0
+    for k in 2:length(cdf)
+        cdf[k] = max(cdf[k], cdf[k - 1] + eps(Float64))
+    end
+    cdf ./= cdf[end]
+    cdf[end] = 1.0
+    return cdf
+end
+
+# The inverse-CDF tables of sample_particles_from_FONLL (utils.jl:33-146), built as the reference
+# builds them; the device draws from them (lmh_sample_fonll).
+function sample_fonll(c, r_grid, p_grid, f, N; cartesian::Bool, antithetic::Bool, seed)
+    rg = collect(Float64, r_grid); pg = collect(Float64, p_grid)
+    Nr, Np = length(rg), length(pg)
+    F = size(f) == (Np, Nr) ? Matrix{Float64}(f) : size(f) == (Nr, Np) ? Matrix{Float64}(permutedims(f)) :
+        error("sample_particles_from_FONLL: f_HQ_init_FONLL has size $(size(f)), expected ($Np, $Nr) (p×r) or ($Nr, $Np) (r×p).")
+    dr = sum(diff(rg)) / (Nr - 1); dp = sum(diff(pg)) / (Np - 1)
+    P = pg .* F
+    cdf_p = zeros(Np, Nr); p_ok = zeros(Nr)
+    for i in 1:Nr
+        if sum(P[:, i]) > 0
+            cdf_p[:, i] .= _monotone_normalised!(cumsum(P[:, i]) .* dp)
+            p_ok[i] = 1.0
+        end
+    end
+    cdf_r = zeros(Nr); n_r = zeros(Nr); rmax = 0.0; fmax = 0.0
+    if cartesian
+        n_r .= vec(sum(P; dims = 1)) .* dp
+        rmax = rg[end]; fmax = maximum(n_r) + eps(Float64)
+    else
+        Prp = rg' .* pg .* F
+        Z = sum(Prp) * dr * dp
+        Z == 0 && error("Distribution normalization is zero.")
+        cdf_r .= _monotone_normalised!(cumsum(vec(sum(Prp ./ Z; dims = 1)) .* dp) .* dr)
+    end
+    x = zeros(2, N); p = zeros(2, N)
+    GC.@preserve rg pg cdf_r n_r cdf_p p_ok x p begin
+        t = LmhFonllTables(pointer(rg), pointer(pg), pointer(cdf_r), pointer(n_r), pointer(cdf_p), pointer(p_ok),
+                           Nr, Np, cartesian, 0, rmax, fmax)
+        check(c, ccall((:lmh_sample_fonll, liblmhip), Cint,
+            (Ptr{Cvoid}, Ref{LmhFonllTables}, Int64, Int32, UInt64, UInt64, Ptr{Float64}, Ptr{Float64}),
+            c.h, t, N, antithetic, UInt64(seed), UInt64(0), x, p))
+    end
+    return x, p
+end
+
+# append_thermal_pz (utils.jl:533-558) on the device: T(|x|, t0) of the context's radial medium
+function append_thermal_pz_device(c, mom::Matrix{Float64}, pos::Matrix{Float64}, m, t0; antithetic::Bool, seed)
+    N = size(mom, 2)
+    out = zeros(3, N)
+    GC.@preserve mom pos out check(c, ccall((:lmh_append_thermal_pz, liblmhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int32, Int64, Float64, Float64, Int32, UInt64, UInt64, Ptr{Float64}),
+        c.h, mom, pos, size(pos, 1), N, m, t0, antithetic, UInt64(seed), UInt64(0), out))
+    return out
+end
+
 # ---- Cartesian: simulate_gpu_wrapper.jl:15-83 signature ----------------------------------------
 function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, heavy_quark_density,
         TemperatureEvolutionn, VelocityEvolutionn, SpaceTimeGrid;
@@ -102,23 +169,50 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
         seed::Integer = 0, device::Integer = 0)
     (collision_mode == :langevin || collision_mode == :rta) ||
         error("simulate_ensemble_bulk(::HIPBackend): collision_mode=$(collision_mode) is not supported on GPU (only :langevin and :rta).")
-    (x_init === nothing || p_init === nothing) && error("pass x_init and p_init (host samplers run before the backend)")
-    dimensions == 1 && error("radial mode: reduce x_init/p_init to (1, N) as simulate_cpu.jl:102-136 does")
     steps = floor(Int, (final_time - initial_time) / Δt)
     save_every = round(Int, save_interval / Δt)
     num_saves = div(steps, save_every)
     xgrid, tgrid = SpaceTimeGrid
-    positions = Matrix{Float64}(x_init); momenta = Matrix{Float64}(p_init)
-    pdim = momentum_dimensions <= 0 ? dimensions : momentum_dimensions
-    check_momentum_dims(dimensions, pdim, false, bjorken_redshift, initial_time)
-    if size(momenta, 1) < pdim
-        momenta = append_thermal_pz(momenta, positions, m,
-            r -> LangevInMedium.KernelsCPU.interpolate_2d_cpu(xgrid, tgrid, TemperatureEvolutionn, r, initial_time))
-    end
-    N = size(positions, 2)
     c = Ctx(device)
+    # the medium first: the thermal p_z completion reads T(r, t0) from it
     set_medium!(c, collect(Float64, xgrid), collect(Float64, tgrid), Matrix{Float64}(TemperatureEvolutionn),
                 Matrix{Float64}(VelocityEvolutionn), V2Evolutionn === nothing ? nothing : Matrix{Float64}(V2Evolutionn))
+    if x_init !== nothing && p_init !== nothing
+        x_matrix = Matrix{Float64}(x_init); p_matrix = Matrix{Float64}(p_init)
+    else
+        # simulate_cpu.jl:57-94: Cartesian sampling by default, antithetic pairs on request
+        cart = cartesian_spatial_sampling === nothing ? (dimensions == 1 || dimensions >= 2) : cartesian_spatial_sampling
+        x_matrix, p_matrix = sample_fonll(c, r_grid_Langevin, p_grid_Langevin, heavy_quark_density, N_particles;
+                                          cartesian = cart, antithetic = antithetic_momenta, seed = seed)
+    end
+    radial_mode = dimensions == 1
+    if radial_mode
+        # simulate_cpu.jl:102-136: r = |x|, p_r = p . e_r, a random e_r on the axis
+        r = sqrt.(x_matrix[1, :] .^ 2 .+ x_matrix[2, :] .^ 2)
+        r_axis_eps = max(1e-12, 0.5 * (length(r_grid_Langevin) >= 2 ? abs(float(r_grid_Langevin[2] - r_grid_Langevin[1])) : 0.0))
+        reg = r .> r_axis_eps
+        ex = similar(r); ey = similar(r)
+        ex[reg] .= x_matrix[1, reg] ./ r[reg]; ey[reg] .= x_matrix[2, reg] ./ r[reg]
+        ns = count(.!reg)
+        if ns > 0
+            rx = randn(ns); ry = randn(ns)
+            invn = 1.0 ./ (sqrt.(rx .^ 2 .+ ry .^ 2) .+ eps())
+            ex[.!reg] .= rx .* invn; ey[.!reg] .= ry .* invn
+        end
+        positions = reshape(r, 1, :)
+        momenta = reshape(ex .* p_matrix[1, :] .+ ey .* p_matrix[2, :], 1, :)
+    else
+        positions = copy(x_matrix); momenta = copy(p_matrix)
+    end
+    pdim = momentum_dimensions <= 0 ? dimensions : momentum_dimensions
+    check_momentum_dims(dimensions, pdim, radial_mode, bjorken_redshift, initial_time)
+    if size(momenta, 1) < pdim
+        # simulate_cpu.jl:147-151
+        momenta = append_thermal_pz_device(c, Matrix{Float64}(momenta), Matrix{Float64}(positions), m, initial_time;
+                                           antithetic = antithetic_momenta && x_init === nothing, seed = seed)
+    end
+    size(momenta, 1) == pdim || error("momentum rows $(size(momenta, 1)) ≠ momentum_dimensions $pdim")
+    N = size(positions, 2)
     Tmin = max(float(minimum(TemperatureEvolutionn)), 0.0)
     Tmax = max(float(maximum(TemperatureEvolutionn)), Tmin + eps(Float64))
     if momentum_langevin && DsT > 0.0

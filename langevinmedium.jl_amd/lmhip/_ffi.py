@@ -58,6 +58,13 @@ class lmh_profile(ctypes.Structure):
     _fields_ = [("launches", c_int64 * 4), ("ms", c_double * 4)]
 
 
+class lmh_fonll_tables(ctypes.Structure):
+    _fields_ = [("r_grid", POINTER(c_double)), ("p_grid", POINTER(c_double)), ("cdf_r", POINTER(c_double)),
+                ("n_r", POINTER(c_double)), ("cdf_p", POINTER(c_double)), ("p_ok", POINTER(c_double)),
+                ("nr", c_int32), ("np", c_int32), ("cartesian", c_int32),
+                ("pad_", c_int32), ("rmax", c_double), ("fmax", c_double)]
+
+
 class lmh_obs_spec(ctypes.Structure):
     _fields_ = [("nbins", c_int32), ("species", c_int32), ("pt_min", c_double),
                 ("pt_max", c_double), ("sigma", c_double)]
@@ -110,6 +117,10 @@ SIGNATURES = {
     "lmh_set_juttner_table": (c_int, [c_void_p, c_int32, _PD, c_int32, c_int32, c_double, c_double]),
     "lmh_set_particles": (c_int, [c_void_p, _PD, c_int32, _PD, c_int32, c_int64, c_int32, _PI64, _PD]),
     "lmh_get_particles": (c_int, [c_void_p, _PD, _PD]),
+    "lmh_sample_fonll": (c_int, [c_void_p, POINTER(lmh_fonll_tables), c_int64, c_int32, c_uint64, c_uint64,
+                                 _PD, _PD]),
+    "lmh_append_thermal_pz": (c_int, [c_void_p, _PD, _PD, c_int32, c_int64, c_double, c_double, c_int32,
+                                      c_uint64, c_uint64, _PD]),
     "lmh_boost_to_lab": (c_int, [c_void_p, POINTER(lmh_params), c_int64]),
     "lmh_run": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64, _PD, c_int32]),
     "lmh_run_device_noise": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64, c_void_p, c_int32]),

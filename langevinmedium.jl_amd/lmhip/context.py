@@ -144,6 +144,40 @@ class Context:
         self._check(self._lib.lmh_get_particles(self._h, dptr(pos), dptr(mom)))
         return pos, mom
 
+    # ---- initial-state samplers on the device ----
+    def sample_fonll(self, tables, N, antithetic=False, seed=0, stream=0):
+        """sample_particles_from_FONLL (utils.jl:33-172) on the device from the tables of
+        lmhip.utils.fonll_tables: (x (2, N), p (2, N))"""
+        keep = {k: (None if tables[k] is None else np.asfortranarray(tables[k], dtype=np.float64))
+                for k in ("r_grid", "p_grid", "cdf_r", "n_r", "cdf_p", "p_ok")}
+        nr, np_ = keep["r_grid"].size, keep["p_grid"].size
+        if keep["cdf_p"].shape != (np_, nr) or keep["p_ok"].size != nr:
+            raise ValueError("fonll tables: cdf_p must be (np, nr) and p_ok (nr,)")
+        null = ctypes.POINTER(ctypes.c_double)()
+        t = _ffi.lmh_fonll_tables(dptr(keep["r_grid"]), dptr(keep["p_grid"]),
+                                  dptr(keep["cdf_r"]) if keep["cdf_r"] is not None else null,
+                                  dptr(keep["n_r"]) if keep["n_r"] is not None else null,
+                                  dptr(keep["cdf_p"]), dptr(keep["p_ok"]), nr, np_,
+                                  1 if tables["cartesian"] else 0, 0, float(tables["rmax"]), float(tables["fmax"]))
+        x = np.empty((2, int(N)), dtype=np.float64, order="F")
+        p = np.empty((2, int(N)), dtype=np.float64, order="F")
+        self._check(self._lib.lmh_sample_fonll(self._h, ctypes.byref(t), int(N), 1 if antithetic else 0,
+                                               int(seed), int(stream), dptr(x), dptr(p)))
+        return x, p
+
+    def append_thermal_pz(self, momenta, positions, m, t0, antithetic=False, seed=0, stream=0):
+        """append_thermal_pz (utils.jl:533-558) on the device, T(r) from the radial medium at t0:
+        (2, N) transverse momenta -> (3, N)"""
+        mom = np.asfortranarray(momenta, dtype=np.float64)
+        pos = np.asfortranarray(positions, dtype=np.float64)
+        if mom.ndim != 2 or mom.shape[0] != 2 or pos.ndim != 2 or pos.shape[1] != mom.shape[1]:
+            raise ValueError(f"append_thermal_pz: expected 2 transverse momentum rows, got {mom.shape[0]}")
+        out = np.empty((3, mom.shape[1]), dtype=np.float64, order="F")
+        self._check(self._lib.lmh_append_thermal_pz(self._h, dptr(mom), dptr(pos), pos.shape[0], mom.shape[1],
+                                                    float(m), float(t0), 1 if antithetic else 0, int(seed),
+                                                    int(stream), dptr(out)))
+        return out
+
     # ---- stepping ----
     def noise_channels(self, params) -> int:
         return int(self._lib.lmh_noise_channels(ctypes.byref(params), self.dims, self.pdim))

@@ -28,8 +28,8 @@ from .context import Context, LMHError
 from .media import Grid2p1Medium
 from .transport import (build_juttner_invcdf, build_relaxation_spline, build_tau_drag_spline,
                         build_taun_current_spline)
-from .utils import (append_thermal_pz, check_momentum_dims, sample_initial_particles_milne,
-                    step_counts, step_counts_gc)
+from .utils import (check_momentum_dims, fonll_tables, sample_initial_particles_milne, step_counts,
+                    step_counts_gc)
 
 _EPS = sys.float_info.epsilon
 
@@ -46,23 +46,6 @@ def simulate_ensemble_bulk(backend, *args, **kwargs):
 
 
 # ------------------------------------------------------------------------------------------------
-def _interp_clamped(xg, tg, F, r, t):
-    """interpolate_2d_cpu (kernels_cpu.jl:22-51) on the host, for the p_z completion."""
-    xi = min(max(r, xg[0]), xg[-1])
-    yi = min(max(t, tg[0]), tg[-1])
-    i = int(np.searchsorted(xg, xi, side="right"))
-    j = int(np.searchsorted(tg, yi, side="right"))
-    i = min(max(i, 1), len(xg) - 1)
-    j = min(max(j, 1), len(tg) - 1)
-    x0, x1, y0, y1 = xg[i - 1], xg[i], tg[j - 1], tg[j]
-    dx, dy = x1 - x0, y1 - y0
-    xd = 0.0 if abs(dx) < _EPS else (xi - x0) / dx
-    yd = 0.0 if abs(dy) < _EPS else (yi - y0) / dy
-    c0 = F[i - 1, j - 1] * (1 - xd) + F[i, j - 1] * xd
-    c1 = F[i - 1, j] * (1 - xd) + F[i, j] * xd
-    return c0 * (1 - yd) + c1 * yd
-
-
 def _collision_code(collision_mode, momentum_langevin, DsT):
     cm = str(collision_mode).lstrip(":")
     if cm not in ("langevin", "rta", "langevin_rta"):
@@ -187,11 +170,47 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
     xgrid, tgrid = (np.asarray(a, dtype=np.float64) for a in SpaceTimeGrid)
     Tf = np.asfortranarray(TemperatureEvolutionn, dtype=np.float64)
     Vf = np.asfortranarray(VelocityEvolutionn, dtype=np.float64)
+    ctx = Context(backend.device)
+    try:
+        return _simulate_cartesian_ctx(
+            ctx, r_grid_Langevin, p_grid_Langevin, heavy_quark_density, xgrid, tgrid, Tf, Vf,
+            coll=coll, cm=cm, steps=steps, save_every=save_every, num_saves=num_saves,
+            N_particles=N_particles, Δt=Δt, initial_time=initial_time, final_time=final_time, m=m,
+            DsT=DsT, DsT_linear=DsT_linear, DsT_slope=DsT_slope, DsT_offset=DsT_offset, Tfo=Tfo,
+            DsT_quad=DsT_quad, DsT_Tref=DsT_Tref, dimensions=dimensions,
+            cartesian_spatial_sampling=cartesian_spatial_sampling, antithetic_momenta=antithetic_momenta,
+            position_diffusion=position_diffusion, momentum_langevin=momentum_langevin,
+            reflecting_boundary=reflecting_boundary, x_init=x_init, p_init=p_init,
+            V2Evolutionn=V2Evolutionn, psi2=psi2, freezeout_capture=freezeout_capture,
+            freezeout_interp=freezeout_interp, integrator_mode=integrator_mode, relativistic=relativistic,
+            momentum_dimensions=momentum_dimensions, bjorken_redshift=bjorken_redshift, noise=noise,
+            seed=seed, steps_per_launch=steps_per_launch, species_counts=species_counts,
+            drag_model=drag_model, rta_time=rta_time)
+    finally:
+        ctx.close()
+
+
+def _simulate_cartesian_ctx(ctx, r_grid_Langevin, p_grid_Langevin, heavy_quark_density, xgrid, tgrid, Tf, Vf,
+                            *, coll, cm, steps, save_every, num_saves, N_particles, Δt, initial_time,
+                            final_time, m, DsT, DsT_linear, DsT_slope, DsT_offset, Tfo, DsT_quad, DsT_Tref,
+                            dimensions, cartesian_spatial_sampling, antithetic_momenta, position_diffusion,
+                            momentum_langevin, reflecting_boundary, x_init, p_init, V2Evolutionn, psi2,
+                            freezeout_capture, freezeout_interp, integrator_mode, relativistic,
+                            momentum_dimensions, bjorken_redshift, noise, seed, steps_per_launch,
+                            species_counts, drag_model, rta_time):
+    # the medium first: the thermal p_z completion reads T(r, t0) from it
+    ctx.set_medium_radial(xgrid, tgrid, Tf, Vf, V2Evolutionn)
     if x_init is None or p_init is None:
-        raise NotImplementedError("the FONLL sampler (utils.jl:33-172) is host setup outside this "
-                                  "backend: pass x_init and p_init")
-    x_matrix = np.array(x_init, dtype=np.float64, order="F", ndmin=2)
-    p_matrix = np.array(p_init, dtype=np.float64, order="F", ndmin=2)
+        # simulate_cpu.jl:57-94: sample_particles_from_FONLL, Cartesian by default, with the
+        # antithetic pairing; the draws run on the device (Philox block 3 under (seed, 0))
+        cart = (dimensions == 1 or dimensions >= 2) if cartesian_spatial_sampling is None \
+            else bool(cartesian_spatial_sampling)
+        tables = fonll_tables(r_grid_Langevin, p_grid_Langevin, heavy_quark_density, cart)
+        x_matrix, p_matrix = ctx.sample_fonll(tables, N_particles, antithetic=antithetic_momenta,
+                                              seed=seed, stream=0)
+    else:
+        x_matrix = np.array(x_init, dtype=np.float64, order="F", ndmin=2)
+        p_matrix = np.array(p_init, dtype=np.float64, order="F", ndmin=2)
     N = x_matrix.shape[1]
     if p_matrix.shape[1] != N or N_particles != N:
         raise ValueError(f"x_init/p_init carry {N} particles, N_particles is {N_particles}")
@@ -224,46 +243,42 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
     if momenta.shape[0] < pdim:
         if len(masses) != 1:
             raise ValueError("the thermal p_z completion needs a single species")
-        momenta = append_thermal_pz(momenta, positions, masses[0],
-                                    lambda rr: _interp_clamped(xgrid, tgrid, Tf, rr, initial_time),
-                                    antithetic=False, rng=rng)
+        # simulate_cpu.jl:147-151: the thermal p_z at T(r, t0), on the device
+        momenta = ctx.append_thermal_pz(momenta, positions, masses[0], initial_time,
+                                        antithetic=antithetic_momenta and x_init is None, seed=seed,
+                                        stream=0)
     if momenta.shape[0] != pdim:
         raise ValueError(f"momentum rows {momenta.shape[0]} != momentum_dimensions {pdim}")
 
     dmodel, g0, lam = _drag_params(drag_model)
-    ctx = Context(backend.device)
-    try:
-        ctx.set_medium_radial(xgrid, tgrid, Tf, Vf, V2Evolutionn)
-        dst_kw = dict(DsT_linear=DsT_linear, DsT_slope=DsT_slope, DsT_offset=DsT_offset, Tfo=Tfo,
-                      DsT_quad=DsT_quad, DsT_Tref=DsT_Tref)
-        _load_tables(ctx, masses, Tf, DsT=DsT, momentum_langevin=momentum_langevin, cm=cm, pdim=pdim,
-                     dst_kw=dst_kw, rta_time=rta_time, needs_drag_table=dmodel == _ffi.LMH_DRAG_TABLE)
-        ctx.set_particles(positions, momenta, masses, counts)
-        params = _ffi.make_params(dt=Δt, t0=initial_time, DsT=DsT, psi2=psi2, gamma0=g0, Lambda=lam,
-                                  seed=seed, coords=_ffi.LMH_CARTESIAN, collision=coll,
-                                  integrator=int(integrator_mode), drag_model=dmodel,
-                                  relativistic=relativistic, bjorken_redshift=bjorken_redshift,
-                                  position_diffusion=position_diffusion,
-                                  reflecting_boundary=reflecting_boundary, check_finite=True)
-        C = ctx.noise_channels(params)
-        params.noise, rec = _noise_mode(noise, steps, C, N)
-        ctx.boost_to_lab(params, 0)                     # simulate_cpu.jl:160-163
-        if freezeout_capture:
-            # simulate_gpu.jl:375-402: the latched states are the result
-            ctx.freezeout_enable(Tfo, freezeout_interp)
-            _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, None)
-            fpos, fmom, ftau, fflag = ctx.freezeout_get()
-            return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
-        pos0, mom0 = ctx.get_particles()
-        mom_hist, pos_hist = [mom0], [pos0]
+    dst_kw = dict(DsT_linear=DsT_linear, DsT_slope=DsT_slope, DsT_offset=DsT_offset, Tfo=Tfo,
+                  DsT_quad=DsT_quad, DsT_Tref=DsT_Tref)
+    _load_tables(ctx, masses, Tf, DsT=DsT, momentum_langevin=momentum_langevin, cm=cm, pdim=pdim,
+                 dst_kw=dst_kw, rta_time=rta_time, needs_drag_table=dmodel == _ffi.LMH_DRAG_TABLE)
+    ctx.set_particles(positions, momenta, masses, counts)
+    params = _ffi.make_params(dt=Δt, t0=initial_time, DsT=DsT, psi2=psi2, gamma0=g0, Lambda=lam,
+                              seed=seed, coords=_ffi.LMH_CARTESIAN, collision=coll,
+                              integrator=int(integrator_mode), drag_model=dmodel,
+                              relativistic=relativistic, bjorken_redshift=bjorken_redshift,
+                              position_diffusion=position_diffusion,
+                              reflecting_boundary=reflecting_boundary, check_finite=True)
+    C = ctx.noise_channels(params)
+    params.noise, rec = _noise_mode(noise, steps, C, N)
+    ctx.boost_to_lab(params, 0)                     # simulate_cpu.jl:160-163
+    if freezeout_capture:
+        # simulate_gpu.jl:375-402: the latched states are the result
+        ctx.freezeout_enable(Tfo, freezeout_interp)
+        _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, None)
+        fpos, fmom, ftau, fflag = ctx.freezeout_get()
+        return {"pos": fpos, "mom": fmom, "tau": ftau, "flag": fflag}
+    pos0, mom0 = ctx.get_particles()
+    mom_hist, pos_hist = [mom0], [pos0]
 
-        def snap(p_, m_):
-            pos_hist.append(p_)
-            mom_hist.append(m_)
+    def snap(p_, m_):
+        pos_hist.append(p_)
+        mom_hist.append(m_)
 
-        _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, snap)
-    finally:
-        ctx.close()
+    _run_chunks(ctx, params, steps, save_every, num_saves, rec, steps_per_launch, snap)
     time_points = np.linspace(initial_time, final_time, num_saves + 1)
     return time_points, mom_hist, pos_hist
 

@@ -1,12 +1,16 @@
-"""Host utilities the drivers use before the step: the momentum-dimension contract, the thermal
-p_z completion and the Milne initial-state sampler (mirrors of src/utils.jl), and the driver's
+"""Host utilities the drivers use before the step: the momentum-dimension contract, the FONLL
+sampler's inverse-CDF tables (the draws run on the device, Context.sample_fonll), the thermal p_z
+completion and the Milne initial-state sampler (mirrors of src/utils.jl), and the driver's
 step-count formulas (src/simulate_cpu.jl:47-50, src/simulate_cpu_general_coords.jl:32-35).
-The samplers draw from numpy generators, so they agree with the reference statistically only."""
+The host samplers draw from numpy generators, so they agree with the reference statistically
+only."""
 from __future__ import annotations
 
 import math
 
 import numpy as np
+
+_EPS = float(np.finfo(np.float64).eps)      # Julia eps(Float64)
 
 
 def check_momentum_dims(dimensions: int, pdim: int, radial_mode: bool, bjorken_redshift: bool,
@@ -48,7 +52,7 @@ def step_counts_gc(initial_time: float, final_time: float, dt: float, save_inter
 
 
 def sample_pz_conditional_juttner(mT: float, T: float, rng: np.random.Generator) -> float:
-    """utils.jl:200-223: one draw from f(p_z) ~ exp(-sqrt(mT^2 + p_z^2)/T)."""
+    """utils.jl:499-530: one draw from f(p_z) ~ exp(-sqrt(mT^2 + p_z^2)/T)."""
     if not (T > 0.0 and mT > 0.0):
         return 0.0
     a = mT / T
@@ -72,7 +76,8 @@ def sample_pz_conditional_juttner(mT: float, T: float, rng: np.random.Generator)
 
 
 def append_thermal_pz(momenta, positions, m, T_of_r, *, antithetic=False, rng=None):
-    """utils.jl:234-252: (2, N) LRF momenta -> (3, N) with a thermal p_z row."""
+    """utils.jl:533-558: (2, N) LRF momenta -> (3, N) with a thermal p_z row (host draws; the
+    drivers use the device version, Context.append_thermal_pz)."""
     momenta = np.asarray(momenta, dtype=np.float64)
     positions = np.asarray(positions, dtype=np.float64)
     if momenta.shape[0] != 2:
@@ -89,6 +94,59 @@ def append_thermal_pz(momenta, positions, m, T_of_r, *, antithetic=False, rng=No
         T = max(float(T_of_r(r[i])), 0.0)
         mT = math.sqrt(float(m) ** 2 + momenta[0, i] ** 2 + momenta[1, i] ** 2)
         out[2, i] = sample_pz_conditional_juttner(mT, T, rng)
+    return out
+
+
+def _monotone_normalised(cdf):
+    """utils.jl:60-67 / 134-141: the first node 0, every node at least eps above its predecessor,
+    divided by the last, the last exactly 1"""
+    cdf[0] = 0.0
+    for k in range(1, cdf.size):
+        cdf[k] = max(cdf[k], cdf[k - 1] + _EPS)
+    cdf /= cdf[-1]
+    cdf[-1] = 1.0
+    return cdf
+
+
+def fonll_tables(r_grid, p_grid, f_HQ_init_FONLL, cartesian_spatial_sampling=False):
+    """The inverse-CDF tables of sample_particles_from_FONLL (utils.jl:33-146), built as the
+    reference builds them: P(p|r) ~ p f(r, p) per r node (utils.jl:52-71), then either the r
+    marginal of P(r, p) ~ r p f(r, p) (polar sampling, utils.jl:130-146) or the spatial density
+    n(r) = sum_p p f(r, p) dp with its envelope max n + eps (Cartesian sampling, utils.jl:80-90).
+    f is (Np, Nr), f[p, r], or its transpose."""
+    rg = np.ascontiguousarray(r_grid, dtype=np.float64).ravel()
+    pg = np.ascontiguousarray(p_grid, dtype=np.float64).ravel()
+    Nr, Np = rg.size, pg.size
+    f = np.asarray(f_HQ_init_FONLL, dtype=np.float64)
+    if f.shape != (Np, Nr):
+        if f.shape == (Nr, Np):
+            f = f.T
+        else:
+            raise ValueError(f"sample_particles_from_FONLL: f_HQ_init_FONLL has size {f.shape}, expected "
+                             f"({Np}, {Nr}) (p×r) or ({Nr}, {Np}) (r×p).")
+    dr = float(np.mean(np.diff(rg)))
+    dp = float(np.mean(np.diff(pg)))
+    P = pg[:, None] * f
+    cdf_p = np.zeros((Np, Nr), dtype=np.float64, order="F")
+    p_ok = np.zeros(Nr, dtype=np.float64)
+    for i in range(Nr):
+        pdf = P[:, i]
+        if np.sum(pdf) > 0:
+            cdf_p[:, i] = _monotone_normalised(np.cumsum(pdf) * dp)
+            p_ok[i] = 1.0
+    out = dict(r_grid=rg, p_grid=pg, cdf_p=cdf_p, p_ok=p_ok, cartesian=bool(cartesian_spatial_sampling),
+               cdf_r=None, n_r=None, rmax=0.0, fmax=0.0)
+    if cartesian_spatial_sampling:
+        n_r = np.sum(P, axis=0) * dp
+        out.update(n_r=n_r, rmax=float(rg[-1]), fmax=float(np.max(n_r)) + _EPS)
+    else:
+        Prp = rg[None, :] * pg[:, None] * f
+        Z = np.sum(Prp) * dr * dp
+        if Z == 0:
+            raise ValueError("Distribution normalization is zero.")
+        Prp /= Z
+        P_r = np.sum(Prp, axis=0) * dp
+        out.update(cdf_r=_monotone_normalised(np.cumsum(P_r) * dr))
     return out
 
 

@@ -550,6 +550,144 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
     }
 }
 
+/* ===================== initial-state samplers (utils.jl:33-172, 499-558) ====================== */
+/* Quark k's draws are the words of noise block 3 under key (seed, stream), counter word 1 selecting
+ * the draw: position attempts 0, 1, ..., the momentum 0xFFFFFFFF, thermal p_z attempts
+ * 0x80000000 + a.  The inverse CDFs follow Interpolations.LinearInterpolation(cdf, values) with
+ * flat extrapolation, as utils.jl:68,141 build them.                                             */
+static int64_t sortedlast(const double* v, int64_t n, double x) {   /* Base.searchsortedlast */
+    int64_t lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        int64_t m = lo + ((hi - lo) >> 1);
+        if (x < v[m - 1]) hi = m; else lo = m;
+    }
+    return lo;
+}
+
+static double inverse_cdf(const double* cdf, const double* vals, int64_t n, double u) {
+    if (!(u > cdf[0])) return vals[0];
+    if (!(u < cdf[n - 1])) return vals[n - 1];
+    int64_t k = sortedlast(cdf, n, u);
+    double t = (u - cdf[k - 1]) / (cdf[k] - cdf[k - 1]);
+    return (1.0 - t) * vals[k - 1] + t * vals[k];
+}
+
+static double linear_or_zero(const double* x, const double* f, int64_t n, double q) {
+    if (!(q >= x[0] && q <= x[n - 1])) return 0.0;
+    int64_t k = sortedlast(x, n, q);
+    if (k > n - 1) k = n - 1;
+    double t = (q - x[k - 1]) / (x[k] - x[k - 1]);
+    return (1.0 - t) * f[k - 1] + t * f[k];
+}
+
+static inline double clampd(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+int lmo_sample_fonll(const double* rg, const double* pg, const double* cdf_r, const double* n_r,
+                     const double* cdf_p, const double* p_ok, int32_t nr, int32_t np, int32_t cartesian,
+                     double rmax, double fmax, int64_t N, int32_t antithetic, uint64_t seed,
+                     uint64_t stream, double* x_out, double* p_out) {
+    const int64_t nhalf = antithetic ? N / 2 : 0;
+    int failed = 0;
+#pragma omp parallel for schedule(static) reduction(| : failed)
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t k = i;
+        double sgn = 1.0, x = 0.0, y = 0.0, r = 0.0, s, c;
+        uint64_t w[4];
+        if (antithetic && i < 2 * nhalf) {
+            k = i / 2;
+            sgn = (i % 2) ? -1.0 : 1.0;
+        } else if (antithetic) {
+            k = nhalf;
+        }
+        if (!cartesian) {
+            /* utils.jl:160-166 */
+            lmo_noise_block(seed, stream, k, 0, 3, w);
+            r = clampd(inverse_cdf(cdf_r, rg, nr, u_co(w[0])), rg[0], rg[nr - 1]);
+            noise_sincospi(2.0 * u_co(w[1]), &s, &c);
+            x = r * c;
+            y = r * s;
+        } else {
+            /* utils.jl:98-127 */
+            int done = 0;
+            for (int32_t a = 0; a < 100000 && !done; ++a) {
+                lmo_noise_block(seed, stream, k, a, 3, w);
+                double xa = rmax * (2.0 * u_co(w[0]) - 1.0);
+                double ya = rmax * (2.0 * u_co(w[1]) - 1.0);
+                double ra = sqrt(xa * xa + ya * ya);
+                if (ra > rmax) continue;
+                if (u_co(w[2]) * fmax > linear_or_zero(rg, n_r, nr, ra)) continue;
+                x = xa;
+                y = ya;
+                r = ra;
+                done = 1;
+            }
+            if (!done) failed |= 1;
+        }
+        /* utils.jl:119-125, 168-172: |p| from the p|r CDF of the r node, azimuth uniform */
+        lmo_noise_block(seed, stream, k, 0xFFFFFFFFLL, 3, w);
+        double rc = clampd(r, rg[0], rg[nr - 1]);
+        int64_t j = sortedlast(rg, nr, rc);
+        j = j < 1 ? 1 : (j > nr ? nr : j);
+        double pm = p_ok[j - 1] != 0.0 ? inverse_cdf(cdf_p + (j - 1) * (int64_t)np, pg, np, u_co(w[0])) : 0.0;
+        noise_sincospi(2.0 * u_co(w[1]), &s, &c);
+        x_out[2 * i] = x;
+        x_out[2 * i + 1] = y;
+        p_out[2 * i] = sgn * (pm * c);
+        p_out[2 * i + 1] = sgn * (pm * s);
+    }
+    return failed ? LMO_E_INVALID : LMO_OK;
+}
+
+/* sample_pz_conditional_juttner, utils.jl:499-530 */
+static double thermal_pz(uint64_t seed, uint64_t stream, int64_t k, double mT, double T) {
+    uint64_t w[4];
+    if (!(T > 0.0 && mT > 0.0)) return 0.0;
+    double a = mT / T;
+    if (a > 1.05) {
+        double c = fmax(2.0, a / (a - 1.0));
+        double sig = sqrt(c / a);
+        for (int it = 0; it < 100000; ++it) {
+            double n0, n1;
+            lmo_noise_block(seed, stream, k, (int64_t)(0x80000000u + (uint32_t)it), 3, w);
+            box_muller(w[0], w[1], &n0, &n1);
+            double y = sig * n0;
+            double ch = cosh(y);
+            double acc = ch * exp(-a * (ch - 1.0) + a * y * y / (2.0 * c));
+            if (u_co(w[2]) <= acc) return mT * sinh(y);
+        }
+        return 0.0;
+    }
+    double b = sqrt(2.0) * T;
+    for (int it = 0; it < 1000000; ++it) {
+        lmo_noise_block(seed, stream, k, (int64_t)(0x80000000u + (uint32_t)it), 3, w);
+        double pz = -b * noise_log(u_oc(w[0])) * (u_co(w[1]) < 0.5 ? -1.0 : 1.0);
+        double acc = exp(-(sqrt(mT * mT + pz * pz) - (mT + fabs(pz)) / sqrt(2.0)) / T);
+        if (u_co(w[2]) <= acc) return pz;
+    }
+    return 0.0;
+}
+
+/* append_thermal_pz, utils.jl:533-558, with T_of_r the radial medium at t0 */
+void lmo_append_thermal_pz(const double* mom, const double* pos, int32_t dims, int64_t N, double m,
+                           double t0, int32_t antithetic, uint64_t seed, uint64_t stream,
+                           const lmo_radial_medium* med, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        int mirror = antithetic && (i % 2);
+        int64_t k = mirror ? i - 1 : i;
+        double r2 = 0.0;
+        for (int32_t d = 0; d < dims; ++d) r2 += pos[(int64_t)dims * k + d] * pos[(int64_t)dims * k + d];
+        double T = lmo_interpolate_2d(med->xgrid, med->nx, med->tgrid, med->nt, med->T, sqrt(r2), t0);
+        T = 0.0 > T ? 0.0 : T;                       /* Julia max(T, 0.0) */
+        double px = mom[2 * k], py = mom[2 * k + 1];
+        double mT = sqrt(m * m + px * px + py * py);
+        double pz = thermal_pz(seed, stream, k, mT, T);
+        out[3 * i] = mom[2 * i];
+        out[3 * i + 1] = mom[2 * i + 1];
+        out[3 * i + 2] = mirror ? -pz : pz;
+    }
+}
+
 /* ===================== Cartesian step on the radial (r, t) medium =========================== */
 
 static inline double interp_med(const lmo_radial_medium* med, const double* f, double r, double t) {

@@ -101,6 +101,15 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim);
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec);
 
+/* ---- initial-state samplers (utils.jl:33-172, 499-558) with the draws of block 3 -------------- */
+int lmo_sample_fonll(const double* rg, const double* pg, const double* cdf_r, const double* n_r,
+                     const double* cdf_p, const double* p_ok, int32_t nr, int32_t np, int32_t cartesian,
+                     double rmax, double fmax, int64_t N, int32_t antithetic, uint64_t seed,
+                     uint64_t stream, double* x_out, double* p_out);
+void lmo_append_thermal_pz(const double* mom, const double* pos, int32_t dims, int64_t N, double m,
+                           double t0, int32_t antithetic, uint64_t seed, uint64_t stream,
+                           const lmo_radial_medium* med, double* out);
+
 /* ---- Cartesian step on the radial (r, t) medium: kernels_cpu.jl + simulate_cpu.jl ------------ */
 void lmo_kernel_boost_to_rest(double* mom, const double* pos, const lmo_radial_medium* med,
                               double m, int64_t N, int64_t step, double dt, double t0,

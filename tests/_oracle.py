@@ -93,6 +93,10 @@ def lib():
             "lmo_run_grid": (c_int, [P, POINTER(GridMedium), POINTER(Species), c_int32, PD, PD, c_int64,
                                      c_int64, c_int64, PD, POINTER(c_int64)]),
             "lmo_observables": (c_int, [PD, c_int32, c_int64, c_int64, POINTER(_ffi.lmh_obs_spec), PD]),
+            "lmo_sample_fonll": (c_int, [PD, PD, PD, PD, PD, PD, c_int32, c_int32, c_int32, c_double, c_double,
+                                         c_int64, c_int32, c_uint64, c_uint64, PD, PD]),
+            "lmo_append_thermal_pz": (None, [PD, PD, c_int32, c_int64, c_double, c_double, c_int32, c_uint64,
+                                             c_uint64, POINTER(RadialMedium), PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -156,6 +160,30 @@ def species(keep, masses, counts, drag=None, rta=None, jt=None):
             v = keep.f64(np.zeros(4))
             s.jt[k] = JTable(dp(v), 0.0, 1.0, 2, 2)
     return s
+
+
+def sample_fonll(tables, N, antithetic=False, seed=0, stream=0):
+    """The FONLL sampler's draws (the device sampler's definition) from lmhip.utils.fonll_tables."""
+    keep = Keep()
+    t = {k: keep.f64(tables[k]) for k in ("r_grid", "p_grid", "cdf_r", "n_r", "cdf_p", "p_ok")}
+    x = np.zeros((2, N), order="F")
+    p = np.zeros((2, N), order="F")
+    rc = lib().lmo_sample_fonll(dp(t["r_grid"]), dp(t["p_grid"]), dp(t["cdf_r"]), dp(t["n_r"]), dp(t["cdf_p"]),
+                                dp(t["p_ok"]), t["r_grid"].size, t["p_grid"].size, 1 if tables["cartesian"] else 0,
+                                float(tables["rmax"]), float(tables["fmax"]), N, 1 if antithetic else 0,
+                                seed, stream, dp(x), dp(p))
+    assert rc == 0, rc
+    return x, p
+
+
+def append_thermal_pz(mom, pos, m, t0, med, antithetic=False, seed=0, stream=0):
+    """append_thermal_pz with the device sampler's draws, T(r) of the radial medium at t0."""
+    keep = Keep()
+    mom, pos = keep.f64(mom), keep.f64(pos)
+    out = np.zeros((3, mom.shape[1]), order="F")
+    lib().lmo_append_thermal_pz(dp(mom), dp(pos), pos.shape[0], mom.shape[1], float(m), float(t0),
+                                1 if antithetic else 0, seed, stream, ctypes.byref(med), dp(out))
+    return out
 
 
 def noise_record(params, dims, pdim, N, step_begin, nsteps):
