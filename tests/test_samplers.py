@@ -134,3 +134,53 @@ def test_thermal_pz_distribution(mT, T):
     # antithetic: quark 2k+1 takes minus the p_z of quark 2k
     anti = O.append_thermal_pz(mom[:, :10], pos[:, :10], m, 0.5, med, antithetic=True, seed=21, stream=1)
     assert np.array_equal(anti[2, 1::2], -anti[2, 0::2]) and np.array_equal(anti[2, 0::2], pz[:10:2])
+
+
+def _uniform_bath(T):
+    keep = O.Keep()
+    xg = np.linspace(0.0, 10.0, 5)
+    tg = np.array([0.0, 1.0])
+    Tf = np.full((xg.size, tg.size), T, order="F")
+    return keep, O.radial_medium(keep, xg, tg, Tf, np.zeros_like(Tf))
+
+
+def _juttner3(rng, M, T, N):
+    """3-D Juttner momenta: |p| ~ p^2 exp(-(E - M)/T) by rejection from the Gamma(3, T) density
+    p^2 exp(-p/T) (acceptance exp(-(E - p)/T) <= 1), isotropic directions"""
+    out = []
+    while sum(a.size for a in out) < N:
+        p = rng.gamma(3.0, T, 2 * N)
+        E = np.sqrt(p * p + M * M)
+        out.append(p[rng.random(p.size) < np.exp(-(E - p) / T)])
+    p = np.concatenate(out)[:N]
+    z = rng.uniform(-1.0, 1.0, N)
+    phi = rng.uniform(0.0, 2 * math.pi, N)
+    rho = np.sqrt(1.0 - z * z)
+    return np.asfortranarray(np.stack([p * rho * np.cos(phi), p * rho * np.sin(phi), p * z]))
+
+
+def test_thermal_pz_second_moment_and_isotropy():
+    """test_momentum_dims3.jl:98-117: <p_z^2> of the conditional draw at fixed m_T against
+    quadrature (rtol 0.02), zero mean; the transverse rows of a 3-D Juttner sample completed with
+    the conditional p_z are isotropic; antithetic mirroring is exact"""
+    for mT, T in ((1.6, 0.3), (2.5, 0.45), (1.5, 0.156)):
+        keep, med = _uniform_bath(T)
+        N = 200_000
+        mom = np.zeros((2, N), order="F")
+        mom[0] = math.sqrt(mT * mT - 1.0)
+        pos = np.ones((2, N), order="F")
+        s = O.append_thermal_pz(mom, pos, 1.0, 0.5, med, seed=7, stream=3)[2]
+        w = lambda z: math.exp(-(math.sqrt(mT * mT + z * z) - mT) / T)
+        num = integrate.quad(lambda z: z * z * w(z), -np.inf, np.inf, epsrel=1e-10)[0]
+        den = integrate.quad(w, -np.inf, np.inf, epsrel=1e-10)[0]
+        assert math.isclose(np.mean(s ** 2), num / den, rel_tol=0.02)
+        assert abs(np.mean(s)) < 4 * np.std(s) / math.sqrt(s.size)
+    M, T = 1.5, 0.30
+    keep, med = _uniform_bath(T)
+    P3 = _juttner3(np.random.default_rng(7), M, T, 200_000)
+    x = np.zeros((2, P3.shape[1]), order="F")
+    P = O.append_thermal_pz(np.asfortranarray(P3[:2]), x, M, 0.5, med, seed=8)
+    assert math.isclose(np.mean(P[2] ** 2), np.mean(P3[0] ** 2), rel_tol=0.02)
+    assert math.isclose(np.mean(P[2] ** 2), np.mean(P[0] ** 2), rel_tol=0.02)
+    Pa = O.append_thermal_pz(np.asfortranarray(P3[:2, :1000]), x[:, :1000], M, 0.5, med, antithetic=True, seed=8)
+    assert np.array_equal(Pa[2, 1::2], -Pa[2, 0::2])
