@@ -30,9 +30,9 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 BYTES_PER_PARTICLE_STEP = 80   # read + write x, y, px, py, pz (FP64)
 # HBM bytes per launch of the step kernel on the default workload, from rocprofv3 FETCH_SIZE
 # (x2.00) + WRITE_SIZE (x1.00), factors from a 1 GiB device copy: profiles/k_cart_grid_1e7_grid2p1_pmc.md
-# (the 80 algorithmic bytes, the 4-byte identity read of the cell-sorted order and the deferred
-# collisions' records)
-MEASURED_TRAFFIC_PER_QUARK = 86.21
+# (the 80 algorithmic bytes, the 4-byte identity read of the cell-sorted order and the colliding
+# quarks' Juttner-table reads)
+MEASURED_TRAFFIC_PER_QUARK = 88.98
 
 
 def build_workload(N, seed=3):
@@ -352,7 +352,7 @@ def main():
     ap.add_argument("--spl", type=int, default=1, help="steps per kernel launch")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--slice", type=int, default=1)
-    ap.add_argument("--defer", type=int, default=1)
+    ap.add_argument("--defer", type=int, default=0)
     ap.add_argument("--sort-every", type=int, default=200,
                     help="steps between device sorts of the quarks by grid cell (0: never)")
     ap.add_argument("--collision", default="langevin_rta", choices=sorted(COLLISIONS),
@@ -467,7 +467,7 @@ def main():
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
     achieved = bytes_per_launch / t_launch / 1e9
     default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox" and args.sort_every == 200
-                     and args.defer == 1 and args.slice == 1 and not args.host_sort)
+                     and args.defer == 0 and args.slice == 1 and not args.host_sort)
     traffic = args.traffic
     if traffic is None and default_shape:
         traffic = round(MEASURED_TRAFFIC_PER_QUARK * N)

@@ -147,7 +147,9 @@ typedef struct lmh_tuning {
                                   lmh_adjoint_forward always runs this way)                    */
     int32_t defer;             /* 2+1D grid, one-step launches with RTA: 1 queues the colliding
                                   particles and finishes them in a second kernel, so the
-                                  redraw runs in colliding lanes only (default 1)              */
+                                  redraw runs in colliding lanes only; 0 redraws them in the
+                                  step kernel (default 0: the faster on MI355X at a few % of
+                                  collisions per step; identical results)                     */
     int32_t specialize;        /* 1: the common configuration (relativistic, pre-point, drag
                                   table, no Bjorken redshift, no position diffusion) runs
                                   kernels compiled for it, with identical results (default 1) */

@@ -816,12 +816,16 @@ __global__ __launch_bounds__(kBlock) void k_permute(const double* __restrict__ s
 // one k_grid_slices launch
 constexpr int kSliceRing = 16;
 
-// three waves per SIMD: the register budget the step needs with a few spilled values, against
-// two waves at its unconstrained register count
+// the adjoint's tape kernel and the batch kernel: at least three waves per SIMD
 constexpr int kStepWaves = 3;
 
+// the specialised sliced 2+1D step kernels (the common configuration): at least seven waves per
+// SIMD, which they reach without spills; left to itself the compiler gives the one with the
+// collision redraw 85 VGPRs and five waves, 4 % slower.  The generic kernels keep three.
+constexpr int kGridWaves = 7;
+
 template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER, bool SPEC>
-__global__ __launch_bounds__(kBlock, kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
+__global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
     int seg;
     const int64_t i_thread = particle_of_thread(a, &seg);
     // with DEFER every thread reaches the block-wide ranking; threads past the segment's end
@@ -1216,7 +1220,7 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 1, 1, 1, 200};
+    lmh_tuning tune = {256, 1, 1, 0, 1, 200};
 
     // cell sorting: the alternate state and identity buffers of the permutation, the radix
     // sort's keys, values and workspace; is_sorted says the state is in cell order with d_id

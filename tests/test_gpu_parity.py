@@ -325,14 +325,15 @@ def test_general_coords_driver_shapes(gpu_ok):
     assert np.all(np.isfinite(mh[-1])) and np.all(mh[-1] >= 1.5 * (1 - 1e-6))
 
 
-@pytest.mark.parametrize("medium", ["grid", "radial"])
-def test_specialized_kernels_identical_to_generic(gpu_ok, medium):
-    """the kernels compiled for the common configuration give the generic kernels' bits"""
+@pytest.mark.parametrize("medium,defer", [("grid", 0), ("grid", 1), ("radial", 0)])
+def test_specialized_kernels_identical_to_generic(gpu_ok, medium, defer):
+    """the kernels compiled for the common configuration give the generic kernels' bits, with the
+    collisions redrawn in the step kernel or queued for a second one"""
     N = 5000
     res = []
     for spec in (0, 1):
         with lmhip.Context(0) as ctx:
-            ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=1, specialize=spec)
+            ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=defer, specialize=spec)
             if medium == "grid":
                 g_ = small_grid()
                 pos = media.hydro_positions(N, np.random.default_rng(77)) * 2.0
