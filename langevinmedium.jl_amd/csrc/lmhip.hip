@@ -577,10 +577,10 @@ __device__ __forceinline__ void grid_sample(const GridMedium& g, double x, doubl
         const int64_t nxy = (int64_t)g.nx * g.ny;
         const double4* s0 = g.nodes + nxy * j + n00;
         const double4* s1 = s0 + nxy;
-        c00 = lerp4(s0[0], s1[0], td);
-        c10 = lerp4(s0[1], s1[1], td);
-        c01 = lerp4(s0[g.nx], s1[g.nx], td);
-        c11 = lerp4(s0[g.nx + 1], s1[g.nx + 1], td);
+        c00 = lerp4(ld_global4(s0), ld_global4(s1), td);
+        c10 = lerp4(ld_global4(s0 + 1), ld_global4(s1 + 1), td);
+        c01 = lerp4(ld_global4(s0 + g.nx), ld_global4(s1 + g.nx), td);
+        c11 = lerp4(ld_global4(s0 + g.nx + 1), ld_global4(s1 + g.nx + 1), td);
     }
     double a0 = c00.x * (1.0 - xd) + c10.x * xd;
     double a1 = c01.x * (1.0 - xd) + c11.x * xd;

@@ -24,6 +24,22 @@ struct BatchArgs {
     const int64_t* step_dev;
 };
 
+// A block-uniform struct read from device memory into scalar registers.  The compiler issues
+// vector loads for it (it cannot tell that the kernel never writes the run table), so every
+// 32-bit word is made wave-uniform with readfirstlane: what derives from the run (medium and
+// table pointers, sizes, mass) then stays in scalar registers instead of occupying VGPRs.
+template <typename T>
+__device__ __forceinline__ T uniform_read(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "uniform_read: size must be a multiple of 4 bytes");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, p, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // this tile's run as a single-run view: state rows and Philox stream of the run
 __device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& run, int r, StepArgs* a) {
     *a = b.base;
@@ -81,8 +97,8 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, 
     const int64_t step0 = *b.step_dev + k_off;
     const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
     for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += stride) {
-        const int r = b.tile_run[tile];
-        const BatchRun& run = b.runs[r];
+        const int r = __builtin_amdgcn_readfirstlane(b.tile_run[tile]);
+        const BatchRun run = uniform_read(&b.runs[r]);
         const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
         if (i >= run.count) continue;
         StepArgs a;
@@ -113,8 +129,8 @@ __global__ void k_batch_advance(int64_t* step_dev, int64_t n) {
 template <int PDIM>
 __global__ __launch_bounds__(kBlock) void k_batch_boost_lab(BatchArgs b, int64_t step) {
     for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
-        const int r = b.tile_run[tile];
-        const BatchRun& run = b.runs[r];
+        const int r = __builtin_amdgcn_readfirstlane(b.tile_run[tile]);
+        const BatchRun run = uniform_read(&b.runs[r]);
         const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
         if (i >= run.count) continue;
         StepArgs a;

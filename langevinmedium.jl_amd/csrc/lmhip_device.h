@@ -71,6 +71,13 @@ __device__ __forceinline__ double ld_global(const double* p) {
     return *(const __attribute__((address_space(1))) double*)p;
 }
 
+// (component by component: a copy of the vector type through an address-space pointer would
+// go back through a generic pointer)
+__device__ __forceinline__ double4 ld_global4(const double4* p) {
+    const __attribute__((address_space(1))) double* q = (const __attribute__((address_space(1))) double*)p;
+    return make_double4(q[0], q[1], q[2], q[3]);
+}
+
 // eval_tau_n_spline, transport.jl:231-253 (index clamped, fraction not clamped)
 struct TimeTable {
     const double* vals;
