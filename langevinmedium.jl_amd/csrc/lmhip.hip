@@ -2302,6 +2302,12 @@ int lmh_sample_fonll(lmh_ctx* c, const lmh_fonll_tables* tb, int64_t N, int32_t 
                                       "rmax > 0 and fmax > 0");
     if (!sorted(tb->r_grid, tb->nr) || !sorted(tb->p_grid, tb->np))
         return c->fail(LMH_E_INVALID, "sample_fonll: the grids must be ascending");
+    if (tb->cartesian) {
+        // the rejection loop accepts nothing from a density that vanishes everywhere
+        double nmax = 0.0;
+        for (int32_t k = 0; k < tb->nr; ++k) nmax = std::isfinite(tb->n_r[k]) ? std::max(nmax, tb->n_r[k]) : nmax;
+        if (!(nmax > 0.0)) return c->fail(LMH_E_INVALID, "sample_fonll: the spatial density n(r) vanishes everywhere");
+    }
     if (N == 0) return LMH_OK;
     LMH_HIP(c, hipSetDevice(c->device));
     const size_t nr = (size_t)tb->nr, np = (size_t)tb->np, n = (size_t)N;

@@ -54,6 +54,8 @@ def test_sample_fonll_empty_and_errors(gpu_ok):
         bad = dict(t, cdf_p=t["cdf_p"][:-1])
         with pytest.raises(ValueError):
             ctx.sample_fonll(bad, 10)
+        with pytest.raises(lmhip.LMHError, match="vanishes everywhere"):
+            ctx.sample_fonll(dict(t, n_r=np.zeros_like(t["n_r"])), 10)
 
 
 @pytest.mark.parametrize("dims,antithetic", [(2, False), (2, True), (1, False)])
