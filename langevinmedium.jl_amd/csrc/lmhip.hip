@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) && rta_collides(p, sp.rta, T, ucol)) {
                 double usamp, g[PDIM];
                 noise_redraw<PDIM, NOISE>(a, i, step, s_off, &usamp, g);
-                rta_redraw<PDIM, RADIAL>(sp.jt, T, q, usamp, g);
+                rta_redraw<PDIM, RADIAL, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, g);
             }
             if (!skip) radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, +1, p.relativistic, vmax_());
         }
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
                 } else if (col) {
                     double usamp, gg[PDIM];
                     noise_redraw<PDIM, NOISE>(a, pid, step, s_off, &usamp, gg);
-                    rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+                    rta_redraw<PDIM, false, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, gg);
                 }
             }
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
             const FlowBoost fb = flow_boost(vx, vy);
             double usamp, gg[PDIM];
             noise_redraw<PDIM, NOISE>(a, pid, step, 0, &usamp, gg);
-            rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+            rta_redraw<PDIM, false, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, gg);
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
             grid_push<PDIM, NOISE, SLICE>(a, p, g, pid, step, 0, j, td, m, &x0, &x1, q);
             a.pos[0][i] = x0;

@@ -113,7 +113,7 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_pa
     if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
         double usamp, gg[PDIM];
         noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
-        rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+        rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
     }
     apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);

@@ -83,7 +83,7 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
             double usamp, gg[PDIM];
             noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
-            rta_redraw<PDIM, false>(sp.jt, T, q, usamp, gg);
+            rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }

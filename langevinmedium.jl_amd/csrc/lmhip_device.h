@@ -462,12 +462,18 @@ __device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTabl
     return rta_decide(p, rta_lookup(taun, T), T, ucol);
 }
 
-template <int PDIM, bool RADIAL>
+// The redraw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701): |p*| from the table, the
+// direction g normalised.  UNIT: g is a unit vector already (the Philox record's direction, unit
+// to an ulp or two), so p* g without the norm's square root and division.
+template <int PDIM, bool RADIAL, bool UNIT = false>
 __device__ __forceinline__ void rta_redraw(const JuttnerTable& jt, double T, double* q,
                                            double usamp, const double* g) {
     double pstar = juttner_lookup(jt, usamp, T);
     if (RADIAL) {
         q[0] = g[0] < 0.0 ? -pstar : pstar;
+    } else if (UNIT) {
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = g[d] * pstar;
     } else {
         double nrm2 = 0.0;
 #pragma unroll
