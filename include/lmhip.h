@@ -60,10 +60,11 @@ extern "C" {
  * each lo | hi << 32 of consecutive outputs; the redraw block evaluates half 0 only) of counter
  * (i, s, stream mod 2^32, (stream >> 32) mod 2^29 | (2b + h) << 29) under key
  * (seed mod 2^32, seed >> 32); i and s must stay below 2^32 and stream below 2^61.  Normals are
- * Box-Muller pairs of 53-bit uniforms; the collision uniform takes the 11 low bits of each of
- * block 0's words; the redraw's quantile is the 53-bit uniform of word 0 of block 1 and its
- * direction the unit vector of word 1 (z = 2u - 1 from the top 32 bits, the azimuth from the
- * low 32 bits) in place of the Gaussian g of a fixed record (the redraw normalises either).  */
+ * Box-Muller pairs of 53-bit uniforms, except block 0's third normal: r cos(pi x) with r from
+ * word 2, x the top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3; the collision
+ * uniform is the low 31 bits of word 3 over 2^31; the redraw's quantile is the 53-bit uniform of
+ * word 0 of block 1 and its direction the unit vector of word 1 (z = 2u - 1 from the top 32 bits,
+ * the azimuth from the low 32 bits) in place of the Gaussian g of a fixed record.             */
 #define LMH_NOISE_FIXED        0
 #define LMH_NOISE_PHILOX       1
 

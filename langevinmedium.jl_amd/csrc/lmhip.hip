@@ -179,13 +179,9 @@ __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t
             box_muller(w[0], w[1], &n0, &n1);
             xi[0] = n0;
             if (PDIM >= 2) xi[1 % PDIM] = n1;
-            if (PDIM >= 3) {
-                double n2, n3;
-                box_muller(w[2], w[3], &n2, &n3);
-                xi[2 % PDIM] = n2;
-            }
+            if (PDIM >= 3) xi[2 % PDIM] = half_box_muller(w[2], w[3]);
         }
-        if (rta) *ucol = u_low44(w);
+        if (rta) *ucol = u_low31(w[3]);
     }
 }
 

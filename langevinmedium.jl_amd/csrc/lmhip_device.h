@@ -170,13 +170,9 @@ __device__ __forceinline__ void noise_block(uint64_t seed, uint64_t stream, int6
 __device__ __forceinline__ double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
 __device__ __forceinline__ double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
 
-// a 44-bit uniform in [0,1) from the 11 low bits of four words (the 53-bit uniforms of the
-// Box-Muller pairs use the high bits)
-__device__ __forceinline__ double u_low44(const uint64_t* w) {
-    uint64_t b = ((w[0] & 0x7FFULL) << 33) | ((w[1] & 0x7FFULL) << 22) | ((w[2] & 0x7FFULL) << 11) |
-                 (w[3] & 0x7FFULL);
-    return (double)b * 0x1.0p-44;
-}
+// the collision uniform: a 31-bit uniform in [0,1) from the low 31 bits of a word (the third
+// kick normal takes the word's top 32 bits and bit 31)
+__device__ __forceinline__ double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
 // 1/sqrt(x) for x > 0 (normal): the hardware estimate refined by two Newton steps, within an ulp
 // of the correctly rounded 1/sqrt(x); half the instructions of sqrt followed by a division
@@ -293,6 +289,29 @@ __device__ __forceinline__ void unit_direction(uint64_t w, double* g) {
     g[0] = rho * c;
     g[1 % PDIM] = rho * s;
     g[2 % PDIM] = z;
+}
+
+// noise_cospi_half(x) = cos(pi x) for x in [0, 1/2]: the Taylor polynomial in x^2 through x^20
+// (truncation below 2e-17), Horner's rule with fused multiply-adds; the same definition on the host
+__device__ __forceinline__ double noise_cospi_half(double x) {
+    constexpr double C[11] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895, 0.2353306303588932,
+                              -0.02580689139001406, 0.0019295743094039231, -0.0001046381049248457,
+                              4.303069587032947e-06, -1.3878952462213771e-07, 3.604730797462501e-09};
+    const double z = x * x;
+    double p = C[10];
+#pragma unroll
+    for (int j = 9; j >= 0; --j) p = fma(p, z, C[j]);
+    return p;
+}
+
+// One normal from two words, half a Box-Muller pair: radius from the (0,1] uniform of wa, cos(pi x)
+// with x = the top 32 bits of wb over 2^33 (in [0, 1/2)), negated when bit 31 of wb is set.  A
+// uniform angle's cosine has the distribution of a random sign times cos(pi x), so this is r cos
+// of a uniform angle without the quadrant logic and the sine; the low 31 bits of wb stay free.
+__device__ __forceinline__ double half_box_muller(uint64_t wa, uint64_t wb) {
+    const double r = sqrt_pos(-2.0 * noise_log(u_oc(wa)));
+    const double n = r * noise_cospi_half((double)(uint32_t)(wb >> 32) * 0x1.0p-33);
+    return ((wb >> 31) & 1u) ? -n : n;
 }
 
 // Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb

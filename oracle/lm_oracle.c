@@ -478,9 +478,32 @@ static inline void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) 
     *n1 = r * s;
 }
 
-void lmo_noise_transforms(double u, double x, double out[3]) {
+/* noise_cospi_half(x) = cos(pi x) for x in [0, 1/2]: Taylor in x^2 through x^20, Horner */
+static const double CH[11] = {  /* pi^(2j) (-1)^j / (2j)!, rounded to double */
+    1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895, 0.2353306303588932,
+    -0.02580689139001406, 0.0019295743094039231, -0.0001046381049248457, 4.303069587032947e-06,
+    -1.3878952462213771e-07, 3.604730797462501e-09};
+
+static inline double noise_cospi_half(double x) {
+    double z = x * x;
+    double p = CH[10];
+    for (int j = 9; j >= 0; --j) p = fma(p, z, CH[j]);
+    return p;
+}
+
+/* the third kick normal: radius from the (0,1] uniform of wa, r cos(pi x) with x = the top 32 bits
+ * of wb / 2^33, negated when bit 31 of wb is set (a random sign times cos(pi x), x uniform in
+ * [0, 1/2), is the cosine of a uniform angle) */
+static inline double half_box_muller(uint64_t wa, uint64_t wb) {
+    double r = sqrt(-2.0 * noise_log(u_oc(wa)));
+    double n = r * noise_cospi_half((double)(uint32_t)(wb >> 32) * 0x1.0p-33);
+    return ((wb >> 31) & 1u) ? -n : n;
+}
+
+void lmo_noise_transforms(double u, double x, double out[4]) {
     out[0] = noise_log(u);
     noise_sincospi(x, &out[1], &out[2]);
+    out[3] = noise_cospi_half(0.25 * x);
 }
 
 static int noise_layout(const lmh_params* p, int32_t dims, int32_t pdim,
@@ -502,13 +525,8 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
     return noise_layout(p, dims, pdim, &a, &b, &c);
 }
 
-/* a 44-bit uniform in [0,1) from the 11 low bits of the four words, which the 53-bit uniforms
- * of the Box-Muller pairs leave unused */
-static inline double u_low44(const uint64_t w[4]) {
-    uint64_t b = ((w[0] & 0x7FFULL) << 33) | ((w[1] & 0x7FFULL) << 22) | ((w[2] & 0x7FFULL) << 11) |
-                 (w[3] & 0x7FFULL);
-    return (double)b * 0x1.0p-44;
-}
+/* the collision uniform: the low 31 bits of a word over 2^31 */
+static inline double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
 /* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
  * lmo_noise_block(seed, stream, i, s, b).
@@ -528,12 +546,12 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
         if (oxi >= 0 || orta >= 0) {
             lmo_noise_block(p->seed, p->stream, i, step, 0, w);
             box_muller(w[0], w[1], &n[0], &n[1]);
-            box_muller(w[2], w[3], &n[2], &n[3]);
+            n[2] = half_box_muller(w[2], w[3]);
             if (oxi >= 0) {
                 int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
                 for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
             }
-            if (orta >= 0) rec[(int64_t)orta * N + i] = u_low44(w);
+            if (orta >= 0) rec[(int64_t)orta * N + i] = u_low31(w[3]);
         }
         if (orta >= 0) {
             lmo_noise_block(p->seed, p->stream, i, step, 1, w);

@@ -96,7 +96,7 @@ int lmo_draw_juttner_rejection(double m, double T, int32_t d, uint64_t seed, int
 void lmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void lmo_noise_block(uint64_t seed, uint64_t stream, int64_t i, int64_t step, int32_t block,
                      uint64_t w[4]);
-void lmo_noise_transforms(double u, double x, double out[3]);
+void lmo_noise_transforms(double u, double x, double out[4]);
 int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim);
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec);

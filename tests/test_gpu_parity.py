@@ -179,7 +179,11 @@ def test_grid_parity(gpu_ok, name, noise, spl):
     rta = p.collision in (_ffi.LMH_RTA, _ffi.LMH_LANGEVIN_RTA)
     g, o = run_grid_pair(p, g_, [1.5, 4.75], counts, pos, mom, 30, spl=spl, rta=rta,
                          rta_time=media.tau_R_law(0.16))
-    check(g, o)
+    # position diffusion: D = (D_s T / T) / hbar c, and this medium's cold corona (T falling to
+    # 1e-10 GeV) makes a quark's jump there hundreds of fm and T at its landing point steep in
+    # the position, so the dynamics amplifies ulp-level differences (the push's refined
+    # reciprocal square root) step after step: the bar for that case is 1e-10
+    check(g, o, tol=1e-10 if name == "diffusion" else TOL)
 
 
 @pytest.mark.parametrize("slice_", [0, 1])

@@ -86,17 +86,18 @@ def test_noise_record_box_muller():
     assert rec.shape == (3 + 5 + 2, 5)
     u = lambda v: mp.mpf(v >> 11) * mp.mpf(2) ** -53
     u1 = lambda v: mp.mpf((v >> 11) + 1) * mp.mpf(2) ** -53
-    low = lambda w: (((w[0] & 0x7FF) << 33) | ((w[1] & 0x7FF) << 22) | ((w[2] & 0x7FF) << 11) | (w[3] & 0x7FF)) * 2.0 ** -44
     twopi = mp.mpf(6.283185307179586)
     for i in range(5):
         w = _philox_words(99, 3, i, 17, 0)
         r0 = mp.sqrt(-2 * mp.log(u1(w[0])))
         r2 = mp.sqrt(-2 * mp.log(u1(w[2])))
+        # the third normal: r2 cos(pi x), x = top 32 bits of w3 / 2^33, sign from bit 31
+        n2 = r2 * mp.cos(mp.pi * mp.mpf(w[3] >> 32) * mp.mpf(2) ** -33) * (-1 if (w[3] >> 31) & 1 else 1)
         for val, ref in ((rec[0, i], r0 * mp.cos(twopi * u(w[1]))), (rec[1, i], r0 * mp.sin(twopi * u(w[1]))),
-                         (rec[2, i], r2 * mp.cos(twopi * u(w[3])))):
+                         (rec[2, i], n2)):
             assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
-        # the collision uniform is the 44 low bits of block 0
-        assert rec[3, i] == low(w)
+        # the collision uniform is the low 31 bits of w3
+        assert rec[3, i] == (w[3] & 0x7FFFFFFF) * 2.0 ** -31
         # the quantile is the 53-bit uniform of block 1's first word, the redraw direction the
         # unit vector of its second: z = 2 u - 1 at the midpoint of the top 32 bits' cell, the
         # azimuth 2 pi u' from the low 32 bits
@@ -150,11 +151,12 @@ def test_noise_record_statistics():
 
 
 def test_noise_transforms_accuracy():
-    # the Box-Muller log and sincospi are built from correctly rounded operations only
+    # the Box-Muller log, sincospi and the half-range cospi are built from correctly rounded
+    # operations only
     f = L.lmo_noise_transforms
     f.argtypes = [ctypes.c_double, ctypes.c_double, O.PD]
     f.restype = None
-    out = np.zeros(3)
+    out = np.zeros(4)
     rng = np.random.default_rng(0)
     us = list(rng.random(4000)) + [2.0 ** -53, 1.0, 0.5, 0.7071067811865476, 0.7071067811865475, 1 - 2.0 ** -53]
     xs = list(2 * rng.random(len(us) - 6)) + [0.0, 0.25, 0.5, 1.0, 1.5, 1.75]
@@ -164,6 +166,9 @@ def test_noise_transforms_accuracy():
         assert abs(out[0] - float(lu)) <= 4e-16 * max(abs(float(lu)), 1e-300) or (u == 1.0 and out[0] == 0.0)
         assert abs(out[1] - float(mp.sinpi(x))) <= 2.3e-16
         assert abs(out[2] - float(mp.cospi(x))) <= 2.3e-16
+        # cos(pi x) on [0, 1/2] by one polynomial in x^2: within 4e-16 (the cancellation of its
+        # first terms near x = 1/2 costs an ulp or two)
+        assert abs(out[3] - float(mp.cospi(mp.mpf(0.25 * x)))) <= 4e-16
 
 
 def test_noise_record_moments():
