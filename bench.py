@@ -32,7 +32,7 @@ BYTES_PER_PARTICLE_STEP = 80   # read + write x, y, px, py, pz (FP64)
 # (x2.00) + WRITE_SIZE (x1.00), factors from a 1 GiB device copy: profiles/k_cart_grid_1e7_grid2p1_pmc.md
 # (the 80 algorithmic bytes, the 4-byte identity read of the cell-sorted order and the colliding
 # quarks' Juttner-table reads)
-MEASURED_TRAFFIC_PER_QUARK = 88.98
+MEASURED_TRAFFIC_PER_QUARK = 89.00
 
 
 def build_workload(N, seed=3):
