@@ -186,6 +186,26 @@ def test_grid_parity(gpu_ok, name, noise, spl):
     check(g, o, tol=1e-10 if name == "diffusion" else TOL)
 
 
+@pytest.mark.parametrize("noise", ["fixed", "philox"])
+@pytest.mark.parametrize("spl", [1, 5])
+def test_grid_parity_diffusion_warm_medium(gpu_ok, noise, spl):
+    """position diffusion at the 1e-12 bar on the same medium with T floored at 0.1 GeV: the
+    looser bar of the diffusion case above comes from the cold corona alone"""
+    import dataclasses
+    g0 = small_grid()
+    g_ = dataclasses.replace(g0, T=np.maximum(g0.T, 0.1))
+    rng = np.random.default_rng(31)
+    N = 6000
+    counts = [4200, 1800]
+    pos = media.hydro_positions(N, rng) * 2.0
+    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(4200, 1.5, rng),
+                                       media.pt_rapidity_momenta(1800, 4.75, rng)]))
+    nm = _ffi.LMH_NOISE_FIXED if noise == "fixed" else _ffi.LMH_NOISE_PHILOX
+    p = _ffi.make_params(dt=0.02, t0=0.6, DsT=0.6366, noise=nm, seed=2024, position_diffusion=True)
+    g, o = run_grid_pair(p, g_, [1.5, 4.75], counts, pos, mom, 30, spl=spl)
+    check(g, o)
+
+
 @pytest.mark.parametrize("slice_", [0, 1])
 def test_grid_deferred_collisions_identical(gpu_ok, slice_):
     """queueing the colliding particles for a second kernel gives the fused kernel's bits"""
