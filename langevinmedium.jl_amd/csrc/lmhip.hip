@@ -1240,7 +1240,7 @@ struct lmh_ctx {
     // per-species mass and tables as the kernels read them
     SpeciesArgs h_species[LMH_MAX_SPECIES];
     SpeciesArgs* d_species = nullptr;
-    // deferred collisions: particle indices and the two step-parity counters
+    // the deferred collisions of the queued mode (tuning defer 1): records and per-block counts
     double4* d_qrec = nullptr;        // two double4 per queued particle
     int32_t* d_qcount = nullptr;      // one count per step-kernel block
 
