@@ -734,3 +734,55 @@ def test_freezeout_bjorken_crossing_time():
     lo, hi = hist[s - 1][1], hist[s][1]
     assert np.all((fo.fo_mom - np.minimum(lo, hi)) >= -1e-12) and np.all((np.maximum(lo, hi) - fo.fo_mom) >= -1e-12)
     assert np.allclose(fo0.fo_mom, hi, rtol=1e-15, atol=1e-15)   # last + 1 * (current - last)
+
+
+def test_position_diffusion_msd_gives_Ds():
+    """Position diffusion (kernels_cpu.jl:591-607): with the momenta glued to a fluid at rest
+    (no streaming), the diffusion kicks alone make the 2-D mean squared displacement 4 D t with
+    D = (D_s T / T) / hbar c (rtol 3 %; 2e4 quarks x 400 steps, statistical error 0.7 %)"""
+    keep = O.Keep()
+    m, T, DsT, dt = 1.5, 0.3, 0.6366197723675814, 0.01
+    med, tab = uniform_setup(keep, T, m, DsT)
+    N, steps = 20_000, 400
+    sp = O.species(keep, [m], [N], drag=[tab])
+    p = _ffi.make_params(dt=dt, DsT=DsT, noise=_ffi.LMH_NOISE_PHILOX, seed=12, collision=_ffi.LMH_FLUID,
+                         position_diffusion=True)
+    P = np.zeros((2, N), order="F")
+    M = np.zeros((2, N), order="F")
+    rec = O.noise_record(p, 2, 2, N, 1, steps)
+    rc, _ = O.run_cartesian(p, med, sp, P, M, 1, steps, np.ascontiguousarray(rec))
+    assert rc == 0
+    assert np.all(M == 0.0)
+    D = DsT / T / FMGEV
+    assert np.mean(np.sum(P * P, axis=0)) == pytest.approx(4 * D * steps * dt, rel=0.03)
+
+
+def test_reflecting_wall_preserves_momentum_magnitude():
+    """The reflecting wall (kernels_cpu.jl:609-634) at r_max, the last radial node: a quark that
+    crosses it comes back to 2 r_max - r with its radial momentum flipped, its tangential momentum
+    and |p| unchanged -- against the same step without the wall"""
+    keep = O.Keep()
+    m, T, DsT, dt = 1.5, 0.3, 0.6366197723675814, 0.01
+    med, tab = uniform_setup(keep, T, m, DsT)               # radial nodes 0 .. 30 fm
+    N = 64
+    rng = np.random.default_rng(4)
+    phi = rng.uniform(0.0, 2 * np.pi, N)
+    P0 = np.asfortranarray(29.995 * np.stack([np.cos(phi), np.sin(phi)]))
+    pr, pt = rng.uniform(3.0, 6.0, N), rng.uniform(-1.0, 1.0, N)
+    M0 = np.asfortranarray(np.stack([pr * np.cos(phi) - pt * np.sin(phi), pr * np.sin(phi) + pt * np.cos(phi)]))
+    sp = O.species(keep, [m], [N], drag=[tab])
+    out = []
+    for wall in (False, True):
+        p = _ffi.make_params(dt=dt, DsT=DsT, noise=_ffi.LMH_NOISE_FIXED, reflecting_boundary=wall)
+        P, M = P0.copy(order="F"), M0.copy(order="F")
+        rc, _ = O.run_cartesian(p, med, sp, P, M, 1, 1, np.zeros((1, 2, N)))
+        assert rc == 0
+        out.append((P, M))
+    (Pf, Mf), (Pr, Mr) = out
+    rf, rr = np.hypot(Pf[0], Pf[1]), np.hypot(Pr[0], Pr[1])
+    assert np.all(rf > 30.0) and np.all(rr <= 30.0)
+    np.testing.assert_allclose(rr, 60.0 - rf, rtol=1e-14)
+    np.testing.assert_allclose(np.hypot(Mr[0], Mr[1]), np.hypot(Mf[0], Mf[1]), rtol=1e-14)
+    nx, ny = Pf[0] / rf, Pf[1] / rf
+    np.testing.assert_allclose(Mr[0] * nx + Mr[1] * ny, -(Mf[0] * nx + Mf[1] * ny), rtol=1e-12)
+    np.testing.assert_allclose(-Mr[0] * ny + Mr[1] * nx, -Mf[0] * ny + Mf[1] * nx, rtol=1e-12, atol=1e-14)
