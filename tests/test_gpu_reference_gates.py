@@ -1,8 +1,9 @@
-"""The reference's physics gates of test_momentum_dims3.jl, run through this backend's public
-entry point on the GPU with Philox noise: the gate (G) of that file (the GPU path must satisfy
-what the CPU path does), applied to (E) the equilibria, (D) the l=1 decay rate, (R) the Bjorken
-redshift and (M) the spatial diffusion coefficient.  Same workloads, tolerances and reference
-quantities (by quadrature) as test_momentum_dims3.jl:24-201."""
+"""The reference's physics gates of test_momentum_dims3.jl and test_relativistic_switch.jl, run
+through this backend's public entry point on the GPU with Philox noise: the gate (G) of the
+first file (the GPU path must satisfy what the CPU path does), applied to (E) the equilibria,
+(D) the l=1 decay rate, (R) the Bjorken redshift and (M) the spatial diffusion coefficient, and
+the relativistic switch's equilibria and streaming.  Same workloads, tolerances and reference
+quantities (by quadrature) as test_momentum_dims3.jl:24-201 and test_relativistic_switch.jl:31-93."""
 import math
 
 import numpy as np
@@ -58,7 +59,8 @@ def juttner_sample(rng, M, T, d, N):
     return np.asfortranarray(p * nv)
 
 
-def box_run(*, N, dt, tfinal, save, x0, p0, pdim, t0=0.0, redshift=False, momentum_langevin=True, seed=1):
+def box_run(*, N, dt, tfinal, save, x0, p0, pdim, t0=0.0, redshift=False, momentum_langevin=True, seed=1,
+            relativistic=True, DsT=DST_):
     """uniform bath at T_, zero flow, the state handed in (test_momentum_dims3.jl:59-71)"""
     xgrid = np.arange(0.0, 400.0 + 0.25, 0.5)
     tgrid = np.arange(t0, t0 + tfinal + 2 + 0.5, 1.0)
@@ -67,9 +69,10 @@ def box_run(*, N, dt, tfinal, save, x0, p0, pdim, t0=0.0, redshift=False, moment
     dens = np.ones((xgrid.size, 10))
     return lmhip.simulate_ensemble_bulk(
         lmhip.HIPBackend(), xgrid, np.linspace(0, 10, 10), dens, Tf, Vf, (xgrid, tgrid), x_init=x0, p_init=p0,
-        N_particles=N, Δt=dt, initial_time=t0, final_time=t0 + tfinal, save_interval=save, m=M_, DsT=DST_,
+        N_particles=N, Δt=dt, initial_time=t0, final_time=t0 + tfinal, save_interval=save, m=M_, DsT=DsT,
         dimensions=2, momentum_dimensions=pdim, bjorken_redshift=redshift, position_diffusion=False,
-        momentum_langevin=momentum_langevin, reflecting_boundary=False, Tfo=0.0, seed=seed)
+        momentum_langevin=momentum_langevin, reflecting_boundary=False, Tfo=0.0, seed=seed,
+        relativistic=relativistic)
 
 
 def est(mom, M):
@@ -158,3 +161,29 @@ def test_gate_M_spatial_diffusion(gpu_ok):
     ds = c[1] / 4.0
     print(f"(M) D_s measured/nominal {ds / DS_FM:.4f}", flush=True)
     assert math.isclose(ds, DS_FM, rel_tol=0.03)
+
+
+def test_gate_relativistic_switch(gpu_ok):
+    """test_relativistic_switch.jl:31-93: d = 2, a hot isotropic start in a bath at rest; the
+    relativistic run relaxes to the 2-D Juttner <p^2> = 2T(M^2+3MT+3T^2)/(M+T), the
+    non-relativistic one to the Maxwell 2MT (rtol 4 %), and the non-relativistic run streams at
+    p/M > p/E, so it spreads further"""
+    N, dt, nsteps = 20_000, 2e-3, 4000
+    rng = np.random.default_rng(20260806)
+    p0 = np.asfortranarray(0.9 * rng.standard_normal((2, N)))
+    x0 = np.zeros((2, N), order="F")
+    res = {}
+    for rel in (True, False):
+        _, mh, ph = box_run(N=N, dt=dt, tfinal=nsteps * dt, save=nsteps * dt, x0=x0, p0=p0, pdim=2, DsT=0.5,
+                            relativistic=rel, seed=7)
+        res[rel] = (np.mean(np.sum(mh[-1] ** 2, axis=0)), np.mean(np.sum(ph[-1] ** 2, axis=0)))
+    juttner = 2 * T_ * (M_ ** 2 + 3 * M_ * T_ + 3 * T_ ** 2) / (M_ + T_)
+    maxwell = 2 * M_ * T_
+    print(f"(S) <p2> rel {res[True][0]:.4f} (Juttner {juttner:.4f}) non-rel {res[False][0]:.4f} "
+          f"(Maxwell {maxwell:.4f}); <x2> {res[True][1]:.3f} / {res[False][1]:.3f}", flush=True)
+    assert not math.isclose(res[True][0], res[False][0], rel_tol=1e-6)
+    assert math.isclose(res[True][0], juttner, rel_tol=0.04)
+    assert math.isclose(res[False][0], maxwell, rel_tol=0.04)
+    assert juttner / maxwell > 1.2
+    assert not math.isclose(res[True][1], res[False][1], rel_tol=1e-3)
+    assert res[False][1] > res[True][1]
