@@ -38,9 +38,18 @@ extern "C" {
 
 /* ---- enumerations ---------------------------------------------------------------------- */
 /* coordinates: Cartesian step (simulate_cpu.jl:215-316) or the general-coordinate (tau, r)
- * step (simulate_cpu_general_coords.jl:83-126)                                             */
+ * step (simulate_cpu_general_coords.jl:83-126), or MILNE (extension): position rows
+ * (x, y, eta_s) and momentum rows (p_x, p_y, p_z) with p_z the longitudinal momentum in the
+ * frame comoving with the Bjorken flow at eta_s, on the radial medium.  A MILNE step is the
+ * Cartesian step with bjorken_redshift (kernels_cpu.jl:427-435) on (x, y, p) -- the medium at
+ * the transverse radius -- followed by the space-time rapidity of the free path over the step
+ * tau_a = t0 + (step - 1) dt -> tau_a + dt (p_z tau conserved, y - eta_s = asinh(p_z/m_T), the
+ * note at kernels_cpu.jl:419-426): eta_s += asinh(k b) - asinh(b), b = p_z/m_T at the end of the
+ * step, k = 1 + dt/tau_a.  Needs dims == pdim == 3, t0 > 0, no position diffusion, reflecting
+ * wall or freeze-out capture.                                                               */
 #define LMH_CARTESIAN          0
 #define LMH_GENERAL_COORDS     1
+#define LMH_MILNE              2
 
 /* momentum update branch (simulate_cpu.jl:229-278):
  *   LANGEVIN      forces + update + boost back                     (default :langevin)
@@ -102,7 +111,7 @@ typedef struct lmh_params {
     double   Lambda;              /* LMH_DRAG_GAMMA_P momentum scale [GeV]                   */
     uint64_t seed;                /* Philox key word 0                                       */
     uint64_t stream;              /* Philox key word 1                                       */
-    int32_t  coords;              /* LMH_CARTESIAN | LMH_GENERAL_COORDS                      */
+    int32_t  coords;              /* LMH_CARTESIAN | LMH_GENERAL_COORDS | LMH_MILNE          */
     int32_t  collision;           /* LMH_LANGEVIN | LMH_RTA | LMH_FLUID | LMH_LANGEVIN_RTA    */
     int32_t  noise;               /* LMH_NOISE_FIXED | LMH_NOISE_PHILOX                      */
     int32_t  integrator;          /* LMH_PREPOINT | LMH_MIDPOINT | LMH_POSTPOINT             */

@@ -213,12 +213,12 @@ __device__ __forceinline__ void noise_diffusion(const StepArgs& a, int64_t i, in
         uint64_t w[4];
         noise_block(a.p.seed, a.p.stream, i, step, 2, w);
         double n0, n1;
-        box_muller(w[0], w[1], &n0, &n1);
+        box_muller<true>(w[0], w[1], &n0, &n1);
         xp[0] = n0;
         if (DIMS >= 2) xp[1 % DIMS] = n1;
         if (DIMS >= 3) {
             double n2, n3;
-            box_muller(w[2], w[3], &n2, &n3);
+            box_muller<true>(w[2], w[3], &n2, &n3);
             xp[2 % DIMS] = n2;
         }
     }
@@ -270,6 +270,24 @@ __device__ __forceinline__ void bjorken(const lmh_params& p, int64_t step, doubl
     }
 }
 
+// The space-time rapidity over the step tau_a -> tau_a + dt of a quark whose comoving
+// longitudinal momentum at the end of the step is p_z (Milne coordinates): p_z tau is conserved
+// along a free path and y - eta_s = asinh(p_z/m_T) (the note at kernels_cpu.jl:419-426), so
+// eta_s advances by asinh(k b) - asinh(b) with b = p_z/m_T, k = 1 + dt/tau_a, written without
+// cancellation as asinh(b (k - 1)(k + 1) / (k sqrt(1 + b^2) + sqrt(1 + (k b)^2))).
+__device__ __forceinline__ double milne_deta(double px, double py, double pz, double m, double tau_a,
+                                             double dt) {
+    if (!(tau_a > 0.0)) return 0.0;
+    const double mT = sqrt(m * m + px * px + py * py);
+    const double b = pz / mT;
+    const double km1 = dt / tau_a;
+    const double k = 1.0 + km1;
+    const double kb = k * b;
+    const double num = b * (km1 * (k + 1.0));
+    const double den = k * sqrt(1.0 + b * b) + sqrt(1.0 + kb * kb);
+    return asinh(num / den);
+}
+
 __device__ __forceinline__ bool finite_all(const double* a, int n) {
     bool ok = true;
     for (int d = 0; d < n; ++d) ok = ok && isfinite(a[d]);
@@ -288,6 +306,9 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
     const lmh_params pp = step_params<SPEC>(a.p);
     const lmh_params& p = pp;
     const double m = sp.m;
+    // Milne coordinates: rows (x, y, eta_s); the medium at the transverse radius, the Bjorken
+    // redshift of p_z, then the push of eta_s
+    const bool milne = DIMS == 3 && PDIM == 3 && !SPEC && p.coords == LMH_MILNE;
     double x[DIMS], q[PDIM];
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) x[d] = a.pos[d][i];
@@ -320,7 +341,8 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
 
         double r2 = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIMS; ++d) r2 += x[d] * x[d];
+        for (int d = 0; d < DIMS; ++d)
+            if (!(milne && d == 2)) r2 += x[d] * x[d];
         const double r = sqrt(r2);
         double xd;
         const int ir = axis_cell(med.xg, med.nx, r, &xd);
@@ -335,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             }
             radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, -1, p.relativistic, vmax_());
         }
-        if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
+        if (PDIM == 3 && (p.bjorken_redshift || milne)) bjorken(p, step, &q[2 % PDIM]);
 
         if (COLL == LMH_FLUID) {
             // kernel_set_to_fluid_velocity_cpu!, kernels_cpu.jl:683-733: r from rows 1-2
@@ -415,7 +437,10 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             x[0] = r_new;
         } else {
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) x[d] = x[d] + p.dt * q[d % PDIM] / E;
+            for (int d = 0; d < DIMS; ++d)
+                if (!(milne && d == 2)) x[d] = x[d] + p.dt * q[d % PDIM] / E;
+            if (milne) x[2 % DIMS] += milne_deta(q[0], q[1 % PDIM], q[2 % PDIM], m,
+                                                 p.t0 + (double)(step - 1) * p.dt, p.dt);
             if (p.position_diffusion) {
                 double rr = 0.0;
 #pragma unroll
@@ -489,7 +514,8 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
     double r2 = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIMS; ++d) r2 += x[d] * x[d];
+    for (int d = 0; d < DIMS; ++d)
+        if (!(DIMS == 3 && p.coords == LMH_MILNE && d == 2)) r2 += x[d] * x[d];
     const double r = sqrt(r2);
     if (r < kEps) return;
     const double t_now = (double)step * p.dt + p.t0;
@@ -667,15 +693,25 @@ template <int PDIM, int NOISE, bool SLICE>
 __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                           int64_t i, int64_t step, int s_off, int j, double td,
                                           double m, double* x0, double* x1, const double* q) {
-    double E2 = m * m;
+    if (!p.position_diffusion) {
+        double E2 = m * m;
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) E2 = fma(q[d], q[d], E2);
-    // dt*p/E as p*(dt/E) with dt/E = dt/sqrt(E^2) from one reciprocal square root, within an
-    // ulp or two of the reference form
-    const double dtE = p.relativistic ? p.dt * rsqrt_pos(E2) : p.dt / m;
-    *x0 = fma(q[0], dtE, *x0);
-    *x1 = fma(q[1], dtE, *x1);
-    if (p.position_diffusion) {
+        for (int d = 0; d < PDIM; ++d) E2 = fma(q[d], q[d], E2);
+        // dt*p/E as p*(dt/E) with dt/E = dt/sqrt(E^2) from one reciprocal square root, within an
+        // ulp or two of the reference form
+        const double dtE = p.relativistic ? p.dt * rsqrt_pos(E2) : p.dt / m;
+        *x0 = fma(q[0], dtE, *x0);
+        *x1 = fma(q[1], dtE, *x1);
+    } else {
+        // with position diffusion the reference form x + dt*p/E (kernels_cpu.jl:586-589): the
+        // jumps that follow, large where T is small, would carry the ulps of the shorter form
+        double E2 = m * m;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) E2 += q[d] * q[d];
+        const double E = p.relativistic ? sqrt(E2) : m;
+        *x0 = *x0 + p.dt * q[0] / E;
+        *x1 = *x1 + p.dt * q[1] / E;
+
         double Td, ux, uy;
         grid_sample<SLICE>(g, *x0, *x1, j, td, &Td, &ux, &uy);
         double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
@@ -1930,6 +1966,13 @@ int check_run(lmh_ctx* c, const lmh_params* p) {
             p->integrator != LMH_PREPOINT || p->position_diffusion || p->reflecting_boundary ||
             p->bjorken_redshift)
             return c->fail(LMH_E_INVALID, "general coordinates support the pre-point Langevin step only");
+    } else if (p->coords == LMH_MILNE) {
+        if (c->medium != 0) return c->fail(LMH_E_STATE, "Milne coordinates need the radial medium");
+        if (c->dims != 3 || c->pdim != 3)
+            return c->fail(LMH_E_INVALID, "Milne coordinates need dims == pdim == 3 (x, y, eta_s)");
+        if (!(p->t0 > 0.0)) return c->fail(LMH_E_INVALID, "Milne coordinates need t0 > 0 (tau)");
+        if (p->position_diffusion || p->reflecting_boundary || c->fo_on)
+            return c->fail(LMH_E_INVALID, "Milne coordinates: no position diffusion, reflecting wall or freeze-out capture");
     } else if (p->coords == LMH_CARTESIAN) {
         bool ok = (c->dims == c->pdim) || (c->dims == 2 && c->pdim == 3);
         if (!ok) return c->fail(LMH_E_INVALID, "momentum_dimensions with dimensions is not supported (utils.jl:561-573)");
@@ -1963,7 +2006,8 @@ int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nst
 
 // the parameters are those the SPEC kernels fix at compile time
 bool spec_config(const lmh_ctx* c, const lmh_params* p) {
-    return c->tune.specialize && !c->fo_on && p->relativistic && p->integrator == LMH_PREPOINT &&
+    return c->tune.specialize && !c->fo_on && p->coords == LMH_CARTESIAN && p->relativistic &&
+           p->integrator == LMH_PREPOINT &&
            p->drag_model == LMH_DRAG_TABLE && !p->bjorken_redshift && !p->position_diffusion;
 }
 
@@ -2160,7 +2204,7 @@ int lmh_boost_to_lab(lmh_ctx* c, const lmh_params* p, int64_t step) {
     ++c->gen;
     int rc = check_run(c, p);
     if (rc) return rc;
-    if (p->coords != LMH_CARTESIAN) return c->fail(LMH_E_INVALID, "boost_to_lab: Cartesian only");
+    if (p->coords == LMH_GENERAL_COORDS) return c->fail(LMH_E_INVALID, "boost_to_lab: Cartesian or Milne only");
     LMH_HIP(c, hipSetDevice(c->device));
     if ((rc = ensure_unsorted(c))) return rc;
     RadialMedium rmed = make_radial(c);

@@ -216,6 +216,8 @@ __device__ __forceinline__ double sqrt_pos(double x) {
 constexpr double kLn2Hi = 6.93147180369123816490e-01;   // e * kLn2Hi is exact for |e| < 2^11
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 
+// EXACT: the correctly rounded division the host uses (bit-identical to the host record)
+template <bool EXACT = false>
 __device__ __forceinline__ double noise_log(double u) {
     uint64_t b = (uint64_t)__double_as_longlong(u);
     int e = (int)((b >> 52) & 0x7FF) - 1023;
@@ -224,7 +226,7 @@ __device__ __forceinline__ double noise_log(double u) {
         m = 0.5 * m;
         e += 1;
     }
-    double s = (m - 1.0) * rcp_pos(m + 1.0);
+    double s = EXACT ? (m - 1.0) / (m + 1.0) : (m - 1.0) * rcp_pos(m + 1.0);
     double z = s * s;
     double p = 1.0 / 19.0;
     p = fma(p, z, 1.0 / 17.0);
@@ -314,9 +316,14 @@ __device__ __forceinline__ double half_box_muller(uint64_t wa, uint64_t wb) {
     return ((wb >> 31) & 1u) ? -n : n;
 }
 
-// Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb
+// Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb.
+// EXACT takes the correctly rounded division and square root of the host restatement, so the
+// pair is the host's to the bit (the position-diffusion block, whose normals are multiplied by
+// jumps that D ~ 1/T makes large in a cold medium)
+template <bool EXACT = false>
 __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
-    double r = sqrt_pos(-2.0 * noise_log(u_oc(wa)));
+    const double l = -2.0 * noise_log<EXACT>(u_oc(wa));
+    double r = EXACT ? sqrt(l) : sqrt_pos(l);
     double s, c;
     noise_sincospi(2.0 * u_co(wb), &s, &c);
     *n0 = r * c;

@@ -21,6 +21,7 @@ LMH_E_NOMEM = -5
 
 LMH_CARTESIAN = 0
 LMH_GENERAL_COORDS = 1
+LMH_MILNE = 2
 LMH_LANGEVIN = 0
 LMH_RTA = 1
 LMH_FLUID = 2

@@ -14,6 +14,11 @@ meanings and defaults; the extensions are keyword arguments the reference does n
   drag_model       "table" (the reference) or ("gamma_p", gamma0, Lambda)
   rta_time         relaxation-time law tau(T) [fm] for the RTA table instead of tau_n_main3
   collision_mode   also "langevin_rta": the Langevin kick followed by the BGK redraw
+  milne            Milne coordinates: with dimensions=2, momentum_dimensions=3 (the reference's
+                   Bjorken mode, bjorken_redshift implied) the positions gain a third row, the
+                   space-time rapidity eta_s of each quark's free path (x_init may hold it as a
+                   third row, else eta_s = 0); p_z is the momentum in the frame comoving with the
+                   Bjorken flow at eta_s
 """
 from __future__ import annotations
 
@@ -161,8 +166,14 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
                         integrator_mode: int = 0, relativistic: bool = True,
                         momentum_dimensions: int = 0, bjorken_redshift: bool = False,
                         noise="philox", seed: int = 0, steps_per_launch: int = 0,
-                        species_counts=None, drag_model="table", rta_time=None):
+                        species_counts=None, drag_model="table", rta_time=None, milne: bool = False):
     coll, cm = _collision_code(collision_mode, momentum_langevin, DsT)
+    if milne:
+        if dimensions != 2 or momentum_dimensions != 3:
+            raise ValueError("milne needs dimensions=2 and momentum_dimensions=3 (x, y, eta_s; p_x, p_y, p_z)")
+        if freezeout_capture or position_diffusion or reflecting_boundary:
+            raise ValueError("milne: no freeze-out capture, position diffusion or reflecting boundary")
+        bjorken_redshift = True
     steps, save_every, num_saves = step_counts(initial_time, final_time, Δt, save_interval)
     if freezeout_capture:
         # simulate_gpu.jl:68-75: no trajectory history, one chunk of all the steps
@@ -185,7 +196,7 @@ def _simulate_cartesian(backend, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
             freezeout_interp=freezeout_interp, integrator_mode=integrator_mode, relativistic=relativistic,
             momentum_dimensions=momentum_dimensions, bjorken_redshift=bjorken_redshift, noise=noise,
             seed=seed, steps_per_launch=steps_per_launch, species_counts=species_counts,
-            drag_model=drag_model, rta_time=rta_time)
+            drag_model=drag_model, rta_time=rta_time, milne=milne)
     finally:
         ctx.close()
 
@@ -197,7 +208,7 @@ def _simulate_cartesian_ctx(ctx, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
                             momentum_langevin, reflecting_boundary, x_init, p_init, V2Evolutionn, psi2,
                             freezeout_capture, freezeout_interp, integrator_mode, relativistic,
                             momentum_dimensions, bjorken_redshift, noise, seed, steps_per_launch,
-                            species_counts, drag_model, rta_time):
+                            species_counts, drag_model, rta_time, milne=False):
     # the medium first: the thermal p_z completion reads T(r, t0) from it
     ctx.set_medium_radial(xgrid, tgrid, Tf, Vf, V2Evolutionn)
     if x_init is None or p_init is None:
@@ -212,6 +223,10 @@ def _simulate_cartesian_ctx(ctx, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
         x_matrix = np.array(x_init, dtype=np.float64, order="F", ndmin=2)
         p_matrix = np.array(p_init, dtype=np.float64, order="F", ndmin=2)
     N = x_matrix.shape[1]
+    eta0 = np.zeros(N)
+    if milne and x_matrix.shape[0] == 3:
+        eta0 = x_matrix[2].copy()
+        x_matrix = np.asfortranarray(x_matrix[:2])
     if p_matrix.shape[1] != N or N_particles != N:
         raise ValueError(f"x_init/p_init carry {N} particles, N_particles is {N_particles}")
     radial_mode = dimensions == 1
@@ -255,9 +270,12 @@ def _simulate_cartesian_ctx(ctx, r_grid_Langevin, p_grid_Langevin, heavy_quark_d
                   DsT_quad=DsT_quad, DsT_Tref=DsT_Tref)
     _load_tables(ctx, masses, Tf, DsT=DsT, momentum_langevin=momentum_langevin, cm=cm, pdim=pdim,
                  dst_kw=dst_kw, rta_time=rta_time, needs_drag_table=dmodel == _ffi.LMH_DRAG_TABLE)
+    if milne:
+        positions = np.asfortranarray(np.vstack([positions, eta0]))
     ctx.set_particles(positions, momenta, masses, counts)
     params = _ffi.make_params(dt=Δt, t0=initial_time, DsT=DsT, psi2=psi2, gamma0=g0, Lambda=lam,
-                              seed=seed, coords=_ffi.LMH_CARTESIAN, collision=coll,
+                              seed=seed, coords=_ffi.LMH_MILNE if milne else _ffi.LMH_CARTESIAN,
+                              collision=coll,
                               integrator=int(integrator_mode), drag_model=dmodel,
                               relativistic=relativistic, bjorken_redshift=bjorken_redshift,
                               position_diffusion=position_diffusion,

@@ -1195,10 +1195,53 @@ static int check_cartesian_dims(int32_t dims, int32_t pdim) {
     return dims == 2 && pdim == 3;
 }
 
+/* ---- Milne coordinates (extension): position rows (x, y, eta_s), momentum rows (p_x, p_y, p_z)
+ * with p_z the longitudinal momentum in the frame comoving with the Bjorken flow at eta_s.  The
+ * step is the reference's Bjorken-mode Cartesian step on (x, y, p) (dimensions 2, momentum rows
+ * 3, bjorken_redshift: simulate_cpu.jl:215-290, kernels_cpu.jl:427-435), the medium sampled at
+ * the transverse radius, followed by the push of eta_s. */
+
+/* copies of rows 0 and 1 of a (3, N) Milne position matrix into (2, N) and back */
+static void milne_split(const double* pos, double* xy, int64_t N) {
+    for (int64_t i = 0; i < N; ++i) { xy[2 * i] = pos[3 * i]; xy[2 * i + 1] = pos[3 * i + 1]; }
+}
+static void milne_join(const double* xy, double* pos, int64_t N) {
+    for (int64_t i = 0; i < N; ++i) { pos[3 * i] = xy[2 * i]; pos[3 * i + 1] = xy[2 * i + 1]; }
+}
+
+/* The space-time rapidity over the step tau_a -> tau_b = tau_a + dt (tau_a = t0 + (step - 1) dt,
+ * the interval of the Bjorken redshift) of a quark whose comoving longitudinal momentum at tau_b
+ * is p_z: along a free path p_z tau is conserved and y - eta_s = asinh(p_z/m_T) (the note at
+ * kernels_cpu.jl:419-426: dη/dτ = tanh(y−η)/τ), so eta_s advances by asinh(k b) - asinh(b) with
+ * b = p_z/m_T and k = tau_b/tau_a, written without cancellation as
+ *     asinh(b (k - 1)(k + 1) / (k sqrt(1 + b^2) + sqrt(1 + (k b)^2))),   k - 1 = dt/tau_a.    */
+double lmo_milne_deta(double px, double py, double pz, double m, double tau_a, double dt) {
+    if (!(tau_a > 0.0)) return 0.0;
+    double mT = sqrt(m * m + px * px + py * py);
+    double b = pz / mT;
+    double km1 = dt / tau_a;
+    double k = 1.0 + km1;
+    double kb = k * b;
+    double num = b * (km1 * (k + 1.0));
+    double den = k * sqrt(1.0 + b * b) + sqrt(1.0 + kb * kb);
+    return asinh(num / den);
+}
+
 /* The driver's initial boost at step 0, simulate_cpu.jl:160-163 */
 int lmo_boost_to_lab_all(const lmh_params* p, const lmo_radial_medium* med,
                          const lmo_species* sp, int32_t dims, int32_t pdim,
                          double* pos, double* mom, int64_t N, int64_t step) {
+    if (p->coords == LMH_MILNE) {
+        if (dims != 3 || pdim != 3) return LMO_E_INVALID;
+        double* xy = (double*)malloc((size_t)(2 * N) * sizeof(double));
+        if (!xy) return LMO_E_INVALID;
+        milne_split(pos, xy, N);
+        lmh_params q = *p;
+        q.coords = LMH_CARTESIAN;
+        int rc = lmo_boost_to_lab_all(&q, med, sp, 2, 3, xy, mom, N, step);
+        free(xy);
+        return rc;
+    }
     if (!check_cartesian_dims(dims, pdim)) return LMO_E_INVALID;
     if (species_begin(sp, sp->nspecies) != N) return LMO_E_INVALID;
     for (int32_t k = 0; k < sp->nspecies; ++k) {
@@ -1212,10 +1255,19 @@ int lmo_boost_to_lab_all(const lmh_params* p, const lmo_radial_medium* med,
 
 /* Steps step_begin .. step_begin+nsteps-1 of simulate_cpu.jl:215-303; noise is the fixed-noise
  * record (nsteps, C, N).  The NaN/Inf scan of :293-303 stops the run at the first bad step. */
+static int run_milne(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+                     double* pos, double* mom, int64_t N, int64_t step_begin, int64_t nsteps,
+                     const double* noise, int64_t* bad_step);
+
 int lmo_run_cartesian(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
                       int32_t dims, int32_t pdim, double* pos, double* mom, int64_t N,
                       int64_t step_begin, int64_t nsteps, const double* noise,
                       int64_t* bad_step) {
+    if (p->coords == LMH_MILNE) {
+        if (dims != 3 || pdim != 3 || p->position_diffusion || p->reflecting_boundary || !(p->t0 > 0.0))
+            return LMO_E_INVALID;
+        return run_milne(p, med, sp, pos, mom, N, step_begin, nsteps, noise, bad_step);
+    }
     if (!check_cartesian_dims(dims, pdim)) return LMO_E_INVALID;
     if (bad_step) *bad_step = -1;
     int32_t C = lmo_noise_channels(p, dims, pdim);
@@ -1237,6 +1289,46 @@ int lmo_run_cartesian(const lmh_params* p, const lmo_radial_medium* med, const l
         }
     }
     work_free(&w);
+    return rc;
+}
+
+/* Milne steps: the Bjorken-mode Cartesian step on (x, y, p) (the noise record of dims 2), then
+ * eta_s += lmo_milne_deta(end-of-step momentum) for every quark */
+static int run_milne(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
+                     double* pos, double* mom, int64_t N, int64_t step_begin, int64_t nsteps,
+                     const double* noise, int64_t* bad_step) {
+    if (bad_step) *bad_step = -1;
+    lmh_params q = *p;
+    q.coords = LMH_CARTESIAN;
+    q.bjorken_redshift = 1;
+    const int32_t C = lmo_noise_channels(&q, 2, 3);
+    double* xy = (double*)malloc((size_t)(2 * N) * sizeof(double));
+    work_t w;
+    if (!xy || work_alloc(&w, N, 3, 2) != LMO_OK) { free(xy); work_free(&w); return LMO_E_INVALID; }
+    milne_split(pos, xy, N);
+    int rc = LMO_OK;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        int64_t step = step_begin + s;
+        const double* rec = noise ? noise + s * (int64_t)C * N : NULL;
+        for (int32_t k = 0; k < sp->nspecies; ++k) {
+            int64_t b = species_begin(sp, k);
+            step_cartesian_segment(&q, med, sp, k, 2, 3, xy + 2 * b, mom + 3 * b, sp->counts[k], step,
+                                   rec, N, b, &w);
+            const double tau_a = p->t0 + (double)(step - 1) * p->dt;
+            const double m = sp->mass[k];
+#pragma omp parallel for schedule(static)
+            for (int64_t i = b; i < b + sp->counts[k]; ++i)
+                pos[2 + 3 * i] += lmo_milne_deta(mom[3 * i], mom[3 * i + 1], mom[3 * i + 2], m, tau_a, p->dt);
+        }
+        milne_join(xy, pos, N);
+        if (!all_finite(mom, 3 * N) || !all_finite(pos, 3 * N)) {
+            if (bad_step) *bad_step = step;
+            rc = LMO_E_NONFINITE;
+            break;
+        }
+    }
+    work_free(&w);
+    free(xy);
     return rc;
 }
 

@@ -127,6 +127,11 @@ int lmo_run_cartesian(const lmh_params* p, const lmo_radial_medium* med, const l
                       int64_t step_begin, int64_t nsteps, const double* noise,
                       int64_t* bad_step);
 
+/* Milne coordinates (extension, coords LMH_MILNE): lmo_run_cartesian and lmo_boost_to_lab_all
+ * with dims == pdim == 3, position rows (x, y, eta_s); the step is the Bjorken-mode Cartesian
+ * step on (x, y, p) followed by eta_s += lmo_milne_deta(p_x, p_y, p_z, m, tau_a, dt) */
+double lmo_milne_deta(double px, double py, double pz, double m, double tau_a, double dt);
+
 /* ---- general-coordinate (tau, r) step: kernels_cpu_GC.jl + simulate_cpu_general_coords.jl ---- */
 int lmo_run_gc(const lmh_params* p, const lmo_radial_medium* med, const lmo_species* sp,
                double* pos, double* mom, int64_t N, int64_t step_begin, int64_t nsteps,

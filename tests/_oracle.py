@@ -87,6 +87,7 @@ def lib():
                                              PD, PD, c_int64, c_int64]),
             "lmo_run_cartesian": (c_int, [P, POINTER(RadialMedium), POINTER(Species), c_int32, c_int32, PD, PD,
                                           c_int64, c_int64, c_int64, PD, POINTER(c_int64)]),
+            "lmo_milne_deta": (c_double, [c_double, c_double, c_double, c_double, c_double, c_double]),
             "lmo_run_gc": (c_int, [P, POINTER(RadialMedium), POINTER(Species), PD, PD, c_int64, c_int64,
                                    c_int64, PD, POINTER(c_int64)]),
             "lmo_grid_sample": (None, [POINTER(GridMedium), c_double, c_double, c_double, PD]),

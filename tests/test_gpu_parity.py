@@ -181,9 +181,10 @@ def test_grid_parity(gpu_ok, name, noise, spl):
                          rta_time=media.tau_R_law(0.16))
     # position diffusion: D = (D_s T / T) / hbar c, and this medium's cold corona (T falling to
     # 1e-10 GeV) makes a quark's jump there hundreds of fm and T at its landing point steep in
-    # the position, so the dynamics amplifies ulp-level differences (the push's refined
-    # reciprocal square root) step after step: the bar for that case is 1e-10
-    check(g, o, tol=1e-10 if name == "diffusion" else TOL)
+    # the position, so the dynamics amplifies ulp-level differences step after step; the kernels
+    # therefore push with the reference form x + dt p/E and draw the diffusion normals with the
+    # host's correctly rounded operations when position diffusion is on, and the bar is 1e-12
+    check(g, o)
 
 
 @pytest.mark.parametrize("noise", ["fixed", "philox"])

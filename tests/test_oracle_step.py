@@ -411,3 +411,100 @@ def test_radial_mode_euler_equilibrium_is_1d_juttner():
     want = juttner_p2(m, T, 1)
     assert np.mean(M * M) == pytest.approx(want, rel=0.03)
     assert abs(want - m * T) > 0.06 * m * T
+
+
+# ---------------------------------------------------------------- Milne coordinates --------------
+def milne_run(params_kw, pos, mom, steps, noise=None, T=0.3, V=0.0, m=1.5, DsT=0.5, t0=0.6, dt=0.005):
+    keep = O.Keep()
+    med = bath(keep, T, V)
+    N = pos.shape[1]
+    sp = O.species(keep, [m], [N], drag=[drag_table(m, DsT, T)])
+    kw = dict(dt=dt, t0=t0, DsT=DsT, noise=_ffi.LMH_NOISE_FIXED)
+    kw.update(params_kw)
+    p = _ffi.make_params(**kw)
+    P, M = np.asfortranarray(pos, dtype=float).copy(order="F"), np.asfortranarray(mom, dtype=float).copy(order="F")
+    C = L.lmo_noise_channels(O.ctypes.byref(p), P.shape[0], M.shape[0])
+    rec = np.zeros((steps, C, N)) if noise is None else noise
+    rc, _ = O.run_cartesian(p, med, sp, P, M, 1, steps, np.ascontiguousarray(rec))
+    assert rc == 0, rc
+    return P, M
+
+
+def straight_line_eta(y, tau0, tau):
+    """eta_s at proper time tau of a free particle of rapidity y that left eta_s = 0 at tau0:
+    z = tanh(y) (t - tau0) on the line, t^2 - z^2 = tau^2, eta_s = atanh(z/t)"""
+    v = mp.tanh(y)
+    t = mp.findroot(lambda t: t * t - (v * (t - tau0)) ** 2 - tau * tau, tau * mp.cosh(y))
+    return mp.atanh(v * (t - tau0) / t)
+
+
+def test_milne_free_streaming_closed_form():
+    """free streaming (no drag: D_s T <= 0 gives eta_D = kappa = 0; Galilean boosts at rest are
+    exact): p_z tau is conserved (the reference's gate R, test_momentum_dims3.jl:173-188) and
+    eta_s follows the straight line z = tanh(y)(t - tau0) through t = tau cosh(eta_s),
+    z = tau sinh(eta_s) (50 digits), for quarks with and without transverse momentum"""
+    m, t0, dt, n = 1.5, 0.6, 0.005, 1942
+    ys = np.array([0.9, -0.4, 0.05, 2.5])
+    pts = np.array([0.0, 0.0, 1.3, 0.7])
+    N = ys.size
+    mT = np.sqrt(m * m + pts ** 2)
+    mom = np.vstack([pts, 0 * pts, mT * np.sinh(ys)])
+    pos = np.zeros((3, N))
+    P, M = milne_run(dict(coords=_ffi.LMH_MILNE, collision=_ffi.LMH_LANGEVIN, relativistic=False, DsT=-1.0),
+                     pos, mom, n, t0=t0, dt=dt)
+    tau = t0 + n * dt
+    for i in range(N):
+        want_pz = mp.mpf(mT[i]) * mp.sinh(mp.mpf(ys[i])) * t0 / mp.mpf(tau)
+        assert M[2, i] == pytest.approx(float(want_pz), rel=1e-9)              # gate R
+        want_eta = straight_line_eta(mp.mpf(ys[i]), mp.mpf(t0), mp.mpf(tau))
+        assert P[2, i] == pytest.approx(float(want_eta), rel=1e-12, abs=1e-14)
+    assert np.array_equal(M[:2], mom[:2])                   # the transverse momenta stream freely
+
+
+def test_milne_deta_is_the_difference_of_rapidity_gaps():
+    """one step of lmo_milne_deta against asinh(k b) - asinh(b) at 50 digits, including b near 0,
+    large |b| and both signs"""
+    for px, py, pz, m, tau_a, dt in [(0.3, -0.2, 1.7, 1.5, 0.6, 0.005), (0.0, 0.0, -40.0, 1.5, 2.0, 0.01),
+                                     (2.0, 1.0, 1e-9, 4.75, 9.0, 0.005), (0.0, 0.5, 3.0, 1.5, 0.01, 0.5)]:
+        got = L.lmo_milne_deta(px, py, pz, m, tau_a, dt)
+        b = mp.mpf(pz) / mp.sqrt(mp.mpf(m) ** 2 + mp.mpf(px) ** 2 + mp.mpf(py) ** 2)
+        k = (mp.mpf(tau_a) + mp.mpf(dt)) / mp.mpf(tau_a)
+        want = mp.asinh(k * b) - mp.asinh(b)
+        assert got == pytest.approx(float(want), rel=4e-16, abs=1e-300)
+    assert L.lmo_milne_deta(1.0, 0.0, 2.0, 1.5, 0.0, 0.01) == 0.0
+
+
+def test_milne_step_is_the_bjorken_mode_step_on_the_transverse_rows():
+    """the x, y and momentum rows of a Milne run equal the reference's Bjorken-mode Cartesian run
+    (dimensions 2, momentum rows 3, bjorken_redshift) bit for bit, on a flowing medium with RTA
+    and the Langevin kick: eta_s is a passenger"""
+    keep = O.Keep()
+    rng = np.random.default_rng(17)
+    N, n, m = 600, 40, 1.5
+    xg = np.sqrt(np.linspace(0.0, 400.0, 81))
+    tg = np.arange(0.0, 10.01, 0.5)
+    R, Tt = np.meshgrid(xg, tg, indexing="ij")
+    Tf = np.asfortranarray(0.4 * np.exp(-R * R / 50.0) * (1 - 0.03 * Tt) + 0.05)
+    Vf = np.asfortranarray(0.6 * np.tanh(R / 5.0) * (1 + 0.02 * Tt))
+    med = O.radial_medium(keep, xg, tg, Tf, Vf)
+    DsT = 0.6366
+    jt = jt_table(m, 3, 0.3)
+    sp = O.species(keep, [m], [N], drag=[drag_table(m, DsT, 0.3)], rta=[const_table(0.5, 0.3)], jt=[jt])
+    pos = np.asfortranarray(rng.normal(0, 3, (2, N)))
+    mom = np.asfortranarray(rng.normal(0, 1.5, (3, N)))
+    base = dict(dt=0.01, t0=0.6, DsT=DsT, noise=_ffi.LMH_NOISE_FIXED, collision=_ffi.LMH_LANGEVIN_RTA)
+    pc = _ffi.make_params(bjorken_redshift=True, **base)
+    pmil = _ffi.make_params(coords=_ffi.LMH_MILNE, **base)
+    C = L.lmo_noise_channels(O.ctypes.byref(pc), 2, 3)
+    rec = rng.standard_normal((n, C, N))
+    rec[:, 3:5] = rng.random((n, 2, N))                     # u_col, u_samp in [0, 1)
+    Pc, Mc = pos.copy(order="F"), mom.copy(order="F")
+    assert O.boost_to_lab_all(pc, med, sp, Pc, Mc, 0) == 0
+    assert O.run_cartesian(pc, med, sp, Pc, Mc, 1, n, rec)[0] == 0
+    Pm = np.asfortranarray(np.vstack([pos, 0.1 * rng.standard_normal(N)]))
+    eta0 = Pm[2].copy()
+    Mm = mom.copy(order="F")
+    assert O.boost_to_lab_all(pmil, med, sp, Pm, Mm, 0) == 0
+    assert O.run_cartesian(pmil, med, sp, Pm, Mm, 1, n, rec)[0] == 0
+    assert np.array_equal(Pm[:2], Pc) and np.array_equal(Mm, Mc)
+    assert np.all(np.sign(Pm[2] - eta0) == np.sign(Mm[2]) + (Mm[2] == 0)) or np.mean(Pm[2] != eta0) > 0.99
