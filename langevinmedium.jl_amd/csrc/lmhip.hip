@@ -1147,58 +1147,137 @@ __global__ __launch_bounds__(kBlock) void k_gc(StepArgs a, RadialMedium med) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// observables: per-block partial sums (each thread accumulates its own column in LDS, columns
-// are then summed in a fixed tree), then one block sums the partials in block order.  The
-// result does not depend on scheduling.
-constexpr int kObsBlock = 64;
+// observables: per-block partial sums, then one block per observable adds the partials in block
+// order.  A thread accumulates the five moments of its particles in registers, in a fixed order;
+// each wave reduces them with a butterfly of shuffles (every lane ends with the same bits) and the
+// block adds its waves in order.  The plain histogram counts in LDS with integer atomics (exact,
+// so the order does not matter); the smoothed histogram reduces each bin over the wave with the
+// same butterfly and lane 0 adds it to its wave's LDS row.  The result does not depend on
+// scheduling.
+constexpr int kObsBlock = 256;
+constexpr int kObsWaves = kObsBlock / 64;
 constexpr int kObsMaxBins = 120;
+constexpr int kObsMaxK = 5 + kObsMaxBins;
+constexpr int kObsBlocks = 1024;           // blocks of a partial-sum launch (grid-stride)
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+struct ObsMoments {
+    double n = 0.0, pt = 0.0, pt2 = 0.0, c2 = 0.0, ptc2 = 0.0;
+    __device__ __forceinline__ void add(double a, double b, double* pt_out) {
+        const double p2 = a * a + b * b;
+        const double p = sqrt(p2);
+        const double c = p2 > 0.0 ? (a * a - b * b) / p2 : 0.0;
+        n += 1.0;
+        pt += p;
+        pt2 += p2;
+        c2 += c;
+        ptc2 += p * c;
+        *pt_out = p;
+    }
+};
+
+// The sums of particles [i0, i1) that block blk of an nblk-block launch takes (particle
+// i0 + blk*kObsBlock + t + k*nblk*kObsBlock for thread t), written to partial[0, 5 + nbins):
+// N, sum p_T, sum p_T^2, sum cos 2phi, sum p_T cos 2phi, then the bins.  Every thread of the
+// block must call it.
+__device__ __forceinline__ void obs_block(const double* __restrict__ px, const double* __restrict__ py,
+                                          int64_t i0, int64_t i1, const lmh_obs_spec& s, int64_t blk,
+                                          int64_t nblk, double* __restrict__ partial) {
+    __shared__ double wmom[kObsWaves][5];
+    __shared__ double whist[kObsWaves][kObsMaxBins];
+    __shared__ unsigned int hist[kObsMaxBins];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int nb = s.nbins;
+    const bool smooth = nb > 0 && s.sigma > 0.0;
+    for (int k = t; k < nb; k += kObsBlock) {
+        hist[k] = 0u;
+#pragma unroll
+        for (int w = 0; w < kObsWaves; ++w) whist[w][k] = 0.0;
+    }
+    __syncthreads();
+    const double width = (s.pt_max - s.pt_min) / (double)(nb > 0 ? nb : 1);
+    const double inv_w = (double)nb / (s.pt_max - s.pt_min);
+    ObsMoments m;
+    const int64_t stride = nblk * kObsBlock;
+    if (smooth) {
+        // every bin b adds Phi((hi - p_T)/sigma) - Phi((lo - p_T)/sigma); all lanes of the block
+        // take the same rounds (past the end they add 0), so that the shuffles see every lane
+        const double inv_s = 1.0 / (s.sigma * sqrt(2.0));
+        for (int64_t base = i0 + blk * kObsBlock; base < i1; base += stride) {
+            const int64_t i = base + t;
+            const bool ok = i < i1;
+            double p = 0.0;
+            if (ok) m.add(px[i], py[i], &p);
+            for (int k = 0; k < nb; ++k) {
+                const double lo = s.pt_min + (double)k * width;
+                const double hi = lo + width;
+                double c = ok ? 0.5 * (erfc((p - hi) * inv_s) - erfc((p - lo) * inv_s)) : 0.0;
+                c = wave_sum(c);
+                if (lane == 0) whist[wave][k] += c;
+            }
+        }
+    } else {
+        int64_t i = i0 + blk * kObsBlock + t;
+        // four particles in flight per thread
+        for (; i + 3 * stride < i1; i += 4 * stride) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = px[i + u * stride];
+                b[u] = py[i + u * stride];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double p;
+                m.add(a[u], b[u], &p);
+                if (nb > 0) {
+                    const double fl = floor((p - s.pt_min) * inv_w);
+                    if (fl >= 0.0 && fl < (double)nb) atomicAdd(&hist[(int)fl], 1u);
+                }
+            }
+        }
+        for (; i < i1; i += stride) {
+            double p;
+            m.add(px[i], py[i], &p);
+            if (nb > 0) {
+                const double fl = floor((p - s.pt_min) * inv_w);
+                if (fl >= 0.0 && fl < (double)nb) atomicAdd(&hist[(int)fl], 1u);
+            }
+        }
+    }
+    const double v[5] = {wave_sum(m.n), wave_sum(m.pt), wave_sum(m.pt2), wave_sum(m.c2), wave_sum(m.ptc2)};
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) wmom[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (t < 5) {
+        double a = wmom[0][t];
+#pragma unroll
+        for (int w = 1; w < kObsWaves; ++w) a += wmom[w][t];
+        partial[t] = a;
+    }
+    for (int k = t; k < nb; k += kObsBlock) {
+        double a = whist[0][k];
+#pragma unroll
+        for (int w = 1; w < kObsWaves; ++w) a += whist[w][k];
+        partial[5 + k] = smooth ? a : (double)hist[k];
+    }
+}
 
 __global__ __launch_bounds__(kObsBlock) void k_obs_partial(const double* __restrict__ px,
                                                             const double* __restrict__ py,
                                                             int64_t i0, int64_t i1, lmh_obs_spec s,
                                                             double* __restrict__ partial) {
-    extern __shared__ double acc[];           // [5 + nbins][kObsBlock]
-    const int t = threadIdx.x;
-    const int K = 5 + s.nbins;
-    for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] = 0.0;
-    const double width = (s.pt_max - s.pt_min) / (double)(s.nbins > 0 ? s.nbins : 1);
-    const double inv_w = (double)s.nbins / (s.pt_max - s.pt_min);
-    const double inv_s = s.sigma > 0.0 ? 1.0 / (s.sigma * sqrt(2.0)) : 0.0;
-    for (int64_t i = i0 + (int64_t)blockIdx.x * kObsBlock + t; i < i1;
-         i += (int64_t)gridDim.x * kObsBlock) {
-        double a = px[i], b = py[i];
-        double pt2 = a * a + b * b;
-        double pt = sqrt(pt2);
-        double c2 = pt2 > 0.0 ? (a * a - b * b) / pt2 : 0.0;
-        acc[0 * kObsBlock + t] += 1.0;
-        acc[1 * kObsBlock + t] += pt;
-        acc[2 * kObsBlock + t] += pt2;
-        acc[3 * kObsBlock + t] += c2;
-        acc[4 * kObsBlock + t] += pt * c2;
-        if (s.nbins == 0) continue;
-        if (s.sigma > 0.0) {
-            for (int k = 0; k < s.nbins; ++k) {
-                double lo = s.pt_min + (double)k * width;
-                double hi = lo + width;
-                acc[(5 + k) * kObsBlock + t] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
-            }
-        } else {
-            double f = (pt - s.pt_min) * inv_w;
-            double fl = floor(f);
-            if (fl >= 0.0 && fl < (double)s.nbins) acc[(5 + (int)fl) * kObsBlock + t] += 1.0;
-        }
-    }
-    __syncthreads();
-    for (int stride = kObsBlock / 2; stride > 0; stride >>= 1) {
-        if (t < stride)
-            for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] += acc[k * kObsBlock + t + stride];
-        __syncthreads();
-    }
-    if (t == 0)
-        for (int k = 0; k < K; ++k) partial[(int64_t)blockIdx.x * K + k] = acc[k * kObsBlock];
+    obs_block(px, py, i0, i1, s, blockIdx.x, gridDim.x, partial + (int64_t)blockIdx.x * (5 + s.nbins));
 }
 
-// one block per observable: the block's threads sum strided partials, then a fixed tree
+// one block per observable: the block's threads add strided partials, then a fixed tree
 constexpr int kObsFinalBlock = 256;
 
 __global__ __launch_bounds__(kObsFinalBlock) void k_obs_final(const double* __restrict__ partial, int nblk,
@@ -1944,6 +2023,33 @@ int lmh_get_particles(lmh_ctx* c, double* pos, double* mom) {
 
 namespace {
 
+int obs_check(lmh_ctx* c, const lmh_obs_spec* s) {
+    if (!s) return c->fail(LMH_E_INVALID, "observables: null spec");
+    if (!c->d_state) return c->fail(LMH_E_STATE, "observables: no particles set");
+    if (c->pdim < 2) return c->fail(LMH_E_INVALID, "observables: need >= 2 momentum rows");
+    if (s->nbins < 0 || s->nbins > kObsMaxBins || (s->nbins > 0 && !(s->pt_max > s->pt_min)))
+        return c->fail(LMH_E_INVALID, "observables: 0 <= nbins <= 120 and pt_max > pt_min");
+    if (s->species >= c->nspecies) return c->fail(LMH_E_INVALID, "observables: species out of range");
+    return LMH_OK;
+}
+
+// the partial sums of the selected particles into `partial` (kObsBlocks x (5 + nbins)) and their
+// total into `out` (5 + nbins, device memory), on the context's stream, in the current slot order
+int obs_enqueue(lmh_ctx* c, const lmh_obs_spec* s, double* partial, double* out) {
+    int64_t i0 = 0, i1 = c->N;
+    if (s->species >= 0) {
+        i0 = species_begin(c, s->species);
+        i1 = i0 + c->counts[s->species];
+    }
+    const int K = 5 + s->nbins;
+    const double* px = c->d_state + (int64_t)c->dims * c->N;
+    const double* py = px + c->N;
+    hipLaunchKernelGGL(k_obs_partial, dim3(kObsBlocks), dim3(kObsBlock), 0, c->stream, px, py, i0, i1, *s, partial);
+    hipLaunchKernelGGL(k_obs_final, dim3(K), dim3(kObsFinalBlock), 0, c->stream, partial, kObsBlocks, K, out);
+    LMH_HIP(c, hipGetLastError());
+    return LMH_OK;
+}
+
 int check_run(lmh_ctx* c, const lmh_params* p) {
     if (!p) return c->fail(LMH_E_INVALID, "null params");
     if (!c->d_state) return c->fail(LMH_E_STATE, "no particles set");
@@ -2286,22 +2392,13 @@ int64_t lmh_nonfinite_step(lmh_ctx* c) {
 
 int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     if (!c) return LMH_E_INVALID;
-    if (!s || !out) return c->fail(LMH_E_INVALID, "observables: null spec or output");
-    if (!c->d_state) return c->fail(LMH_E_STATE, "observables: no particles set");
-    if (c->pdim < 2) return c->fail(LMH_E_INVALID, "observables: need >= 2 momentum rows");
-    if (s->nbins < 0 || s->nbins > kObsMaxBins || (s->nbins > 0 && !(s->pt_max > s->pt_min)))
-        return c->fail(LMH_E_INVALID, "observables: 0 <= nbins <= 120 and pt_max > pt_min");
-    if (s->species >= c->nspecies) return c->fail(LMH_E_INVALID, "observables: species out of range");
+    if (!out) return c->fail(LMH_E_INVALID, "observables: null output");
+    int rc = obs_check(c, s);
+    if (rc) return rc;
     LMH_HIP(c, hipSetDevice(c->device));
-    if (int rc = ensure_unsorted(c)) return rc;
-    int64_t i0 = 0, i1 = c->N;
-    if (s->species >= 0) {
-        i0 = species_begin(c, s->species);
-        i1 = i0 + c->counts[s->species];
-    }
+    if ((rc = ensure_unsorted(c))) return rc;
     const int K = 5 + s->nbins;
-    const int nblk = 1024;
-    size_t need = (size_t)K * (nblk + 1);
+    const size_t need = (size_t)K * (kObsBlocks + 1);
     if (need > c->obs_cap) {
         LMH_HIP(c, hipStreamSynchronize(c->stream));
         dfree(c->d_obs);
@@ -2312,13 +2409,9 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
         }
         c->obs_cap = need;
     }
-    const double* px = c->d_state + (int64_t)c->dims * c->N;
-    const double* py = px + c->N;
-    size_t shmem = (size_t)K * kObsBlock * sizeof(double);
-    hipLaunchKernelGGL(k_obs_partial, dim3(nblk), dim3(kObsBlock), shmem, c->stream, px, py, i0, i1, *s, c->d_obs);
-    hipLaunchKernelGGL(k_obs_final, dim3(K), dim3(kObsFinalBlock), 0, c->stream, c->d_obs, nblk, K, c->d_obs + (size_t)K * nblk);
-    LMH_HIP(c, hipGetLastError());
-    LMH_HIP(c, hipMemcpyAsync(out, c->d_obs + (size_t)K * nblk, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double* fin = c->d_obs + (size_t)K * kObsBlocks;
+    if ((rc = obs_enqueue(c, s, c->d_obs, fin))) return rc;
+    LMH_HIP(c, hipMemcpyAsync(out, fin, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
     return LMH_OK;
 }
@@ -3458,8 +3551,7 @@ int lmh_batch_observables(lmh_batch* b, const lmh_obs_spec* s, double* out) {
     int rc = batch_dev(b, &b->d_obs, &b->obs_cap, (size_t)K * R * (nblk + 1), "batch_observables");
     if (rc) return rc;
     const double* px = b->d_state + 2 * b->N;
-    const size_t shmem = (size_t)K * kObsBlock * sizeof(double);
-    hipLaunchKernelGGL(k_batch_obs_partial, dim3(nblk, R), dim3(kObsBlock), shmem, b->stream, px, px + b->N,
+    hipLaunchKernelGGL(k_batch_obs_partial, dim3(nblk, R), dim3(kObsBlock), 0, b->stream, px, px + b->N,
                        b->d_begin, *s, b->d_obs);
     double* fin = b->d_obs + (size_t)K * R * nblk;
     hipLaunchKernelGGL(k_batch_obs_final, dim3(R), dim3(kObsBlock), 0, b->stream, b->d_obs, nblk, K, fin);

@@ -155,47 +155,9 @@ __global__ __launch_bounds__(kObsBlock) void k_batch_obs_partial(const double* _
                                                                   const int64_t* __restrict__ begin,
                                                                   lmh_obs_spec s,
                                                                   double* __restrict__ partial) {
-    extern __shared__ double acc[];           // [5 + nbins][kObsBlock]
-    const int t = threadIdx.x;
-    const int K = 5 + s.nbins;
     const int r = blockIdx.y;
-    const int64_t i0 = begin[r], i1 = begin[r + 1];
-    for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] = 0.0;
-    const double width = (s.pt_max - s.pt_min) / (double)(s.nbins > 0 ? s.nbins : 1);
-    const double inv_w = (double)s.nbins / (s.pt_max - s.pt_min);
-    const double inv_s = s.sigma > 0.0 ? 1.0 / (s.sigma * sqrt(2.0)) : 0.0;
-    for (int64_t i = i0 + (int64_t)blockIdx.x * kObsBlock + t; i < i1; i += (int64_t)gridDim.x * kObsBlock) {
-        double a = px[i], b = py[i];
-        double pt2 = a * a + b * b;
-        double pt = sqrt(pt2);
-        double c2 = pt2 > 0.0 ? (a * a - b * b) / pt2 : 0.0;
-        acc[0 * kObsBlock + t] += 1.0;
-        acc[1 * kObsBlock + t] += pt;
-        acc[2 * kObsBlock + t] += pt2;
-        acc[3 * kObsBlock + t] += c2;
-        acc[4 * kObsBlock + t] += pt * c2;
-        if (s.nbins == 0) continue;
-        if (s.sigma > 0.0) {
-            for (int k = 0; k < s.nbins; ++k) {
-                double lo = s.pt_min + (double)k * width;
-                double hi = lo + width;
-                acc[(5 + k) * kObsBlock + t] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
-            }
-        } else {
-            double f = (pt - s.pt_min) * inv_w;
-            double fl = floor(f);
-            if (fl >= 0.0 && fl < (double)s.nbins) acc[(5 + (int)fl) * kObsBlock + t] += 1.0;
-        }
-    }
-    __syncthreads();
-    for (int stride = kObsBlock / 2; stride > 0; stride >>= 1) {
-        if (t < stride)
-            for (int k = 0; k < K; ++k) acc[k * kObsBlock + t] += acc[k * kObsBlock + t + stride];
-        __syncthreads();
-    }
-    if (t == 0)
-        for (int k = 0; k < K; ++k)
-            partial[((int64_t)r * gridDim.x + blockIdx.x) * K + k] = acc[k * kObsBlock];
+    obs_block(px, py, begin[r], begin[r + 1], s, blockIdx.x, gridDim.x,
+              partial + ((int64_t)r * gridDim.x + blockIdx.x) * (5 + s.nbins));
 }
 
 // one block per run: out[r*K + k] = sum over the run's partials, in block order
