@@ -327,6 +327,17 @@ int lmh_profile_stop(lmh_ctx* ctx, lmh_profile* out);     /* synchronises */
 /* out has 5 + spec->nbins doubles (see lmh_obs_spec); synchronises                          */
 int lmh_observables(lmh_ctx* ctx, const lmh_obs_spec* spec, double* out);
 
+/* The same observables into device memory with no allocation -- no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  d_out (5 + nbins doubles, device memory) receives
+ * the sums of the particles in their current slot order (the cell sort of lmh_run may have
+ * reordered them: the sums agree with lmh_observables to rounding); the kernels are enqueued on
+ * the context's stream and the call returns without synchronising.  The per-block partial sums
+ * live in one static device array per GPU (1024 x 125 doubles, 1 MB, held until the process
+ * ends): nbins is at most 120, and calls on one GPU that overlap in time (several contexts or
+ * streams) race on that array.  Calls no hipMalloc, hipFree or allocating library routine.   */
+int lmh_observables_noalloc(lmh_ctx* ctx, const lmh_obs_spec* spec, double* d_out);
+
 /* ---- snapshots (the saved states of simulate_cpu.jl:305-316, simulate_gpu.jl:385-410) ------ */
 /* Takes the current state in the order of the context's stream (after every step enqueued so
  * far) and copies it to pinned host memory on a second stream while later steps run; returns

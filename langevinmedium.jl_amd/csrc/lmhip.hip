@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -24,6 +25,13 @@
 #include "lmhip_device.h"
 
 using namespace lmh;
+
+// Scratch of lmh_observables_noalloc: the per-block partial sums of one observables launch
+// (1024 blocks x at most 5 + 120 values), a file-scope device array instead of an allocation.
+// Each GPU holds its own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (the array's name is a host-side stand-in).
+constexpr size_t kScratchDoubles = 1024 * (5 + 120);
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -1159,6 +1167,7 @@ constexpr int kObsWaves = kObsBlock / 64;
 constexpr int kObsMaxBins = 120;
 constexpr int kObsMaxK = 5 + kObsMaxBins;
 constexpr int kObsBlocks = 1024;           // blocks of a partial-sum launch (grid-stride)
+static_assert((size_t)kObsBlocks * kObsMaxK <= kScratchDoubles, "observables partials exceed the scratch array");
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -1316,6 +1325,7 @@ struct lmh_ctx {
     double mass[LMH_MAX_SPECIES] = {0, 0, 0, 0};
     double* d_state = nullptr;      // (dims + pdim) rows of N doubles
     unsigned long long* d_bad = nullptr;
+    double* d_scratch = nullptr;    // this device's copy of the static scratch array
 
     // medium
     int medium = -1;                // 0 radial, 1 grid
@@ -1704,6 +1714,25 @@ int ensure_unsorted(lmh_ctx* c) {
 
 }  // namespace
 
+// The address of the static scratch array on `device` (which must be current), looked up once
+// per device and kept for every later context on it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 extern "C" {
 
 int32_t lmh_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
@@ -1728,7 +1757,8 @@ int lmh_create(lmh_ctx** out, int32_t device) {
         return LMH_E_NOMEM;
     }
     unsigned long long none = ULLONG_MAX;
-    if (hipMemcpy(c->d_bad, &none, sizeof(none), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(c->d_bad, &none, sizeof(none), hipMemcpyHostToDevice) != hipSuccess ||
+        scratch_address(device, &c->d_scratch) != hipSuccess) {
         (void)hipFree(c->d_bad);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -2414,6 +2444,18 @@ int lmh_observables(lmh_ctx* c, const lmh_obs_spec* s, double* out) {
     LMH_HIP(c, hipMemcpyAsync(out, fin, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LMH_HIP(c, hipStreamSynchronize(c->stream));
     return LMH_OK;
+}
+
+int lmh_observables_noalloc(lmh_ctx* c, const lmh_obs_spec* s, double* d_out) {
+    if (!c) return LMH_E_INVALID;
+    if (!d_out) return c->fail(LMH_E_INVALID, "observables_noalloc: null output");
+    int rc = obs_check(c, s);
+    if (rc) return rc;
+    if ((size_t)(5 + s->nbins) * kObsBlocks > kScratchDoubles)
+        return c->fail(LMH_E_INVALID, "observables_noalloc: the partial sums exceed the scratch array");
+    if (!c->d_scratch) return c->fail(LMH_E_STATE, "observables_noalloc: no scratch address for this device");
+    LMH_HIP(c, hipSetDevice(c->device));
+    return obs_enqueue(c, s, c->d_scratch, d_out);
 }
 
 }  // extern "C"

@@ -127,6 +127,7 @@ SIGNATURES = {
     "lmh_run_device_noise": (c_int, [c_void_p, POINTER(lmh_params), c_int64, c_int64, c_void_p, c_int32]),
     "lmh_nonfinite_step": (c_int64, [c_void_p]),
     "lmh_observables": (c_int, [c_void_p, POINTER(lmh_obs_spec), _PD]),
+    "lmh_observables_noalloc": (c_int, [c_void_p, POINTER(lmh_obs_spec), c_void_p]),
     "lmh_profile_start": (c_int, [c_void_p]),
     "lmh_profile_stop": (c_int, [c_void_p, POINTER(lmh_profile)]),
     "lmh_snapshot": (c_int, [c_void_p, POINTER(c_int32)]),
