@@ -64,16 +64,18 @@ extern "C" {
 #define LMH_LANGEVIN_RTA       3
 
 /* noise source: FIXED reads a host- or device-supplied record per step; PHILOX generates the
- * record in registers.  Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at
+ * record in registers.  Block b (0 kick and collision, 2 position diffusion) of particle i at
  * step s is four 64-bit words from two Philox4x32-10 evaluations (half h gives words 2h, 2h+1,
- * each lo | hi << 32 of consecutive outputs; the redraw block evaluates half 0 only) of counter
- * (i, s, stream mod 2^32, (stream >> 32) mod 2^29 | (2b + h) << 29) under key
- * (seed mod 2^32, seed >> 32); i and s must stay below 2^32 and stream below 2^61.  Normals are
- * Box-Muller pairs of 53-bit uniforms, except block 0's third normal: r cos(pi x) with r from
- * word 2, x the top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3; the collision
- * uniform is the low 31 bits of word 3 over 2^31; the redraw's quantile is the 53-bit uniform of
- * word 0 of block 1 and its direction the unit vector of word 1 (z = 2u - 1 from the top 32 bits,
- * the azimuth from the low 32 bits) in place of the Gaussian g of a fixed record.             */
+ * each lo | hi << 32 of consecutive outputs) of counter (i, s, stream mod 2^32,
+ * (stream >> 32) mod 2^29 | (2b + h) << 29) under key (seed mod 2^32, seed >> 32); i and s must
+ * stay below 2^32 and stream below 2^61.  Normals are Box-Muller pairs of 53-bit uniforms (the
+ * top 53 bits of a word), except block 0's third normal: r cos(pi x) with r from word 2, x the
+ * top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3.  The collision uniform is the
+ * low 31 bits of word 3 over 2^31.  The redraw's quantile u_samp is the 33-bit uniform
+ * (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 of the low 11 bits of block 0's words 0-2, and its
+ * Gaussian g is the three kick normals of block 0 (drawn for the redraw when the step has no
+ * kick): a colliding particle's kick is replaced by the redraw, so its normals serve it, and the
+ * normalised Gaussian is the reference's isotropic direction (kernels_cpu.jl:497-503).       */
 #define LMH_NOISE_FIXED        0
 #define LMH_NOISE_PHILOX       1
 
@@ -191,7 +193,7 @@ typedef struct lmh_profile {
 typedef struct lmh_adjoint_spec {
     int32_t checkpoint_every;  /* segment length K: a checkpoint every K steps                  */
     int32_t keep_tape;         /* 1: keep every step's tape row and medium slice in device
-                                  memory (nsteps x (2 + 2 pdim [+1 with RTA]) x N doubles plus
+                                  memory (nsteps x (2 + pdim) x N doubles: the state, no noise; plus
                                   nsteps x 96 B per grid cell), so that the pullback only runs
                                   the reverse sweep; 0: checkpoints only, the pullback re-runs
                                   each segment                                                 */

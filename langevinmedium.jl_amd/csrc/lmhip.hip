@@ -160,15 +160,16 @@ struct GridMedium {
 };
 
 // ---------------------------------------------------------------------------------------------
-// Noise of one particle-step, in three parts generated where they are used: the kick normals
-// and the collision uniform (block 0), the redraw's quantile and direction (block 1, colliding
-// particles only) and the position-diffusion normals (block 2).  FIXED reads the record layout
-// of include/lmhip.h; PHILOX generates the same values.
+// Noise of one particle-step.  FIXED reads the record layout of include/lmhip.h.  PHILOX draws
+// block 0 once: the kick normals, the collision uniform and the redraw's quantile; the redraw's
+// direction is the kick normals' direction (a colliding particle's kick is replaced by the
+// redraw, so its normals serve it); block 2 holds the position-diffusion normals.
 // KICK, RTA: whether the step draws the kick normals and the collision uniform, when the kernel
-// knows it at compile time (-1: from the record layout at run time)
+// knows it at compile time (-1: from the record layout at run time).  usamp receives the
+// redraw's quantile in PHILOX mode (FIXED reads it with the direction in noise_redraw).
 template <int PDIM, int NOISE, int KICK = -1, int RTA = -1>
 __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t step, int s_off,
-                                           double* xi, double* ucol) {
+                                           double* xi, double* ucol, double* usamp) {
     const bool kick = KICK < 0 ? a.oxi >= 0 : KICK != 0;
     const bool rta = RTA < 0 ? a.orta >= 0 : RTA != 0;
     if (NOISE == LMH_NOISE_FIXED) {
@@ -182,31 +183,32 @@ __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t
         if (!kick && !rta) return;
         uint64_t w[4];
         noise_block(a.p.seed, a.p.stream, i, step, 0, w);
-        if (kick) {
-            double n0, n1;
-            box_muller(w[0], w[1], &n0, &n1);
-            xi[0] = n0;
-            if (PDIM >= 2) xi[1 % PDIM] = n1;
-            if (PDIM >= 3) xi[2 % PDIM] = half_box_muller(w[2], w[3]);
+        double n0, n1;
+        box_muller(w[0], w[1], &n0, &n1);
+        xi[0] = n0;
+        if (PDIM >= 2) xi[1 % PDIM] = n1;
+        if (PDIM >= 3) xi[2 % PDIM] = half_box_muller(w[2], w[3]);
+        if (rta) {
+            *ucol = u_low31(w[3]);
+            *usamp = u_low33(w[0], w[1], w[2]);
         }
-        if (rta) *ucol = u_low31(w[3]);
     }
 }
 
+// The redraw's quantile and direction: FIXED reads them from the record; PHILOX takes the
+// quantile noise_kick drew and the kick normals xi
 template <int PDIM, int NOISE>
-__device__ __forceinline__ void noise_redraw(const StepArgs& a, int64_t i, int64_t step, int s_off,
-                                             double* usamp, double* g) {
+__device__ __forceinline__ void noise_redraw(const StepArgs& a, int64_t i, int s_off, const double* xi,
+                                             double usamp_philox, double* usamp, double* g) {
     if (NOISE == LMH_NOISE_FIXED) {
         const double* rec = a.noise + (int64_t)s_off * a.C * a.N + i;
         *usamp = rec[(int64_t)(a.orta + 1) * a.N];
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) g[d] = rec[(int64_t)(a.orta + 2 + d) * a.N];
     } else {
-        // one Philox evaluation: the quantile from w0, the direction from w1
-        uint64_t w[4];
-        noise_block<1>(a.p.seed, a.p.stream, i, step, 1, w);
-        *usamp = u_co(w[0]);
-        unit_direction<PDIM>(w[1], g);
+        *usamp = usamp_philox;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) g[d] = xi[d];
     }
 }
 
@@ -344,8 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             for (int d = 0; d < PDIM; ++d) qin[d] = q[d];
         }
         double xi[PDIM];
-        double ucol = 1.0;
-        noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol);
+        double ucol = 1.0, usamp_p = 0.0;
+        noise_kick<PDIM, NOISE>(a, i, step, s_off, xi, &ucol, &usamp_p);
 
         double r2 = 0.0;
 #pragma unroll
@@ -401,8 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             }
             if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) && rta_collides(p, sp.rta, T, ucol)) {
                 double usamp, g[PDIM];
-                noise_redraw<PDIM, NOISE>(a, i, step, s_off, &usamp, g);
-                rta_redraw<PDIM, RADIAL, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, g);
+                noise_redraw<PDIM, NOISE>(a, i, s_off, xi, usamp_p, &usamp, g);
+                rta_redraw<PDIM, RADIAL>(sp.jt, T, q, usamp, g);
             }
             if (!skip) radial_boost<DIMS, PDIM, RADIAL>(q, x, r, v, m, +1, p.relativistic, vmax_());
         }
@@ -927,8 +929,8 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         // ... then the kick noise, which needs no memory, is computed while it is in flight (an
         // empty asm that reads the noise keeps the compiler from sinking it to its uses)
         double xi[PDIM];
-        double ucol = 1.0;
-        noise_kick<PDIM, NOISE, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, pid, step, s_off, xi, &ucol);
+        double ucol = 1.0, usamp_p = 0.0;
+        noise_kick<PDIM, NOISE, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, pid, step, s_off, xi, &ucol, &usamp_p);
         if (NOISE == LMH_NOISE_PHILOX) {
 #pragma unroll
             for (int d = 0; d < PDIM; ++d)
@@ -964,8 +966,8 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
                     }
                 } else if (col) {
                     double usamp, gg[PDIM];
-                    noise_redraw<PDIM, NOISE>(a, pid, step, s_off, &usamp, gg);
-                    rta_redraw<PDIM, false, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, gg);
+                    noise_redraw<PDIM, NOISE>(a, pid, s_off, xi, usamp_p, &usamp, gg);
+                    rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
                 }
             }
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
@@ -1048,9 +1050,13 @@ __global__ __launch_bounds__(kBlock) void k_redraw_grid(StepArgs a, GridMedium g
             const double m = sp.m;
             double q[PDIM];
             const FlowBoost fb = flow_boost(vx, vy);
-            double usamp, gg[PDIM];
-            noise_redraw<PDIM, NOISE>(a, pid, step, 0, &usamp, gg);
-            rta_redraw<PDIM, false, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, gg);
+            // PHILOX: the step's block-0 noise again (the redraw's quantile and the kick normals);
+            // FIXED: noise_redraw reads the record
+            constexpr int DRAW = NOISE == LMH_NOISE_PHILOX ? 1 : 0;
+            double xi[PDIM], ucol = 1.0, usamp_p = 0.0, usamp, gg[PDIM];
+            noise_kick<PDIM, NOISE, DRAW, DRAW>(a, pid, step, 0, xi, &ucol, &usamp_p);
+            noise_redraw<PDIM, NOISE>(a, pid, 0, xi, usamp_p, &usamp, gg);
+            rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
             grid_push<PDIM, NOISE, SLICE>(a, p, g, pid, step, 0, j, td, m, &x0, &x1, q);
             a.pos[0][i] = x0;
@@ -2823,7 +2829,7 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     if (s->keep_tape) {
         // the whole tape and every step's medium slice stay in device memory: the pullback only
         // runs the reverse sweep.  The forward steps are the tape kernel's, segment by segment.
-        const size_t TR = R + (size_t)c->pdim + (p->collision == LMH_LANGEVIN_RTA ? 1 : 0);
+        const size_t TR = R;                 // the tape kernels' rows: the state (no noise)
         const int64_t ncell = (int64_t)(c->gx - 1) * (c->gy - 1);
         if ((rc = ensure_dev(c, &c->d_tape, &c->tape_cap, (size_t)nsteps * TR * (size_t)c->N,
                              "adjoint_forward whole tape (keep_tape)")))
@@ -2895,7 +2901,7 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
     const lmh_params& p = c->adj.p;
     const int64_t N = c->N;
     const int R = c->dims + c->pdim;
-    const int TR = R + c->pdim + (p.collision == LMH_LANGEVIN_RTA ? 1 : 0);     // the tape kernels' rows
+    const int TR = R;                    // the tape kernels' rows: the state (no noise)
     const int64_t K = c->adj.K;
     const int64_t ncp = (c->adj.nsteps + K - 1) / K;
     const int nw = 5 + os->nbins;

@@ -89,13 +89,13 @@ __device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, 
 }
 
 // One forward step, exactly the fused k_cart_grid step (non-deferred, Philox, no position
-// diffusion, the medium from the step's slice g.slice), for the tape recomputation.  xi and ucol
-// are the step's kick noise.
+// diffusion, the medium from the step's slice g.slice), for the tape recomputation.  xi, ucol and
+// usamp are the step's block-0 noise (noise_kick).
 template <int PDIM, int COLL>
 __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_params& p,
                                                  const GridMedium& g, const SpeciesArgs& sp,
                                                  int64_t i, int64_t step, double* x0, double* x1,
-                                                 double* q, const double* xi, double ucol) {
+                                                 double* q, const double* xi, double ucol, double usamp_p) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m;
     // the slice holds the step's time; no time cell is needed (no position diffusion here)
@@ -112,7 +112,7 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_pa
     ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
     if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
         double usamp, gg[PDIM];
-        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
+        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
         rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
     }
     apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
@@ -132,7 +132,8 @@ __device__ __forceinline__ double drv_rsq(double x) {
     return fma(r, fma(-(0.5 * x * r), r, 0.5), r);
 }
 
-// One reverse step.  (x0, x1, qin) is the step's input state and (xi, ucol) its kick noise;
+// One reverse step.  (x0, x1, qin) is the step's input state and (xi, ucol, usamp_p) its
+// block-0 noise;
 // lam holds the adjoint of the step's output state (x0, x1, q) on entry and of its input state
 // on return; th accumulates dL/dtheta.  The pre-point exact-OU kick, relativistic boosts.  The
 // forward values are recomputed from the input state; the collision decision sees the forward
@@ -143,7 +144,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
                                                  const GridMedium& g, const SpeciesArgs& sp,
                                                  int64_t i, int64_t step, double x0, double x1,
                                                  const double* qin, const double* xi, double ucol,
-                                                 double dtau_dDsT, double* lam, double* th) {
+                                                 double usamp_p, double dtau_dDsT, double* lam, double* th) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m, m2 = sp.m * sp.m;
     const double dt = p.dt;
@@ -235,7 +236,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
     if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
         col = true;
         double usamp, gg[PDIM];
-        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, &usamp, gg);
+        noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
         double nrm2 = 0.0;
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) nrm2 += gg[d] * gg[d];
@@ -449,8 +450,8 @@ __device__ __forceinline__ lmh_params adjoint_params(const lmh_params& in) {
 }
 
 // A backward segment, first half: re-run the segment's nk steps from its checkpoint, writing
-// each step's input state and kick noise to the tape (rows x0, x1, q..., xi...[, ucol with
-// collisions]).  With state_out (the forward pass that keeps the whole tape) the kernel is the
+// each step's input state to the tape (rows x0, x1, q...; no noise: the reverse kernel draws the
+// step's noise again from its Philox counters).  With state_out (the forward pass that keeps the whole tape) the kernel is the
 // forward pass itself: it also takes the last step, writes the final state there (in place over
 // ckpt is fine: each thread reads its particle before writing it) and flags non-finite states.
 template <int PDIM, int COLL, int DRAG>
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
     const lmh_params p = adjoint_params<DRAG>(a.p);
     const int64_t N = a.N;
     constexpr int R = 2 + PDIM;
-    constexpr int TR = R + PDIM + (COLL == LMH_LANGEVIN_RTA ? 1 : 0);
+    constexpr int TR = R;
     double x0 = ckpt[i], x1 = ckpt[N + i];
     double q[PDIM];
 #pragma unroll
@@ -480,17 +481,14 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
         tk[N] = x1;
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) tk[(int64_t)(2 + d) * N] = q[d];
-        double xi[PDIM];
-        double ucol = 1.0;
-        noise_kick<PDIM, LMH_NOISE_PHILOX>(a, i, step, 0, xi, &ucol);
-#pragma unroll
-        for (int d = 0; d < PDIM; ++d) tk[(int64_t)(R + d) * N] = xi[d];
-        if (COLL == LMH_LANGEVIN_RTA) tk[(int64_t)(R + PDIM) * N] = ucol;
         // the last step's output is the next checkpoint: only its inputs go to the tape
         if (k + 1 < nk || state_out) {
+            double xi[PDIM];
+            double ucol = 1.0, usamp = 0.0;
+            noise_kick<PDIM, LMH_NOISE_PHILOX, 1, COLL == LMH_LANGEVIN_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp);
             GridMedium gk = g;
             gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
-            adj_forward_step<PDIM, COLL>(a, p, gk, sp, i, step, &x0, &x1, q, xi, ucol);
+            adj_forward_step<PDIM, COLL>(a, p, gk, sp, i, step, &x0, &x1, q, xi, ucol, usamp);
             if (state_out && bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
         }
     }
@@ -519,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
     const lmh_params p = adjoint_params<DRAG>(a.p);
     const int64_t N = a.N;
     constexpr int R = 2 + PDIM;
-    constexpr int TR = R + PDIM + (COLL == LMH_LANGEVIN_RTA ? 1 : 0);
+    constexpr int TR = R;
     double lm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) lm[r] = lam[(int64_t)r * N + i];
@@ -541,12 +539,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
 #pragma unroll
             for (int r = 0; r < TR; ++r) row[r] = tn[(int64_t)r * N];
         }
-        // the tape row holds the collision uniform only with collisions (index kept in range
-        // for the row without it)
-        const double ucol = COLL == LMH_LANGEVIN_RTA ? cur[COLL == LMH_LANGEVIN_RTA ? R + PDIM : 0] : 1.0;
+        // the step's noise, drawn again from its Philox counters (common random numbers)
+        double xi[PDIM];
+        double ucol = 1.0, usamp = 0.0;
+        noise_kick<PDIM, LMH_NOISE_PHILOX, 1, COLL == LMH_LANGEVIN_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp);
         GridMedium gk = g;
         gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
-        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, cur + R, ucol,
+        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, xi, ucol, usamp,
                                      dtau_dDsT, lm, th);
     }
 #pragma unroll

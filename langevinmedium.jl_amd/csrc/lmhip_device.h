@@ -174,6 +174,13 @@ __device__ __forceinline__ double u_oc(uint64_t w) { return (double)((w >> 11) +
 // kick normal takes the word's top 32 bits and bit 31)
 __device__ __forceinline__ double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
+// the redraw's quantile: a 33-bit uniform in [0,1) from the low 11 bits of three words, which
+// their 53-bit uniforms leave unused, (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 (exact)
+__device__ __forceinline__ double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
+    const uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
+    return fma((double)((uint32_t)w2 & 0x7FFu), 0x1.0p-11, (double)a * 0x1.0p-33);
+}
+
 // 1/sqrt(x) for x > 0 (normal): the hardware estimate refined by two Newton steps, within an ulp
 // of the correctly rounded 1/sqrt(x); half the instructions of sqrt followed by a division
 __device__ __forceinline__ double rsqrt_pos(double x) {
@@ -267,30 +274,6 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
     double c1 = (q & 1) ? sr : cr;
     *sn = (q & 2) ? -s1 : s1;
     *cs = ((q + 1) & 2) ? -c1 : c1;
-}
-
-// An isotropic unit vector from one 64-bit word: PDIM 3 from z = 2u - 1 (u from the top 32 bits
-// at the midpoints of 2^-32 cells, so |z| < 1 and z is exact) and the azimuth 2 pi u' (u' from
-// the low 32 bits); PDIM 2 from the angle 2 pi u (u the 53-bit uniform); PDIM 1 the sign bit
-template <int PDIM>
-__device__ __forceinline__ void unit_direction(uint64_t w, double* g) {
-    if (PDIM == 1) {
-        g[0] = (w >> 63) ? -1.0 : 1.0;
-        return;
-    }
-    double s, c;
-    if (PDIM == 2) {
-        noise_sincospi(2.0 * u_co(w), &s, &c);
-        g[0] = c;
-        g[1 % PDIM] = s;
-        return;
-    }
-    const double z = ((double)(uint32_t)(w >> 32) + 0.5) * 0x1.0p-31 - 1.0;
-    const double rho = sqrt_pos((1.0 - z) * (1.0 + z));
-    noise_sincospi((double)(uint32_t)w * 0x1.0p-31, &s, &c);
-    g[0] = rho * c;
-    g[1 % PDIM] = rho * s;
-    g[2 % PDIM] = z;
 }
 
 // noise_cospi_half(x) = cos(pi x) for x in [0, 1/2]: the Taylor polynomial in x^2 through x^20
@@ -489,17 +472,22 @@ __device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTabl
 }
 
 // The redraw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701): |p*| from the table, the
-// direction g normalised.  UNIT: g is a unit vector already (the Philox record's direction, unit
-// to an ulp or two), so p* g without the norm's square root and division.
-template <int PDIM, bool RADIAL, bool UNIT = false>
+// direction g normalised (the normalised Gaussians of kernels_cpu.jl:497-503).  FAST takes
+// p* / |g| from the refined reciprocal square root (within an ulp or two of the division by the
+// square root).
+template <int PDIM, bool RADIAL, bool FAST = false>
 __device__ __forceinline__ void rta_redraw(const JuttnerTable& jt, double T, double* q,
                                            double usamp, const double* g) {
     double pstar = juttner_lookup(jt, usamp, T);
     if (RADIAL) {
         q[0] = g[0] < 0.0 ? -pstar : pstar;
-    } else if (UNIT) {
+    } else if (FAST) {
+        double nrm2 = g[0] * g[0];
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) q[d] = g[d] * pstar;
+        for (int d = 1; d < PDIM; ++d) nrm2 = fma(g[d], g[d], nrm2);
+        const double inv = pstar * rsqrt_pos(jl_max(nrm2, 1e-300));
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) q[d] = g[d] * inv;
     } else {
         double nrm2 = 0.0;
 #pragma unroll

@@ -361,16 +361,15 @@ void lmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Block b (0 kick, 1 redraw, 2 position diffusion) of particle i at step s: four 64-bit words
- * from two Philox4x32-10 evaluations (half h = 0 gives w0, w1; h = 1 gives w2, w3; each word is
- * lo | hi << 32 of consecutive outputs; block 1 evaluates half 0 only, w2 = w3 = 0) of counter
+/* Block b (0 kick and collision, 2 position diffusion, 3 initial-state samplers) of particle i
+ * at step s: four 64-bit words from two Philox4x32-10 evaluations (half h = 0 gives w0, w1;
+ * h = 1 gives w2, w3; each word is lo | hi << 32 of consecutive outputs) of counter
  *     (i mod 2^32, s mod 2^32, stream mod 2^32, (stream >> 32) mod 2^29 | (2 b + h) << 29)
  * under key (seed mod 2^32, seed >> 32).  The host keeps i, s < 2^32 and stream < 2^61.       */
 void lmo_noise_block(uint64_t seed, uint64_t stream, int64_t i, int64_t step, int32_t block,
                      uint64_t w[4]) {
-    w[2] = w[3] = 0;
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    for (int h = 0; h < (block == 1 ? 1 : 2); ++h) {
+    for (int h = 0; h < 2; ++h) {
         const uint32_t ctr[4] = {(uint32_t)(uint64_t)i, (uint32_t)(uint64_t)step, (uint32_t)stream,
                                  ((uint32_t)(stream >> 32) & 0x1FFFFFFFu) | ((uint32_t)(2 * block + h) << 29)};
         uint32_t x[4];
@@ -446,29 +445,6 @@ static inline void noise_sincospi(double x, double* sn, double* cs) {
     *cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
-/* An isotropic unit vector from one 64-bit word: pdim 3 from z = 2u - 1 (u from the top 32 bits
- * at the midpoints of 2^-32 cells, so |z| < 1 and z is exact) and the azimuth 2 pi u' (u' from
- * the low 32 bits); pdim 2 from the angle 2 pi u (u the 53-bit uniform); pdim 1 the sign bit  */
-static inline void unit_direction(uint64_t w, int32_t pdim, double* g) {
-    double s, c;
-    if (pdim == 1) {
-        g[0] = (w >> 63) ? -1.0 : 1.0;
-        return;
-    }
-    if (pdim == 2) {
-        noise_sincospi(2.0 * u_co(w), &s, &c);
-        g[0] = c;
-        g[1] = s;
-        return;
-    }
-    const double z = ((double)(uint32_t)(w >> 32) + 0.5) * 0x1.0p-31 - 1.0;
-    const double rho = sqrt((1.0 - z) * (1.0 + z));
-    noise_sincospi((double)(uint32_t)w * 0x1.0p-31, &s, &c);
-    g[0] = rho * c;
-    g[1] = rho * s;
-    g[2] = z;
-}
-
 /* Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb */
 static inline void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
     double r = sqrt(-2.0 * noise_log(u_oc(wa)));
@@ -528,12 +504,19 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
 /* the collision uniform: the low 31 bits of a word over 2^31 */
 static inline double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
+/* the redraw's quantile: the 33-bit uniform of the low 11 bits of three words (which their
+ * 53-bit uniforms leave unused), (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33, exact */
+static inline double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
+    uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
+    return (double)((uint32_t)w2 & 0x7FFu) * 0x1.0p-11 + (double)a * 0x1.0p-33;
+}
+
 /* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
  * lmo_noise_block(seed, stream, i, s, b).
- *   block 0: Box-Muller pairs (w0,w1), (w2,w3) -> xi[0..pdim) (GC: xi = n0);
- *            u_col = the low-bit uniform of the block
- *   block 1: u_samp = the 53-bit uniform of w0, g = the unit direction of w1 (a direction in
- *            place of the normals a fixed record holds: rta_redraw normalises either)
+ *   block 0: the Box-Muller pair of (w0, w1) and the half pair of (w2, w3): the normals
+ *            n0, n1, n2 -> xi[0..pdim) (GC: xi = n0); u_col = the low 31 bits of w3;
+ *            u_samp = the 33-bit uniform of the low 11 bits of w0, w1, w2; g = n0..n(pdim-1),
+ *            the kick normals (rta_collision normalises them)
  *   block 2: (xpos0, xpos1) = BoxMuller(w0, w1), xpos2 = first of BoxMuller(w2, w3)         */
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec) {
@@ -551,13 +534,11 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
                 int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
                 for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
             }
-            if (orta >= 0) rec[(int64_t)orta * N + i] = u_low31(w[3]);
-        }
-        if (orta >= 0) {
-            lmo_noise_block(p->seed, p->stream, i, step, 1, w);
-            rec[(int64_t)(orta + 1) * N + i] = u_co(w[0]);
-            unit_direction(w[1], pdim, n);
-            for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = n[d];
+            if (orta >= 0) {
+                rec[(int64_t)orta * N + i] = u_low31(w[3]);
+                rec[(int64_t)(orta + 1) * N + i] = u_low33(w[0], w[1], w[2]);
+                for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = n[d];
+            }
         }
         if (opos >= 0) {
             lmo_noise_block(p->seed, p->stream, i, step, 2, w);

@@ -62,21 +62,19 @@ def _philox_words(seed, stream, i, step, block):
 
 
 def test_noise_block_layout():
-    for seed, stream, i, step, block in ((0, 0, 0, 0, 0), (99, 3, 7, 17, 1), (2**64 - 1, 2**61 - 1, 2**32 - 1, 2**32 - 1, 2)):
+    for seed, stream, i, step, block in ((0, 0, 0, 0, 0), (99, 3, 7, 17, 3), (2**64 - 1, 2**61 - 1, 2**32 - 1, 2**32 - 1, 2)):
         w = (ctypes.c_uint64 * 4)()
         L.lmo_noise_block(seed, stream, i, step, block, w)
-        want = _philox_words(seed, stream, i, step, block)
-        # the redraw block evaluates its first half only
-        assert list(w) == (want[:2] + [0, 0] if block == 1 else want)
+        assert list(w) == _philox_words(seed, stream, i, step, block)
     # distinct blocks, halves, streams and steps give distinct words
     seen = set()
-    for block in range(3):
+    for block in (0, 2, 3):
         for stream in (0, 1, 2**32, 2**60):
             for step in (1, 2):
                 w = (ctypes.c_uint64 * 4)()
                 L.lmo_noise_block(5, stream, 11, step, block, w)
-                seen.update(list(w)[:2] if block == 1 else list(w))
-    assert len(seen) == (4 + 2 + 4) * 4 * 2
+                seen.update(list(w))
+    assert len(seen) == 3 * 4 * 4 * 2
 
 
 def test_noise_record_box_muller():
@@ -98,34 +96,29 @@ def test_noise_record_box_muller():
             assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
         # the collision uniform is the low 31 bits of w3
         assert rec[3, i] == (w[3] & 0x7FFFFFFF) * 2.0 ** -31
-        # the quantile is the 53-bit uniform of block 1's first word, the redraw direction the
-        # unit vector of its second: z = 2 u - 1 at the midpoint of the top 32 bits' cell, the
-        # azimuth 2 pi u' from the low 32 bits
-        w1 = _philox_words(99, 3, i, 17, 1)
-        assert rec[4, i] == float(u(w1[0]))
-        z = (mp.mpf(w1[1] >> 32) + mp.mpf(0.5)) * mp.mpf(2) ** -31 - 1
-        phi = 2 * mp.pi * mp.mpf(w1[1] & 0xFFFFFFFF) * mp.mpf(2) ** -32
-        rho = mp.sqrt(1 - z * z)
-        for k, ref in enumerate((rho * mp.cos(phi), rho * mp.sin(phi), z)):
-            assert abs(rec[5 + k, i] - float(ref)) <= 4e-15
+        # the redraw's quantile is the 33-bit uniform of the low 11 bits of w0, w1, w2 (the
+        # bits their 53-bit uniforms leave), its Gaussian the three kick normals themselves
+        lo = [v & 0x7FF for v in w[:3]]
+        assert rec[4, i] == (lo[0] + (lo[1] << 11) + (lo[2] << 22)) * 2.0 ** -33
+        assert np.array_equal(rec[5:8, i], rec[0:3, i])
         w2 = _philox_words(99, 3, i, 17, 2)
         x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(twopi * u(w2[1]))
         assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
 
 
 def test_noise_record_statistics():
-    """the record's normals are standard normal, its uniforms uniform and its redraw direction
-    isotropic (seeded, 4e5 draws per channel; 5 sigma on the moments, Kolmogorov-Smirnov
-    p > 1e-4), and distinct channels are uncorrelated: a wrong word, shift or counter in the
-    generator shows up here"""
+    """the record's normals are standard normal, its uniforms uniform and the redraw direction
+    (the normalised kick normals) isotropic (seeded, 4e5 draws per channel; 5 sigma on the
+    moments, Kolmogorov-Smirnov p > 1e-4), and distinct channels are uncorrelated: a wrong word,
+    shift or counter in the generator shows up here"""
     from scipy import stats
     p = _ffi.make_params(dt=0.01, seed=7, stream=2, noise=_ffi.LMH_NOISE_PHILOX,
                          collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
     rec = O.noise_record(p, 2, 3, 200000, 3, 2)
     normals, uniforms, direction = (0, 1, 2, 8, 9), (3, 4), (5, 6, 7)
-    # the redraw direction is a unit vector with independent, uniformly distributed z and azimuth
-    g = rec[:, direction, :]
-    assert np.max(np.abs(np.sum(g * g, axis=1) - 1.0)) < 1e-15
+    assert np.array_equal(rec[:, direction, :], rec[:, 0:3, :])
+    # the normalised Gaussian: z = cos(theta) uniform on [-1, 1], the azimuth uniform
+    g = rec[:, direction, :] / np.sqrt(np.sum(rec[:, direction, :] ** 2, axis=1, keepdims=True))
     n = g[:, 0, :].size
     for k in range(3):
         x = g[:, k, :].ravel()
@@ -134,6 +127,9 @@ def test_noise_record_statistics():
     assert stats.kstest(g[:, 2, :].ravel(), "uniform", args=(-1.0, 2.0)).pvalue > 1e-4
     phi = np.arctan2(g[:, 1, :], g[:, 0, :]).ravel()
     assert stats.kstest(phi, "uniform", args=(-np.pi, 2 * np.pi)).pvalue > 1e-4
+    # the quantile is independent of the normals drawn from the same words' high bits
+    for c in normals[:3]:
+        assert abs(np.corrcoef(rec[:, 4, :].ravel(), rec[:, c, :].ravel() ** 2)[0, 1]) < 5 / np.sqrt(n)
     for c in normals:
         x = rec[:, c, :].ravel()
         n = x.size
@@ -144,7 +140,7 @@ def test_noise_record_statistics():
         x = rec[:, c, :].ravel()
         assert 0.0 <= x.min() and x.max() < 1.0
         assert stats.kstest(x, "uniform").pvalue > 1e-4
-    flat = rec.transpose(1, 0, 2).reshape(rec.shape[1], -1)
+    flat = rec[:, [0, 1, 2, 3, 4, 8, 9], :].transpose(1, 0, 2).reshape(7, -1)
     cc = np.corrcoef(flat)
     off = cc[~np.eye(cc.shape[0], dtype=bool)]
     assert np.max(np.abs(off)) < 5 / np.sqrt(flat.shape[1])
@@ -178,9 +174,7 @@ def test_noise_record_moments():
     for c in (0, 1, 2):                                  # the kick normals
         assert abs(rec[c].mean()) < 5 / math.sqrt(n)
         assert abs(rec[c].var() - 1) < 6 * math.sqrt(2 / n)
-    for c in (5, 6, 7):                                  # the redraw direction's components
-        assert abs(rec[c].mean()) < 5 / math.sqrt(3 * n)
-        assert abs(rec[c].var() - 1 / 3) < 6 * math.sqrt(4 / 45 / n)
+    assert np.array_equal(rec[5:8], rec[0:3])            # the redraw's Gaussian: the kick normals
     for c in (3, 4):                                     # the collision and quantile uniforms
         assert abs(rec[c].mean() - 0.5) < 5 * math.sqrt(1 / 12 / n)
         assert abs(rec[c].var() - 1 / 12) < 0.003
