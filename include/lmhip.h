@@ -68,9 +68,11 @@ extern "C" {
  * step s is four 64-bit words from two Philox4x32-10 evaluations (half h gives words 2h, 2h+1,
  * each lo | hi << 32 of consecutive outputs) of counter (i, s, stream mod 2^32,
  * (stream >> 32) mod 2^29 | (2b + h) << 29) under key (seed mod 2^32, seed >> 32); i and s must
- * stay below 2^32 and stream below 2^61.  Normals are Box-Muller pairs of 53-bit uniforms (the
- * top 53 bits of a word), except block 0's third normal: r cos(pi x) with r from word 2, x the
- * top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3.  The collision uniform is the
+ * stay below 2^32 and stream below 2^61.  A uniform is d - 1 in [0, 1) or 2 - d in (0, 1] with
+ * d = 1.m, m the word's top 52 bits.  Normals are Box-Muller pairs: the radius from the (0, 1]
+ * uniform of the first word, the angle pi (q/2 + r) from the second (q its top 2 bits, r = d/2 -
+ * 3/4 in [-1/4, 1/4) with d = 1.m from its next 52 bits); block 0's third normal is r cos(pi x)
+ * with r from word 2, x the top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3.  The collision uniform is the
  * low 31 bits of word 3 over 2^31.  The redraw's quantile u_samp is the 33-bit uniform
  * (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 of the low 11 bits of block 0's words 0-2, and its
  * Gaussian g is the three kick normals of block 0 (drawn for the redraw when the step has no

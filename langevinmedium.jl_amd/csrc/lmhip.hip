@@ -564,16 +564,21 @@ __device__ __forceinline__ double4 lerp4(const double4& a, const double4& b, dou
 }
 
 // the slice path in two parts, so that a kernel can place independent work between the gather
-// and its use: the cell's record (and the fractions), then the bilinear combination
+// and its use: the cell's record (and the fractions), then the bilinear combination.  The query
+// clamp of interpolate_2d_cpu (kernels_cpu.jl:27-28) in cell units: the cell index clamped to
+// [0, n - 2] and the fraction to [0, 1] (a NaN coordinate takes cell 0, fraction 0)
+__device__ __forceinline__ void cell_frac(double x, double x0, double inv_d, int n, int* i, double* frac) {
+    const double f = (x - x0) * inv_d;
+    const int k = (int)floor(f);
+    *i = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+    *frac = fmin(fmax(f - (double)*i, 0.0), 1.0);
+}
+
 __device__ __forceinline__ const double4* slice_cell(const GridMedium& g, double x, double y, double* xd,
                                                      double* yd) {
-    const double xc = fmin(fmax(x, g.x0), g.xlast);
-    const double yc = fmin(fmax(y, g.y0), g.ylast);
-    const double fx = (xc - g.x0) * g.inv_dx;
-    const double fy = (yc - g.y0) * g.inv_dy;
-    const int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
-    *xd = fx - (double)ix;
-    *yd = fy - (double)iy;
+    int ix, iy;
+    cell_frac(x, g.x0, g.inv_dx, g.nx, &ix, xd);
+    cell_frac(y, g.y0, g.inv_dy, g.ny, &iy, yd);
     return g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
 }
 
@@ -598,12 +603,12 @@ __device__ __forceinline__ void grid_sample(const GridMedium& g, double x, doubl
     int64_t n00 = ix + (int64_t)g.nx * iy;
     double4 c00, c10, c01, c11;
     if (SLICE) {
-        // the cell's bilinear coefficients (a, b, c, d) of T, vx, vy: f = a + b xd + (c + d xd) yd
-        const double4* cell = g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
+        // the cell's bilinear coefficients (a, b, c, d) of T, vx, vy: f = a + b xd + (c + d xd) yd,
+        // the cell and fractions of slice_cell (the specialised kernels' form)
+        double sxd, syd;
+        const double4* cell = slice_cell(g, x, y, &sxd, &syd);
         const double4 cT = cell[0], cx = cell[1], cy = cell[2];
-        *T = fma(fma(cT.w, xd, cT.z), yd, fma(cT.y, xd, cT.x));
-        *vx = fma(fma(cx.w, xd, cx.z), yd, fma(cx.y, xd, cx.x));
-        *vy = fma(fma(cy.w, xd, cy.z), yd, fma(cy.y, xd, cy.x));
+        slice_combine(cT, cx, cy, sxd, syd, T, vx, vy);
         return;
     } else {
         const int64_t nxy = (int64_t)g.nx * g.ny;
@@ -664,7 +669,7 @@ __device__ __forceinline__ void apply_boost(double* q, double m, const FlowBoost
     double p2 = q[0] * q[0];
 #pragma unroll
     for (int d = 1; d < PDIM; ++d) p2 = fma(q[d], q[d], p2);
-    const double E = sqrt_pos(fma(m, m, p2));
+    const double E = sqrt_pos<true>(fma(m, m, p2));
     const double px = q[0], py = q[1];
     const double p_par = fma(py, b.ny, px * b.nx);
     const double p_perp_x = fma(-p_par, b.nx, px);

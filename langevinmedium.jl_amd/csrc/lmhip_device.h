@@ -167,15 +167,20 @@ __device__ __forceinline__ void noise_block(uint64_t seed, uint64_t stream, int6
     }
 }
 
-__device__ __forceinline__ double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
-__device__ __forceinline__ double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
+// uniforms from the top 52 bits of a word: d = 1.m in [1, 2) by setting the exponent, then
+// d - 1 in [0, 1) or 2 - d in (0, 1], both exact (three instructions, no integer conversion)
+__device__ __forceinline__ double unit_mantissa(uint64_t w) {
+    return __longlong_as_double((long long)((w >> 12) | 0x3FF0000000000000ULL));
+}
+__device__ __forceinline__ double u_co(uint64_t w) { return unit_mantissa(w) - 1.0; }
+__device__ __forceinline__ double u_oc(uint64_t w) { return 2.0 - unit_mantissa(w); }
 
 // the collision uniform: a 31-bit uniform in [0,1) from the low 31 bits of a word (the third
 // kick normal takes the word's top 32 bits and bit 31)
 __device__ __forceinline__ double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
 // the redraw's quantile: a 33-bit uniform in [0,1) from the low 11 bits of three words, which
-// their 53-bit uniforms leave unused, (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 (exact)
+// their 52-bit uniforms leave unused, (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 (exact)
 __device__ __forceinline__ double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
     const uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
     return fma((double)((uint32_t)w2 & 0x7FFu), 0x1.0p-11, (double)a * 0x1.0p-33);
@@ -203,13 +208,15 @@ __device__ __forceinline__ double rcp_pos(double x) {
 // sqrt(x) for x >= 0: g = x/sqrt(x) from the once-refined estimate, then one residual step
 // g + (x - g^2)/(2 sqrt(x)); 0 and NaN pass through.  No rescaling of tiny or huge arguments
 // (the callers' arguments are energies, rates and variances far from the exponent limits).
+// POS: x > 0 is known (an energy squared, at least m^2), so 0 needs no pass-through.
+template <bool POS = false>
 __device__ __forceinline__ double sqrt_pos(double x) {
     double y = __builtin_amdgcn_rsq(x);
     const double h = 0.5 * x;
     y = fma(y, fma(-(h * y), y, 0.5), y);
     const double g = x * y;
     const double r = fma(fma(-g, g, x), 0.5 * y, g);
-    return x > 0.0 ? r : x;
+    return (POS || x > 0.0) ? r : x;
 }
 
 // The two transcendental functions of the Box-Muller transform, the same definition as the host
@@ -218,7 +225,7 @@ __device__ __forceinline__ double sqrt_pos(double x) {
 // correctly rounded division and square root the host uses), so the two records agree within
 // an ulp or two.
 //
-// noise_log(u), u in [2^-53, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with
+// noise_log(u), u in [2^-52, 1]: u = 2^e m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with
 // s = (m - 1)/(m + 1), |s| <= 0.1716, by the atanh series through s^19 (truncation < 3e-17).
 constexpr double kLn2Hi = 6.93147180369123816490e-01;   // e * kLn2Hi is exact for |e| < 2^11
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
@@ -233,7 +240,18 @@ __device__ __forceinline__ double noise_log(double u) {
         m = 0.5 * m;
         e += 1;
     }
-    double s = EXACT ? (m - 1.0) / (m + 1.0) : (m - 1.0) * rcp_pos(m + 1.0);
+    // the quotient from the refined reciprocal y and one residual step q + (a - b q) y, which
+    // rounds it as the correctly rounded division does (Markstein), short of the scaling and
+    // special cases the division's operands here (|a| < 1/2, b in [1.7, 2.5]) never need
+    double s;
+    if (EXACT) {
+        s = (m - 1.0) / (m + 1.0);
+    } else {
+        const double a = m - 1.0, b = m + 1.0;
+        const double y = rcp_pos(b);
+        const double q = a * y;
+        s = fma(fma(-b, q, a), y, q);
+    }
     double z = s * s;
     double p = 1.0 / 19.0;
     p = fma(p, z, 1.0 / 17.0);
@@ -276,6 +294,32 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
     *cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
+// (sin, cos) of a uniform angle from one word: the quadrant q is the word's top 2 bits and the
+// angle pi (q/2 + r), r = d/2 - 3/4 in [-1/4, 1/4) with d = 1.m from the next 52 bits (exact);
+// the polynomials and the quadrant of noise_sincospi
+__device__ __forceinline__ void sincos_word(uint64_t w, double* sn, double* cs) {
+    constexpr double SP[8] = {3.141592653589793, -5.16771278004997, 2.5501640398773455,
+                              -0.5992645293207921, 0.08214588661112823, -0.0073704309457143504,
+                              0.00046630280576761255, -2.1915353447830217e-05};
+    constexpr double CP[9] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895,
+                              0.2353306303588932, -0.02580689139001406, 0.0019295743094039231,
+                              -0.0001046381049248457, 4.303069587032947e-06};
+    const double r = fma(0.5, unit_mantissa(w << 2), -0.75);
+    const double z = r * r;
+    double ps = SP[7];
+#pragma unroll
+    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[8];
+#pragma unroll
+    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    const double sr = r * ps, cr = pc;
+    const int q = (int)(uint32_t)(w >> 62);
+    const double s1 = (q & 1) ? cr : sr;
+    const double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
 // noise_cospi_half(x) = cos(pi x) for x in [0, 1/2]: the Taylor polynomial in x^2 through x^20
 // (truncation below 2e-17), Horner's rule with fused multiply-adds; the same definition on the host
 __device__ __forceinline__ double noise_cospi_half(double x) {
@@ -299,7 +343,7 @@ __device__ __forceinline__ double half_box_muller(uint64_t wa, uint64_t wb) {
     return ((wb >> 31) & 1u) ? -n : n;
 }
 
-// Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb.
+// Box-Muller: radius from the (0,1] uniform of wa, the uniform angle of wb (sincos_word).
 // EXACT takes the correctly rounded division and square root of the host restatement, so the
 // pair is the host's to the bit (the position-diffusion block, whose normals are multiplied by
 // jumps that D ~ 1/T makes large in a cold medium)
@@ -308,7 +352,7 @@ __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0,
     const double l = -2.0 * noise_log<EXACT>(u_oc(wa));
     double r = EXACT ? sqrt(l) : sqrt_pos(l);
     double s, c;
-    noise_sincospi(2.0 * u_co(wb), &s, &c);
+    sincos_word(wb, &s, &c);
     *n0 = r * c;
     *n1 = r * s;
 }

@@ -8,8 +8,8 @@
 // inverse-CDF tables are built on the host exactly as the reference builds them; the draws are
 // Philox4x32-10 words of noise block 3 under the run's key (seed, stream) and the draw's index,
 // with counter word 1 selecting the draw: position attempts 0, 1, 2, ..., the momentum
-// 0xFFFFFFFF, the thermal p_z attempts 0x80000000 + k.  Uniforms are the 53-bit uniforms of the
-// words.  The steps use blocks 0-2, so no sampler word repeats a step's.
+// 0xFFFFFFFF, the thermal p_z attempts 0x80000000 + k.  Uniforms are the 52-bit uniforms of the
+// words.  The steps use blocks 0 and 2, so no sampler word repeats a step's.
 
 constexpr int kSampleBlock = 3;
 constexpr uint32_t kSampleMomentum = 0xFFFFFFFFu;

@@ -379,15 +379,22 @@ void lmo_noise_block(uint64_t seed, uint64_t stream, int64_t i, int64_t step, in
     }
 }
 
-/* uniforms from a 64-bit word: [0,1) and (0,1], 53 bits each */
-static inline double u_co(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
-static inline double u_oc(uint64_t w) { return (double)((w >> 11) + 1) * 0x1.0p-53; }
+/* uniforms from the top 52 bits of a word: d = 1.m in [1, 2) (the bits as the mantissa of
+ * exponent 0), then d - 1 in [0, 1) or 2 - d in (0, 1], both exact */
+static inline double unit_mantissa(uint64_t w) {
+    uint64_t b = (w >> 12) | 0x3FF0000000000000ULL;
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+static inline double u_co(uint64_t w) { return unit_mantissa(w) - 1.0; }
+static inline double u_oc(uint64_t w) { return 2.0 - unit_mantissa(w); }
 
 /* The two transcendental functions of the Box-Muller transform, written with correctly
  * rounded operations only (+, -, *, /, sqrt, fma, floor and exact bit manipulation) so that the
  * same definition evaluates to the same bits on any IEEE-754 machine.
  *
- * noise_log(u), u in [2^-53, 1]: u = 2^e m with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+ * noise_log(u), u in [2^-52, 1]: u = 2^e m with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
  * s = (m - 1)/(m + 1), |s| <= 0.1716, by the Taylor series of atanh through s^19 (truncation
  * < 3e-17 relative); ln 2 split into a head whose products with e are exact and a tail.       */
 static const double LN2_HI = 6.93147180369123816490e-01;
@@ -445,11 +452,29 @@ static inline void noise_sincospi(double x, double* sn, double* cs) {
     *cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
-/* Box-Muller: radius from the (0,1] uniform of wa, angle 2 pi u from the [0,1) uniform of wb */
+/* (sin, cos) of a uniform angle from one word: the quadrant q = the top 2 bits, the angle
+ * pi (q/2 + r) with r = d/2 - 3/4 in [-1/4, 1/4), d = 1.m from the next 52 bits (exact); the
+ * polynomials and quadrant logic of noise_sincospi */
+static inline void sincos_word(uint64_t w, double* sn, double* cs) {
+    double r = fma(0.5, unit_mantissa(w << 2), -0.75);
+    double z = r * r;
+    double ps = SP[7];
+    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[8];
+    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    double sr = r * ps, cr = pc;
+    int q = (int)(uint32_t)(w >> 62);
+    double s1 = (q & 1) ? cr : sr;
+    double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+/* Box-Muller: radius from the (0,1] uniform of wa, the uniform angle of wb (sincos_word) */
 static inline void box_muller(uint64_t wa, uint64_t wb, double* n0, double* n1) {
     double r = sqrt(-2.0 * noise_log(u_oc(wa)));
     double s, c;
-    noise_sincospi(2.0 * u_co(wb), &s, &c);
+    sincos_word(wb, &s, &c);
     *n0 = r * c;
     *n1 = r * s;
 }
@@ -505,7 +530,7 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
 static inline double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
 /* the redraw's quantile: the 33-bit uniform of the low 11 bits of three words (which their
- * 53-bit uniforms leave unused), (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33, exact */
+ * 52-bit uniforms leave unused), (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33, exact */
 static inline double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
     uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
     return (double)((uint32_t)w2 & 0x7FFu) * 0x1.0p-11 + (double)a * 0x1.0p-33;

@@ -83,7 +83,9 @@ class OracleEnsemble:
                                 noise=_ffi.LMH_NOISE_PHILOX, drag_model=self.drag_model,
                                 check_finite=False)
 
-    def obs(self, theta, pos=None, mom=None):
+    def obs(self, theta, pos=None, mom=None, only=None):
+        """the observables of the final state; only = i: of particle i alone (the particles are
+        independent, so a change of particle i's initial state changes these and nothing else)"""
         p = self.params(theta)
         Tmin, Tmax = _tminmax(self.g)
         drag = [transport.build_tau_drag_spline(m, theta[0], Tmin=Tmin, Tmax=Tmax, nT=1024)
@@ -95,7 +97,8 @@ class OracleEnsemble:
         q = (self.mom if mom is None else mom).copy(order="F")
         rc, bad = O.run_grid(p, gm, sp, x, q, 1, self.steps, self.noise)
         assert rc == 0, (rc, bad)
-        return O.observables(q, 0, x.shape[1], OBS["nbins"], OBS["pt_min"], OBS["pt_max"], OBS["sigma"])
+        i0, i1 = (0, x.shape[1]) if only is None else (only, only + 1)
+        return O.observables(q, i0, i1, OBS["nbins"], OBS["pt_min"], OBS["pt_max"], OBS["sigma"])
 
 
 def _gpu_ensemble(ctx, g, masses, counts, pos, mom, steps, drag_model, collision, seed, K, rta):
@@ -209,20 +212,22 @@ def test_adjoint_state_gradient_matches_oracle(gpu_ok):
                             _ffi.LMH_LANGEVIN, seed, K, False)
         ens.forward(theta)
         _, dpos, dmom = ens.pullback(w, state_grad=True)
+    # the reference: central differences of particle i's own observables (the other particles'
+    # sums, unchanged, would add their rounding to the difference) at h = 1e-5, where the
+    # difference quotients have converged: below 1e-6 the trajectory's rounding dominates them
+    # (~1e-10 on a 2.5e-7 derivative), above 3e-5 the +-h paths straddle the piecewise-linear
+    # medium's cell edges
+    def cd(fn, base, d, i, h=1e-5):
+        a, b = base.copy(order="F"), base.copy(order="F")
+        a[d, i] += h
+        b[d, i] -= h
+        return (fn(a) - fn(b)) @ w / (2 * h)
     for i in (0, 57, 150, 199):
         for d in range(3):
-            h = 1e-6
-            mp, mm = mom.copy(order="F"), mom.copy(order="F")
-            mp[d, i] += h
-            mm[d, i] -= h
-            fd = (orc.obs(theta, mom=mp) - orc.obs(theta, mom=mm)) @ w / (2 * h)
+            fd = cd(lambda m_: orc.obs(theta, mom=m_, only=i), mom, d, i)
             assert abs(dmom[d, i] - fd) <= 1e-4 * abs(fd) + 1e-10, (i, d, dmom[d, i], fd)
         for d in range(2):
-            h = 1e-6
-            pp, pm = pos.copy(order="F"), pos.copy(order="F")
-            pp[d, i] += h
-            pm[d, i] -= h
-            fd = (orc.obs(theta, pos=pp) - orc.obs(theta, pos=pm)) @ w / (2 * h)
+            fd = cd(lambda p_: orc.obs(theta, pos=p_, only=i), pos, d, i)
             assert abs(dpos[d, i] - fd) <= 1e-4 * abs(fd) + 1e-10, (i, d, dpos[d, i], fd)
 
 

@@ -82,27 +82,30 @@ def test_noise_record_box_muller():
                          collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
     rec = O.noise_record(p, 2, 3, 5, 17, 1)[0]
     assert rec.shape == (3 + 5 + 2, 5)
-    u = lambda v: mp.mpf(v >> 11) * mp.mpf(2) ** -53
-    u1 = lambda v: mp.mpf((v >> 11) + 1) * mp.mpf(2) ** -53
-    twopi = mp.mpf(6.283185307179586)
+    # uniforms: d = 1.m from the top 52 bits, u = d - 1 or 1 - u = d - 1 (so u1 = 2 - d)
+    mant = lambda v: 1 + mp.mpf((v >> 12) & (2**52 - 1)) * mp.mpf(2) ** -52
+    u = lambda v: mant(v) - 1
+    u1 = lambda v: 2 - mant(v)
+    # the pair's angle: pi (q/2 + d/2 - 3/4), q = top 2 bits, d = 1.m from the next 52 bits
+    ang = lambda v: mp.pi * (mp.mpf(v >> 62) / 2 + mant((v << 2) & (2**64 - 1)) / 2 - mp.mpf(0.75))
     for i in range(5):
         w = _philox_words(99, 3, i, 17, 0)
         r0 = mp.sqrt(-2 * mp.log(u1(w[0])))
         r2 = mp.sqrt(-2 * mp.log(u1(w[2])))
         # the third normal: r2 cos(pi x), x = top 32 bits of w3 / 2^33, sign from bit 31
         n2 = r2 * mp.cos(mp.pi * mp.mpf(w[3] >> 32) * mp.mpf(2) ** -33) * (-1 if (w[3] >> 31) & 1 else 1)
-        for val, ref in ((rec[0, i], r0 * mp.cos(twopi * u(w[1]))), (rec[1, i], r0 * mp.sin(twopi * u(w[1]))),
+        for val, ref in ((rec[0, i], r0 * mp.cos(ang(w[1]))), (rec[1, i], r0 * mp.sin(ang(w[1]))),
                          (rec[2, i], n2)):
             assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
         # the collision uniform is the low 31 bits of w3
         assert rec[3, i] == (w[3] & 0x7FFFFFFF) * 2.0 ** -31
         # the redraw's quantile is the 33-bit uniform of the low 11 bits of w0, w1, w2 (the
-        # bits their 53-bit uniforms leave), its Gaussian the three kick normals themselves
+        # bits their 52-bit uniforms leave), its Gaussian the three kick normals themselves
         lo = [v & 0x7FF for v in w[:3]]
         assert rec[4, i] == (lo[0] + (lo[1] << 11) + (lo[2] << 22)) * 2.0 ** -33
         assert np.array_equal(rec[5:8, i], rec[0:3, i])
         w2 = _philox_words(99, 3, i, 17, 2)
-        x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(twopi * u(w2[1]))
+        x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(ang(w2[1]))
         assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
 
 
