@@ -128,14 +128,17 @@ __device__ __forceinline__ lmh_params step_params(const lmh_params& in) {
     return p;
 }
 
-// this block's species segment and this thread's particle (-1 past the segment's end)
+// this block's species segment and this thread's particle (-1 past the segment's end); the
+// segment and its first particle are block-uniform (32-bit block numbers: scalar compares)
 __device__ __forceinline__ int64_t particle_of_thread(const StepArgs& a, int* seg) {
+    const uint32_t b = blockIdx.x;
     int k = 0;
 #pragma unroll
     for (int j = 1; j < LMH_MAX_SPECIES; ++j)
-        if (j < a.nseg && (int64_t)blockIdx.x >= a.block0[j]) k = j;
+        if (j < a.nseg && b >= (uint32_t)a.block0[j]) k = j;
     *seg = k;
-    const int64_t i = a.begin[k] + ((int64_t)blockIdx.x - a.block0[k]) * blockDim.x + threadIdx.x;
+    const int64_t first = a.begin[k] + (int64_t)(b - (uint32_t)a.block0[k]) * blockDim.x;
+    const int64_t i = first + threadIdx.x;
     return i < a.begin[k + 1] ? i : -1;
 }
 
@@ -579,7 +582,9 @@ __device__ __forceinline__ const double4* slice_cell(const GridMedium& g, double
     int ix, iy;
     cell_frac(x, g.x0, g.inv_dx, g.nx, &ix, xd);
     cell_frac(y, g.y0, g.inv_dy, g.ny, &iy, yd);
-    return g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
+    // a 32-bit byte offset into the slice (3 records of 32 bytes per cell)
+    const uint32_t off = (uint32_t)(ix + (g.nx - 1) * iy) * 96u;
+    return (const double4*)((const char*)g.slice + off);
 }
 
 __device__ __forceinline__ void slice_combine(const double4& cT, const double4& cx, const double4& cy,
@@ -886,12 +891,18 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
     const lmh_params pp = step_params<SPEC>(a.p);
     const lmh_params& p = pp;
     const double m = sp.m;
-    double x0 = a.pos[0][i], x1 = a.pos[1][i];
+    // the specialised kernels run for N < 2^29 only, so a state row is addressed by a 32-bit byte
+    // offset from its base (no 64-bit address arithmetic per row)
+    const uint32_t o8 = (uint32_t)i * 8u;
+    double x0 = SPEC ? ld_off(a.pos[0], o8) : a.pos[0][i];
+    double x1 = SPEC ? ld_off(a.pos[1], o8) : a.pos[1][i];
     double q[PDIM];
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    for (int d = 0; d < PDIM; ++d) q[d] = SPEC ? ld_off(a.mom[d], o8) : a.mom[d][i];
     // the particle's identity keys its noise (the slot unless the particles are cell-sorted)
-    const int64_t pid = (int64_t)a.id[i];
+    const int64_t pid = SPEC ? (int64_t)*(const __attribute__((address_space(1))) uint32_t*)(
+                                   (const char*)a.id + (uint32_t)i * 4u)
+                             : (int64_t)a.id[i];
     // every state load is issued before the medium sample's address arithmetic waits for x, y
     __builtin_amdgcn_sched_barrier(0);
     // freeze-out capture runs in the generic, non-deferred kernels only
@@ -987,10 +998,17 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         }
         if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
     }
-    a.pos[0][i] = x0;
-    a.pos[1][i] = x1;
+    if (SPEC) {
+        st_off(a.pos[0], o8, x0);
+        st_off(a.pos[1], o8, x1);
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+        for (int d = 0; d < PDIM; ++d) st_off(a.mom[d], o8, q[d]);
+    } else {
+        a.pos[0][i] = x0;
+        a.pos[1][i] = x1;
+#pragma unroll
+        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    }
     if (fo) a.fo.lastT[i] = lastT;
     if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
 }
@@ -2153,7 +2171,8 @@ int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nst
 
 // the parameters are those the SPEC kernels fix at compile time
 bool spec_config(const lmh_ctx* c, const lmh_params* p) {
-    return c->tune.specialize && !c->fo_on && p->coords == LMH_CARTESIAN && p->relativistic &&
+    return c->tune.specialize && !c->fo_on && c->N < ((int64_t)1 << 29) && p->coords == LMH_CARTESIAN &&
+           p->relativistic &&
            p->integrator == LMH_PREPOINT &&
            p->drag_model == LMH_DRAG_TABLE && !p->bjorken_redshift && !p->position_diffusion;
 }

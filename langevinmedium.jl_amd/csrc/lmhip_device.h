@@ -71,6 +71,18 @@ __device__ __forceinline__ double ld_global(const double* p) {
     return *(const __attribute__((address_space(1))) double*)p;
 }
 
+// A read-only load at a 32-bit byte offset from a base the whole launch shares: the global
+// access with the base in scalar registers and the offset in one vector register, with no 64-bit
+// address arithmetic per load (tables and slices are far below 4 GB)
+__device__ __forceinline__ double ld_off(const double* base, uint32_t byte_off) {
+    return *(const __attribute__((address_space(1))) double*)((const char*)base + byte_off);
+}
+
+// the store counterpart of ld_off
+__device__ __forceinline__ void st_off(double* base, uint32_t byte_off, double v) {
+    *(__attribute__((address_space(1))) double*)((char*)base + byte_off) = v;
+}
+
 // (component by component: a copy of the vector type through an address-space pointer would
 // go back through a generic pointer)
 __device__ __forceinline__ double4 ld_global4(const double4* p) {
@@ -95,8 +107,8 @@ __device__ __forceinline__ double eval_time_table(const TimeTable& t, double T) 
     const bool one = t.n < 2;
     const int i = one ? 0 : (int)fc + 1;
     double w = u - fc;
-    double y0 = ld_global(t.vals + (one ? 0 : i - 1));
-    double y1 = ld_global(t.vals + i);
+    double y0 = ld_off(t.vals, (uint32_t)(one ? 0 : i - 1) * 8u);
+    double y1 = ld_off(t.vals, (uint32_t)i * 8u);
     const double r = (1.0 - w) * y0 + w * y1;
     return one ? y0 : (isfinite(T) ? r : 0.0);
 }
@@ -122,10 +134,10 @@ __device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double 
     else jT = (int)fl;
     double tT = fT - (double)jT;
     tT = tT < 0.0 ? 0.0 : (tT > 1.0 ? 1.0 : tT);
-    const double* b0 = jt.vals + (int64_t)jT * jt.nU + iu;
-    const double* b1 = b0 + jt.nU;
-    double v00 = ld_global(b0), v10 = ld_global(b0 + 1);
-    double v01 = ld_global(b1), v11 = ld_global(b1 + 1);
+    const uint32_t o0 = (uint32_t)(jT * jt.nU + iu) * 8u;
+    const uint32_t o1 = o0 + (uint32_t)jt.nU * 8u;
+    double v00 = ld_off(jt.vals, o0), v10 = ld_off(jt.vals, o0 + 8u);
+    double v01 = ld_off(jt.vals, o1), v11 = ld_off(jt.vals, o1 + 8u);
     double c0 = v00 * (1.0 - tu) + v10 * tu;
     double c1 = v01 * (1.0 - tu) + v11 * tu;
     return c0 * (1.0 - tT) + c1 * tT;
@@ -248,7 +260,9 @@ __device__ __forceinline__ double noise_log(double u) {
         s = (m - 1.0) / (m + 1.0);
     } else {
         const double a = m - 1.0, b = m + 1.0;
-        const double y = rcp_pos(b);
+        // the estimate refined once (within ~20 ulp); the residual step corrects the quotient
+        double y = __builtin_amdgcn_rcp(b);
+        y = fma(y, fma(-b, y, 1.0), y);
         const double q = a * y;
         s = fma(fma(-b, q, a), y, q);
     }
