@@ -389,9 +389,8 @@ def bench_adjoint(args):
         _, t_fwd, _ = timed(lambda: ctx.run(p, 1, steps, None, 0))
         obs_plain = ctx.observables(**obs_kw)
         p = ens.reset(theta)
-        _, t_cfwd, _ = timed(lambda: ctx.adjoint_forward(p, 1, steps, K))
+        _, t_cfwd, _ = timed(lambda: ens.run_forward(p, theta, keep_tape=False))
         obs_ck = ctx.observables(**obs_kw)
-        ens._theta = tuple(theta)
         ens.pullback(w["mean_pT"])                                  # workspace allocation, untimed
         grads, t_bwd = {}, {}
         for lname, wl in w.items():
@@ -399,7 +398,7 @@ def bench_adjoint(args):
             grads[lname], t_bwd[lname] = gr, t_b
         # the whole tape kept: forward pass writes it, the pullbacks only reverse
         p = ens.reset(theta)
-        _, t_kfwd, _ = timed(lambda: ctx.adjoint_forward(p, 1, steps, K, keep_tape=True))
+        _, t_kfwd, _ = timed(lambda: ens.run_forward(p, theta, keep_tape=True))
         obs_keep = ctx.observables(**obs_kw)
         ens.pullback(w["mean_pT"])                                  # untimed, as above
         t_kbwd, same_grads = {}, True

@@ -9,6 +9,16 @@
 #   using LangevInMedium, LangevInMediumHIP
 #   t, mom, pos = simulate_ensemble_bulk(HIPBackend(), r, p, dens, Tevo, Vevo, (xgrid, tgrid);
 #                                        N_particles = N, ...)
+#
+# Keyword arguments beyond the reference's (extensions of the backend):
+#   drag_model  :table (the reference's eta_D(T) m/E) or (:gamma_p, gamma0, Lambda):
+#               eta(p, T) = gamma0 T^2/(1 + p^2/Lambda^2)
+#   noise       :philox, or a fixed-noise record: an Array{Float64,3} of size (N, C, steps)
+#               (Julia's column-major layout of the ABI's (steps, C, N) record; C from
+#               lmh_noise_channels) read instead of the driver's randn draws
+#   milne       Milne coordinates with dimensions = 2, momentum_dimensions = 3 (the reference's
+#               Bjorken mode, bjorken_redshift implied): the positions gain the space-time rapidity
+#               row eta_s (x_init may hold it as a third row, else eta_s = 0)
 module LangevInMediumHIP
 
 using LangevInMedium
@@ -35,9 +45,10 @@ struct LmhParams
     reflecting_boundary::Int32; check_finite::Int32
 end
 
-const LMH_CARTESIAN, LMH_GENERAL_COORDS = Int32(0), Int32(1)
+const LMH_CARTESIAN, LMH_GENERAL_COORDS, LMH_MILNE = Int32(0), Int32(1), Int32(2)
 const LMH_LANGEVIN, LMH_RTA, LMH_FLUID = Int32(0), Int32(1), Int32(2)
-const LMH_NOISE_PHILOX = Int32(1)
+const LMH_NOISE_FIXED, LMH_NOISE_PHILOX = Int32(0), Int32(1)
+const LMH_DRAG_TABLE, LMH_DRAG_GAMMA_P = Int32(0), Int32(1)
 const LMH_TABLE_DRAG, LMH_TABLE_RTA = Int32(0), Int32(1)
 
 mutable struct Ctx
@@ -82,6 +93,23 @@ end
 
 run!(c, p::LmhParams, step0, n) = check(c, ccall((:lmh_run, liblmhip), Cint,
     (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ptr{Float64}, Int32), c.h, p, step0, n, C_NULL, 0))
+
+# steps step0 .. step0+n-1 reading the fixed-noise record's columns of those steps
+function run!(c, p::LmhParams, step0, n, noise::Array{Float64,3})
+    rec = noise[:, :, step0:step0 + n - 1]
+    GC.@preserve rec check(c, ccall((:lmh_run, liblmhip), Cint,
+        (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ptr{Float64}, Int32), c.h, p, step0, n, rec, 0))
+end
+
+noise_channels(p::LmhParams, dims, pdim) =
+    ccall((:lmh_noise_channels, liblmhip), Int32, (Ref{LmhParams}, Int32, Int32), p, dims, pdim)
+
+function drag_params(drag_model)
+    drag_model === :table && return LMH_DRAG_TABLE, 0.0, 1.0
+    (drag_model isa Tuple && length(drag_model) == 3 && drag_model[1] === :gamma_p) ||
+        error("drag_model must be :table or (:gamma_p, gamma0, Lambda)")
+    return LMH_DRAG_GAMMA_P, Float64(drag_model[2]), Float64(drag_model[3])
+end
 
 boost_to_lab!(c, p::LmhParams, step) = check(c, ccall((:lmh_boost_to_lab, liblmhip), Cint,
     (Ptr{Cvoid}, Ref{LmhParams}, Int64), c.h, p, step))
@@ -166,9 +194,17 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
         x_init = nothing, p_init = nothing, V2Evolutionn = nothing, psi2::Float64 = 0.0,
         freezeout_capture::Bool = false, freezeout_interp::Bool = true, integrator_mode::Int = 0,
         relativistic::Bool = true, momentum_dimensions::Int = 0, bjorken_redshift::Bool = false,
-        seed::Integer = 0, device::Integer = 0)
+        seed::Integer = 0, device::Integer = 0, drag_model = :table, noise = :philox, milne::Bool = false)
     (collision_mode == :langevin || collision_mode == :rta) ||
         error("simulate_ensemble_bulk(::HIPBackend): collision_mode=$(collision_mode) is not supported on GPU (only :langevin and :rta).")
+    if milne
+        (dimensions == 2 && momentum_dimensions == 3) ||
+            error("milne needs dimensions = 2 and momentum_dimensions = 3 (x, y, eta_s; p_x, p_y, p_z)")
+        (freezeout_capture || position_diffusion || reflecting_boundary) &&
+            error("milne: no freeze-out capture, position diffusion or reflecting boundary")
+        bjorken_redshift = true
+    end
+    dmodel, gamma0, Lambda = drag_params(drag_model)
     steps = floor(Int, (final_time - initial_time) / Δt)
     save_every = round(Int, save_interval / Δt)
     num_saves = div(steps, save_every)
@@ -177,8 +213,12 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
     # the medium first: the thermal p_z completion reads T(r, t0) from it
     set_medium!(c, collect(Float64, xgrid), collect(Float64, tgrid), Matrix{Float64}(TemperatureEvolutionn),
                 Matrix{Float64}(VelocityEvolutionn), V2Evolutionn === nothing ? nothing : Matrix{Float64}(V2Evolutionn))
+    eta0 = nothing
     if x_init !== nothing && p_init !== nothing
         x_matrix = Matrix{Float64}(x_init); p_matrix = Matrix{Float64}(p_init)
+        if milne && size(x_matrix, 1) == 3
+            eta0 = x_matrix[3:3, :]; x_matrix = x_matrix[1:2, :]
+        end
     else
         # simulate_cpu.jl:57-94: Cartesian sampling by default, antithetic pairs on request
         cart = cartesian_spatial_sampling === nothing ? (dimensions == 1 || dimensions >= 2) : cartesian_spatial_sampling
@@ -218,7 +258,11 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
     if momentum_langevin && DsT > 0.0
         kw = (Tmin = Tmin, Tmax = Tmax, nT = 1024, DsT_linear = DsT_linear, DsT_slope = DsT_slope,
               DsT_offset = DsT_offset, Tfo = Tfo, DsT_quad = DsT_quad, DsT_Tref = DsT_Tref)
-        set_table!(c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(m, DsT; kw...)...)
+        if dmodel == LMH_DRAG_TABLE
+            set_table!(c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(m, DsT; kw...)...)
+        else
+            set_table!(c, LMH_TABLE_DRAG, 0, 0.0, 1.0, [0.0, 0.0])
+        end
         if collision_mode == :rta
             set_table!(c, LMH_TABLE_RTA, 0, build_taun_current_spline(m, DsT; kw...)...)
             inv, nU, nT, Tt, idt = build_juttner_invcdf(m, pdim; Tmin = Tmin, Tmax = Tmax, nT = 256, nU = 2048, np = 8192)
@@ -227,16 +271,31 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
     else
         set_table!(c, LMH_TABLE_DRAG, 0, 0.0, 1.0, [0.0, 0.0])
     end
+    if milne
+        positions = vcat(positions, eta0 === nothing ? zeros(1, N) : eta0)
+        dimensions = 3
+    end
     set_particles!(c, positions, momenta, m)
     coll = (!momentum_langevin || DsT == 0.0) ? LMH_FLUID : (collision_mode == :rta ? LMH_RTA : LMH_LANGEVIN)
-    p = LmhParams(Δt, initial_time, DsT, psi2, 0.0, 1.0, UInt64(seed), 0, LMH_CARTESIAN, coll,
-                  LMH_NOISE_PHILOX, Int32(integrator_mode), 0, relativistic, bjorken_redshift,
+    fixed = noise isa AbstractArray
+    p = LmhParams(Δt, initial_time, DsT, psi2, gamma0, Lambda, UInt64(seed), 0,
+                  milne ? LMH_MILNE : LMH_CARTESIAN, coll, fixed ? LMH_NOISE_FIXED : LMH_NOISE_PHILOX,
+                  Int32(integrator_mode), dmodel, relativistic, bjorken_redshift,
                   position_diffusion, reflecting_boundary, 1)
+    if fixed
+        C = noise_channels(p, dimensions, pdim)
+        size(noise) == (N, C, steps) ||
+            error("the fixed-noise record must be (N, C, steps) = ($N, $C, $steps), got $(size(noise))")
+        noise = Array{Float64,3}(noise)
+    else
+        noise === :philox || error("noise must be :philox or a fixed-noise record")
+    end
+    step_run! = fixed ? ((s, n) -> run!(c, p, s, n, noise)) : ((s, n) -> run!(c, p, s, n))
     boost_to_lab!(c, p, 0)
     if freezeout_capture
         # simulate_gpu.jl:68-75,375-402: no history; the latched states are the result
         check(c, ccall((:lmh_freezeout_enable, liblmhip), Cint, (Ptr{Cvoid}, Float64, Int32), c.h, Tfo, freezeout_interp))
-        run!(c, p, 1, steps)
+        step_run!(1, steps)
         fpos = zeros(dimensions, N); fmom = zeros(pdim, N); ftau = zeros(N); fflag = zeros(N)
         check(c, ccall((:lmh_freezeout_get, liblmhip), Cint,
             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), c.h, fpos, fmom, ftau, fflag))
@@ -247,7 +306,7 @@ function simulate_ensemble_bulk(::HIPBackend, r_grid_Langevin, p_grid_Langevin, 
     step = 1
     while step <= steps
         n = min(save_every, steps - step + 1)
-        run!(c, p, step, n)
+        step_run!(step, n)
         step += n
         if n == save_every && length(pos_hist) <= num_saves
             ps, ms = get_particles(c, dimensions, pdim, N)
@@ -315,7 +374,10 @@ struct GridEnsemble
     pos::Matrix{Float64}; mom::Matrix{Float64}; m::Float64
     dt::Float64; t0::Float64; nsteps::Int; drag_model::Int32; seed::UInt64
     checkpoint_every::Int; keep_tape::Bool; obs::LmhObsSpec
+    current::Base.RefValue{Int}     # the forward pass whose adjoint record the context holds (0: none)
 end
+
+const _RECORDS = Ref(0)
 
 function GridEnsemble(x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, pos, mom, m; dt, t0, nsteps,
                       drag_model = 0, seed = 0, checkpoint_every = 50, keep_tape = false, nbins = 40,
@@ -327,10 +389,11 @@ function GridEnsemble(x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, pos, mom, m; dt,
         c.h, x0, dx, nx, y0, dy, ny, tgrid, length(tgrid), T, vx, vy))
     return GridEnsemble(c, x0, dx, nx, y0, dy, ny, tgrid, T, vx, vy, Matrix{Float64}(pos),
                         Matrix{Float64}(mom), m, dt, t0, nsteps, Int32(drag_model), UInt64(seed),
-                        checkpoint_every, keep_tape, LmhObsSpec(nbins, -1, pt_min, pt_max, sigma))
+                        checkpoint_every, keep_tape, LmhObsSpec(nbins, -1, pt_min, pt_max, sigma), Ref(0))
 end
 
 function _reset!(e::GridEnsemble, theta)
+    e.current[] = 0                 # the context's adjoint record no longer describes its state
     DsT, g0, Lam = theta
     Tmin = max(minimum(e.T), 0.0); Tmax = max(maximum(e.T), Tmin + eps(Float64))
     e.drag_model == 0 && set_table!(e.c, LMH_TABLE_DRAG, 0, build_tau_drag_spline(e.m, DsT; Tmin = Tmin, Tmax = Tmax, nT = 1024)...)
@@ -354,14 +417,24 @@ function final_observables(theta::AbstractVector{<:Real}, e::GridEnsemble)
     return _observables(e)
 end
 
-function ChainRulesCore.rrule(::typeof(final_observables), theta::AbstractVector{<:Real}, e::GridEnsemble)
+# the checkpointed forward pass at theta; the context then holds its adjoint record
+function _adjoint_forward!(e::GridEnsemble, theta, record)
     p = _reset!(e, theta)
     spec = LmhAdjointSpec(e.checkpoint_every, Int32(e.keep_tape), theta[1] > 0 ? 1.0 / theta[1] : 0.0)
     check(e.c, ccall((:lmh_adjoint_forward, liblmhip), Cint,
         (Ptr{Cvoid}, Ref{LmhParams}, Int64, Int64, Ref{LmhAdjointSpec}), e.c.h, p, 1, e.nsteps, spec))
+    e.current[] = record
+end
+
+function ChainRulesCore.rrule(::typeof(final_observables), theta::AbstractVector{<:Real}, e::GridEnsemble)
+    th = collect(Float64, theta)
+    record = (_RECORDS[] += 1)
+    _adjoint_forward!(e, th, record)
     obs = _observables(e)
     function final_observables_pullback(dobs)
         w = collect(Float64, ChainRulesCore.unthunk(dobs))
+        # another forward pass (or a plain run) replaced this one's record: run it again first
+        e.current[] == record || _adjoint_forward!(e, th, record)
         g = zeros(3)
         check(e.c, ccall((:lmh_adjoint_backward, liblmhip), Cint,
             (Ptr{Cvoid}, Ref{LmhObsSpec}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),

@@ -8,7 +8,14 @@ dL/dtheta for L = cotangent . observables from the pathwise adjoint (liblmhip's
 `lmh_adjoint_backward`).  `observables(theta, ensemble)` wraps the pair as a
 torch.autograd.Function, the PyTorch counterpart of a ChainRulesCore.rrule.  With
 keep_tape=True the forward pass keeps every step's tape row in device memory (nsteps x
-(2 + 2 pdim [+1]) x N doubles) and the pullback runs the reverse sweeps only.
+(2 + pdim) x N doubles: the state; the noise is drawn again from its Philox counters) and the
+pullback runs the reverse sweeps only.
+
+A context holds one adjoint record, the last forward pass's.  Each forward pass gets a record
+id; a pullback that names the forward it belongs to (theta and record, as the autograd node
+keeps them) re-runs that forward first when the context's record is another's (a second
+forward before the backward, on this or another ensemble of the context, or a plain run that
+replaced the state), so every node gets its own theta's gradient.
 
 DsT enters through the drag-time table (rebuilt for each DsT, `transport.jl:165-209`);
 gamma0 and Lambda through the GAMMA_P drag eta = gamma0 T^2/(1 + p^2/Lambda^2).  The noise is
@@ -17,10 +24,14 @@ Philox with a fixed key, so forward runs at different theta share their random n
 """
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 
 from . import _ffi, transport
-from .context import Context
+from .context import Context, LMHError
+
+_RECORDS = itertools.count(1)
 
 
 class GridEnsemble:
@@ -58,6 +69,7 @@ class GridEnsemble:
                 ctx.set_juttner_table(k, *juttner_tables[k])
         self._dst_loaded = None
         self._theta = None
+        self.last_record = None
 
     def params(self, theta):
         DsT, gamma0, Lambda = (float(v) for v in theta)
@@ -76,6 +88,8 @@ class GridEnsemble:
 
     def reset(self, theta):
         """Upload the initial ensemble and the theta-dependent tables."""
+        # the context's adjoint record no longer describes its state
+        self.ctx._adjoint_record = None
         p = self.params(theta)
         self._load_drag(p.DsT)
         self.ctx.set_particles(self.pos0, self.mom0, self.masses, self.counts)
@@ -90,16 +104,46 @@ class GridEnsemble:
         return self.ctx.observables(**self.obs)
 
     def forward(self, theta):
+        """The final observables at theta; keeps the context's adjoint record (its id in
+        last_record)."""
         p = self.reset(theta)
-        self.ctx.adjoint_forward(p, 1, self.nsteps, self.K, keep_tape=self.keep_tape)
-        self._theta = tuple(float(v) for v in theta)
+        self.run_forward(p, theta)
         return self.ctx.observables(**self.obs)
 
-    def pullback(self, cotangent, state_grad=False):
-        if self._theta is None:
-            raise RuntimeError("pullback before forward")
-        return self.ctx.adjoint_backward(np.asarray(cotangent, dtype=np.float64), state_grad=state_grad,
-                                         **self.obs)
+    def run_forward(self, p, theta, keep_tape=None):
+        """The checkpointed forward pass from the state reset(theta) uploaded (p its
+        parameters); registers the context's adjoint record as this ensemble's."""
+        keep = self.keep_tape if keep_tape is None else bool(keep_tape)
+        self.ctx.adjoint_forward(p, 1, self.nsteps, self.K, keep_tape=keep)
+        self.keep_tape = keep
+        self._theta = tuple(float(v) for v in theta)
+        self.last_record = next(_RECORDS)
+        self.ctx._adjoint_record = (id(self), self.last_record)
+
+    def pullback(self, cotangent, state_grad=False, theta=None, record=None):
+        """dL/dtheta (and with state_grad the state gradients) of the forward pass `record` at
+        `theta`; without them, of the last forward pass of this ensemble.  When the context's
+        record is not that pass's, the pass is run again at its theta first."""
+        if record is None:
+            if self._theta is None:
+                raise RuntimeError("pullback before forward")
+            theta, record = self._theta, self.last_record
+        w = np.asarray(cotangent, dtype=np.float64)
+        if getattr(self.ctx, "_adjoint_record", None) != (id(self), record):
+            self._rerun(theta, record)
+        try:
+            return self.ctx.adjoint_backward(w, state_grad=state_grad, **self.obs)
+        except LMHError as e:
+            # the library found the record stale (the state, medium or tables changed since)
+            if e.code != _ffi.LMH_E_STATE:
+                raise
+            self._rerun(theta, record)
+            return self.ctx.adjoint_backward(w, state_grad=state_grad, **self.obs)
+
+    def _rerun(self, theta, record):
+        self.forward(theta)
+        self.last_record = record
+        self.ctx._adjoint_record = (id(self), record)
 
 
 def _torch_function():
@@ -108,13 +152,15 @@ def _torch_function():
     class _Observables(torch.autograd.Function):
         @staticmethod
         def forward(fctx, theta, ensemble):
-            obs = ensemble.forward(theta.detach().cpu().numpy())
-            fctx.ensemble = ensemble
+            th = theta.detach().cpu().numpy().astype(np.float64)
+            obs = ensemble.forward(th)
+            fctx.ensemble, fctx.theta, fctx.record = ensemble, th.copy(), ensemble.last_record
             return torch.from_numpy(obs).to(theta.dtype)
 
         @staticmethod
         def backward(fctx, gobs):
-            g = fctx.ensemble.pullback(gobs.detach().cpu().numpy().astype(np.float64))
+            g = fctx.ensemble.pullback(gobs.detach().cpu().numpy().astype(np.float64),
+                                       theta=fctx.theta, record=fctx.record)
             return torch.from_numpy(g).to(gobs.dtype), None
 
     return _Observables
