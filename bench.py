@@ -43,34 +43,13 @@ MEASURED_TRAFFIC_PER_QUARK = 89.00
 
 
 def build_workload(N, seed=3):
-    import numpy as np
     from lmhip import media
-
-    rng = np.random.default_rng(seed)
-    g = media.hydro_grid(nx=200, ny=200, nt=120, L=15.0, tau0=0.6, tau1=12.0, T0=0.45)
-    nc = int(round(0.7 * N))
-    counts = [nc, N - nc]
-    masses = [1.5, 4.75]
-    pos = media.hydro_positions(N, rng)
-    mom = np.asfortranarray(np.hstack([media.pt_rapidity_momenta(counts[0], masses[0], rng),
-                                       media.pt_rapidity_momenta(counts[1], masses[1], rng)]))
-    return g, masses, counts, pos, mom
+    return media.hydro_ensemble(N, seed)
 
 
 def tables_for(g, masses, DsT, pdim):
-    import numpy as np
-    from lmhip import media, transport
-
-    eps = np.finfo(float).eps
-    Tmin = max(float(np.min(g.T)), 0.0)
-    Tmax = max(float(np.max(g.T)), Tmin + eps)
-    out = []
-    for m in masses:
-        drag = transport.build_tau_drag_spline(m, DsT, Tmin=Tmin, Tmax=Tmax, nT=1024)
-        rta = transport.build_relaxation_spline(media.tau_R_law(0.16), Tmin=Tmin, Tmax=Tmax, nT=1024)
-        jt = transport.build_juttner_invcdf(m, pdim, Tmin=Tmin, Tmax=Tmax, nT=256, nU=2048, np_=8192)
-        out.append((drag, rta, jt))
-    return out
+    from lmhip import media
+    return media.hydro_tables(g, masses, DsT, pdim)
 
 
 COLLISIONS = {"langevin_rta": 3, "langevin": 0, "rta": 1, "fluid": 2}

@@ -129,3 +129,34 @@ def hydro_positions(N, rng, sx=3.0, sy=4.0, ecc=0.0):
 def tau_R_law(eta_over_s=0.16, c=5.0):
     """RTA relaxation time tau_R = c (eta/s)/T [GeV^-1] in fm."""
     return lambda T: c * eta_over_s / np.asarray(T, dtype=np.float64) / fmGeV
+
+
+def hydro_ensemble(N, seed=3, nx=200, ny=200, nt=120):
+    """The 2+1D hydro workload: the hydro_grid medium (200 x 200 cells over +-15 fm, 120
+    tau-slices from 0.6 to 12 fm/c, T0 = 0.45 GeV), N quarks of which 70% charm (m = 1.5 GeV)
+    and 30% bottom (m = 4.75 GeV), positions ~ T^4 at tau0, p_T ~ p_T/(1 + p_T^2/4)^3 on
+    [0, 20] GeV with rapidity uniform in [-1, 1].  Returns (grid, masses, counts, pos, mom)."""
+    rng = np.random.default_rng(seed)
+    g = hydro_grid(nx=nx, ny=ny, nt=nt, L=15.0, tau0=0.6, tau1=12.0, T0=0.45)
+    nc = int(round(0.7 * N))
+    counts = [nc, N - nc]
+    masses = [1.5, 4.75]
+    pos = hydro_positions(N, rng)
+    mom = np.asfortranarray(np.hstack([pt_rapidity_momenta(counts[0], masses[0], rng),
+                                       pt_rapidity_momenta(counts[1], masses[1], rng)]))
+    return g, masses, counts, pos, mom
+
+
+def hydro_tables(g, masses, DsT, pdim, eta_over_s=0.16):
+    """per species: (drag table, RTA table of tau_R = 5 (eta/s)/T, Juttner inverse CDF) over the
+    medium's temperature range, at the resolutions the hydro workload uses"""
+    from . import transport
+    Tmin = max(float(np.min(g.T)), 0.0)
+    Tmax = max(float(np.max(g.T)), Tmin + np.finfo(float).eps)
+    out = []
+    for m in masses:
+        drag = transport.build_tau_drag_spline(m, DsT, Tmin=Tmin, Tmax=Tmax, nT=1024)
+        rta = transport.build_relaxation_spline(tau_R_law(eta_over_s), Tmin=Tmin, Tmax=Tmax, nT=1024)
+        jt = transport.build_juttner_invcdf(m, pdim, Tmin=Tmin, Tmax=Tmax, nT=256, nU=2048, np_=8192)
+        out.append((drag, rta, jt))
+    return out

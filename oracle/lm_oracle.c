@@ -1594,6 +1594,24 @@ static void step_grid_segment(const lmh_params* p, const lmo_grid_medium* g,
     }
 }
 
+int lmo_boost_to_lab_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
+                          int32_t pdim, const double* pos, double* mom, int64_t N, int64_t step) {
+    if (pdim != 2 && pdim != 3) return LMO_E_INVALID;
+    if (species_begin(sp, sp->nspecies) != N) return LMO_E_INVALID;
+    double t = (double)step * p->dt + p->t0;
+    for (int32_t k = 0; k < sp->nspecies; ++k) {
+        int64_t b = species_begin(sp, k), n = sp->counts[k];
+        double m = sp->mass[k];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = b; i < b + n; ++i) {
+            double med[3];
+            lmo_grid_sample(g, pos[2 * i], pos[2 * i + 1], t, med);
+            grid_boost(mom + (int64_t)pdim * i, pdim, m, med[1], med[2], +1, p->relativistic);
+        }
+    }
+    return LMO_OK;
+}
+
 int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
                  int32_t pdim, double* pos, double* mom, int64_t N, int64_t step_begin,
                  int64_t nsteps, const double* noise, int64_t* bad_step) {

@@ -142,6 +142,10 @@ void lmo_grid_sample(const lmo_grid_medium* g, double x, double y, double t, dou
 int lmo_run_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
                  int32_t pdim, double* pos, double* mom, int64_t N, int64_t step_begin,
                  int64_t nsteps, const double* noise, int64_t* bad_step);
+/* the driver's initial boost to the lab (simulate_cpu.jl:160-163) on the grid: every quark's
+ * momentum boosted along the flow sampled at its (x, y) and t = step*dt + t0 */
+int lmo_boost_to_lab_grid(const lmh_params* p, const lmo_grid_medium* g, const lmo_species* sp,
+                          int32_t pdim, const double* pos, double* mom, int64_t N, int64_t step);
 
 /* ---- on-the-fly freeze-out capture ------------------------------------------------------------ */
 void lmo_freezeout_capture(const lmo_radial_medium* rmed, const lmo_grid_medium* gmed,

@@ -85,6 +85,8 @@ def lib():
                                                c_double, c_double, c_int32, c_int32, c_int32, c_double, c_int32]),
             "lmo_boost_to_lab_all": (c_int, [P, POINTER(RadialMedium), POINTER(Species), c_int32, c_int32,
                                              PD, PD, c_int64, c_int64]),
+            "lmo_boost_to_lab_grid": (c_int, [P, POINTER(GridMedium), POINTER(Species), c_int32, PD, PD,
+                                              c_int64, c_int64]),
             "lmo_run_cartesian": (c_int, [P, POINTER(RadialMedium), POINTER(Species), c_int32, c_int32, PD, PD,
                                           c_int64, c_int64, c_int64, PD, POINTER(c_int64)]),
             "lmo_milne_deta": (c_double, [c_double, c_double, c_double, c_double, c_double, c_double]),
@@ -221,6 +223,11 @@ def run_grid(params, med, sp, pos, mom, step_begin, nsteps, noise):
     rc = L.lmo_run_grid(ctypes.byref(params), ctypes.byref(med), ctypes.byref(sp), mom.shape[0],
                         dp(pos), dp(mom), pos.shape[1], step_begin, nsteps, dp(noise), ctypes.byref(bad))
     return rc, bad.value
+
+
+def boost_to_lab_grid(params, med, sp, pos, mom, step):
+    return lib().lmo_boost_to_lab_grid(ctypes.byref(params), ctypes.byref(med), ctypes.byref(sp),
+                                       mom.shape[0], dp(pos), dp(mom), pos.shape[1], step)
 
 
 def boost_to_lab_all(params, med, sp, pos, mom, step):

@@ -508,3 +508,50 @@ def test_milne_step_is_the_bjorken_mode_step_on_the_transverse_rows():
     assert O.run_cartesian(pmil, med, sp, Pm, Mm, 1, n, rec)[0] == 0
     assert np.array_equal(Pm[:2], Pc) and np.array_equal(Mm, Mc)
     assert np.all(np.sign(Pm[2] - eta0) == np.sign(Mm[2]) + (Mm[2] == 0)) or np.mean(Pm[2] != eta0) > 0.99
+
+
+# ------------------------------------------------------------ initial boost on the grid -------
+@pytest.mark.parametrize("relativistic", [True, False])
+def test_grid_initial_boost_is_the_lorentz_boost(relativistic):
+    """lmo_boost_to_lab_grid on a medium of uniform flow v = (vx, vy): the textbook boost
+    p' = p + (gamma - 1)(p.n)n + gamma E v n with E = sqrt(p^2 + m^2) (relativistic; the
+    reference's gamma regulator 1e-10 makes the bar 1e-9), p' = p + m v (Galilean); a quark at
+    rest lands on p' = gamma m v; the species keep their own masses"""
+    from lmhip import media
+    vx, vy = 0.5, -0.6
+    n = 5
+    x = np.linspace(-2.0, 2.0, n)
+    tg = np.array([0.5, 1.0, 2.0])
+    shape = (n, n, tg.size)
+    g = media.Grid2p1Medium(-2.0, 1.0, n, -2.0, 1.0, n, tg, np.full(shape, 0.3, order="F"),
+                            np.full(shape, vx, order="F"), np.full(shape, vy, order="F"))
+    keep = O.Keep()
+    gm = O.grid_medium(keep, g)
+    rng = np.random.default_rng(5)
+    counts, masses = [60, 40], [1.5, 4.75]
+    N = sum(counts)
+    sp = O.species(keep, masses, counts, drag=[(0.0, 1.0, np.zeros(2))] * 2)
+    pos = np.asfortranarray(rng.uniform(-3.0, 3.0, (2, N)))       # outside the grid: clamped
+    mom = np.asfortranarray(rng.normal(0.0, 2.0, (3, N)))
+    mom[:, 0] = 0.0
+    p = _ffi.make_params(dt=0.01, t0=0.6, relativistic=relativistic)
+    out = mom.copy(order="F")
+    assert O.boost_to_lab_grid(p, gm, sp, pos, out, 7) == 0
+    m = np.repeat(masses, counts)
+    v = np.hypot(vx, vy)
+    nv = np.array([vx, vy, 0.0]) / v
+    if relativistic:
+        gam = 1.0 / np.sqrt(1.0 - v * v)
+        E = np.sqrt((mom ** 2).sum(0) + m * m)
+        pn = nv @ mom
+        want = mom + np.outer(nv, (gam - 1.0) * pn + gam * v * E)
+        # per quark, relative to |p'| (single components cancel)
+        dev = np.linalg.norm(out - want, axis=0) / np.linalg.norm(want, axis=0)
+        assert dev.max() <= 1e-9, dev.max()
+        assert np.allclose(out[:, 0], gam * 1.5 * v * nv, rtol=1e-9)
+        # the mass shell: E'^2 - p'^2 is unchanged
+        Ep = gam * (E + v * pn)
+        assert np.allclose(Ep ** 2 - (out ** 2).sum(0), m * m, rtol=1e-6)
+    else:
+        want = mom + np.outer(nv * v, m)
+        assert np.allclose(out, want, rtol=1e-13, atol=1e-14)
