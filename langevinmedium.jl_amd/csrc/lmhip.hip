@@ -42,7 +42,13 @@ struct SpeciesArgs {
     TimeTable drag;
     TimeTable rta;
     JuttnerTable jt;
+    int32_t shared_axis;  // 1: drag and rta lie on one temperature axis (eval_time_table_pair)
 };
+
+// whether a species' drag and RTA tables share their temperature axis
+inline int32_t tables_share_axis(const TimeTable& a, const TimeTable& b) {
+    return a.vals && b.vals && a.Tmin == b.Tmin && a.invdT == b.invdT && a.n == b.n ? 1 : 0;
+}
 
 // Species are contiguous particle segments; one launch covers all of them: block b belongs to
 // the segment whose first block is the largest block0[k] <= b.
@@ -962,9 +968,16 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         } else {
             const double T = jl_max(Tm, 0.0);
             // both time-table reads are issued together, before the kick needs the first
-            const double tn = HAS_RTA ? rta_lookup(sp.rta, T) : 0.0;
+            const bool read_drag = HAS_KICK && drag_table_read(p);
+            double tn = 0.0, tau_d = 0.0;
+            if (HAS_RTA && read_drag) {
+                eval_time_table_pair(sp.rta, sp.drag, sp.shared_axis != 0, T, &tn, &tau_d);
+            } else {
+                if (HAS_RTA) tn = rta_lookup(sp.rta, T);
+                if (read_drag) tau_d = eval_time_table(sp.drag, T);
+            }
             double etaD = 0.0, kT = 0.0;
-            if (HAS_KICK) drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
+            if (HAS_KICK) drag_coeffs_tau<false>(p, tau_d, m, T, &etaD, &kT);
             if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
             if (HAS_RTA) {
                 const bool col = rta_decide(p, tn, T, ucol);
@@ -1522,6 +1535,7 @@ int upload_species(lmh_ctx* c) {
         s.drag = TimeTable{c->drag[k].d, c->drag[k].Tmin, c->drag[k].invdT, c->drag[k].n};
         s.rta = TimeTable{c->rta[k].d, c->rta[k].Tmin, c->rta[k].invdT, c->rta[k].n};
         s.jt = JuttnerTable{c->jt[k].d, c->jt[k].Tmin, c->jt[k].invdT, c->jt[k].nU, c->jt[k].nT};
+        s.shared_axis = tables_share_axis(s.drag, s.rta);
     }
     if (!c->d_species && hipMalloc((void**)&c->d_species, sizeof(c->h_species)) != hipSuccess) {
         c->d_species = nullptr;
@@ -3152,6 +3166,7 @@ int batch_layout(lmh_batch* b, int block) {
             const lmh_batch::JTab& t = b->jtabs[d.juttner_table];
             o.sp.jt = JuttnerTable{t.d, t.Tmin, t.invdT, t.nU, t.nT};
         }
+        o.sp.shared_axis = tables_share_axis(o.sp.drag, o.sp.rta);
         o.DsT = d.DsT;
         const int64_t nt = (d.count + block - 1) / block;
         for (int64_t k = 0; k < nt; ++k) tr.push_back(r);

@@ -74,13 +74,21 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         glue_to_flow<PDIM>(q, m, vx, vy, p.relativistic);
     } else {
         const double T = jl_max(Tm, 0.0);
+        const bool read_drag = HAS_KICK && drag_table_read(p);
+        double tn = 0.0, tau_d = 0.0;
+        if (HAS_RTA && read_drag) {
+            eval_time_table_pair(sp.rta, sp.drag, sp.shared_axis != 0, T, &tn, &tau_d);
+        } else {
+            if (HAS_RTA) tn = rta_lookup(sp.rta, T);
+            if (read_drag) tau_d = eval_time_table(sp.drag, T);
+        }
         double etaD = 0.0, kT = 0.0;
-        if (HAS_KICK) drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
+        if (HAS_KICK) drag_coeffs_tau<false>(p, tau_d, m, T, &etaD, &kT);
         double xi[PDIM];
         double ucol = 1.0, usamp_p = 0.0;
         noise_kick<PDIM, LMH_NOISE_PHILOX, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp_p);
         if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
-        if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+        if (HAS_RTA && rta_decide(p, tn, T, ucol)) {
             double usamp, gg[PDIM];
             noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
             rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);

@@ -113,6 +113,32 @@ __device__ __forceinline__ double eval_time_table(const TimeTable& t, double T) 
     return one ? y0 : (isfinite(T) ? r : 0.0);
 }
 
+// Two time tables read at one T.  Tables built over one medium's temperature range share their
+// axis (the same Tmin, 1/dT and node count; the drag and RTA tables of a species are, which the
+// host records in `shared`): the segment and the weight are then found once, and each value is
+// the one eval_time_table gives.
+__device__ __forceinline__ void eval_time_table_pair(const TimeTable& a, const TimeTable& b, bool shared,
+                                                     double T, double* ya, double* yb) {
+    if (!shared) {
+        *ya = eval_time_table(a, T);
+        *yb = eval_time_table(b, T);
+        return;
+    }
+    double u = (T - a.Tmin) * a.invdT;
+    const double fc = fmin(fmax(floor(u), 0.0), (double)(a.n - 2));
+    const bool one = a.n < 2;
+    const int i = one ? 0 : (int)fc + 1;
+    double w = u - fc;
+    const uint32_t o0 = (uint32_t)(one ? 0 : i - 1) * 8u, o1 = (uint32_t)i * 8u;
+    const double a0 = ld_off(a.vals, o0), a1 = ld_off(a.vals, o1);
+    const double b0 = ld_off(b.vals, o0), b1 = ld_off(b.vals, o1);
+    const double ra = (1.0 - w) * a0 + w * a1;
+    const double rb = (1.0 - w) * b0 + w * b1;
+    const bool fin = isfinite(T);
+    *ya = one ? a0 : (fin ? ra : 0.0);
+    *yb = one ? b0 : (fin ? rb : 0.0);
+}
+
 // inverse-CDF Jüttner table and its bilinear lookup, kernels_gpu.jl:88-117
 struct JuttnerTable {
     const double* vals;
@@ -198,23 +224,22 @@ __device__ __forceinline__ double u_low33(uint64_t w0, uint64_t w1, uint64_t w2)
     return fma((double)((uint32_t)w2 & 0x7FFu), 0x1.0p-11, (double)a * 0x1.0p-33);
 }
 
-// 1/sqrt(x) for x > 0 (normal): the hardware estimate refined by two Newton steps, within an ulp
-// of the correctly rounded 1/sqrt(x); half the instructions of sqrt followed by a division
+// 1/sqrt(x) for x > 0 (normal): the hardware estimate y (relative error e ~ 2^-24) refined by
+// one third-order step y (1 + d/2 + 3 d^2/8), d = 1 - x y^2 (truncation ~ 5 e^3/16 < 2^-70;
+// d from one fused multiply-add, so its rounding stays ~2^-54 absolute): within an ulp of the
+// correctly rounded 1/sqrt(x) in five instructions (two Newton steps take seven)
 __device__ __forceinline__ double rsqrt_pos(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    y = fma(y, fma(-(h * y), y, 0.5), y);
-    y = fma(y, fma(-(h * y), y, 0.5), y);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    const double d = fma(-(x * y), y, 1.0);
+    return fma(y, d * fma(d, 0.375, 0.5), y);
 }
 
-// 1/x for finite x > 0 (normal): the hardware estimate refined by two Newton steps, within an
-// ulp of the correctly rounded 1/x at half the instructions of a division
+// 1/x for finite x > 0 (normal): the hardware estimate y refined by one third-order step
+// y (1 + e + e^2), e = 1 - x y: within an ulp of the correctly rounded 1/x in three instructions
 __device__ __forceinline__ double rcp_pos(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    y = fma(y, fma(-x, y, 1.0), y);
-    y = fma(y, fma(-x, y, 1.0), y);
-    return y;
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, y, 1.0);
+    return fma(fma(e, e, e), y, y);
 }
 
 // sqrt(x) for x >= 0: g = x/sqrt(x) from the once-refined estimate, then one residual step
@@ -229,6 +254,16 @@ __device__ __forceinline__ double sqrt_pos(double x) {
     const double g = x * y;
     const double r = fma(fma(-g, g, x), 0.5 * y, g);
     return (POS || x > 0.0) ? r : x;
+}
+
+// fma(a, b, c) for a wave-uniform constant c, read from a scalar register pair by the v_fma_f64
+// itself.  The polynomial coefficients below are such constants; left to itself the compiler
+// copies some of them into vector register pairs first (two v_mov_b32 per term, and two more
+// VGPRs).  The same IEEE fused multiply-add as fma().
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
 }
 
 // The two transcendental functions of the Box-Muller transform, the same definition as the host
@@ -296,10 +331,10 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
     double z = r * r;
     double ps = SP[7];
 #pragma unroll
-    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    for (int j = 6; j >= 0; --j) ps = fma_sc(ps, z, SP[j]);
     double pc = CP[8];
 #pragma unroll
-    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    for (int j = 7; j >= 0; --j) pc = fma_sc(pc, z, CP[j]);
     double sr = r * ps, cr = pc;
     int q = (int)k & 3;
     double s1 = (q & 1) ? cr : sr;
@@ -322,10 +357,10 @@ __device__ __forceinline__ void sincos_word(uint64_t w, double* sn, double* cs) 
     const double z = r * r;
     double ps = SP[7];
 #pragma unroll
-    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    for (int j = 6; j >= 0; --j) ps = fma_sc(ps, z, SP[j]);
     double pc = CP[8];
 #pragma unroll
-    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    for (int j = 7; j >= 0; --j) pc = fma_sc(pc, z, CP[j]);
     const double sr = r * ps, cr = pc;
     const int q = (int)(uint32_t)(w >> 62);
     const double s1 = (q & 1) ? cr : sr;
@@ -343,7 +378,7 @@ __device__ __forceinline__ double noise_cospi_half(double x) {
     const double z = x * x;
     double p = C[10];
 #pragma unroll
-    for (int j = 9; j >= 0; --j) p = fma(p, z, C[j]);
+    for (int j = 9; j >= 0; --j) p = fma_sc(p, z, C[j]);
     return p;
 }
 
@@ -390,22 +425,26 @@ __device__ __forceinline__ double exp_neg(double x) {
     return fma(r, y, 1.0);
 }
 
+// drag_table_read: whether the drag time table is read (DsT > 0 and the TABLE model; a NaN DsT
+// reads it); drag_coeffs_tau takes the table value tau read at T (0 where it is not read).
+__device__ __forceinline__ bool drag_table_read(const lmh_params& p) {
+    return p.drag_model != LMH_DRAG_GAMMA_P && !(p.DsT <= 0.0);
+}
+
 // drag/noise coefficients from the local (clamped) T: kernels_cpu.jl:270-290, eta_D = 1/tau and
 // kT = sqrt(kappa), kappa = 2 m T / tau; or the GAMMA_P model eta_D = gamma0*T^2,
 // kappa = 2 m T eta_D.  eta_D = 0 gives kappa = 0 in both models.  EXACT keeps the reference's
 // divisions (the radial-mode Euler kick uses kT); otherwise 1/tau is the refined reciprocal and
 // kappa = 2 m T eta_D (the exact-OU kick below needs kT only in its radial form).
 template <bool EXACT>
-__device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable& tab, double m,
-                                            double T, double* etaD, double* kT) {
+__device__ __forceinline__ void drag_coeffs_tau(const lmh_params& p, double tau, double m, double T,
+                                                double* etaD, double* kT) {
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
         double e = p.gamma0 * T * T;
         *etaD = e;
         *kT = sqrt(2.0 * m * T * e);
         return;
     }
-    double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
-    double tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(tab, T);
     bool ok = tau > 0.0 && isfinite(tau);
     if (EXACT) {
         *etaD = ok ? (1.0 / tau) : 0.0;
@@ -416,6 +455,13 @@ __device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable
     const double e = ok ? rcp_pos(tau) : 0.0;
     *etaD = e;
     *kT = sqrt_pos((2.0 * m * T) * e);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable& tab, double m,
+                                            double T, double* etaD, double* kT) {
+    const double tau = drag_table_read(p) ? eval_time_table(tab, T) : 0.0;
+    drag_coeffs_tau<EXACT>(p, tau, m, T, etaD, kT);
 }
 
 // effective drag at momentum q -- eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
