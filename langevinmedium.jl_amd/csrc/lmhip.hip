@@ -55,6 +55,8 @@ inline int32_t tables_share_axis(const TimeTable& a, const TimeTable& b) {
 struct StepArgs {
     double* pos[3];
     double* mom[3];
+    double* wpos[3];            // the rows k_cart_grid writes: pos and mom themselves (in place),
+    double* wmom[3];            // or the next row of an adjoint segment's tape
     int64_t N;                  // particles (the noise row stride)
     int64_t step_begin;
     int32_t nsteps;
@@ -849,6 +851,16 @@ __global__ __launch_bounds__(kBlock) void k_unsort_keys(const uint32_t* id, int6
     vals[s] = (uint32_t)s;
 }
 
+// the state in the caller's order: dst row r, slot id[s] = src row r, slot s (an adjoint
+// checkpoint taken while the particles are cell-sorted)
+__global__ __launch_bounds__(kBlock) void k_scatter_rows(const double* __restrict__ src, double* __restrict__ dst,
+                                                         int32_t rows, int64_t N, const uint32_t* __restrict__ id) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= N) return;
+    const int64_t to = id[s];
+    for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + to] = src[(int64_t)r * N + s];
+}
+
 // dst row r, slot s = src row r, slot perm[s]; the identity likewise.  Every row is read before
 // any is written (the buffers never alias), so a thread's loads are in flight together.
 constexpr int kMaxRows = 6;
@@ -1012,15 +1024,15 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step;
     }
     if (SPEC) {
-        st_off(a.pos[0], o8, x0);
-        st_off(a.pos[1], o8, x1);
+        st_off(a.wpos[0], o8, x0);
+        st_off(a.wpos[1], o8, x1);
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) st_off(a.mom[d], o8, q[d]);
+        for (int d = 0; d < PDIM; ++d) st_off(a.wmom[d], o8, q[d]);
     } else {
-        a.pos[0][i] = x0;
-        a.pos[1][i] = x1;
+        a.wpos[0][i] = x0;
+        a.wpos[1][i] = x1;
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+        for (int d = 0; d < PDIM; ++d) a.wmom[d][i] = q[d];
     }
     if (fo) a.fo.lastT[i] = lastT;
     if (bad >= 0) atomicMin(a.bad, (unsigned long long)bad);
@@ -1552,6 +1564,8 @@ StepArgs make_args(lmh_ctx* c, const lmh_params* p, int block, unsigned* nblocks
     for (int d = 0; d < 3; ++d) {
         a.pos[d] = d < c->dims ? c->d_state + (int64_t)d * c->N : nullptr;
         a.mom[d] = d < c->pdim ? c->d_state + (int64_t)(c->dims + d) * c->N : nullptr;
+        a.wpos[d] = a.pos[d];
+        a.wmom[d] = a.mom[d];
     }
     a.N = c->N;
     a.p = *p;
@@ -2798,6 +2812,7 @@ struct AdjLaunch {
     double* lam = nullptr;
     double* gacc = nullptr;
     double dtau = 0.0;
+    const double* row0 = nullptr;   // the reverse kernel: step 0's rows (the checkpoint) if set
 };
 
 template <int PDIM, int COLL, int DRAG>
@@ -2808,7 +2823,7 @@ void launch_adj_t(lmh_ctx* c, bool tape, const AdjLaunch& l) {
                            l.tp, l.state_out);
     else
         hipLaunchKernelGGL((k_adj_reverse<PDIM, COLL, DRAG>), G, B, 0, c->stream, *l.a, *l.g, l.sl, l.step0, l.nk,
-                           l.tp, l.lam, l.gacc, l.dtau);
+                           l.tp, l.row0, l.lam, l.gacc, l.dtau);
 }
 
 template <int PDIM>
@@ -2827,6 +2842,54 @@ void launch_adj(lmh_ctx* c, bool tape, const lmh_params& p, const AdjLaunch& l) 
     else launch_adj_p<3>(c, tape, p.collision, p.drag_model, l);
 }
 
+// The forward steps of an adjoint segment again, into its tape, on stream st: step k of the
+// segment reads tape row k (the checkpoint for k = 0) and writes row k + 1, one launch of
+// lmh_run's step kernel per step with the segment's medium slices -- the forward pass's kernel
+// and medium, so the tape is its trajectory bit for bit.  The last step's output (the next
+// checkpoint) is not needed.
+template <int PDIM, int COLL, bool SPEC>
+void adj_tape_steps_t(const StepArgs& a, unsigned nb, const GridMedium& g, int block, hipStream_t st) {
+    hipLaunchKernelGGL((k_cart_grid<PDIM, LMH_NOISE_PHILOX, true, COLL, false, SPEC>), dim3(nb), dim3(block), 0, st, a, g);
+}
+
+template <int PDIM>
+void adj_tape_steps_p(const StepArgs& a, unsigned nb, const GridMedium& g, int block, hipStream_t st, int coll,
+                      bool spec) {
+    if (coll == LMH_LANGEVIN_RTA) {
+        if (spec) adj_tape_steps_t<PDIM, LMH_LANGEVIN_RTA, true>(a, nb, g, block, st);
+        else adj_tape_steps_t<PDIM, LMH_LANGEVIN_RTA, false>(a, nb, g, block, st);
+    } else {
+        if (spec) adj_tape_steps_t<PDIM, LMH_LANGEVIN, true>(a, nb, g, block, st);
+        else adj_tape_steps_t<PDIM, LMH_LANGEVIN, false>(a, nb, g, block, st);
+    }
+}
+
+void adj_tape_steps(lmh_ctx* c, const lmh_params& p, StepArgs a, unsigned nb, const GridMedium& g,
+                    const double4* sl, int64_t step0, int32_t nk, const double* ckpt, double* tape,
+                    hipStream_t st) {
+    const int64_t N = c->N;
+    const int64_t R = c->dims + c->pdim;
+    const int64_t ncell = (int64_t)(c->gx - 1) * (c->gy - 1);
+    const bool spec = spec_config(c, &p);
+    a.fo.on = 0;
+    a.nsteps = 1;
+    for (int32_t k = 0; k + 1 < nk; ++k) {
+        double* in = const_cast<double*>(k == 0 ? ckpt : tape + (size_t)k * R * N);
+        double* out = tape + (size_t)(k + 1) * R * N;
+        for (int d = 0; d < 3; ++d) {
+            a.pos[d] = d < c->dims ? in + d * N : nullptr;
+            a.mom[d] = d < c->pdim ? in + (c->dims + d) * N : nullptr;
+            a.wpos[d] = d < c->dims ? out + d * N : nullptr;
+            a.wmom[d] = d < c->pdim ? out + (c->dims + d) * N : nullptr;
+        }
+        a.step_begin = step0 + k;
+        GridMedium gk = g;
+        gk.slice = sl + (size_t)k * 3 * (size_t)ncell;
+        if (c->pdim == 2) adj_tape_steps_p<2>(a, nb, gk, c->tune.block, st, p.collision, spec);
+        else adj_tape_steps_p<3>(a, nb, gk, c->tune.block, st, p.collision, spec);
+    }
+}
+
 // the forward pass of the adjoint steps through the per-step medium slices, one step per launch,
 // which the backward pass rebuilds for its recomputation: both see the same medium, bit for bit
 struct SliceTuning {
@@ -2835,10 +2898,25 @@ struct SliceTuning {
     explicit SliceTuning(lmh_ctx* c_) : c(c_), saved(c_->tune) {
         c->tune.slice = 1;
         c->tune.steps_per_launch = 1;
-        c->tune.sort_every = 0;       // the checkpoints are taken in the caller's order
     }
     ~SliceTuning() { c->tune = saved; }
 };
+
+// an adjoint checkpoint of the current state in the caller's order (scattered through the
+// identities when the particles are cell-sorted)
+int take_checkpoint(lmh_ctx* c, double* dst) {
+    const int64_t N = c->N;
+    const int rows = c->dims + c->pdim;
+    if (c->is_sorted) {
+        hipLaunchKernelGGL(k_scatter_rows, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, dst,
+                           (int32_t)rows, N, c->d_id);
+        LMH_HIP(c, hipGetLastError());
+    } else {
+        LMH_HIP(c, hipMemcpyAsync(dst, c->d_state, (size_t)rows * N * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    return LMH_OK;
+}
 
 }  // namespace
 
@@ -2902,13 +2980,23 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
                 return c->fail(LMH_E_NONFINITE, "Breaking simulation due to NaN/Inf at step " + std::to_string((long long)bad));
         }
     } else {
+        // lmh_run's steps (cell-sorted as it sorts, one step per launch through the medium slices),
+        // a checkpoint in the caller's order before every segment, one finiteness check at the end
         if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
         SliceTuning sliced(c);
+        lmh_params pseg = *p;
+        pseg.check_finite = 0;
         for (int64_t cp = 0; cp < ncp; ++cp) {
-            LMH_HIP(c, hipMemcpyAsync(c->d_ckpt + per * (size_t)cp, c->d_state, per * sizeof(double),
-                                      hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = take_checkpoint(c, c->d_ckpt + per * (size_t)cp))) return rc;
             const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
-            if ((rc = run_impl(c, p, step_begin + cp * K, n, nullptr, 1))) return rc;
+            if ((rc = run_impl(c, &pseg, step_begin + cp * K, n, nullptr, 1))) return rc;
+        }
+        if (p->check_finite) {
+            unsigned long long bad = ULLONG_MAX;
+            LMH_HIP(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+            LMH_HIP(c, hipStreamSynchronize(c->stream));
+            if (bad != ULLONG_MAX)
+                return c->fail(LMH_E_NONFINITE, "Breaking simulation due to NaN/Inf at step " + std::to_string((long long)bad));
         }
     }
     c->adj.p = *p;
@@ -3014,7 +3102,8 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
             if (cp + 2 <= ncp - 1) LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[buf], 0));
             hipLaunchKernelGGL(k_grid_slices, dim3(grid_blocks(3 * ncell * nk)), dim3(kBlock), 0, c->tape_stream,
                                g, sl, step0, nk, p.dt, p.t0);
-            launch_adj(c, true, p, l);
+            adj_tape_steps(c, p, a, nb, g, sl, step0, nk, l.ckpt, l.tp, c->tape_stream);
+            l.row0 = l.ckpt;
             LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
             // reverse segment cp on the main stream
             LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
@@ -3191,6 +3280,8 @@ BatchArgs batch_args(lmh_batch* b, const lmh_params* p) {
     for (int d = 0; d < 3; ++d) {
         a.base.pos[d] = d < 2 ? b->d_state + (int64_t)d * b->N : nullptr;
         a.base.mom[d] = d < b->pdim ? b->d_state + (int64_t)(2 + d) * b->N : nullptr;
+        a.base.wpos[d] = a.base.pos[d];
+        a.base.wmom[d] = a.base.mom[d];
     }
     a.base.N = b->N;
     a.base.p = *p;

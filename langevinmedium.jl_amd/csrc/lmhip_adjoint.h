@@ -36,6 +36,21 @@ __device__ __forceinline__ double eval_time_table_slope(const TimeTable& t, doub
     return (t.vals[i] - t.vals[i - 1]) * t.invdT;
 }
 
+// eval_time_table and eval_time_table_slope from one read of the segment's two nodes
+__device__ __forceinline__ double eval_time_table_and_slope(const TimeTable& t, double T, double* slope) {
+    double u = (T - t.Tmin) * t.invdT;
+    const double fc = fmin(fmax(floor(u), 0.0), (double)(t.n - 2));
+    const bool one = t.n < 2;
+    const int i = one ? 0 : (int)fc + 1;
+    double w = u - fc;
+    const double y0 = ld_off(t.vals, (uint32_t)(one ? 0 : i - 1) * 8u);
+    const double y1 = ld_off(t.vals, (uint32_t)i * 8u);
+    const bool fin = isfinite(T);
+    *slope = (one || !fin) ? 0.0 : (y1 - y0) * t.invdT;
+    const double r = (1.0 - w) * y0 + w * y1;
+    return one ? y0 : (fin ? r : 0.0);
+}
+
 // d/dT of juttner_lookup at fixed quantile u (0 where the T fraction is clamped)
 __device__ __forceinline__ double juttner_lookup_dT(const JuttnerTable& jt, double u, double T) {
     double uu = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
@@ -66,15 +81,32 @@ struct SampleGrad {
     double dTx, dTy, dvxx, dvxy, dvyx, dvyy;
 };
 
-__device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, SampleGrad* s) {
+// in two parts, so that independent work can be placed between the gather and its use
+struct SampleCell {
+    double4 cT, cx, cy;
+    double xd, yd;
+};
+
+__device__ __forceinline__ SampleCell grid_sample_cell(const GridMedium& g, double x, double y) {
+    SampleCell c;
     double xc = fmin(fmax(x, g.x0), g.xlast);
     double yc = fmin(fmax(y, g.y0), g.ylast);
     double fx = (xc - g.x0) * g.inv_dx;
     double fy = (yc - g.y0) * g.inv_dy;
     int ix = grid_cell(fx, g.nx), iy = grid_cell(fy, g.ny);
-    double xd = fx - (double)ix, yd = fy - (double)iy;
+    c.xd = fx - (double)ix;
+    c.yd = fy - (double)iy;
     const double4* cell = g.slice + 3 * (ix + (int64_t)(g.nx - 1) * iy);
-    const double4 cT = cell[0], cx = cell[1], cy = cell[2];
+    c.cT = cell[0];
+    c.cx = cell[1];
+    c.cy = cell[2];
+    return c;
+}
+
+__device__ __forceinline__ void grid_sample_combine(const GridMedium& g, double x, double y, const SampleCell& c,
+                                                    SampleGrad* s) {
+    const double xd = c.xd, yd = c.yd;
+    const double4 cT = c.cT, cx = c.cx, cy = c.cy;
     s->T = fma(fma(cT.w, xd, cT.z), yd, fma(cT.y, xd, cT.x));
     s->vx = fma(fma(cx.w, xd, cx.z), yd, fma(cx.y, xd, cx.x));
     s->vy = fma(fma(cy.w, xd, cy.z), yd, fma(cy.y, xd, cy.x));
@@ -86,6 +118,10 @@ __device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, 
     s->dTy = fma(cT.w, xd, cT.z) * gy;
     s->dvxy = fma(cx.w, xd, cx.z) * gy;
     s->dvyy = fma(cy.w, xd, cy.z) * gy;
+}
+
+__device__ __forceinline__ void grid_sample_grad(const GridMedium& g, double x, double y, SampleGrad* s) {
+    grid_sample_combine(g, x, y, grid_sample_cell(g, x, y), s);
 }
 
 // One forward step, exactly the fused k_cart_grid step (non-deferred, Philox, no position
@@ -143,15 +179,23 @@ template <int PDIM, int COLL>
 __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_params& p,
                                                  const GridMedium& g, const SpeciesArgs& sp,
                                                  int64_t i, int64_t step, double x0, double x1,
-                                                 const double* qin, const double* xi, double ucol,
-                                                 double usamp_p, double dtau_dDsT, double* lam, double* th) {
+                                                 const double* qin, double dtau_dDsT, double* lam, double* th) {
     constexpr bool HAS_RTA = COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m, m2 = sp.m * sp.m;
     const double dt = p.dt;
+    const double idt = drv_rcp(dt);
 
-    // ---- forward values
+    // ---- forward values: the medium cell is requested first; the step's noise, drawn again from
+    // its Philox counters (common random numbers), is computed while the cell is in flight (an
+    // empty asm that reads the noise keeps the compiler from sinking it to its uses)
+    const SampleCell cell = grid_sample_cell(g, x0, x1);
+    double xi[PDIM];
+    double ucol = 1.0, usamp_p = 0.0;
+    noise_kick<PDIM, LMH_NOISE_PHILOX, 1, COLL == LMH_LANGEVIN_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp_p);
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) asm volatile("" ::"v"(xi[d]));
     SampleGrad s;
-    grid_sample_grad(g, x0, x1, &s);
+    grid_sample_combine(g, x0, x1, cell, &s);
     const double vm2 = s.vx * s.vx + s.vy * s.vy;
     const double ivm0 = rsqrt_pos(fmax(vm2, 1e-300));
     const double vmag = vm2 * ivm0;
@@ -190,14 +234,15 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
         kT = sqrt_pos(kappa);
     } else {
         const double DsT_safe = p.DsT <= 0.0 ? 0.0 : p.DsT;
-        tau = DsT_safe == 0.0 ? 0.0 : eval_time_table(sp.drag, T);
+        double slope = 0.0;
+        tau = DsT_safe == 0.0 ? 0.0 : eval_time_table_and_slope(sp.drag, T, &slope);
         tab_ok = tau > 0.0 && isfinite(tau);
         if (tab_ok) {
             itau = drv_rcp(tau);
             etaD = itau;
             kappa = 2.0 * m * T * itau;
             kT = sqrt_pos(kappa);
-            dtau_dT = eval_time_table_slope(sp.drag, T);
+            dtau_dT = slope;
         }
     }
     const double ikT = kT > 0.0 ? drv_rcp(kT) : 0.0;
@@ -217,9 +262,9 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
             iE2 = drv_rsq(q2B + m2);
             eta = etaD * m * iE2;
         }
-        aa = exp(-eta * dt);
+        aa = exp_neg(eta * dt);
         ieta = drv_rcp(eta);
-        rr = (1.0 - aa * aa) * (0.5 * ieta / dt);
+        rr = (1.0 - aa * aa) * (0.5 * ieta * idt);
         isrr = rr > 0.0 ? drv_rsq(rr) : 0.0;
         srr = rr * isrr;
 #pragma unroll
@@ -322,7 +367,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
             if (rr > 0.0) {
                 kTb += npb * srr;
                 const double rrb = npb * kT * 0.5 * isrr;
-                ab += -rrb * aa * ieta / dt;
+                ab += -rrb * aa * ieta * idt;
                 etab += -rrb * rr * ieta;
             }
             etab += -ab * dt * aa;
@@ -508,6 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
                                                         const double4* __restrict__ slices,
                                                         int64_t seg_step0,
                                                         int32_t nk, const double* __restrict__ tape,
+                                                        const double* __restrict__ row0,
                                                         double* __restrict__ lam,
                                                         double* __restrict__ gacc, double dtau_dDsT) {
     int seg;
@@ -522,10 +568,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
 #pragma unroll
     for (int r = 0; r < R; ++r) lm[r] = lam[(int64_t)r * N + i];
     double th[kAdjThetas] = {0.0, 0.0, 0.0};
-    // the tape row of step k - 1 is loaded while step k is reversed
+    // the rows of step k: the tape's, or for k = 0 the checkpoint's when row0 is given (the tape
+    // recomputation then starts from it and leaves tape row 0 unwritten); the row of step k - 1 is
+    // loaded while step k is reversed
+    auto row_of = [&](int k) -> const double* {
+        return (k == 0 && row0) ? row0 + i : tape + (int64_t)k * TR * N + i;
+    };
     double row[TR];
     {
-        const double* tk = tape + (int64_t)(nk - 1) * TR * N + i;
+        const double* tk = row_of(nk - 1);
 #pragma unroll
         for (int r = 0; r < TR; ++r) row[r] = tk[(int64_t)r * N];
     }
@@ -535,18 +586,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
 #pragma unroll
         for (int r = 0; r < TR; ++r) cur[r] = row[r];
         if (k > 0) {
-            const double* tn = tape + (int64_t)(k - 1) * TR * N + i;
+            const double* tn = row_of(k - 1);
 #pragma unroll
             for (int r = 0; r < TR; ++r) row[r] = tn[(int64_t)r * N];
         }
-        // the step's noise, drawn again from its Philox counters (common random numbers)
-        double xi[PDIM];
-        double ucol = 1.0, usamp = 0.0;
-        noise_kick<PDIM, LMH_NOISE_PHILOX, 1, COLL == LMH_LANGEVIN_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp);
         GridMedium gk = g;
         gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
-        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, xi, ucol, usamp,
-                                     dtau_dDsT, lm, th);
+        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, dtau_dDsT, lm, th);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) lam[(int64_t)r * N + i] = lm[r];
