@@ -8,6 +8,13 @@
 * 256 batched runs on grids of that shape (D_s(2 pi T), T0, eccentricity and a log-uniform
   particle count drawn per run, the counts cut to 10^3..10^4 so the check runs in seconds):
   every run bit for bit the same run alone, and the largest run against the oracle at 1e-12.
+* The differentiation workload's shape (10^6 charm quarks x 1000 steps of dt = 0.01 fm/c on a
+  200 x 200 x 120 grid, checkpoints every 50 steps; the <p_T>, smoothed 40-bin spectrum and
+  <cos 2 phi> losses; d/dD_sT with the drag table, d/dgamma0 and d/dLambda with gamma(p, T)) on
+  a medium whose fields are affine in (x, y, tau): its interpolation has no kinks at cell
+  boundaries, so central differences at the stated relative step 1e-4 (and 5e-5, combined by
+  Richardson's rule to cancel their O(h^2) error) are the smooth pathwise derivative, and the
+  adjoint must agree with them to 1e-4 relative.
 """
 import math
 
@@ -147,3 +154,62 @@ def test_batch_256_runs_full_grid_shape(gpu_ok):
     assert rc == 0, (rc, bad)
     sl = slice(begin[r], begin[r + 1])
     _report(f"batch run {r} ({counts[r]} quarks)", bpos[:, sl], bmom[:, sl], opos, omom)
+
+
+def _affine_grid():
+    """T = 0.30 + 0.004 x - 0.003 y - 0.008 (tau - 0.6) GeV (0.105 .. 0.405 over the grid), flow
+    (0.2 + 0.004 x, 0.1 + 0.003 y): trilinear interpolation reproduces it exactly, |v| is nowhere
+    near 0 (where the boost direction would be singular), T nowhere near 0"""
+    x = np.linspace(-15.0, 15.0, 200)
+    y = np.linspace(-15.0, 15.0, 200)
+    tg = np.linspace(0.6, 12.0, 120)
+    X, Y, TAU = np.meshgrid(x, y, tg, indexing="ij")
+    T = np.asfortranarray(0.30 + 0.004 * X - 0.003 * Y - 0.008 * (TAU - 0.6))
+    vx = np.asfortranarray(0.2 + 0.004 * X + 0.0 * TAU)
+    vy = np.asfortranarray(0.1 + 0.003 * Y + 0.0 * TAU)
+    return media.Grid2p1Medium(float(x[0]), float(x[1] - x[0]), 200, float(y[0]), float(y[1] - y[0]), 200,
+                               tg, T, vx, vy)
+
+
+def test_adjoint_full_shape_fd_h1e4(gpu_ok):
+    from lmhip.adjoint import GridEnsemble
+    N, steps, K = 1_000_000, 1000, 50
+    g = _affine_grid()
+    rng = np.random.default_rng(12)
+    # |x|, |y| <= 3 fm: in 10 fm/c no quark reaches the grid's edge (no position clamp)
+    pos = np.asfortranarray(rng.uniform(-3.0, 3.0, (2, N)))
+    mom = media.pt_rapidity_momenta(N, 1.5, rng)
+    obs_kw = dict(nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.25)
+    w = {"mean_pT": np.zeros(45), "cos2phi": np.zeros(45), "spectrum": np.zeros(45)}
+    w["mean_pT"][1] = 1.0 / N
+    w["cos2phi"][3] = 1.0 / N
+    w["spectrum"][5:] = np.random.default_rng(6).normal(size=40) / N
+    print(f"adjoint full shape: N={N} steps={steps} K={K} on the GPU", flush=True)
+    with lmhip.Context(0) as ctx:
+        for drag_model, theta, ks in ((_ffi.LMH_DRAG_TABLE, np.array([media.DsT_from_2piT(4.0), 0.0, 5.0]), (0,)),
+                                      (_ffi.LMH_DRAG_GAMMA_P, np.array([media.DsT_from_2piT(4.0), 5.3, 5.0]), (1, 2))):
+            ens = GridEnsemble(ctx, g, [1.5], [N], pos, mom, dt=0.01, t0=0.6, nsteps=steps,
+                               drag_model=drag_model, collision=_ffi.LMH_LANGEVIN, seed=3,
+                               checkpoint_every=K, **obs_kw)
+            ens.forward(theta)
+            grads = {name: ens.pullback(wl) for name, wl in w.items()}
+            for k in ks:
+                # central differences at h = 1e-4 theta_k and h/2, and their Richardson combination
+                # (4 D(h/2) - D(h))/3, which cancels the O(h^2) term: <cos 2 phi> has a large
+                # third derivative from quarks at small p_T, where the angle turns fastest
+                fds = []
+                for hr in (1e-4, 5e-5):
+                    h = hr * theta[k]
+                    tp, tm = theta.copy(), theta.copy()
+                    tp[k] += h
+                    tm[k] -= h
+                    fds.append((ens.run_plain(tp) - ens.run_plain(tm)) / (2 * h))
+                for name, wl in w.items():
+                    d1, d2 = float(fds[0] @ wl), float(fds[1] @ wl)
+                    fd = (4.0 * d2 - d1) / 3.0
+                    adj = float(grads[name][k])
+                    err = abs(adj - fd) / abs(fd)
+                    print(f"  model {drag_model} theta[{k}] {name}: adjoint {adj:.9e} central difference "
+                          f"h=1e-4 {d1:.9e} (rel {abs(adj - d1) / abs(d1):.1e}), Richardson {fd:.9e} "
+                          f"(rel {err:.1e})", flush=True)
+                    assert err <= 1e-4, (drag_model, k, name, adj, fd)
