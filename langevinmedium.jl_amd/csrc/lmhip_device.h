@@ -467,16 +467,18 @@ __device__ __forceinline__ void drag_coeffs(const lmh_params& p, const TimeTable
 // effective drag at momentum q -- eta_D*m/E (relativistic) or eta_D (kernels_cpu.jl:334-339),
 // or eta_D/(1+|q|^2/Lambda^2) for GAMMA_P -- and c = kappa/(2 eta_eff) = m T eta_D/eta_eff:
 // T E (relativistic), m T, or m T (1+|q|^2/Lambda^2) (GAMMA_P)
-template <int PDIM>
+// FAST: the GAMMA_P denominator's two divisions as products with refined reciprocals (an ulp or
+// two from the divisions)
+template <int PDIM, bool FAST = false>
 __device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, double m, double T,
                                          const double* q, double* c) {
     double q2 = q[0] * q[0];
 #pragma unroll
     for (int d = 1; d < PDIM; ++d) q2 = fma(q[d], q[d], q2);
     if (p.drag_model == LMH_DRAG_GAMMA_P) {
-        const double den = 1.0 + q2 / (p.Lambda * p.Lambda);
+        const double den = FAST ? fma(q2, rcp_pos(p.Lambda * p.Lambda), 1.0) : 1.0 + q2 / (p.Lambda * p.Lambda);
         *c = (m * T) * den;
-        return etaD / den;
+        return FAST ? etaD * rcp_pos(den) : etaD / den;
     }
     if (p.relativistic) {
         const double E2 = fma(m, m, q2);
@@ -497,7 +499,7 @@ __device__ __forceinline__ double eta_of(const lmh_params& p, double etaD, doubl
 // The exact-OU noise sqrt(dt)*sto = sqrt(dt) kT sqrt((1 - a^2)/(2 eta_eff dt)) xi (:340-349) is
 // written as sqrt((1 - a^2) kappa/(2 eta_eff)) xi = sqrt((1 - a^2) c) xi with c from eta_of: the
 // same quantity without the divisions (within a few ulps of the reference's rounding).
-template <int PDIM, bool RADIAL>
+template <int PDIM, bool RADIAL, bool FAST = false>
 __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double kT, double T,
                                         double m, double sqrt_dt, double* q, const double* xi) {
     const double dt = p.dt;
@@ -528,7 +530,7 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) qk[d] = q[d];
         for (int it = 0; it < 3; ++it) {
-            const double eta_eff = eta_of<PDIM>(p, etaD, m, T, qk, &c);
+            const double eta_eff = eta_of<PDIM, FAST>(p, etaD, m, T, qk, &c);
             const double a = exp_neg(eta_eff * dt);
             const double s = sqrt_pos(fma(-a, a, 1.0) * c);
 #pragma unroll
@@ -538,13 +540,13 @@ __device__ __forceinline__ void ou_kick(const lmh_params& p, double etaD, double
         for (int d = 0; d < PDIM; ++d) q[d] = qk[d];
         return;
     }
-    double eta_eff = eta_of<PDIM>(p, etaD, m, T, q, &c);
+    double eta_eff = eta_of<PDIM, FAST>(p, etaD, m, T, q, &c);
     if (p.integrator == LMH_MIDPOINT && dt > 0.0) {
         const double a_half = exp_neg(eta_eff * (0.5 * dt));
         double qm[PDIM];
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) qm[d] = a_half * q[d];
-        eta_eff = eta_of<PDIM>(p, etaD, m, T, qm, &c);
+        eta_eff = eta_of<PDIM, FAST>(p, etaD, m, T, qm, &c);
     }
     const double a = exp_neg(eta_eff * dt);
     const double s = dt > 0.0 ? sqrt_pos(fma(-a, a, 1.0) * c) : 0.0;
