@@ -218,7 +218,7 @@ def bench_bjorken(args):
                    "mean_abs_eta_s": round(float(np.mean(np.abs(gpos[2]))), 6)},
         "roofline": {"bound": "hbm", "achieved": round(bytes_per_launch / t_launch / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(bytes_per_launch / t_launch / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_cart_radial<3,3> (Milne, post-point, GAMMA_P)",
+                     "traffic": None, "kernel": "k_milne (post-point, GAMMA_P, Philox)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(t_launch * 1e3, 4)},
         "cpu_baseline": cpu,
     }
