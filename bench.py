@@ -230,7 +230,8 @@ def bench_batch(args):
     GeV, a charm count 10^U[3, 6] and a profile eccentricity ~ U[0, 0.4] (seed 4), on its own
     200 x 200 x 120 grid; Langevin + RTA as in the headline workload.  The drag tables are per run,
     the RTA and Juttner tables are shared over the common temperature range.  The launch shape is
-    chosen by the library's autotuner, then `--steps` steps run as replays of a captured graph."""
+    chosen by the library's autotuner (block, steps per launch, persistent launch and the cadence of
+    the runs' device cell sorts), then `--steps` steps run as replays of a captured graph."""
     import numpy as np
     import torch
     from lmhip import _ffi, media, transport
@@ -272,7 +273,7 @@ def bench_batch(args):
     b.boost_to_lab(params, 0)
     tune, tuned_ms = b.autotune(params, trial_steps=3)
     b.set_tuning(block=tune["block"], steps_per_launch=tune["steps_per_launch"],
-                 persistent=tune["persistent"], graph_steps=10)
+                 persistent=tune["persistent"], graph_steps=10, sort_every=tune["sort_every"])
     setup = time.perf_counter() - t_setup
     print(f"batch setup {setup:.1f} s; {R} runs, {N} quarks, tuning {tune} ({tuned_ms:.3f} ms/step)",
           file=sys.stderr, flush=True)

@@ -222,6 +222,10 @@ typedef struct lmh_batch_tuning {
     int32_t persistent;        /* 1: a fixed grid of blocks walks the tiles; 0: a block a tile */
     int32_t graph_steps;       /* > 0: lmh_batch_run replays a captured HIP graph of this many
                                   steps (the step index advances on the device); 0: direct    */
+    int32_t sort_every;        /* > 0: lmh_batch_run keeps each run's quarks ordered by grid
+                                  cell, sorting them on the device before its first step and
+                                  then every sort_every steps (rounded up to whole graph
+                                  replays); the results do not depend on it; 0: never        */
 } lmh_batch_tuning;
 
 typedef struct lmh_ctx lmh_ctx;
@@ -414,8 +418,9 @@ int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step);
 /* steps step_begin .. step_begin+nsteps-1 of every run; p->DsT is ignored (each run's own)  */
 int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps);
 /* times trial_steps steps of every launch shape (block 64/128/256 x persistent 0/1 x
- * steps_per_launch 1/2/4) on this batch, keeps the fastest (graph_steps unchanged) and
- * restores the state; the results do not depend on the shape.  Synchronises.              */
+ * steps_per_launch 1/2/4) on this batch, then 200 steps at the fastest shape for each sort
+ * cadence (sort_every 0, 50, 200), keeps the fastest (graph_steps unchanged) and restores the
+ * state; the results do not depend on the shape or the cadence.  Synchronises.             */
 int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps,
                        lmh_batch_tuning* best, double* best_ms_per_step);
 /* out (nruns, 5 + nbins): lmh_observables of every run (spec->species is ignored)           */

@@ -3346,7 +3346,20 @@ struct lmh_batch {
     int64_t ntiles = 0;
     int tiles_block = 0;              // block size the tile table was built for
 
-    lmh_batch_tuning tune = {256, 1, 0, 0};
+    lmh_batch_tuning tune = {256, 1, 0, 0, 0};
+
+    // cell order (tune.sort_every > 0): each run's quarks ordered by grid cell within the run's
+    // segment; d_id[s] is the original slot of the quark in slot s (its Philox counter is d_id[s]
+    // minus the run's first slot).  A sort permutes into the alternate buffers and copies back,
+    // so the state's address, which a captured graph holds, never changes.
+    uint32_t* d_id = nullptr;
+    uint32_t* d_id_alt = nullptr;
+    double* d_state_alt = nullptr;
+    uint32_t* d_keys = nullptr;       // 4 N words: keys, sorted keys, values, sorted values
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    bool is_sorted = false;
+    int64_t steps_since_sort = 0;
 
     // captured step loop: valid for `gkey` (parameters, tuning, steps per replay)
     hipGraph_t graph = nullptr;
@@ -3380,6 +3393,93 @@ void batch_drop_graph(lmh_batch* b) {
     b->gexec = nullptr;
     b->graph = nullptr;
     b->gkey.clear();
+}
+
+void batch_free_order(lmh_batch* b) {
+    dfree(b->d_id);
+    dfree(b->d_id_alt);
+    dfree(b->d_state_alt);
+    dfree(b->d_keys);
+    dfree(b->d_sort_tmp);
+    b->d_id = b->d_id_alt = nullptr;
+    b->d_state_alt = nullptr;
+    b->d_keys = nullptr;
+    b->d_sort_tmp = nullptr;
+    b->sort_tmp_bytes = 0;
+    b->is_sorted = false;
+    b->steps_since_sort = 0;
+}
+
+// the buffers of the cell order (alternate state and identities, keys and values, the radix
+// sort's workspace), allocated at the first sort
+int batch_sort_alloc(lmh_batch* b) {
+    if (b->d_keys) return LMH_OK;
+    const int64_t N = b->N;
+    const size_t rows = (size_t)(2 + b->pdim);
+    if (hipMalloc((void**)&b->d_id_alt, (size_t)N * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_state_alt, rows * (size_t)N * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&b->d_keys, 4 * (size_t)N * sizeof(uint32_t)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch cell sort: device allocation failed");
+    size_t bytes = 0;
+    uint32_t* k = b->d_keys;
+    LMB_HIP(b, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, k + N, k + 2 * N, k + 3 * N, (int)N, 0, 32,
+                                                   b->stream));
+    if (hipMalloc(&b->d_sort_tmp, std::max<size_t>(bytes, 1)) != hipSuccess)
+        return b->fail(LMH_E_NOMEM, "batch cell sort: device allocation failed");
+    b->sort_tmp_bytes = bytes;
+    return LMH_OK;
+}
+
+// sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit), permute the
+// state and the identities through the sorted values into the alternate buffers and copy them back
+int batch_sort_apply(lmh_batch* b, int end_bit) {
+    const int64_t N = b->N;
+    uint32_t* k = b->d_keys;
+    size_t bytes = b->sort_tmp_bytes;
+    LMB_HIP(b, hipcub::DeviceRadixSort::SortPairs(b->d_sort_tmp, bytes, k, k + N, k + 2 * N, k + 3 * N, (int)N, 0,
+                                                   end_bit, b->stream));
+    const int rows = 2 + b->pdim;
+    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_state, b->d_state_alt,
+                       (int32_t)rows, N, k + 3 * N, b->d_id, b->d_id_alt);
+    LMB_HIP(b, hipGetLastError());
+    LMB_HIP(b, hipMemcpyAsync(b->d_state, b->d_state_alt, (size_t)rows * N * sizeof(double), hipMemcpyDeviceToDevice,
+                              b->stream));
+    LMB_HIP(b, hipMemcpyAsync(b->d_id, b->d_id_alt, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, b->stream));
+    return LMH_OK;
+}
+
+BatchArgs batch_args(lmh_batch* b, const lmh_params* p);
+
+// order every run's quarks by grid cell (within the run's segment)
+int batch_sort(lmh_batch* b, const lmh_params* p) {
+    int rc = batch_sort_alloc(b);
+    if (rc) return rc;
+    int cell_bits = 1;
+    for (const auto& m : b->media) cell_bits = std::max(cell_bits, bits_for((uint64_t)(m.nx - 1) * (uint64_t)(m.ny - 1)));
+    const int end_bit = cell_bits + bits_for((uint64_t)b->runs.size());
+    if (end_bit > 32) return b->fail(LMH_E_INVALID, "batch cell sort: too many runs or grid cells");
+    const BatchArgs a = batch_args(b, p);
+    const unsigned G = (unsigned)std::min<int64_t>(b->ntiles, 65535);
+    hipLaunchKernelGGL(k_batch_sort_keys, dim3(G), dim3(b->tune.block), 0, b->stream, a, (int32_t)cell_bits,
+                       b->d_keys, b->d_keys + 2 * b->N);
+    LMB_HIP(b, hipGetLastError());
+    if ((rc = batch_sort_apply(b, end_bit))) return rc;
+    b->is_sorted = true;
+    b->steps_since_sort = 0;
+    return LMH_OK;
+}
+
+// restore the caller's order (a no-op unless the runs are cell-sorted)
+int batch_ensure_unsorted(lmh_batch* b) {
+    if (!b->is_sorted) return LMH_OK;
+    const int64_t N = b->N;
+    hipLaunchKernelGGL(k_unsort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_id, N, b->d_keys,
+                       b->d_keys + 2 * N);
+    LMB_HIP(b, hipGetLastError());
+    int rc = batch_sort_apply(b, bits_for((uint64_t)N));
+    if (rc) return rc;
+    b->is_sorted = false;
+    return LMH_OK;
 }
 
 int batch_dev(lmh_batch* b, double** ptr, size_t* cap, size_t n, const char* what) {
@@ -3466,6 +3566,7 @@ BatchArgs batch_args(lmh_batch* b, const lmh_params* p) {
     a.base.N = b->N;
     a.base.p = *p;
     a.base.bad = b->d_bad;
+    a.base.id = b->d_id;
     a.base.nseg = 1;
     a.base.sqrt_dt = std::sqrt(p->dt);
     a.base.C = noise_layout(p, 2, b->pdim, &a.base.oxi, &a.base.orta, &a.base.opos);
@@ -3543,13 +3644,9 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a, int coll, int64_t nsteps) {
     hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->d_step, nsteps);
 }
 
-int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps) {
-    int rc = batch_layout(b, b->tune.block);
-    if (rc) return rc;
-    if (b->ntiles == 0 || nsteps == 0) return LMH_OK;
-    const BatchArgs a = batch_args(b, p);
-    LMB_HIP(b, hipMemcpyAsync(b->d_step, &step_begin, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
-    LMB_HIP(b, hipStreamSynchronize(b->stream));
+// the steps through captured graph replays of graph_steps steps (the graph built for the
+// current parameters and tuning), the rest as direct launches
+int batch_steps(lmh_batch* b, const lmh_params* p, const BatchArgs& a, int64_t nsteps) {
     const int64_t S = b->tune.graph_steps;
     int64_t done = 0;
     if (S > 0 && nsteps >= S) {
@@ -3576,6 +3673,34 @@ int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_
     }
     if (done < nsteps) batch_enqueue(b, a, p->collision, nsteps - done);
     LMB_HIP(b, hipGetLastError());
+    return LMH_OK;
+}
+
+int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps) {
+    int rc = batch_layout(b, b->tune.block);
+    if (rc) return rc;
+    if (b->ntiles == 0 || nsteps == 0) return LMH_OK;
+    const BatchArgs a = batch_args(b, p);
+    LMB_HIP(b, hipMemcpyAsync(b->d_step, &step_begin, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    LMB_HIP(b, hipStreamSynchronize(b->stream));
+    // cell order: a sort before the first step and whenever sort_every steps have passed, checked
+    // between chunks of whole graph replays
+    const int64_t E = b->tune.sort_every;
+    const bool sorting = E > 0 && b->d_id != nullptr;
+    if (!sorting && (rc = batch_ensure_unsorted(b))) return rc;
+    const int64_t S = std::max<int64_t>(1, b->tune.graph_steps);
+    for (int64_t done = 0; done < nsteps;) {
+        int64_t chunk = nsteps - done;
+        if (sorting) {
+            if (!b->is_sorted || b->steps_since_sort >= E) {
+                if ((rc = batch_sort(b, p))) return rc;
+            }
+            chunk = std::min<int64_t>(chunk, ((E - b->steps_since_sort + S - 1) / S) * S);
+        }
+        if ((rc = batch_steps(b, p, a, chunk))) return rc;
+        done += chunk;
+        b->steps_since_sort += chunk;
+    }
     if (p->check_finite) {
         unsigned long long bad = ULLONG_MAX;
         LMB_HIP(b, hipMemcpyAsync(&bad, b->d_bad, sizeof(bad), hipMemcpyDeviceToHost, b->stream));
@@ -3635,6 +3760,7 @@ void lmh_batch_destroy(lmh_batch* b) {
     dfree(b->d_tile_run);
     dfree(b->d_obs);
     dfree(b->d_save);
+    batch_free_order(b);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -3753,6 +3879,7 @@ int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, co
     dfree(b->d_begin);
     dfree(b->d_runs);
     dfree(b->d_tile_run);
+    batch_free_order(b);
     b->tiles_block = 0;
     b->runs.assign(runs, runs + nruns);
     b->begin.assign(nruns + 1, 0);
@@ -3764,6 +3891,13 @@ int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, co
         return b->fail(LMH_E_NOMEM, "batch_set_runs: device allocation failed");
     LMB_HIP(b, hipMemcpy(b->d_state, soa.data(), rows * (size_t)N * sizeof(double), hipMemcpyHostToDevice));
     LMB_HIP(b, hipMemcpy(b->d_begin, b->begin.data(), (size_t)(nruns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    // identities: every quark in its own slot
+    if (N < ((int64_t)1 << 32)) {
+        if (hipMalloc((void**)&b->d_id, (size_t)N * sizeof(uint32_t)) != hipSuccess)
+            return b->fail(LMH_E_NOMEM, "batch_set_runs: device allocation failed");
+        hipLaunchKernelGGL(k_iota, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_id, N);
+        LMB_HIP(b, hipGetLastError());
+    }
     unsigned long long none = ULLONG_MAX;
     LMB_HIP(b, hipMemcpy(b->d_bad, &none, sizeof(none), hipMemcpyHostToDevice));
     return batch_layout(b, b->tune.block);
@@ -3774,6 +3908,7 @@ int lmh_batch_get_particles(lmh_batch* b, double* pos, double* mom) {
     if (!b->d_state) return b->fail(LMH_E_STATE, "batch_get_particles: no runs set");
     if (!pos || !mom) return b->fail(LMH_E_INVALID, "batch_get_particles: null output");
     LMB_HIP(b, hipSetDevice(b->device));
+    if (int rc = batch_ensure_unsorted(b)) return rc;
     const size_t rows = (size_t)(2 + b->pdim);
     const int64_t N = b->N;
     std::vector<double> soa;
@@ -3794,9 +3929,9 @@ int lmh_batch_get_particles(lmh_batch* b, double* pos, double* mom) {
 int lmh_batch_set_tuning(lmh_batch* b, const lmh_batch_tuning* t) {
     if (!b) return LMH_E_INVALID;
     if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
-        (t->persistent != 0 && t->persistent != 1) || t->graph_steps < 0)
+        (t->persistent != 0 && t->persistent != 1) || t->graph_steps < 0 || t->sort_every < 0)
         return b->fail(LMH_E_INVALID, "batch_set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, "
-                                      "persistent in {0, 1}, graph_steps >= 0");
+                                      "persistent in {0, 1}, graph_steps >= 0, sort_every >= 0");
     b->tune = *t;
     return LMH_OK;
 }
@@ -3841,6 +3976,7 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     if (rc) return rc;
     if (trial_steps < 1) return b->fail(LMH_E_INVALID, "batch_autotune: trial_steps >= 1");
     LMB_HIP(b, hipSetDevice(b->device));
+    if ((rc = batch_ensure_unsorted(b))) return rc;
     const size_t n = (size_t)(2 + b->pdim) * (size_t)b->N;
     if ((rc = batch_dev(b, &b->d_save, &b->save_cap, n, "batch_autotune"))) return rc;
     LMB_HIP(b, hipMemcpyAsync(b->d_save, b->d_state, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
@@ -3857,7 +3993,7 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     for (int bi = 0; bi < 3 && rc == LMH_OK; ++bi)
         for (int pe = 0; pe < 2 && rc == LMH_OK; ++pe)
             for (int si = 0; si < 3 && rc == LMH_OK; ++si) {
-                b->tune = lmh_batch_tuning{blocks[bi], spls[si], pe, 0};
+                b->tune = lmh_batch_tuning{blocks[bi], spls[si], pe, 0, 0};
                 // one untimed launch for the code object and caches, then the timed steps
                 if ((rc = batch_run_impl(b, &q, 1, spls[si]))) break;
                 if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = b->fail(LMH_E_HIP, "batch_autotune: event"); break; }
@@ -3878,6 +4014,44 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
                     break;
                 }
             }
+    // the sort cadence at the fastest shape: kTuneSortSteps steps from the same state for each
+    // cadence, the sorts included (the first before the first step)
+    constexpr int64_t kTuneSortSteps = 200;
+    const int cadences[3] = {0, 50, 200};
+    double pick_sort_ms = 0.0;
+    int pick_sort = 0;
+    for (int ci = 0; ci < 3 && rc == LMH_OK && b->d_id; ++ci) {
+        b->tune = pick;
+        b->tune.graph_steps = 0;
+        b->tune.sort_every = cadences[ci];
+        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = b->fail(LMH_E_HIP, "batch_autotune: event"); break; }
+        if ((rc = batch_run_impl(b, &q, 1, kTuneSortSteps))) break;
+        if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = b->fail(LMH_E_HIP, "batch_autotune: event");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = ms / kTuneSortSteps;
+        if (ci == 0 || per < pick_sort_ms) {
+            pick_sort_ms = per;
+            pick_sort = cadences[ci];
+        }
+        // the state as before, every quark in its own slot again
+        if (hipMemcpyAsync(b->d_state, b->d_save, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+            rc = b->fail(LMH_E_HIP, "batch_autotune: restore");
+            break;
+        }
+        hipLaunchKernelGGL(k_iota, dim3(grid_blocks(b->N)), dim3(kBlock), 0, b->stream, b->d_id, b->N);
+        b->is_sorted = false;
+        b->steps_since_sort = 0;
+    }
+    pick.sort_every = pick_sort;
+    if (pick_sort_ms > 0.0) pick_ms = pick_sort_ms;
+    if (b->d_id && b->is_sorted) {
+        hipLaunchKernelGGL(k_iota, dim3(grid_blocks(b->N)), dim3(kBlock), 0, b->stream, b->d_id, b->N);
+        b->is_sorted = false;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // the state as before the trials, the first non-finite step cleared
@@ -3903,6 +4077,8 @@ int lmh_batch_observables(lmh_batch* b, const lmh_obs_spec* s, double* out) {
     if (s->nbins < 0 || s->nbins > kObsMaxBins || (s->nbins > 0 && !(s->pt_max > s->pt_min)))
         return b->fail(LMH_E_INVALID, "batch_observables: 0 <= nbins <= 120 and pt_max > pt_min");
     LMB_HIP(b, hipSetDevice(b->device));
+    // the sums in the caller's order (the same bits as the run alone)
+    if (int rc = batch_ensure_unsorted(b)) return rc;
     const int K = 5 + s->nbins;
     const int R = (int)b->runs.size();
     const int nblk = 16;

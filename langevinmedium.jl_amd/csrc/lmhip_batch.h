@@ -117,9 +117,12 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, 
         double q[PDIM];
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+        // the quark's identity keys its noise: its original slot in the run (the slot itself
+        // unless the run is ordered by grid cell)
+        const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
         int64_t bad = -1;
         for (int s = 0; s < nsteps; ++s) {
-            fused_grid_step<PDIM, COLL>(a, p, run.g, sp, i, step0 + s, &x0, &x1, q);
+            fused_grid_step<PDIM, COLL>(a, p, run.g, sp, pid, step0 + s, &x0, &x1, q);
             if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
         }
         a.pos[0][i] = x0;
@@ -127,6 +130,27 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, 
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
         if (bad >= 0) atomicMin(b.base.bad, (unsigned long long)bad);
+    }
+}
+
+// the cell-order keys of every run: key = run << cell_bits | cell (the cell grid_sample reads on
+// the run's grid), value = slot; a stable sort of them orders each run's quarks by cell and keeps
+// the runs' segments in place
+__global__ __launch_bounds__(kBlock) void k_batch_sort_keys(BatchArgs b, int32_t cell_bits, uint32_t* keys,
+                                                            uint32_t* vals) {
+    for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
+        const int r = __builtin_amdgcn_readfirstlane(b.tile_run[tile]);
+        const BatchRun run = uniform_read(&b.runs[r]);
+        const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
+        if (i >= run.count) continue;
+        const int64_t s = run.begin + i;
+        const GridMedium& g = run.g;
+        const double xc = fmin(fmax(b.base.pos[0][s], g.x0), g.xlast);
+        const double yc = fmin(fmax(b.base.pos[1][s], g.y0), g.ylast);
+        const int ix = grid_cell((xc - g.x0) * g.inv_dx, g.nx);
+        const int iy = grid_cell((yc - g.y0) * g.inv_dy, g.ny);
+        keys[s] = ((uint32_t)r << cell_bits) | (uint32_t)(ix + (g.nx - 1) * iy);
+        vals[s] = (uint32_t)s;
     }
 }
 

@@ -174,6 +174,34 @@ def test_batch_graph_replays_match_direct_launches(gpu_ok):
     assert np.array_equal(got[0], direct[3][0]) and np.array_equal(got[1], direct[3][1])
 
 
+@pytest.mark.parametrize("graph_steps", [0, 3])
+def test_batch_cell_sort_identical(gpu_ok, graph_steps):
+    """each run's quarks ordered by grid cell every 7 steps (rounded up to whole graph replays of
+    3): particles and observables bit for bit the unsorted batch's, also between calls and after
+    the caller's order is restored for a download in the middle"""
+    runs = _runs(seed=13)
+    p = _params(_ffi.LMH_LANGEVIN_RTA)
+    with Batch(0) as b:
+        _load_batch(b, runs)
+        b.run(p, 1, 11)
+        mid = b.get_particles()
+        b.run(p, 12, 19)
+        ref = b.get_particles()
+        ref_obs = b.observables(nbins=12, pt_max=6.0, sigma=0.3)
+    with Batch(0) as b:
+        b.set_tuning(block=128, steps_per_launch=2, graph_steps=graph_steps, sort_every=7)
+        assert b.get_tuning()["sort_every"] == 7
+        _load_batch(b, runs)
+        b.run(p, 1, 11)
+        got_mid = b.get_particles()
+        b.run(p, 12, 19)
+        obs = b.observables(nbins=12, pt_max=6.0, sigma=0.3)
+        got = b.get_particles()
+    assert np.array_equal(got_mid[0], mid[0]) and np.array_equal(got_mid[1], mid[1])
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(obs, ref_obs)
+
+
 def test_batch_autotune_keeps_state_and_results(gpu_ok):
     runs = _runs(seed=5)
     p = _params(_ffi.LMH_LANGEVIN_RTA)
@@ -187,6 +215,7 @@ def test_batch_autotune_keeps_state_and_results(gpu_ok):
         tune, ms = b.autotune(p, trial_steps=2)
         assert tune["block"] in (64, 128, 256) and tune["persistent"] in (0, 1)
         assert tune["steps_per_launch"] in (1, 2, 4) and ms > 0.0
+        assert tune["sort_every"] in (0, 50, 200)
         assert b.get_tuning() == tune
         now = b.get_particles()
         assert np.array_equal(now[0], ref_before[0]) and np.array_equal(now[1], ref_before[1])

@@ -116,8 +116,9 @@ def test_batch_256_runs_full_grid_shape(gpu_ok):
         b.set_runs(desc, np.hstack(pos), np.hstack(mom))
         b.boost_to_lab(p, 0)
         tune, _ = b.autotune(p, trial_steps=3)
+        # the tuned launch shape with a cell sort every 10 steps (several sorts in the run)
         b.set_tuning(block=tune["block"], steps_per_launch=tune["steps_per_launch"],
-                     persistent=tune["persistent"], graph_steps=10)
+                     persistent=tune["persistent"], graph_steps=10, sort_every=10)
         b.run(p, 1, steps)
         bpos, bmom = b.get_particles()
     print(f"batch done (tuning {tune}); the runs alone", flush=True)
