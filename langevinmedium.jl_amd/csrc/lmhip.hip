@@ -1411,18 +1411,22 @@ __device__ __forceinline__ void obs_block(const double* __restrict__ px, const d
     ObsMoments m;
     const int64_t stride = nblk * kObsBlock;
     if (smooth) {
-        // every bin b adds Phi((hi - p_T)/sigma) - Phi((lo - p_T)/sigma); all lanes of the block
-        // take the same rounds (past the end they add 0), so that the shuffles see every lane
+        // every bin b adds Phi((hi - p_T)/sigma) - Phi((lo - p_T)/sigma) between the edges
+        // pt_min + b width and pt_min + (b + 1) width, each edge's erfc serving the two bins it
+        // bounds; all lanes of the block take the same rounds (past the end they add 0), so that
+        // the shuffles see every lane
         const double inv_s = 1.0 / (s.sigma * sqrt(2.0));
         for (int64_t base = i0 + blk * kObsBlock; base < i1; base += stride) {
             const int64_t i = base + t;
             const bool ok = i < i1;
             double p = 0.0;
             if (ok) m.add(px[i], py[i], &p);
+            double e_lo = erfc((p - s.pt_min) * inv_s);
             for (int k = 0; k < nb; ++k) {
-                const double lo = s.pt_min + (double)k * width;
-                const double hi = lo + width;
-                double c = ok ? 0.5 * (erfc((p - hi) * inv_s) - erfc((p - lo) * inv_s)) : 0.0;
+                const double hi = s.pt_min + (double)(k + 1) * width;
+                const double e_hi = erfc((p - hi) * inv_s);
+                double c = ok ? 0.5 * (e_hi - e_lo) : 0.0;
+                e_lo = e_hi;
                 c = wave_sum(c);
                 if (lane == 0) whist[wave][k] += c;
             }

@@ -1716,11 +1716,15 @@ int lmo_observables(const double* mom, int32_t pdim, int64_t i0, int64_t i1,
         out[4] += pt * c2;
         if (nb == 0) continue;
         if (spec->sigma > 0.0) {
+            /* bin b spans the edges pt_min + b width and pt_min + (b + 1) width: each edge's erfc
+               serves the two bins it bounds */
             double inv_s = 1.0 / (spec->sigma * sqrt(2.0));
+            double e_lo = erfc((pt - spec->pt_min) * inv_s);
             for (int32_t b = 0; b < nb; ++b) {
-                double lo = spec->pt_min + (double)b * width;
-                double hi = lo + width;
-                out[5 + b] += 0.5 * (erfc((pt - hi) * inv_s) - erfc((pt - lo) * inv_s));
+                double hi = spec->pt_min + (double)(b + 1) * width;
+                double e_hi = erfc((pt - hi) * inv_s);
+                out[5 + b] += 0.5 * (e_hi - e_lo);
+                e_lo = e_hi;
             }
         } else {
             double f = (pt - spec->pt_min) * inv_w;
