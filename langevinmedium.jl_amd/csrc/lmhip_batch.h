@@ -99,9 +99,13 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
 }
 
 // PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
-// otherwise one block per tile.  SPEC as in k_cart_grid.
+// otherwise one block per tile.  SPEC as in k_cart_grid.  At least four waves per SIMD (128
+// VGPRs, a few spilled bytes; left to itself the compiler takes 145 and three waves, 6 % slower
+// on the 256-run workload; five waves spill more and are 9 % slower).
+constexpr int kBatchWaves = 4;
+
 template <int PDIM, int COLL, bool PERSIST, bool SPEC>
-__global__ __launch_bounds__(kBlock, kStepWaves) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
+__global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
     const int64_t step0 = *b.step_dev + k_off;
     const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
     for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += stride) {
