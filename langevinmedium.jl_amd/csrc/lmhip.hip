@@ -1039,6 +1039,11 @@ constexpr int kGridWaves = 7;
 
 template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER, bool SPEC>
 __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
+    // The specialised kernels are VALU-issue bound with seven waves on a SIMD: a wave issues its
+    // memory requests (the state loads here, the slice gather below) at wave priority 1, ahead
+    // of the other waves' arithmetic, so that the loads' latency starts as early as it can, and
+    // drops back to 0 for its own arithmetic (1.8 % per launch at 10^7 quarks)
+    if (SPEC) __builtin_amdgcn_s_setprio(1);
     int seg;
     const int64_t i_thread = particle_of_thread(a, &seg);
     // with DEFER every thread reaches the block-wide ranking; threads past the segment's end
@@ -1066,6 +1071,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
                              : (int64_t)a.id[i];
     // every state load is issued before the medium sample's address arithmetic waits for x, y
     __builtin_amdgcn_sched_barrier(0);
+    if (SPEC) __builtin_amdgcn_s_setprio(0);
     // freeze-out capture runs in the generic, non-deferred kernels only
     const bool fo = !SPEC && !DEFER && a.fo.on;
     bool frozen = false;
@@ -1095,16 +1101,19 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         double xd, yd;
         if (SLICE) {
             // the slice record is requested first ...
+            if (SPEC) __builtin_amdgcn_s_setprio(1);
             const double4* cell = slice_cell(g, x0, x1, &xd, &yd);
             cT = cell[0];
             cx = cell[1];
             cy = cell[2];
             __builtin_amdgcn_sched_barrier(0);
+            if (SPEC) __builtin_amdgcn_s_setprio(0);
         } else {
             grid_sample<false>(g, x0, x1, j, td, &Tm, &vx, &vy);
         }
-        // ... then the kick noise, which needs no memory, is computed while it is in flight (an
-        // empty asm that reads the noise keeps the compiler from sinking it to its uses)
+        // ... and the kick noise, which needs no memory, is computed around it (the compiler
+        // starts the Philox rounds, which need the identity only, ahead of the gather's address
+        // arithmetic; an empty asm that reads the noise keeps it from sinking the noise to its uses)
         double xi[PDIM];
         double ucol = 1.0, usamp_p = 0.0;
         noise_kick<PDIM, NOISE, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, pid, step, s_off, xi, &ucol, &usamp_p);

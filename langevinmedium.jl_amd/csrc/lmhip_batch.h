@@ -66,7 +66,11 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
     double td;
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
     double Tm, vx, vy;
+    // the medium's corner reads are issued at wave priority 1, ahead of the other waves'
+    // arithmetic (the batch kernel is issue bound too: 1.8 % per step on 256 runs)
+    __builtin_amdgcn_s_setprio(1);
     grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    __builtin_amdgcn_s_setprio(0);
     const FlowBoost fb = flow_boost(vx, vy);
     apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
     if (PDIM == 3 && p.bjorken_redshift) bjorken(p, step, &q[2 % PDIM]);
