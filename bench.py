@@ -319,8 +319,9 @@ def bench_adjoint(args):
     every 50 steps, Langevin with D_s(2 pi T) = 4 (drag table; d/dDsT) and with the GAMMA_P drag
     gamma0 = 5.3, Lambda = 5 GeV (d/dgamma0, d/dLambda).  Losses: <p_T>, a Gaussian-smoothed
     (sigma = 0.25 GeV) 40-bin p_T spectrum on [0, 10] GeV weighted by fixed random weights, and
-    <cos 2 phi>.  Times: the plain forward run (lmh_run), the checkpointed forward, one pullback per
-    loss (after one untimed pullback that allocates the backward workspace), and the same with the
+    <cos 2 phi>.  Times: the plain forward run (lmh_run), the checkpointed forward (after one
+    untimed one that allocates the checkpoints), one pullback per loss (after one untimed pullback
+    that allocates the backward workspace), and the same with the
     whole tape kept in device memory (keep_tape: 64 GB at this size; the pullback runs the reverse
     sweeps only, with bit-identical gradients); gradients checked against central differences of
     the plain run at relative steps 1e-4, 1e-5 and 1e-6."""
@@ -368,6 +369,8 @@ def bench_adjoint(args):
         p = ens.reset(theta)
         _, t_fwd, _ = timed(lambda: ctx.run(p, 1, steps, None, 0))
         obs_plain = ctx.observables(**obs_kw)
+        p = ens.reset(theta)
+        ens.run_forward(p, theta, keep_tape=False)                  # checkpoint buffers, untimed
         p = ens.reset(theta)
         _, t_cfwd, _ = timed(lambda: ens.run_forward(p, theta, keep_tape=False))
         obs_ck = ctx.observables(**obs_kw)

@@ -1004,6 +1004,16 @@ __global__ __launch_bounds__(kBlock) void k_scatter_rows(const double* __restric
     for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + to] = src[(int64_t)r * N + s];
 }
 
+// the inverse of k_scatter_rows: dst row r, slot s = src row r, slot id[s] (rows in the caller's
+// order into the order whose identities are id)
+__global__ __launch_bounds__(kBlock) void k_gather_rows(const double* __restrict__ src, double* __restrict__ dst,
+                                                        int32_t rows, int64_t N, const uint32_t* __restrict__ id) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= N) return;
+    const int64_t from = id[s];
+    for (int r = 0; r < rows; ++r) dst[(int64_t)r * N + s] = src[(int64_t)r * N + from];
+}
+
 // dst row r, slot s = src row r, slot perm[s]; the identity likewise.  Every row is read before
 // any is written (the buffers never alias), so a thread's loads are in flight together.
 constexpr int kMaxRows = 6;
@@ -1558,6 +1568,7 @@ struct lmh_ctx {
     // holding each slot's identity
     bool is_sorted = false;
     int64_t steps_since_sort = 0;
+    uint64_t order_gen = 0;           // bumped by every permutation of the state
     double* d_state_alt = nullptr;
     uint32_t* d_id = nullptr;
     uint32_t* d_id_alt = nullptr;
@@ -1617,9 +1628,14 @@ struct lmh_ctx {
         bool keep = false;            // the forward pass kept the whole tape and every slice
         double dtau_dDsT = 0.0;
         uint64_t gen = 0;
+        // the order of each checkpoint's rows: -1 the caller's, else the index of the identity
+        // array (d_ckpt_id + index * N) of the cell order the segment ran in
+        std::vector<int32_t> ord;
     } adj;
     double* d_ckpt = nullptr;
     size_t ckpt_cap = 0;
+    uint32_t* d_ckpt_id = nullptr;
+    size_t ckpt_id_cap = 0;
     double* d_tape = nullptr;
     size_t tape_cap = 0;
     double* d_adjslice = nullptr;     // the medium slices of a segment's steps, two buffers
@@ -1878,6 +1894,7 @@ int sort_apply(lmh_ctx* c, int end_bit) {
     LMH_HIP(c, hipGetLastError());
     std::swap(c->d_state, c->d_state_alt);
     std::swap(c->d_id, c->d_id_alt);
+    ++c->order_gen;
     return LMH_OK;
 }
 
@@ -1995,6 +2012,7 @@ void lmh_destroy(lmh_ctx* c) {
     dfree(c->d_nodes);
     dfree(c->d_slice);
     dfree(c->d_ckpt);
+    dfree(c->d_ckpt_id);
     dfree(c->d_tape);
     dfree(c->d_adjslice);
     dfree(c->d_adjw);
@@ -2462,16 +2480,20 @@ void launch_grid(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g
     else launch_grid_n<PDIM, false>(c, a, nb, g, noise, slice, coll, defer);
 }
 
+// cell sorting applies to the 2+1D grid with Philox noise (the noise follows the identity) and
+// without freeze-out capture (its records are per slot); anything else runs in the caller's order
+bool run_sortable(const lmh_ctx* c, const lmh_params* p) {
+    return c->medium == 1 && p->coords == LMH_CARTESIAN && p->noise == LMH_NOISE_PHILOX && !c->fo_on &&
+           c->tune.sort_every > 0;
+}
+
 int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps,
              const double* d_noise, int32_t spl) {
     const bool grid = c->medium == 1 && p->coords == LMH_CARTESIAN;
     const int32_t k = spl > 0 ? spl : std::max(1, c->tune.steps_per_launch);
     RadialMedium rmed = make_radial(c);
     GridMedium gmed = make_grid(c);
-    // cell sorting applies to the 2+1D grid with Philox noise (the noise follows the identity)
-    // and without freeze-out capture (its records are per slot); anything else runs in the
-    // caller's order
-    const bool sortable = grid && p->noise == LMH_NOISE_PHILOX && !c->fo_on && c->tune.sort_every > 0;
+    const bool sortable = run_sortable(c, p);
     if (!sortable) {
         const int rc = ensure_unsorted(c);
         if (rc) return rc;
@@ -3095,19 +3117,27 @@ struct SliceTuning {
     ~SliceTuning() { c->tune = saved; }
 };
 
-// an adjoint checkpoint of the current state in the caller's order (scattered through the
-// identities when the particles are cell-sorted)
-int take_checkpoint(lmh_ctx* c, double* dst) {
+// Adjoint checkpoint cp: the state in its current order, and that order -- -1 for the caller's,
+// else the index of a copy of the identity array, shared by the consecutive checkpoints no
+// permutation separates (*last_gen: the order generation of the last copy).  The backward pass
+// re-runs and reverses the segment in this order, so its medium reads stay cell-coherent.
+int take_checkpoint(lmh_ctx* c, int64_t cp, double* dst, int32_t* nid, uint64_t* last_gen) {
     const int64_t N = c->N;
     const int rows = c->dims + c->pdim;
+    LMH_HIP(c, hipMemcpyAsync(dst, c->d_state, (size_t)rows * N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+    int32_t ord = -1;
     if (c->is_sorted) {
-        hipLaunchKernelGGL(k_scatter_rows, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, dst,
-                           (int32_t)rows, N, c->d_id);
-        LMH_HIP(c, hipGetLastError());
-    } else {
-        LMH_HIP(c, hipMemcpyAsync(dst, c->d_state, (size_t)rows * N * sizeof(double), hipMemcpyDeviceToDevice,
-                                  c->stream));
+        if (*nid > 0 && *last_gen == c->order_gen) {
+            ord = *nid - 1;
+        } else {
+            ord = (*nid)++;
+            *last_gen = c->order_gen;
+            LMH_HIP(c, hipMemcpyAsync(c->d_ckpt_id + (size_t)ord * N, c->d_id, (size_t)N * sizeof(uint32_t),
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
     }
+    c->adj.ord[cp] = ord;
     return LMH_OK;
 }
 
@@ -3135,6 +3165,11 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
     const size_t R = (size_t)(c->dims + c->pdim);
     const size_t per = R * (size_t)c->N;
     ++c->gen;
+    try {
+        c->adj.ord.assign((size_t)ncp, -1);
+    } catch (...) {
+        return c->fail(LMH_E_NOMEM, "adjoint_forward: host allocation failed");
+    }
     if (s->keep_tape) {
         // the whole tape and every step's medium slice stay in device memory: the pullback only
         // runs the reverse sweep.  The forward steps are the tape kernel's, segment by segment.
@@ -3174,13 +3209,30 @@ int lmh_adjoint_forward(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int
         }
     } else {
         // lmh_run's steps (cell-sorted as it sorts, one step per launch through the medium slices),
-        // a checkpoint in the caller's order before every segment, one finiteness check at the end
+        // a checkpoint before every segment (after the segment's cell sort when one is due, so
+        // that the checkpoint's order is the one the segment starts in), one finiteness check at
+        // the end
         if ((rc = ensure_dev(c, &c->d_ckpt, &c->ckpt_cap, per * (size_t)ncp, "adjoint_forward checkpoints"))) return rc;
+        const bool sortable = run_sortable(c, p);
+        if (sortable && c->ckpt_id_cap < (size_t)ncp * (size_t)c->N) {
+            LMH_HIP(c, hipStreamSynchronize(c->stream));
+            dfree(c->d_ckpt_id);
+            c->ckpt_id_cap = 0;
+            if (hipMalloc((void**)&c->d_ckpt_id, (size_t)ncp * (size_t)c->N * sizeof(uint32_t)) != hipSuccess) {
+                c->d_ckpt_id = nullptr;
+                return c->fail(LMH_E_NOMEM, "adjoint_forward checkpoint orders: device allocation failed");
+            }
+            c->ckpt_id_cap = (size_t)ncp * (size_t)c->N;
+        }
         SliceTuning sliced(c);
         lmh_params pseg = *p;
         pseg.check_finite = 0;
+        int32_t nid = 0;
+        uint64_t last_gen = 0;
         for (int64_t cp = 0; cp < ncp; ++cp) {
-            if ((rc = take_checkpoint(c, c->d_ckpt + per * (size_t)cp))) return rc;
+            if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every))
+                if ((rc = sort_by_cell(c))) return rc;
+            if ((rc = take_checkpoint(c, cp, c->d_ckpt + per * (size_t)cp, &nid, &last_gen))) return rc;
             const int64_t n = std::min<int64_t>(K, nsteps - cp * K);
             if ((rc = run_impl(c, &pseg, step_begin + cp * K, n, nullptr, 1))) return rc;
         }
@@ -3241,9 +3293,17 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
             LMH_HIP(c, hipEventCreateWithFlags(&c->ev_rev[k], hipEventDisableTiming));
         }
     }
+    // lam and the per-particle gradients are adjacent rows, which move together between orders
+    // through the reorder buffer
+    const int nmov = R + kAdjThetas;
     const size_t o_lam = 0, o_g = (size_t)R * N, o_part = o_g + (size_t)kAdjThetas * N;
     const size_t o_out = o_part + (size_t)kAdjThetas * kSumBlocks, o_w = o_out + kAdjThetas;
-    if ((rc = ensure_dev(c, &c->d_adjw, &c->adjw_cap, o_w + nw, "adjoint_backward workspace"))) return rc;
+    const size_t o_mov = o_w + nw;
+    bool reorder = false;
+    for (const int32_t o : c->adj.ord) reorder = reorder || o >= 0;
+    if ((rc = ensure_dev(c, &c->d_adjw, &c->adjw_cap, o_mov + (reorder ? (size_t)nmov * N : 0),
+                         "adjoint_backward workspace")))
+        return rc;
     double* lam = c->d_adjw + o_lam;
     double* gacc = c->d_adjw + o_g;
     LMH_HIP(c, hipMemcpyAsync(c->d_adjw + o_w, cotangent, nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -3279,10 +3339,40 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
         // were enqueued on the main stream)
         LMH_HIP(c, hipEventRecord(c->ev_rev[0], c->stream));
         LMH_HIP(c, hipStreamWaitEvent(c->tape_stream, c->ev_rev[0], 0));
+        // lam and the gradients start in the caller's order; each segment is re-run and reversed
+        // in its checkpoint's order (the identities key the noise), and lam and the gradients
+        // move to that order first when it differs (through the caller's order)
+        const uint32_t* caller_id = c->d_id;
+        double* mov = lam;
+        double* buf_mov = c->d_adjw + o_mov;
+        int32_t lam_ord = -1;
+        auto move_to = [&](int32_t ord) -> int {
+            if (ord == lam_ord) return LMH_OK;
+            const unsigned gb = grid_blocks(N);
+            if (lam_ord >= 0) {
+                hipLaunchKernelGGL(k_scatter_rows, dim3(gb), dim3(kBlock), 0, c->stream, mov, buf_mov,
+                                   (int32_t)nmov, N, c->d_ckpt_id + (size_t)lam_ord * N);
+            } else {
+                LMH_HIP(c, hipMemcpyAsync(buf_mov, mov, (size_t)nmov * N * sizeof(double), hipMemcpyDeviceToDevice,
+                                          c->stream));
+            }
+            if (ord >= 0) {
+                hipLaunchKernelGGL(k_gather_rows, dim3(gb), dim3(kBlock), 0, c->stream, buf_mov, mov, (int32_t)nmov,
+                                   N, c->d_ckpt_id + (size_t)ord * N);
+            } else {
+                LMH_HIP(c, hipMemcpyAsync(mov, buf_mov, (size_t)nmov * N * sizeof(double), hipMemcpyDeviceToDevice,
+                                          c->stream));
+            }
+            LMH_HIP(c, hipGetLastError());
+            lam_ord = ord;
+            return LMH_OK;
+        };
         for (int64_t cp = ncp - 1; cp >= 0; --cp) {
             const int32_t nk = (int32_t)std::min<int64_t>(K, c->adj.nsteps - cp * K);
             const int64_t step0 = c->adj.step_begin + cp * K;
             const int buf = (int)(cp & 1);
+            const int32_t ord = c->adj.ord[(size_t)cp];
+            a.id = ord >= 0 ? c->d_ckpt_id + (size_t)ord * N : caller_id;
             double4* sl = reinterpret_cast<double4*>(c->d_adjslice + slices_n * buf);
             AdjLaunch l{&a, nb, &g, sl, step0, nk, c->d_tape + tape_n * buf};
             l.ckpt = c->d_ckpt + per * cp;
@@ -3299,6 +3389,7 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
             l.row0 = l.ckpt;
             LMH_HIP(c, hipEventRecord(c->ev_tape[buf], c->tape_stream));
             // reverse segment cp on the main stream
+            if ((rc = move_to(ord))) return rc;
             LMH_HIP(c, hipStreamWaitEvent(c->stream, c->ev_tape[buf], 0));
             {
                 LaunchMark mk(c, LMH_PROF_STEP);
@@ -3307,6 +3398,7 @@ int lmh_adjoint_backward(lmh_ctx* c, const lmh_obs_spec* os, const double* cotan
             LMH_HIP(c, hipEventRecord(c->ev_rev[buf], c->stream));
             LMH_HIP(c, hipGetLastError());
         }
+        if ((rc = move_to(-1))) return rc;
     }
     hipLaunchKernelGGL(k_rows_partial, dim3(kSumBlocks), dim3(kSumBlock), 0, c->stream, gacc, kAdjThetas, N, c->d_adjw + o_part);
     hipLaunchKernelGGL(k_rows_final, dim3(kAdjThetas), dim3(kSumBlock), 0, c->stream, c->d_adjw + o_part, kSumBlocks, c->d_adjw + o_out);

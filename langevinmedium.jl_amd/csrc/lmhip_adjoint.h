@@ -7,15 +7,17 @@
 // theta = (DsT, gamma0, Lambda): DsT scales the drag-time table, (gamma0, Lambda) are the
 // GAMMA_P drag eta = gamma0 T^2 / (1 + p^2/Lambda^2).
 //
-// Common random numbers: the noise of particle i at step s is Philox4x32-10 of a counter made of
-// (i, s, stream, block), so each segment's recomputation draws the forward pass's noise again and
-// no noise is kept between the passes.  The forward pass keeps the state every K steps
-// (checkpoints); the backward pass walks the segments from the last to the first: it prepares the
-// medium slices of the segment's steps (the per-cell coefficients the forward step kernels read),
-// re-runs the segment forward from its checkpoint while writing the step inputs (state and kick
-// noise) to a segment tape -- the forward pass's trajectory, bit for bit -- then runs the reverse
-// steps over the tape.
-// One thread owns one particle throughout, so no adjoint is ever exchanged between threads.
+// Common random numbers: the noise of the particle with identity i at step s is Philox4x32-10 of
+// a counter made of (i, s, stream, block), so each segment's recomputation draws the forward
+// pass's noise again and no noise is kept between the passes.  The forward pass keeps the state
+// every K steps (checkpoints, in the cell order the segment runs in, with its identities); the
+// backward pass walks the segments from the last to the first: it prepares the medium slices of
+// the segment's steps (the per-cell coefficients the forward step kernels read), re-runs the
+// segment forward from its checkpoint with the step kernel, writing each step's input state to
+// a segment tape -- the forward pass's trajectory, bit for bit -- then runs the reverse steps over
+// the tape, drawing each step's noise again.
+// One thread owns one particle throughout a segment, so no adjoint is exchanged between threads;
+// between segments of different orders the adjoint moves with its particle.
 //
 // The RTA collision decision is a discontinuous function of the state; the pathwise derivative
 // keeps each step's decision fixed (a colliding step forgets the incoming momentum and depends
@@ -168,8 +170,8 @@ __device__ __forceinline__ double drv_rsq(double x) {
     return fma(r, fma(-(0.5 * x * r), r, 0.5), r);
 }
 
-// One reverse step.  (x0, x1, qin) is the step's input state and (xi, ucol, usamp_p) its
-// block-0 noise;
+// One reverse step of the particle with identity i.  (x0, x1, qin) is the step's input state; its
+// block-0 noise is drawn again from the Philox counters of (i, step).
 // lam holds the adjoint of the step's output state (x0, x1, q) on entry and of its input state
 // on return; th accumulates dL/dtheta.  The pre-point exact-OU kick, relativistic boosts.  The
 // forward values are recomputed from the input state; the collision decision sees the forward
@@ -547,7 +549,8 @@ __global__ __launch_bounds__(kBlock, kStepWaves) void k_adj_tape(StepArgs a, Gri
 }
 
 // Second half: the reverse steps over the tape from the last to the first.  lam carries the
-// state adjoint between segments (rows as the state), gacc the per-particle dL/dtheta (3 rows).
+// state adjoint between segments (rows as the state), gacc the per-particle dL/dtheta (3 rows),
+// both in the tape's order; a.id holds each slot's identity, which keys its noise.
 template <int PDIM, int COLL, int DRAG>
 __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g,
                                                         const double4* __restrict__ slices,
@@ -562,6 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
     const SpeciesArgs sp = a.species[seg];
     const lmh_params p = adjoint_params<DRAG>(a.p);
     const int64_t N = a.N;
+    const int64_t pid = a.id[i];
     constexpr int R = 2 + PDIM;
     constexpr int TR = R;
     double lm[R];
@@ -592,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_reverse(StepArgs a, GridMedium g
         }
         GridMedium gk = g;
         gk.slice = slices + (int64_t)k * 3 * ((int64_t)(g.nx - 1) * (g.ny - 1));
-        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, i, step, cur[0], cur[1], cur + 2, dtau_dDsT, lm, th);
+        adj_reverse_step<PDIM, COLL>(a, p, gk, sp, pid, step, cur[0], cur[1], cur + 2, dtau_dDsT, lm, th);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) lam[(int64_t)r * N + i] = lm[r];

@@ -308,6 +308,32 @@ def test_adjoint_keep_tape_equals_checkpointed(gpu_ok, drag_model, rta):
     assert np.array_equal(out[True][1], plain[0]) and np.array_equal(out[True][2], plain[1])
 
 
+def test_adjoint_cell_order_independent(gpu_ok):
+    """the checkpoints keep the cell order their segment runs in and the backward pass re-runs and
+    reverses each segment in it: the gradients (parameters and initial state) do not depend on the
+    sort cadence -- never, mid-segment, at every checkpoint, every other -- bit for bit, and equal
+    the whole-tape pullback's (caller's order throughout)"""
+    N, steps, K = 3000, 45, 10
+    g, masses, counts, pos, mom = _setup(N=N)
+    rt, jt = _rta_tables(g, masses)
+    w = _cotangents(N, OBS["nbins"])["mixed"]
+    theta = (0.6366, 5.3, 5.0)
+    out = []
+    with lmhip.Context(0) as ctx:
+        for sort_every, keep in ((0, False), (3, False), (10, False), (20, False), (3, True)):
+            ctx.set_tuning(block=256, steps_per_launch=1, slice=1, defer=0, sort_every=sort_every)
+            ens = GridEnsemble(ctx, g, masses, counts, pos, mom, dt=DT, t0=T0, nsteps=steps,
+                               drag_model=_ffi.LMH_DRAG_TABLE, collision=_ffi.LMH_LANGEVIN_RTA, seed=8,
+                               checkpoint_every=K, rta_tables=rt, juttner_tables=jt, keep_tape=keep, **OBS)
+            obs = ens.forward(theta)
+            state = ctx.get_particles()
+            grad, dpos, dmom = ens.pullback(w, state_grad=True)
+            out.append((obs, state[0], state[1], grad, dpos, dmom))
+    for r in out[1:]:
+        for a, b in zip(out[0], r):
+            assert np.array_equal(a, b)
+
+
 def test_torch_autograd_function(gpu_ok):
     import torch
     from lmhip.adjoint import observables
