@@ -68,16 +68,22 @@ extern "C" {
  * step s is four 64-bit words from two Philox4x32-10 evaluations (half h gives words 2h, 2h+1,
  * each lo | hi << 32 of consecutive outputs) of counter (i, s, stream mod 2^32,
  * (stream >> 32) mod 2^29 | (2b + h) << 29) under key (seed mod 2^32, seed >> 32); i and s must
- * stay below 2^32 and stream below 2^61.  A uniform is d - 1 in [0, 1) or 2 - d in (0, 1] with
- * d = 1.m, m the word's top 52 bits.  Normals are Box-Muller pairs: the radius from the (0, 1]
- * uniform of the first word, the angle pi (q/2 + r) from the second (q its top 2 bits, r = d/2 -
- * 3/4 in [-1/4, 1/4) with d = 1.m from its next 52 bits); block 0's third normal is r cos(pi x)
- * with r from word 2, x the top 32 bits of word 3 over 2^33 and the sign bit 31 of word 3.  The collision uniform is the
- * low 31 bits of word 3 over 2^31.  The redraw's quantile u_samp is the 33-bit uniform
- * (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 of the low 11 bits of block 0's words 0-2, and its
- * Gaussian g is the three kick normals of block 0 (drawn for the redraw when the step has no
- * kick): a colliding particle's kick is replaced by the redraw, so its normals serve it, and the
- * normalised Gaussian is the reference's isotropic direction (kernels_cpu.jl:497-503).       */
+ * stay below 2^32 and stream below 2^61.  With lo(w), hi(w) a word's 32-bit halves and
+ * d = 1.m (m a word's top 52 bits):
+ *   block 0 (kick and collision): the kick normals are the first three components of a 4-D
+ *     standard normal vector R (sqrt(v) cos(beta), sqrt(v) sin(beta), sqrt(1 - v) cos(gamma), .)
+ *     with R^2 = -2 ln(u1 u2) (u1 = 2 - d of word 0, u2 = (lo(w1) + 1)/2^32), beta = pi (q/2 + r)
+ *     (q the top 2 bits of hi(w1), r = lo30(hi(w1))/2^31 - 1/4), v = lo(w2)/2^32, and cos(gamma)
+ *     = +-cos(pi x) with x = lo31(hi(w2))/2^32 and the sign bit 31 of hi(w2).  The collision
+ *     uniform is lo31(w3)/2^31, the redraw's quantile u_samp = hi(w3)/2^32, and the redraw's
+ *     Gaussian g is the three kick normals (drawn for the redraw when the step has no kick): a
+ *     colliding particle's kick is replaced by the redraw, so its normals serve it, and the
+ *     normalised Gaussian is the reference's isotropic direction (kernels_cpu.jl:497-503).  The
+ *     general-coordinate step's one normal is the first of the Box-Muller pair below of words
+ *     0 and 1.
+ *   block 2 (position diffusion): Box-Muller pairs of words (0, 1) and (2, 3): the radius
+ *     sqrt(-2 ln(2 - d)) of the first word, the angle pi (q/2 + r) of the second (q its top 2
+ *     bits, r = d/2 - 3/4 in [-1/4, 1/4) with d from its next 52 bits).                     */
 #define LMH_NOISE_FIXED        0
 #define LMH_NOISE_PHILOX       1
 

@@ -194,14 +194,10 @@ __device__ __forceinline__ void noise_kick(const StepArgs& a, int64_t i, int64_t
         if (!kick && !rta) return;
         uint64_t w[4];
         noise_block(a.p.seed, a.p.stream, i, step, 0, w);
-        double n0, n1;
-        box_muller(w[0], w[1], &n0, &n1);
-        xi[0] = n0;
-        if (PDIM >= 2) xi[1 % PDIM] = n1;
-        if (PDIM >= 3) xi[2 % PDIM] = half_box_muller(w[2], w[3]);
+        kick_normals<PDIM>(w, xi);
         if (rta) {
             *ucol = u_low31(w[3]);
-            *usamp = u_low33(w[0], w[1], w[2]);
+            *usamp = u_hi32(w[3]);
         }
     }
 }

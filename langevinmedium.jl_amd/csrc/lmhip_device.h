@@ -213,16 +213,9 @@ __device__ __forceinline__ double unit_mantissa(uint64_t w) {
 __device__ __forceinline__ double u_co(uint64_t w) { return unit_mantissa(w) - 1.0; }
 __device__ __forceinline__ double u_oc(uint64_t w) { return 2.0 - unit_mantissa(w); }
 
-// the collision uniform: a 31-bit uniform in [0,1) from the low 31 bits of a word (the third
-// kick normal takes the word's top 32 bits and bit 31)
+// the collision uniform: a 31-bit uniform in [0,1) from the low 31 bits of a word
 __device__ __forceinline__ double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
-// the redraw's quantile: a 33-bit uniform in [0,1) from the low 11 bits of three words, which
-// their 52-bit uniforms leave unused, (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33 (exact)
-__device__ __forceinline__ double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
-    const uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
-    return fma((double)((uint32_t)w2 & 0x7FFu), 0x1.0p-11, (double)a * 0x1.0p-33);
-}
 
 // 1/sqrt(x) for x > 0 (normal): the hardware estimate y (relative error e ~ 2^-24) refined by
 // one third-order step y (1 + d/2 + 3 d^2/8), d = 1 - x y^2 (truncation ~ 5 e^3/16 < 2^-70;
@@ -302,15 +295,14 @@ __device__ __forceinline__ double noise_log(double u) {
         s = fma(fma(-b, q, a), y, q);
     }
     double z = s * s;
-    double p = 1.0 / 19.0;
-    p = fma(p, z, 1.0 / 17.0);
-    p = fma(p, z, 1.0 / 15.0);
-    p = fma(p, z, 1.0 / 13.0);
-    p = fma(p, z, 1.0 / 11.0);
-    p = fma(p, z, 1.0 / 9.0);
-    p = fma(p, z, 1.0 / 7.0);
-    p = fma(p, z, 1.0 / 5.0);
-    p = fma(p, z, 1.0 / 3.0);
+    double p = fma_sc(1.0 / 19.0, z, 1.0 / 17.0);
+    p = fma_sc(p, z, 1.0 / 15.0);
+    p = fma_sc(p, z, 1.0 / 13.0);
+    p = fma_sc(p, z, 1.0 / 11.0);
+    p = fma_sc(p, z, 1.0 / 9.0);
+    p = fma_sc(p, z, 1.0 / 7.0);
+    p = fma_sc(p, z, 1.0 / 5.0);
+    p = fma_sc(p, z, 1.0 / 3.0);
     double lm = 2.0 * s + (2.0 * s * z) * p;
     double de = (double)e;
     return fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
@@ -341,6 +333,35 @@ __device__ __forceinline__ void noise_sincospi(double x, double* sn, double* cs)
     double c1 = (q & 1) ? sr : cr;
     *sn = (q & 2) ? -s1 : s1;
     *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// (sin, cos) of the angle pi (q/2 + r), q in 0..3, r in [-1/4, 1/4): the polynomials and the
+// quadrant logic of noise_sincospi
+__device__ __forceinline__ void sincos_qr(int q, double r, double* sn, double* cs) {
+    constexpr double SP[8] = {3.141592653589793, -5.16771278004997, 2.5501640398773455,
+                              -0.5992645293207921, 0.08214588661112823, -0.0073704309457143504,
+                              0.00046630280576761255, -2.1915353447830217e-05};
+    constexpr double CP[9] = {1.0, -4.934802200544679, 4.0587121264167685, -1.3352627688545895,
+                              0.2353306303588932, -0.02580689139001406, 0.0019295743094039231,
+                              -0.0001046381049248457, 4.303069587032947e-06};
+    const double z = r * r;
+    double ps = SP[7];
+#pragma unroll
+    for (int j = 6; j >= 0; --j) ps = fma_sc(ps, z, SP[j]);
+    double pc = CP[8];
+#pragma unroll
+    for (int j = 7; j >= 0; --j) pc = fma_sc(pc, z, CP[j]);
+    const double sr = r * ps, cr = pc;
+    const double s1 = (q & 1) ? cr : sr;
+    const double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// (sin, cos) of a uniform angle from a 32-bit word x: the quadrant q = the top 2 bits and
+// r = lo30(x)/2^31 - 1/4 in [-1/4, 1/4) (exact)
+__device__ __forceinline__ void sincos_u32(uint32_t x, double* sn, double* cs) {
+    sincos_qr((int)(x >> 30), fma((double)(x & 0x3FFFFFFFu), 0x1.0p-31, -0.25), sn, cs);
 }
 
 // (sin, cos) of a uniform angle from one word: the quadrant q is the word's top 2 bits and the
@@ -382,15 +403,35 @@ __device__ __forceinline__ double noise_cospi_half(double x) {
     return p;
 }
 
-// One normal from two words, half a Box-Muller pair: radius from the (0,1] uniform of wa, cos(pi x)
-// with x = the top 32 bits of wb over 2^33 (in [0, 1/2)), negated when bit 31 of wb is set.  A
-// uniform angle's cosine has the distribution of a random sign times cos(pi x), so this is r cos
-// of a uniform angle without the quadrant logic and the sine; the low 31 bits of wb stay free.
-__device__ __forceinline__ double half_box_muller(uint64_t wa, uint64_t wb) {
-    const double r = sqrt_pos(-2.0 * noise_log(u_oc(wa)));
-    const double n = r * noise_cospi_half((double)(uint32_t)(wb >> 32) * 0x1.0p-33);
-    return ((wb >> 31) & 1u) ? -n : n;
+// Block 0's kick normals: the first NN (<= 3) components of a 4-D standard normal vector
+// R (sqrt(v) cos(beta), sqrt(v) sin(beta), sqrt(1 - v) cos(gamma), sqrt(1 - v) sin(gamma)).
+// R^2 = -2 ln(u1 u2) is chi^2(4) (the sum of two independent chi^2(2) = -2 ln u variables); v,
+// the squared modulus of the first complex coordinate of a uniform point on the 3-sphere, is
+// uniform on [0, 1); beta and gamma are uniform angles.  u1 is the (0, 1] uniform of w0's top
+// 52 bits, u2 = (lo(w1) + 1)/2^32, beta the angle of hi(w1) (sincos_u32), v = lo(w2)/2^32, and
+// cos(gamma) a random sign (bit 31 of hi(w2)) times cos(pi x), x = lo31(hi(w2))/2^32 in
+// [0, 1/2).  One logarithm for three normals (Box-Muller takes one per pair); w3 stays for the
+// collision uniform and the redraw's quantile.
+template <int NN>
+__device__ __forceinline__ void kick_normals(const uint64_t w[4], double* n) {
+    const double P = u_oc(w[0]) * (((double)(uint32_t)w[1] + 1.0) * 0x1.0p-32);
+    const double L = -2.0 * noise_log(P);
+    const double v = (double)(uint32_t)w[2] * 0x1.0p-32;
+    const double a = sqrt_pos(v * L);
+    double s, c;
+    sincos_u32((uint32_t)(w[1] >> 32), &s, &c);
+    n[0] = a * c;
+    if (NN >= 2) n[1 % NN] = a * s;
+    if (NN >= 3) {
+        const double b = sqrt_pos((1.0 - v) * L);
+        const uint32_t hg = (uint32_t)(w[2] >> 32);
+        const double nb = b * noise_cospi_half((double)(hg & 0x7FFFFFFFu) * 0x1.0p-32);
+        n[2 % NN] = (hg >> 31) ? -nb : nb;
+    }
 }
+
+// the redraw's quantile: the 32-bit uniform hi(w3)/2^32 in [0, 1)
+__device__ __forceinline__ double u_hi32(uint64_t w) { return (double)(uint32_t)(w >> 32) * 0x1.0p-32; }
 
 // Box-Muller: radius from the (0,1] uniform of wa, the uniform angle of wb (sincos_word).
 // EXACT takes the correctly rounded division and square root of the host restatement, so the
@@ -412,17 +453,17 @@ __device__ __forceinline__ void box_muller(uint64_t wa, uint64_t wb, double* n0,
 // of the result; larger or non-finite arguments take exp.
 __device__ __forceinline__ double exp_neg(double x) {
     if (!(x <= 0.0625)) return exp(-x);
-    const double y = -x;
-    double r = 1.0 / 362880.0;
-    r = fma(r, y, 1.0 / 40320.0);
-    r = fma(r, y, 1.0 / 5040.0);
-    r = fma(r, y, 1.0 / 720.0);
-    r = fma(r, y, 1.0 / 120.0);
-    r = fma(r, y, 1.0 / 24.0);
-    r = fma(r, y, 1.0 / 6.0);
-    r = fma(r, y, 0.5);
-    r = fma(r, y, 1.0);
-    return fma(r, y, 1.0);
+    // Horner's rule in y = -x, r_k = r_(k+1) y + 1/k!, written in x: s_k = (-1)^k r_k =
+    // s_(k+1) x + (-1)^k/k! (each step the same fused multiply-add up to sign, so the same bits)
+    double r = fma_sc(-1.0 / 362880.0, x, 1.0 / 40320.0);
+    r = fma_sc(r, x, -1.0 / 5040.0);
+    r = fma_sc(r, x, 1.0 / 720.0);
+    r = fma_sc(r, x, -1.0 / 120.0);
+    r = fma_sc(r, x, 1.0 / 24.0);
+    r = fma_sc(r, x, -1.0 / 6.0);
+    r = fma(r, x, 0.5);
+    r = fma(r, x, -1.0);
+    return fma(r, x, 1.0);
 }
 
 // drag_table_read: whether the drag time table is read (DsT > 0 and the TABLE model; a NaN DsT

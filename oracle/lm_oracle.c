@@ -492,13 +492,42 @@ static inline double noise_cospi_half(double x) {
     return p;
 }
 
-/* the third kick normal: radius from the (0,1] uniform of wa, r cos(pi x) with x = the top 32 bits
- * of wb / 2^33, negated when bit 31 of wb is set (a random sign times cos(pi x), x uniform in
- * [0, 1/2), is the cosine of a uniform angle) */
-static inline double half_box_muller(uint64_t wa, uint64_t wb) {
-    double r = sqrt(-2.0 * noise_log(u_oc(wa)));
-    double n = r * noise_cospi_half((double)(uint32_t)(wb >> 32) * 0x1.0p-33);
-    return ((wb >> 31) & 1u) ? -n : n;
+/* (sin, cos) of a uniform angle from a 32-bit word x: the quadrant q = the top 2 bits and the
+ * angle pi (q/2 + r), r = lo30(x)/2^31 - 1/4 in [-1/4, 1/4) (exact); the polynomials and quadrant
+ * logic of noise_sincospi */
+static inline void sincos_u32(uint32_t x, double* sn, double* cs) {
+    double r = fma((double)(x & 0x3FFFFFFFu), 0x1.0p-31, -0.25);
+    double z = r * r;
+    double ps = SP[7];
+    for (int j = 6; j >= 0; --j) ps = fma(ps, z, SP[j]);
+    double pc = CP[8];
+    for (int j = 7; j >= 0; --j) pc = fma(pc, z, CP[j]);
+    double sr = r * ps, cr = pc;
+    int q = (int)(x >> 30);
+    double s1 = (q & 1) ? cr : sr;
+    double c1 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+/* block 0's kick normals: the first three components of a 4-D standard normal vector
+ * R (sqrt(v) cos(beta), sqrt(v) sin(beta), sqrt(1 - v) cos(gamma), sqrt(1 - v) sin(gamma)), with
+ * R^2 = -2 ln(u1 u2) ~ chi^2(4) (u1 the (0,1] uniform of w0's top 52 bits, u2 = (lo(w1) + 1)/2^32),
+ * beta the uniform angle of hi(w1), v = lo(w2)/2^32 uniform on [0, 1) (the squared modulus of the
+ * first complex coordinate of a uniform point on the 3-sphere), and cos(gamma) a random sign
+ * (bit 31 of hi(w2)) times cos(pi x), x = lo31(hi(w2))/2^32 uniform on [0, 1/2) */
+static inline void kick_normals(const uint64_t w[4], double n[3]) {
+    double P = u_oc(w[0]) * (((double)(uint32_t)w[1] + 1.0) * 0x1.0p-32);
+    double L = -2.0 * noise_log(P);
+    double v = (double)(uint32_t)w[2] * 0x1.0p-32;
+    double a = sqrt(v * L), b = sqrt((1.0 - v) * L);
+    double s, c;
+    sincos_u32((uint32_t)(w[1] >> 32), &s, &c);
+    n[0] = a * c;
+    n[1] = a * s;
+    uint32_t hg = (uint32_t)(w[2] >> 32);
+    double nb = b * noise_cospi_half((double)(hg & 0x7FFFFFFFu) * 0x1.0p-32);
+    n[2] = (hg >> 31) ? -nb : nb;
 }
 
 void lmo_noise_transforms(double u, double x, double out[4]) {
@@ -529,19 +558,14 @@ int32_t lmo_noise_channels(const lmh_params* p, int32_t dims, int32_t pdim) {
 /* the collision uniform: the low 31 bits of a word over 2^31 */
 static inline double u_low31(uint64_t w) { return (double)(uint32_t)(w & 0x7FFFFFFFu) * 0x1.0p-31; }
 
-/* the redraw's quantile: the 33-bit uniform of the low 11 bits of three words (which their
- * 52-bit uniforms leave unused), (lo(w0) + 2^11 lo(w1) + 2^22 lo(w2)) / 2^33, exact */
-static inline double u_low33(uint64_t w0, uint64_t w1, uint64_t w2) {
-    uint32_t a = ((uint32_t)w0 & 0x7FFu) | (((uint32_t)w1 & 0x7FFu) << 11);
-    return (double)((uint32_t)w2 & 0x7FFu) * 0x1.0p-11 + (double)a * 0x1.0p-33;
-}
+/* the redraw's quantile: the 32-bit uniform hi(w)/2^32 in [0, 1) */
+static inline double u_hi32(uint64_t w) { return (double)(uint32_t)(w >> 32) * 0x1.0p-32; }
 
 /* The PHILOX noise record of one step (include/lmhip.h): block b of particle i at step s is
  * lmo_noise_block(seed, stream, i, s, b).
- *   block 0: the Box-Muller pair of (w0, w1) and the half pair of (w2, w3): the normals
- *            n0, n1, n2 -> xi[0..pdim) (GC: xi = n0); u_col = the low 31 bits of w3;
- *            u_samp = the 33-bit uniform of the low 11 bits of w0, w1, w2; g = n0..n(pdim-1),
- *            the kick normals (rta_collision normalises them)
+ *   block 0: the normals n0, n1, n2 of kick_normals -> xi[0..pdim) (GC: xi = the first normal of
+ *            the Box-Muller pair of (w0, w1)); u_col = the low 31 bits of w3; u_samp =
+ *            hi(w3)/2^32; g = n0..n(pdim-1), the kick normals (rta_collision normalises them)
  *   block 2: (xpos0, xpos1) = BoxMuller(w0, w1), xpos2 = first of BoxMuller(w2, w3)         */
 void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N, int64_t step,
                       double* rec) {
@@ -553,15 +577,15 @@ void lmo_noise_record(const lmh_params* p, int32_t dims, int32_t pdim, int64_t N
         double n[4];
         if (oxi >= 0 || orta >= 0) {
             lmo_noise_block(p->seed, p->stream, i, step, 0, w);
-            box_muller(w[0], w[1], &n[0], &n[1]);
-            n[2] = half_box_muller(w[2], w[3]);
+            if (p->coords == LMH_GENERAL_COORDS) box_muller(w[0], w[1], &n[0], &n[1]);
+            else kick_normals(w, n);
             if (oxi >= 0) {
                 int32_t k = p->coords == LMH_GENERAL_COORDS ? 1 : pdim;
                 for (int32_t d = 0; d < k; ++d) rec[(int64_t)(oxi + d) * N + i] = n[d];
             }
             if (orta >= 0) {
                 rec[(int64_t)orta * N + i] = u_low31(w[3]);
-                rec[(int64_t)(orta + 1) * N + i] = u_low33(w[0], w[1], w[2]);
+                rec[(int64_t)(orta + 1) * N + i] = u_hi32(w[3]);
                 for (int32_t d = 0; d < pdim; ++d) rec[(int64_t)(orta + 2 + d) * N + i] = n[d];
             }
         }

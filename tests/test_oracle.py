@@ -77,36 +77,67 @@ def test_noise_block_layout():
     assert len(seen) == 3 * 4 * 4 * 2
 
 
-def test_noise_record_box_muller():
+def test_noise_record_layout_vs_mpmath():
+    """each record channel at 50 digits from the Philox words: the kick normals as the 4-D
+    construction of include/lmhip.h (radius^2 = -2 ln(u1 u2), the split v, the angles beta and
+    gamma), the collision uniform, the redraw's quantile and Gaussian, the diffusion pair"""
     p = _ffi.make_params(dt=0.01, seed=99, stream=3, noise=_ffi.LMH_NOISE_PHILOX,
                          collision=_ffi.LMH_LANGEVIN_RTA, position_diffusion=True)
     rec = O.noise_record(p, 2, 3, 5, 17, 1)[0]
     assert rec.shape == (3 + 5 + 2, 5)
+    two = mp.mpf(2)
     # uniforms: d = 1.m from the top 52 bits, u = d - 1 or 1 - u = d - 1 (so u1 = 2 - d)
-    mant = lambda v: 1 + mp.mpf((v >> 12) & (2**52 - 1)) * mp.mpf(2) ** -52
-    u = lambda v: mant(v) - 1
+    mant = lambda v: 1 + mp.mpf((v >> 12) & (2**52 - 1)) * two ** -52
     u1 = lambda v: 2 - mant(v)
-    # the pair's angle: pi (q/2 + d/2 - 3/4), q = top 2 bits, d = 1.m from the next 52 bits
+    lo = lambda v: v & 0xFFFFFFFF
+    hi = lambda v: v >> 32
+    # the diffusion pair's angle: pi (q/2 + d/2 - 3/4), q = top 2 bits, d = 1.m from the next 52 bits
     ang = lambda v: mp.pi * (mp.mpf(v >> 62) / 2 + mant((v << 2) & (2**64 - 1)) / 2 - mp.mpf(0.75))
     for i in range(5):
         w = _philox_words(99, 3, i, 17, 0)
-        r0 = mp.sqrt(-2 * mp.log(u1(w[0])))
-        r2 = mp.sqrt(-2 * mp.log(u1(w[2])))
-        # the third normal: r2 cos(pi x), x = top 32 bits of w3 / 2^33, sign from bit 31
-        n2 = r2 * mp.cos(mp.pi * mp.mpf(w[3] >> 32) * mp.mpf(2) ** -33) * (-1 if (w[3] >> 31) & 1 else 1)
-        for val, ref in ((rec[0, i], r0 * mp.cos(ang(w[1]))), (rec[1, i], r0 * mp.sin(ang(w[1]))),
-                         (rec[2, i], n2)):
-            assert abs(val - float(ref)) <= 4e-15 * max(1, abs(float(ref)))
-        # the collision uniform is the low 31 bits of w3
+        R2 = -2 * mp.log(u1(w[0]) * (mp.mpf(lo(w[1])) + 1) * two ** -32)
+        v = mp.mpf(lo(w[2])) * two ** -32
+        beta = mp.pi * (mp.mpf(hi(w[1]) >> 30) / 2 + mp.mpf(hi(w[1]) & 0x3FFFFFFF) * two ** -31 - mp.mpf(0.25))
+        gcos = mp.cos(mp.pi * mp.mpf(hi(w[2]) & 0x7FFFFFFF) * two ** -32) * (-1 if hi(w[2]) >> 31 else 1)
+        want = (mp.sqrt(v * R2) * mp.cos(beta), mp.sqrt(v * R2) * mp.sin(beta), mp.sqrt((1 - v) * R2) * gcos)
+        for c in range(3):
+            assert abs(rec[c, i] - float(want[c])) <= 4e-15 * max(1, abs(float(want[c]))), (i, c)
+        # the collision uniform is the low 31 bits of w3, the quantile its high half
         assert rec[3, i] == (w[3] & 0x7FFFFFFF) * 2.0 ** -31
-        # the redraw's quantile is the 33-bit uniform of the low 11 bits of w0, w1, w2 (the
-        # bits their 52-bit uniforms leave), its Gaussian the three kick normals themselves
-        lo = [v & 0x7FF for v in w[:3]]
-        assert rec[4, i] == (lo[0] + (lo[1] << 11) + (lo[2] << 22)) * 2.0 ** -33
+        assert rec[4, i] == hi(w[3]) * 2.0 ** -32
+        # the redraw's Gaussian: the three kick normals themselves
         assert np.array_equal(rec[5:8, i], rec[0:3, i])
         w2 = _philox_words(99, 3, i, 17, 2)
         x0 = mp.sqrt(-2 * mp.log(u1(w2[0]))) * mp.cos(ang(w2[1]))
         assert abs(rec[8, i] - float(x0)) <= 4e-15 * max(1, abs(float(x0)))
+    # the general-coordinate step's normal: the first of the Box-Muller pair of words 0 and 1
+    pg = _ffi.make_params(dt=0.01, seed=99, stream=3, noise=_ffi.LMH_NOISE_PHILOX, coords=_ffi.LMH_GENERAL_COORDS)
+    recg = O.noise_record(pg, 1, 1, 5, 17, 1)[0]
+    for i in range(5):
+        w = _philox_words(99, 3, i, 17, 0)
+        want = mp.sqrt(-2 * mp.log(u1(w[0]))) * mp.cos(ang(w[1]))
+        assert abs(recg[0, i] - float(want)) <= 4e-15 * max(1, abs(float(want)))
+
+
+def test_kick_normals_jointly_gaussian():
+    """the three kick normals are jointly standard normal: |n|^2 ~ chi^2(3), n0^2 + n1^2 ~ chi^2(2),
+    n2^2 ~ chi^2(1), and n0 n1, n0 n2, n1 n2, n0^2 n2^2 carry no correlation (4e5 draws; KS
+    p > 1e-4, 5 sigma): a radius, split or angle taken from the wrong bits, or v and 1 - v
+    swapped into one component, fails here even where each channel alone looks normal"""
+    from scipy import stats
+    p = _ffi.make_params(dt=0.01, seed=21, stream=0, noise=_ffi.LMH_NOISE_PHILOX, collision=_ffi.LMH_LANGEVIN)
+    rec = O.noise_record(p, 2, 3, 200_000, 1, 2)
+    n = rec[:, 0:3, :].transpose(1, 0, 2).reshape(3, -1)
+    N = n.shape[1]
+    assert stats.kstest(np.sum(n ** 2, axis=0), "chi2", args=(3,)).pvalue > 1e-4
+    assert stats.kstest(n[0] ** 2 + n[1] ** 2, "chi2", args=(2,)).pvalue > 1e-4
+    assert stats.kstest(n[2] ** 2, "chi2", args=(1,)).pvalue > 1e-4
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(np.mean(n[a] * n[b])) < 5 / np.sqrt(N)
+    # fourth moments: E[n_a^2 n_b^2] = 1 for independent standard normals (3 for a = b)
+    for a, b in ((0, 2), (1, 2), (0, 1)):
+        assert abs(np.mean(n[a] ** 2 * n[b] ** 2) - 1) < 5 * np.sqrt(8 / N)
+    assert abs(np.mean(n[2] ** 4) - 3) < 5 * np.sqrt(96 / N)
 
 
 def test_noise_record_statistics():
