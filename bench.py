@@ -460,7 +460,7 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--slice", type=int, default=1)
     ap.add_argument("--defer", type=int, default=0)
-    ap.add_argument("--sort-every", type=int, default=200,
+    ap.add_argument("--sort-every", type=int, default=64,
                     help="steps between device sorts of the quarks by grid cell (0: never)")
     ap.add_argument("--collision", default="langevin_rta", choices=sorted(COLLISIONS),
                     help="diagnostic: another momentum update than the workload's")
@@ -603,7 +603,7 @@ def main():
     t_launch = min(t_window, t_bracketed)
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * N
     achieved = bytes_per_launch / t_launch / 1e9
-    default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox" and args.sort_every == 200
+    default_shape = (args.spl <= 1 and args.collision == "langevin_rta" and args.noise == "philox" and args.sort_every == 64
                      and args.defer == 0 and args.slice == 1 and not args.host_sort)
     traffic = args.traffic
     if traffic is None and default_shape:

@@ -176,11 +176,13 @@ typedef struct lmh_tuning {
     int32_t sort_every;        /* 2+1D grid, Philox noise, no freeze-out capture: lmh_run keeps
                                   the particles ordered by grid cell (species segments kept),
                                   sorting them on the device before its first step and again
-                                  every sort_every steps, so that neighbouring threads read
+                                  every sort_every steps (the specialised sliced step applies
+                                  the sort's permutation by reading through it), so that
+                                  neighbouring threads read
                                   neighbouring medium cells.  Each particle keeps its identity
                                   (its noise counters), so results do not depend on it; every
                                   other entry point first restores the caller's order.  0: never
-                                  (default 200)                                                 */
+                                  (default 64)                                                  */
 } lmh_tuning;
 
 /* device time per launch class, recorded with HIP events on the context's stream between

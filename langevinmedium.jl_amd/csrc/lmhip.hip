@@ -71,6 +71,8 @@ struct StepArgs {
     const SpeciesArgs* species_dev;        // the same in device memory, for per-lane indices
     const uint32_t* id;         // each slot's particle identity (its noise counters): the slot
                                 // itself unless the particles are ordered by grid cell
+    const uint32_t* perm;       // GATHER launches: slot i's particle is read from slot perm[i] of
+    uint32_t* wid;              // pos/mom/id and written to slot i of wpos/wmom/wid
     double4* qrec;              // deferred collisions: block b's records (two double4 each:
                                 // x, y, T, vx | vy, identity, slot, 0) at [b*qblock, ...)
     int32_t* qcount;            // deferred collisions per block of the step kernel
@@ -1043,8 +1045,9 @@ constexpr int kStepWaves = 3;
 // collision redraw 85 VGPRs and five waves, 4 % slower.  The generic kernels keep three.
 constexpr int kGridWaves = 7;
 
-template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER, bool SPEC>
+template <int PDIM, int NOISE, bool SLICE, int COLL, bool DEFER, bool SPEC, bool GATHER = false>
 __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) void k_cart_grid(StepArgs a, GridMedium g) {
+    static_assert(!GATHER || (SPEC && !DEFER), "the gathering step is the specialised in-kernel-collision one");
     // The specialised kernels are VALU-issue bound with seven waves on a SIMD: a wave issues its
     // memory requests (the state loads here, the slice gather below) at wave priority 1, ahead
     // of the other waves' arithmetic, so that the loads' latency starts as early as it can, and
@@ -1066,14 +1069,20 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
     // the specialised kernels run for N < 2^29 only, so a state row is addressed by a 32-bit byte
     // offset from its base (no 64-bit address arithmetic per row)
     const uint32_t o8 = (uint32_t)i * 8u;
-    double x0 = SPEC ? ld_off(a.pos[0], o8) : a.pos[0][i];
-    double x1 = SPEC ? ld_off(a.pos[1], o8) : a.pos[1][i];
+    // GATHER: the particle comes from slot perm[i] of the order the last cell sort found (the
+    // sort's permutation applied by this step's reads instead of a pass of its own)
+    const uint32_t src = GATHER ? *(const __attribute__((address_space(1))) uint32_t*)(
+                                      (const char*)a.perm + (uint32_t)i * 4u)
+                                : (uint32_t)i;
+    const uint32_t r8 = GATHER ? src * 8u : o8;
+    double x0 = SPEC ? ld_off(a.pos[0], r8) : a.pos[0][i];
+    double x1 = SPEC ? ld_off(a.pos[1], r8) : a.pos[1][i];
     double q[PDIM];
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) q[d] = SPEC ? ld_off(a.mom[d], o8) : a.mom[d][i];
+    for (int d = 0; d < PDIM; ++d) q[d] = SPEC ? ld_off(a.mom[d], r8) : a.mom[d][i];
     // the particle's identity keys its noise (the slot unless the particles are cell-sorted)
     const int64_t pid = SPEC ? (int64_t)*(const __attribute__((address_space(1))) uint32_t*)(
-                                   (const char*)a.id + (uint32_t)i * 4u)
+                                   (const char*)a.id + (r8 >> 1))
                              : (int64_t)a.id[i];
     // every state load is issued before the medium sample's address arithmetic waits for x, y
     __builtin_amdgcn_sched_barrier(0);
@@ -1186,6 +1195,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
         st_off(a.wpos[1], o8, x1);
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) st_off(a.wmom[d], o8, q[d]);
+        if (GATHER) *(__attribute__((address_space(1))) uint32_t*)((char*)a.wid + (o8 >> 1)) = (uint32_t)pid;
     } else {
         a.wpos[0][i] = x0;
         a.wpos[1][i] = x1;
@@ -1557,13 +1567,15 @@ struct lmh_ctx {
     int gx = 0, gy = 0;
     double gx0 = 0, gdx = 1, gy0 = 0, gdy = 1;
 
-    lmh_tuning tune = {256, 1, 1, 0, 1, 200};
+    lmh_tuning tune = {256, 1, 1, 0, 1, 64};
 
     // cell sorting: the alternate state and identity buffers of the permutation, the radix
     // sort's keys, values and workspace; is_sorted says the state is in cell order with d_id
     // holding each slot's identity
     bool is_sorted = false;
     int64_t steps_since_sort = 0;
+    bool perm_pending = false;        // the last sort's permutation (d_keys + 3N) not yet applied:
+                                      // the next step reads through it (or k_permute applies it)
     uint64_t order_gen = 0;           // bumped by every permutation of the state
     double* d_state_alt = nullptr;
     uint32_t* d_id = nullptr;
@@ -1877,27 +1889,44 @@ int sort_alloc(lmh_ctx* c) {
     return LMH_OK;
 }
 
-// radix-sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit), then
-// permute the state and the identities through the sorted values and swap the buffers
+// the state and identities now in the alternate buffers become the current ones
+void swap_order(lmh_ctx* c) {
+    std::swap(c->d_state, c->d_state_alt);
+    std::swap(c->d_id, c->d_id_alt);
+    ++c->order_gen;
+}
+
+// apply a pending permutation (d_keys + 3N) to the state and the identities
+int finish_permutation(lmh_ctx* c) {
+    if (!c->perm_pending) return LMH_OK;
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, c->d_state_alt,
+                       (int32_t)(c->dims + c->pdim), N, c->d_keys + 3 * N, c->d_id, c->d_id_alt);
+    LMH_HIP(c, hipGetLastError());
+    swap_order(c);
+    c->perm_pending = false;
+    return LMH_OK;
+}
+
+// radix-sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit): the
+// sorted values d_keys[3N, 4N) are the permutation (slot s takes the particle of slot perm[s]),
+// applied to the state and the identities at once
 int sort_apply(lmh_ctx* c, int end_bit) {
     const int64_t N = c->N;
     uint32_t* k = c->d_keys;
     size_t bytes = c->sort_tmp_bytes;
     LMH_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, bytes, k, k + N, k + 2 * N, k + 3 * N,
                                                    (int)N, 0, end_bit, c->stream));
-    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_state, c->d_state_alt,
-                       (int32_t)(c->dims + c->pdim), N, k + 3 * N, c->d_id, c->d_id_alt);
-    LMH_HIP(c, hipGetLastError());
-    std::swap(c->d_state, c->d_state_alt);
-    std::swap(c->d_id, c->d_id_alt);
-    ++c->order_gen;
-    return LMH_OK;
+    c->perm_pending = true;
+    return finish_permutation(c);
 }
 
-// order the particles by (species segment, grid cell); the 2+1D grid medium must be set
-int sort_by_cell(lmh_ctx* c) {
-    int rc = sort_alloc(c);
+// order the particles by (species segment, grid cell); the 2+1D grid medium must be set.
+// defer_permute: the permutation is left for the next step launch to apply (perm_pending).
+int sort_by_cell(lmh_ctx* c, bool defer_permute = false) {
+    int rc = finish_permutation(c);
     if (rc) return rc;
+    if ((rc = sort_alloc(c))) return rc;
     LaunchMark mk(c, LMH_PROF_SORT);
     const int64_t N = c->N;
     SortKeyArgs ka;
@@ -1917,7 +1946,20 @@ int sort_by_cell(lmh_ctx* c) {
     hipLaunchKernelGGL(k_sort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, ka, make_grid(c),
                        c->d_keys, c->d_keys + 2 * N);
     LMH_HIP(c, hipGetLastError());
-    if ((rc = sort_apply(c, end_bit))) return rc;
+    // each species segment is radix-sorted on its own over the cell bits (the segment bits are
+    // the same within it): a pass fewer than one sort over (segment, cell) when there are two
+    // species.  The sorted values d_keys[3N, 4N) are the permutation.
+    uint32_t* k = c->d_keys;
+    for (int sg = 0; sg < c->nspecies; ++sg) {
+        const int64_t b0 = ka.begin[sg], n = ka.begin[sg + 1] - b0;
+        if (n <= 0) continue;
+        size_t bytes = c->sort_tmp_bytes;
+        LMH_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, bytes, k + b0, k + N + b0, k + 2 * N + b0,
+                                                       k + 3 * N + b0, (int)n, 0, std::max(1, ka.cell_bits),
+                                                       c->stream));
+    }
+    c->perm_pending = true;
+    if (!defer_permute && (rc = finish_permutation(c))) return rc;
     c->is_sorted = true;
     c->steps_since_sort = 0;
     return LMH_OK;
@@ -1927,6 +1969,8 @@ int sort_by_cell(lmh_ctx* c) {
 int ensure_unsorted(lmh_ctx* c) {
     if (!c->is_sorted) return LMH_OK;
     LMH_HIP(c, hipSetDevice(c->device));
+    int rc0 = finish_permutation(c);
+    if (rc0) return rc0;
     LaunchMark mk(c, LMH_PROF_SORT);
     const int64_t N = c->N;
     hipLaunchKernelGGL(k_unsort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, c->stream, c->d_id, N, c->d_keys,
@@ -2202,6 +2246,7 @@ int lmh_set_particles(lmh_ctx* c, const double* pos, int32_t dims, const double*
     }
     dfree(c->d_state);
     c->is_sorted = false;
+    c->perm_pending = false;
     c->steps_since_sort = 0;
     if (c->sort_n != N || (c->d_state_alt && c->dims + c->pdim != dims + pdim)) {
         dfree(c->d_state_alt);
@@ -2457,6 +2502,20 @@ void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium&
     }
 }
 
+// the specialised sliced step with in-kernel collisions, reading through a sort's permutation
+template <int PDIM>
+void launch_grid_gather(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll) {
+    const dim3 G(nb), B(c->tune.block);
+    LaunchMark mk(c, LMH_PROF_STEP);
+    constexpr int P = LMH_NOISE_PHILOX;
+    switch (coll) {
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_cart_grid<PDIM, P, true, LMH_LANGEVIN, false, true, true>), G, B, 0, c->stream, a, g); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_cart_grid<PDIM, P, true, LMH_RTA, false, true, true>), G, B, 0, c->stream, a, g); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_cart_grid<PDIM, P, true, LMH_FLUID, false, true, true>), G, B, 0, c->stream, a, g); break;
+        default: hipLaunchKernelGGL((k_cart_grid<PDIM, P, true, LMH_LANGEVIN_RTA, false, true, true>), G, B, 0, c->stream, a, g); break;
+    }
+}
+
 template <int PDIM, bool SPEC>
 void launch_grid_n(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int noise, bool slice,
                    int coll, bool defer) {
@@ -2503,12 +2562,15 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
     int64_t ring0 = -1;       // first step of the slices in the ring
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        const bool slice = grid && n == 1 && c->tune.slice;
+        // the specialised sliced step with in-kernel collisions applies a sort's permutation by
+        // reading through it (GATHER) instead of a permutation pass of its own
+        const bool gather_ok = grid && slice && spec && !defer && p->noise == LMH_NOISE_PHILOX;
         if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every)) {
-            const int rc = sort_by_cell(c);
+            const int rc = sort_by_cell(c, gather_ok);
             if (rc) return rc;
             a = make_args(c, p, c->tune.block, &nb);
         }
-        const bool slice = grid && n == 1 && c->tune.slice;
         GridMedium gstep = gmed;
         if (slice) {
             // the slices of the next kSliceRing steps in one launch, then one per step from the ring
@@ -2531,7 +2593,28 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_FIXED>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
             else
                 hipLaunchKernelGGL((k_gc<LMH_NOISE_PHILOX>), dim3(nb), dim3(c->tune.block), 0, c->stream, a, rmed);
+        } else if (grid && c->perm_pending && gather_ok) {
+            // read the old order through the permutation, write the new order into the alternate
+            // buffers, then make them current
+            StepArgs ag = a;
+            const int64_t N = c->N;
+            for (int d = 0; d < 3; ++d) {
+                ag.wpos[d] = d < c->dims ? c->d_state_alt + (int64_t)d * N : nullptr;
+                ag.wmom[d] = d < c->pdim ? c->d_state_alt + (int64_t)(c->dims + d) * N : nullptr;
+            }
+            ag.perm = c->d_keys + 3 * N;
+            ag.wid = c->d_id_alt;
+            if (c->pdim == 2) launch_grid_gather<2>(c, ag, nb, gstep, p->collision);
+            else launch_grid_gather<3>(c, ag, nb, gstep, p->collision);
+            swap_order(c);
+            c->perm_pending = false;
+            a = make_args(c, p, c->tune.block, &nb);
         } else if (grid) {
+            if (c->perm_pending) {
+                const int rc = finish_permutation(c);
+                if (rc) return rc;
+                a = make_args(c, p, c->tune.block, &nb);
+            }
             if (c->pdim == 2) launch_grid<2>(c, a, nb, gstep, p->noise, slice, p->collision, defer && n == 1, spec);
             else launch_grid<3>(c, a, nb, gstep, p->noise, slice, p->collision, defer && n == 1, spec);
         } else if (milne_spec(c, p)) {
@@ -2546,6 +2629,10 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
         }
         LMH_HIP(c, hipGetLastError());
         c->steps_since_sort += n;
+    }
+    {
+        const int rc = finish_permutation(c);
+        if (rc) return rc;
     }
     if (p->check_finite) {
         unsigned long long bad = ULLONG_MAX;

@@ -69,7 +69,7 @@ class Context:
         self._check(self._lib.lmh_synchronize(self._h))
 
     # ---- launch shape ----
-    def set_tuning(self, block=256, steps_per_launch=1, slice=1, defer=0, specialize=1, sort_every=200):
+    def set_tuning(self, block=256, steps_per_launch=1, slice=1, defer=0, specialize=1, sort_every=64):
         t = _ffi.lmh_tuning(int(block), int(steps_per_launch), int(slice), int(defer), int(specialize),
                             int(sort_every))
         self._check(self._lib.lmh_set_tuning(self._h, ctypes.byref(t)))
