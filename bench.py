@@ -621,7 +621,10 @@ def main():
             "other_kernels_ms_per_step": {**{k: round(v["ms"] / args.steps, 4) for k, v in prof.items()
                                              if k not in ("step", "sort")},
                                           "sort": round(sort_ms / args.sort_every, 4) if args.sort_every > 0 else 0.0},
-            "sort_ms_per_sort": round(sort_ms, 4)}
+            "sort_ms_per_sort": round(sort_ms, 4),
+            "sort_note": "a sort: its keys and radix sort, and the step launch that applies the "
+                         "permutation by reading through it (that step's work included); the "
+                         "step kernel's launches are the other steps"}
     out = {
         "metric": "particle-steps/s at 1e7 quarks, 2+1D medium",
         "value": round(value, 1),

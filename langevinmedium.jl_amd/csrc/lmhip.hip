@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <optional>
 #include <new>
 #include <string>
 #include <vector>
@@ -1150,7 +1151,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
             const bool read_drag = HAS_KICK && drag_table_read(p);
             double tn = 0.0, tau_d = 0.0;
             if (HAS_RTA && read_drag) {
-                eval_time_table_pair(sp.rta, sp.drag, sp.shared_axis != 0, T, &tn, &tau_d);
+                eval_time_table_pair<SPEC>(sp.rta, sp.drag, sp.shared_axis != 0, T, &tn, &tau_d);
             } else {
                 if (HAS_RTA) tn = rta_lookup(sp.rta, T);
                 if (read_drag) tau_d = eval_time_table(sp.drag, T);
@@ -1175,7 +1176,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
                 } else if (col) {
                     double usamp, gg[PDIM];
                     noise_redraw<PDIM, NOISE>(a, pid, s_off, xi, usamp_p, &usamp, gg);
-                    rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
+                    rta_redraw<PDIM, false, true, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, gg);
                 }
             }
             apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
@@ -1923,11 +1924,13 @@ int sort_apply(lmh_ctx* c, int end_bit) {
 
 // order the particles by (species segment, grid cell); the 2+1D grid medium must be set.
 // defer_permute: the permutation is left for the next step launch to apply (perm_pending).
-int sort_by_cell(lmh_ctx* c, bool defer_permute = false) {
+// mark: time the sort's launches as one LMH_PROF_SORT entry (the caller times them otherwise).
+int sort_by_cell(lmh_ctx* c, bool defer_permute = false, bool mark = true) {
     int rc = finish_permutation(c);
     if (rc) return rc;
     if ((rc = sort_alloc(c))) return rc;
-    LaunchMark mk(c, LMH_PROF_SORT);
+    std::optional<LaunchMark> mk;
+    if (mark) mk.emplace(c, LMH_PROF_SORT);
     const int64_t N = c->N;
     SortKeyArgs ka;
     memset(&ka, 0, sizeof(ka));
@@ -2412,8 +2415,13 @@ int check_steps(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nst
     return LMH_OK;
 }
 
-// the parameters are those the SPEC kernels fix at compile time
+// the parameters are those the SPEC kernels fix at compile time, and every species' drag and
+// relaxation-time tables (where set) have two nodes or more
 bool spec_config(const lmh_ctx* c, const lmh_params* p) {
+    for (int k = 0; k < c->nspecies; ++k) {
+        const SpeciesArgs& s = c->h_species[k];
+        if ((s.drag.vals && s.drag.n < 2) || (s.rta.vals && s.rta.n < 2)) return false;
+    }
     return c->tune.specialize && !c->fo_on && c->N < ((int64_t)1 << 29) && p->coords == LMH_CARTESIAN &&
            p->relativistic &&
            p->integrator == LMH_PREPOINT &&
@@ -2503,10 +2511,10 @@ void launch_grid_s(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium&
 }
 
 // the specialised sliced step with in-kernel collisions, reading through a sort's permutation
+// (timed by the caller with the sort)
 template <int PDIM>
 void launch_grid_gather(lmh_ctx* c, const StepArgs& a, unsigned nb, const GridMedium& g, int coll) {
     const dim3 G(nb), B(c->tune.block);
-    LaunchMark mk(c, LMH_PROF_STEP);
     constexpr int P = LMH_NOISE_PHILOX;
     switch (coll) {
         case LMH_LANGEVIN: hipLaunchKernelGGL((k_cart_grid<PDIM, P, true, LMH_LANGEVIN, false, true, true>), G, B, 0, c->stream, a, g); break;
@@ -2566,11 +2574,6 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
         // the specialised sliced step with in-kernel collisions applies a sort's permutation by
         // reading through it (GATHER) instead of a permutation pass of its own
         const bool gather_ok = grid && slice && spec && !defer && p->noise == LMH_NOISE_PHILOX;
-        if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every)) {
-            const int rc = sort_by_cell(c, gather_ok);
-            if (rc) return rc;
-            a = make_args(c, p, c->tune.block, &nb);
-        }
         GridMedium gstep = gmed;
         if (slice) {
             // the slices of the next kSliceRing steps in one launch, then one per step from the ring
@@ -2583,6 +2586,14 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
                                    c->d_slice, step_begin + s0, nk, p->dt, p->t0);
             }
             gstep.slice = c->d_slice + rec * (s0 - ring0);
+        }
+        // a sort that the step applies is timed with that step, as one sort entry
+        std::optional<LaunchMark> sort_mark;
+        if (sortable && (!c->is_sorted || c->steps_since_sort >= c->tune.sort_every)) {
+            if (gather_ok) sort_mark.emplace(c, LMH_PROF_SORT);
+            const int rc = sort_by_cell(c, gather_ok, !gather_ok);
+            if (rc) return rc;
+            a = make_args(c, p, c->tune.block, &nb);
         }
         a.step_begin = step_begin + s0;
         a.nsteps = n;
@@ -2606,6 +2617,7 @@ int run_impl(lmh_ctx* c, const lmh_params* p, int64_t step_begin, int64_t nsteps
             ag.wid = c->d_id_alt;
             if (c->pdim == 2) launch_grid_gather<2>(c, ag, nb, gstep, p->collision);
             else launch_grid_gather<3>(c, ag, nb, gstep, p->collision);
+            sort_mark.reset();
             swap_order(c);
             c->perm_pending = false;
             a = make_args(c, p, c->tune.block, &nb);

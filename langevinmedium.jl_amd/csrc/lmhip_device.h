@@ -116,7 +116,9 @@ __device__ __forceinline__ double eval_time_table(const TimeTable& t, double T) 
 // Two time tables read at one T.  Tables built over one medium's temperature range share their
 // axis (the same Tmin, 1/dT and node count; the drag and RTA tables of a species are, which the
 // host records in `shared`): the segment and the weight are then found once, and each value is
-// the one eval_time_table gives.
+// the one eval_time_table gives.  MULTI: the host has checked that the tables have two nodes or
+// more (the one-node case compiled out).
+template <bool MULTI = false>
 __device__ __forceinline__ void eval_time_table_pair(const TimeTable& a, const TimeTable& b, bool shared,
                                                      double T, double* ya, double* yb) {
     if (!shared) {
@@ -126,7 +128,7 @@ __device__ __forceinline__ void eval_time_table_pair(const TimeTable& a, const T
     }
     double u = (T - a.Tmin) * a.invdT;
     const double fc = fmin(fmax(floor(u), 0.0), (double)(a.n - 2));
-    const bool one = a.n < 2;
+    const bool one = !MULTI && a.n < 2;
     const int i = one ? 0 : (int)fc + 1;
     double w = u - fc;
     const uint32_t o0 = (uint32_t)(one ? 0 : i - 1) * 8u, o1 = (uint32_t)i * 8u;
@@ -146,11 +148,14 @@ struct JuttnerTable {
     int nU, nT;
 };
 
+// UNIT: u is known to lie in [0, 1) (a Philox quantile), where the clamps of u and of its cell
+// change nothing
+template <bool UNIT = false>
 __device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double u, double T) {
-    double uu = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double uu = UNIT ? u : (u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u));
     double fu = uu * (double)(jt.nU - 1);
     int iu = (int)floor(fu);
-    iu = iu < 0 ? 0 : (iu > jt.nU - 2 ? jt.nU - 2 : iu);
+    if (!UNIT) iu = iu < 0 ? 0 : (iu > jt.nU - 2 ? jt.nU - 2 : iu);
     double tu = fu - (double)iu;
     double fT = (T - jt.Tmin) * jt.invdT;
     double fl = floor(fT);
@@ -247,6 +252,17 @@ __device__ __forceinline__ double sqrt_pos(double x) {
     const double g = x * y;
     const double r = fma(fma(-g, g, x), 0.5 * y, g);
     return (POS || x > 0.0) ? r : x;
+}
+
+// sqrt(x) for x = 0 or x >= 1e-300 (noise radii and kick widths: a product of factors each far
+// from the underflow range, or exactly 0): the estimate taken at max(x, 1e-300), whose residual
+// step returns exactly 0 for x = 0 (g = 0), so no select is needed; otherwise sqrt_pos.
+__device__ __forceinline__ double sqrt_nn(double x) {
+    double y = __builtin_amdgcn_rsq(fmax(x, 1e-300));
+    const double h = 0.5 * x;
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    const double g = x * y;
+    return fma(fma(-g, g, x), 0.5 * y, g);
 }
 
 // fma(a, b, c) for a wave-uniform constant c, read from a scalar register pair by the v_fma_f64
@@ -417,13 +433,13 @@ __device__ __forceinline__ void kick_normals(const uint64_t w[4], double* n) {
     const double P = u_oc(w[0]) * (((double)(uint32_t)w[1] + 1.0) * 0x1.0p-32);
     const double L = -2.0 * noise_log(P);
     const double v = (double)(uint32_t)w[2] * 0x1.0p-32;
-    const double a = sqrt_pos(v * L);
+    const double a = sqrt_nn(v * L);
     double s, c;
     sincos_u32((uint32_t)(w[1] >> 32), &s, &c);
     n[0] = a * c;
     if (NN >= 2) n[1 % NN] = a * s;
     if (NN >= 3) {
-        const double b = sqrt_pos((1.0 - v) * L);
+        const double b = sqrt_nn((1.0 - v) * L);
         const uint32_t hg = (uint32_t)(w[2] >> 32);
         const double nb = b * noise_cospi_half((double)(hg & 0x7FFFFFFFu) * 0x1.0p-32);
         n[2 % NN] = (hg >> 31) ? -nb : nb;
@@ -621,11 +637,11 @@ __device__ __forceinline__ bool rta_collides(const lmh_params& p, const TimeTabl
 // The redraw of kernel_rta_collision_gpu! (kernels_gpu.jl:673-701): |p*| from the table, the
 // direction g normalised (the normalised Gaussians of kernels_cpu.jl:497-503).  FAST takes
 // p* / |g| from the refined reciprocal square root (within an ulp or two of the division by the
-// square root).
-template <int PDIM, bool RADIAL, bool FAST = false>
+// square root); UNIT as in juttner_lookup.
+template <int PDIM, bool RADIAL, bool FAST = false, bool UNIT = false>
 __device__ __forceinline__ void rta_redraw(const JuttnerTable& jt, double T, double* q,
                                            double usamp, const double* g) {
-    double pstar = juttner_lookup(jt, usamp, T);
+    double pstar = juttner_lookup<UNIT>(jt, usamp, T);
     if (RADIAL) {
         q[0] = g[0] < 0.0 ? -pstar : pstar;
     } else if (FAST) {
