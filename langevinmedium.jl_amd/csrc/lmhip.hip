@@ -636,10 +636,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_milne(StepArgs a, RadialMedium me
                 drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
                 ou_kick<3, false, true>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
             }
-            if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) && rta_collides(p, sp.rta, T, ucol)) {
+            if ((COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA) &&
+                rta_decide<NOISE == LMH_NOISE_PHILOX>(p, rta_lookup(sp.rta, T), T, ucol)) {
                 double usamp, g[3];
                 noise_redraw<3, NOISE>(a, i, s_off, xi, usamp_p, &usamp, g);
-                rta_redraw<3, false, true>(sp.jt, T, q, usamp, g);
+                rta_redraw<3, false, true, NOISE == LMH_NOISE_PHILOX>(sp.jt, T, q, usamp, g);
             }
             if (!skip) radial_boost_fast(q, rhx, rhy, v, gam, m, +1);
         }
@@ -1160,7 +1161,7 @@ __global__ __launch_bounds__(kBlock, (SLICE && SPEC) ? kGridWaves : kStepWaves) 
             if (HAS_KICK) drag_coeffs_tau<false>(p, tau_d, m, T, &etaD, &kT);
             if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
             if (HAS_RTA) {
-                const bool col = rta_decide(p, tn, T, ucol);
+                const bool col = rta_decide<NOISE == LMH_NOISE_PHILOX>(p, tn, T, ucol);
                 if (DEFER) {
                     const int rank = block_rank(valid && col, &a.qcount[blockIdx.x]);
                     if (!valid) return;

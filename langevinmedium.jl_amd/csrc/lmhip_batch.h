@@ -92,10 +92,10 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         double ucol = 1.0, usamp_p = 0.0;
         noise_kick<PDIM, LMH_NOISE_PHILOX, HAS_KICK ? 1 : 0, HAS_RTA ? 1 : 0>(a, i, step, 0, xi, &ucol, &usamp_p);
         if (HAS_KICK) ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
-        if (HAS_RTA && rta_decide(p, tn, T, ucol)) {
+        if (HAS_RTA && rta_decide<true>(p, tn, T, ucol)) {
             double usamp, gg[PDIM];
             noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
-            rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
+            rta_redraw<PDIM, false, true, true>(sp.jt, T, q, usamp, gg);
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }

@@ -159,10 +159,8 @@ __device__ __forceinline__ double juttner_lookup(const JuttnerTable& jt, double 
     double tu = fu - (double)iu;
     double fT = (T - jt.Tmin) * jt.invdT;
     double fl = floor(fT);
-    int jT;
-    if (!(fl >= 0.0)) jT = 0;
-    else if (fl > (double)(jt.nT - 2)) jT = jt.nT - 2;
-    else jT = (int)fl;
+    // the cell clamped to [0, nT - 2] (a NaN fraction takes cell 0)
+    const int jT = (int)fmin(fmax(fl, 0.0), (double)(jt.nT - 2));
     double tT = fT - (double)jT;
     tT = tT < 0.0 ? 0.0 : (tT > 1.0 ? 1.0 : tT);
     const uint32_t o0 = (uint32_t)(jT * jt.nU + iu) * 8u;
@@ -621,7 +619,15 @@ __device__ __forceinline__ double rta_lookup(const TimeTable& taun, double T) {
     return eval_time_table(taun, T);
 }
 
+// UNIT: ucol lies in [0, 1) (a Philox uniform), so ucol < 1 holds: a collision is certain unless
+// D_sT > 0, T > 0 and tau_n is a finite positive number (one class test), and otherwise decided
+// by ucol tau_n < dt -- the same decision in fewer instructions.
+template <bool UNIT = false>
 __device__ __forceinline__ bool rta_decide(const lmh_params& p, double tn, double T, double ucol) {
+    if (UNIT) {
+        const bool pos = __builtin_amdgcn_class(tn, 0x180);   // +subnormal or +normal
+        return !(p.DsT > 0.0 && T > 0.0 && pos) || ucol * tn < p.dt;
+    }
     const double tau_n = (p.DsT > 0.0 && T > 0.0) ? tn : 0.0;
     // ucol < clamp(dt/tau_n, 0, 1) with ucol in [0, 1) is ucol * tau_n < dt for tau_n > 0 (no
     // division; the two forms differ only when ucol * tau_n is within an ulp of dt)
@@ -648,7 +654,8 @@ __device__ __forceinline__ void rta_redraw(const JuttnerTable& jt, double T, dou
         double nrm2 = g[0] * g[0];
 #pragma unroll
         for (int d = 1; d < PDIM; ++d) nrm2 = fma(g[d], g[d], nrm2);
-        const double inv = pstar * rsqrt_pos(jl_max(nrm2, 1e-300));
+        // fmax, not jl_max: a NaN norm (a NaN g) gives NaN components either way
+        const double inv = pstar * rsqrt_pos(fmax(nrm2, 1e-300));
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) q[d] = g[d] * inv;
     } else {
