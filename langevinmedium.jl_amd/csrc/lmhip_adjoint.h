@@ -148,10 +148,10 @@ __device__ __forceinline__ void adj_forward_step(const StepArgs& a, const lmh_pa
     double etaD = 0.0, kT = 0.0;
     drag_coeffs<false>(p, sp.drag, m, T, &etaD, &kT);
     ou_kick<PDIM, false>(p, etaD, kT, T, m, a.sqrt_dt, q, xi);
-    if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+    if (HAS_RTA && rta_decide<true>(p, rta_lookup(sp.rta, T), T, ucol)) {
         double usamp, gg[PDIM];
         noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
-        rta_redraw<PDIM, false, true>(sp.jt, T, q, usamp, gg);
+        rta_redraw<PDIM, false, true, true>(sp.jt, T, q, usamp, gg);
     }
     apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
@@ -280,7 +280,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
     double qC[PDIM], ghat[PDIM], dpstar = 0.0;
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) qC[d] = qK[d];
-    if (HAS_RTA && rta_collides(p, sp.rta, T, ucol)) {
+    if (HAS_RTA && rta_decide<true>(p, rta_lookup(sp.rta, T), T, ucol)) {
         col = true;
         double usamp, gg[PDIM];
         noise_redraw<PDIM, LMH_NOISE_PHILOX>(a, i, 0, xi, usamp_p, &usamp, gg);
@@ -288,7 +288,7 @@ __device__ __forceinline__ void adj_reverse_step(const StepArgs& a, const lmh_pa
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) nrm2 += gg[d] * gg[d];
         const double inv = drv_rsq(jl_max(nrm2, 1e-300));
-        const double pstar = juttner_lookup(sp.jt, usamp, T);
+        const double pstar = juttner_lookup<true>(sp.jt, usamp, T);
 #pragma unroll
         for (int d = 0; d < PDIM; ++d) {
             ghat[d] = gg[d] * inv;
