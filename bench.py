@@ -272,8 +272,11 @@ def bench_batch(args):
     params = make_params(0.0, False, "langevin_rta")
     b.boost_to_lab(params, 0)
     tune, tuned_ms = b.autotune(params, trial_steps=3)
+    if args.slice_min >= 0:
+        tune["slice_min"] = args.slice_min
     b.set_tuning(block=tune["block"], steps_per_launch=tune["steps_per_launch"],
-                 persistent=tune["persistent"], graph_steps=10, sort_every=tune["sort_every"])
+                 persistent=tune["persistent"], graph_steps=10, sort_every=tune["sort_every"],
+                 slice_min=tune["slice_min"])
     setup = time.perf_counter() - t_setup
     print(f"batch setup {setup:.1f} s; {R} runs, {N} quarks, tuning {tune} ({tuned_ms:.3f} ms/step)",
           file=sys.stderr, flush=True)
@@ -450,6 +453,9 @@ def main():
                          "in one launch; adjoint: gradient / forward time of the pathwise adjoint")
     ap.add_argument("--bj-n", type=int, default=1_000_000, help="bjorken: quarks")
     ap.add_argument("--runs", type=int, default=256, help="batch: number of runs")
+    ap.add_argument("--slice-min", type=int, default=-1,
+                    help="batch: runs of at least this many quarks read per-step medium slices "
+                         "(-1: the autotuner's choice)")
     ap.add_argument("--adj-n", type=int, default=1_000_000, help="adjoint: quarks")
     ap.add_argument("--adj-steps", type=int, default=1000, help="adjoint: steps")
     ap.add_argument("--adj-k", type=int, default=50, help="adjoint: checkpoint cadence")

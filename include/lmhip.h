@@ -234,6 +234,11 @@ typedef struct lmh_batch_tuning {
                                   cell, sorting them on the device before its first step and
                                   then every sort_every steps (rounded up to whole graph
                                   replays); the results do not depend on it; 0: never        */
+    int32_t slice_min;         /* > 0: the runs of at least slice_min quarks read their medium
+                                  from per-step slices (the grid interpolated in tau once per
+                                  step and run, as lmh_run's one-step launches do, with the
+                                  same results as such a run alone); the other runs interpolate
+                                  each corner in tau per quark; 0: no slices                  */
 } lmh_batch_tuning;
 
 typedef struct lmh_ctx lmh_ctx;
@@ -426,9 +431,11 @@ int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step);
 /* steps step_begin .. step_begin+nsteps-1 of every run; p->DsT is ignored (each run's own)  */
 int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps);
 /* times trial_steps steps of every launch shape (block 64/128/256 x persistent 0/1 x
- * steps_per_launch 1/2/4) on this batch, then 200 steps at the fastest shape for each sort
- * cadence (sort_every 0, 50, 200), keeps the fastest (graph_steps unchanged) and restores the
- * state; the results do not depend on the shape or the cadence.  Synchronises.             */
+ * steps_per_launch 1/2/4 x slice_min 0/2^18) on this batch, then 200 steps at the fastest shape
+ * for each sort cadence (sort_every 0, 50, 200), keeps the fastest (graph_steps unchanged) and
+ * restores the state; the results do not depend on the block, persistence, steps per launch or
+ * cadence (slice_min selects each run's form of the medium interpolation, whose results agree
+ * to rounding).  Synchronises.                                                              */
 int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps,
                        lmh_batch_tuning* best, double* best_ms_per_step);
 /* out (nruns, 5 + nbins): lmh_observables of every run (spec->species is ignored)           */

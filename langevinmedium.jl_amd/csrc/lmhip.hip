@@ -3546,8 +3546,17 @@ struct lmh_batch {
     int32_t* d_tile_run = nullptr;
     int64_t ntiles = 0;
     int tiles_block = 0;              // block size the tile table was built for
+    int layout_slice_min = 0;         // ... and slice_min and steps per launch (the slice rings)
+    int layout_spl = 0;
+    // per-step medium slices of the runs of at least slice_min quarks: run srun[k]'s ring of
+    // steps_per_launch slices in d_bslice, filled by k_batch_slices before each step launch
+    double4* d_bslice = nullptr;
+    size_t bslice_cap = 0;            // double4 records
+    int32_t* d_srun = nullptr;
+    int32_t nsr = 0;
+    int64_t max_rec = 0;              // the largest slice (records) of a sliced run
 
-    lmh_batch_tuning tune = {256, 1, 0, 0, 0};
+    lmh_batch_tuning tune = {256, 1, 0, 0, 0, 0};
 
     // cell order (tune.sort_every > 0): each run's quarks ordered by grid cell within the run's
     // segment; d_id[s] is the original slot of the quark in slot s (its Philox counter is d_id[s]
@@ -3698,11 +3707,32 @@ int batch_dev(lmh_batch* b, double** ptr, size_t* cap, size_t n, const char* wha
 
 // tile table and per-run records for the current block size
 int batch_layout(lmh_batch* b, int block) {
-    if (b->tiles_block == block && b->d_runs) return LMH_OK;
+    const int K = std::max(1, b->tune.steps_per_launch);
+    const int smin = b->tune.slice_min;
+    if (b->tiles_block == block && b->layout_slice_min == smin && b->layout_spl == K && b->d_runs) return LMH_OK;
     const int R = (int)b->runs.size();
     std::vector<BatchRun> h(R);
-    std::vector<int32_t> tr;
+    std::vector<int32_t> tr, srun;
     int64_t t0 = 0;
+    // the slice rings: K steps of 3 records per cell for every run of at least slice_min quarks
+    int64_t nrec = 0, max_rec = 0;
+    for (int r = 0; r < R; ++r) {
+        const lmh_run_desc& d = b->runs[r];
+        if (smin <= 0 || d.count < smin) continue;
+        const lmh_batch::Med& md = b->media[d.medium];
+        const int64_t rec = 3 * (int64_t)(md.nx - 1) * (md.ny - 1);
+        nrec += K * rec;
+        max_rec = std::max(max_rec, rec);
+        srun.push_back(r);
+    }
+    if ((size_t)nrec > b->bslice_cap) {
+        dfree(b->d_bslice);
+        b->bslice_cap = 0;
+        if (hipMalloc((void**)&b->d_bslice, (size_t)nrec * sizeof(double4)) != hipSuccess)
+            return b->fail(LMH_E_NOMEM, "batch: medium slice allocation failed");
+        b->bslice_cap = (size_t)nrec;
+    }
+    int64_t soff = 0;
     for (int r = 0; r < R; ++r) {
         const lmh_run_desc& d = b->runs[r];
         const lmh_batch::Med& md = b->media[d.medium];
@@ -3738,6 +3768,11 @@ int batch_layout(lmh_batch* b, int block) {
         }
         o.sp.shared_axis = tables_share_axis(o.sp.drag, o.sp.rta);
         o.DsT = d.DsT;
+        if (smin > 0 && d.count >= smin) {
+            o.slice_stride = 3 * (int64_t)(md.nx - 1) * (md.ny - 1);
+            o.g.slice = b->d_bslice + soff;
+            soff += K * o.slice_stride;
+        }
         const int64_t nt = (d.count + block - 1) / block;
         for (int64_t k = 0; k < nt; ++k) tr.push_back(r);
         t0 += nt;
@@ -3745,13 +3780,21 @@ int batch_layout(lmh_batch* b, int block) {
     batch_drop_graph(b);
     dfree(b->d_runs);
     dfree(b->d_tile_run);
+    dfree(b->d_srun);
+    b->nsr = 0;
     if (hipMalloc((void**)&b->d_runs, std::max<size_t>(1, R) * sizeof(BatchRun)) != hipSuccess ||
-        hipMalloc((void**)&b->d_tile_run, std::max<size_t>(1, tr.size()) * sizeof(int32_t)) != hipSuccess)
+        hipMalloc((void**)&b->d_tile_run, std::max<size_t>(1, tr.size()) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_srun, std::max<size_t>(1, srun.size()) * sizeof(int32_t)) != hipSuccess)
         return b->fail(LMH_E_NOMEM, "batch: device allocation failed");
     if (R) LMB_HIP(b, hipMemcpy(b->d_runs, h.data(), R * sizeof(BatchRun), hipMemcpyHostToDevice));
     if (!tr.empty()) LMB_HIP(b, hipMemcpy(b->d_tile_run, tr.data(), tr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!srun.empty()) LMB_HIP(b, hipMemcpy(b->d_srun, srun.data(), srun.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     b->ntiles = t0;
     b->tiles_block = block;
+    b->layout_slice_min = smin;
+    b->layout_spl = K;
+    b->nsr = (int32_t)srun.size();
+    b->max_rec = max_rec;
     return LMH_OK;
 }
 
@@ -3834,6 +3877,11 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a, int coll, int64_t nsteps) {
     }
     for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        if (b->nsr > 0) {
+            // the sliced runs' medium for the launch's n steps
+            const dim3 G((unsigned)((b->max_rec * n + kBlock - 1) / kBlock), (unsigned)b->nsr);
+            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, b->d_srun, (int32_t)s0, n);
+        }
         if (b->pdim == 2) {
             if (b->tune.persistent) batch_launch_p<2, true>(b, a, coll, (int32_t)s0, n, grid);
             else batch_launch_p<2, false>(b, a, coll, (int32_t)s0, n, grid);
@@ -3959,6 +4007,8 @@ void lmh_batch_destroy(lmh_batch* b) {
     dfree(b->d_begin);
     dfree(b->d_runs);
     dfree(b->d_tile_run);
+    dfree(b->d_srun);
+    dfree(b->d_bslice);
     dfree(b->d_obs);
     dfree(b->d_save);
     batch_free_order(b);
@@ -4080,6 +4130,8 @@ int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, co
     dfree(b->d_begin);
     dfree(b->d_runs);
     dfree(b->d_tile_run);
+    dfree(b->d_srun);
+    b->nsr = 0;
     batch_free_order(b);
     b->tiles_block = 0;
     b->runs.assign(runs, runs + nruns);
@@ -4130,9 +4182,11 @@ int lmh_batch_get_particles(lmh_batch* b, double* pos, double* mom) {
 int lmh_batch_set_tuning(lmh_batch* b, const lmh_batch_tuning* t) {
     if (!b) return LMH_E_INVALID;
     if (!t || (t->block != 64 && t->block != 128 && t->block != 256) || t->steps_per_launch < 1 ||
-        (t->persistent != 0 && t->persistent != 1) || t->graph_steps < 0 || t->sort_every < 0)
+        (t->persistent != 0 && t->persistent != 1) || t->graph_steps < 0 || t->sort_every < 0 ||
+        t->slice_min < 0)
         return b->fail(LMH_E_INVALID, "batch_set_tuning: block in {64, 128, 256}, steps_per_launch >= 1, "
-                                      "persistent in {0, 1}, graph_steps >= 0, sort_every >= 0");
+                                      "persistent in {0, 1}, graph_steps >= 0, sort_every >= 0, "
+                                      "slice_min >= 0");
     b->tune = *t;
     return LMH_OK;
 }
@@ -4191,14 +4245,16 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     LMB_HIP(b, hipEventCreate(&e1));
     const int blocks[3] = {64, 128, 256};
     const int spls[3] = {1, 2, 4};
+    // medium slices for no run, or for the runs of at least 2^18 quarks
+    const int slice_mins[2] = {0, 1 << 18};
     for (int bi = 0; bi < 3 && rc == LMH_OK; ++bi)
         for (int pe = 0; pe < 2 && rc == LMH_OK; ++pe)
-            for (int si = 0; si < 3 && rc == LMH_OK; ++si) {
-                b->tune = lmh_batch_tuning{blocks[bi], spls[si], pe, 0, 0};
+            for (int si = 0; si < 3 * 2 && rc == LMH_OK; ++si) {
+                b->tune = lmh_batch_tuning{blocks[bi], spls[si % 3], pe, 0, 0, slice_mins[si / 3]};
                 // one untimed launch for the code object and caches, then the timed steps
-                if ((rc = batch_run_impl(b, &q, 1, spls[si]))) break;
+                if ((rc = batch_run_impl(b, &q, 1, spls[si % 3]))) break;
                 if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = b->fail(LMH_E_HIP, "batch_autotune: event"); break; }
-                if ((rc = batch_run_impl(b, &q, 1 + spls[si], trial_steps))) break;
+                if ((rc = batch_run_impl(b, &q, 1 + spls[si % 3], trial_steps))) break;
                 if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                     rc = b->fail(LMH_E_HIP, "batch_autotune: event");
                     break;

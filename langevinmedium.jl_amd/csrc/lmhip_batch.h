@@ -11,9 +11,11 @@
 
 struct BatchRun {
     int64_t begin, count, tile0;
-    GridMedium g;
+    GridMedium g;               // g.slice: the run's ring of per-step medium slices (one per step
+                                // of a launch, slice_stride records apart), or null
     SpeciesArgs sp;
     double DsT;
+    int64_t slice_stride;
 };
 
 struct BatchArgs {
@@ -66,10 +68,12 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
     double td;
     const int j = axis_cell(g.tg, g.nt, t_now, &td);
     double Tm, vx, vy;
-    // the medium's corner reads are issued at wave priority 1, ahead of the other waves'
-    // arithmetic (the batch kernel is issue bound too: 1.8 % per step on 256 runs)
+    // the medium's reads are issued at wave priority 1, ahead of the other waves' arithmetic (the
+    // batch kernel is issue bound too: 1.8 % per step on 256 runs): the step's slice record where
+    // the run has slices (g.slice, wave-uniform), else the corners interpolated in tau
     __builtin_amdgcn_s_setprio(1);
-    grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    if (g.slice) grid_sample<true>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    else grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
     __builtin_amdgcn_s_setprio(0);
     const FlowBoost fb = flow_boost(vx, vy);
     apply_boost<PDIM>(q, m, fb, -1, p.relativistic);
@@ -99,7 +103,8 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }
-    grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+    if (g.slice) grid_push<PDIM, LMH_NOISE_PHILOX, true>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+    else grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
 }
 
 // PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
@@ -129,8 +134,10 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
         // unless the run is ordered by grid cell)
         const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
         int64_t bad = -1;
+        GridMedium gs = run.g;
         for (int s = 0; s < nsteps; ++s) {
-            fused_grid_step<PDIM, COLL>(a, p, run.g, sp, pid, step0 + s, &x0, &x1, q);
+            fused_grid_step<PDIM, COLL>(a, p, gs, sp, pid, step0 + s, &x0, &x1, q);
+            if (gs.slice) gs.slice += run.slice_stride;
             if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
         }
         a.pos[0][i] = x0;
@@ -139,6 +146,24 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
         for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
         if (bad >= 0) atomicMin(b.base.bad, (unsigned long long)bad);
     }
+}
+
+// The slices of the sliced runs for the launch's steps *step_dev + k_off .. + nk - 1: blockIdx.y
+// picks the sliced run srun[y], the x threads its records of every step (step s at slice_stride
+// records past step s - 1), each record slice_coeffs of the run's grid at the step's time, the
+// step kernels' cell and fraction in tau
+__global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const int32_t* __restrict__ srun,
+                                                         int32_t k_off, int32_t nk) {
+    const int r = srun[blockIdx.y];
+    const BatchRun run = uniform_read(&b.runs[r]);
+    const int64_t rec = run.slice_stride;
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= rec * nk) return;
+    const int64_t s = k / rec, rr = k - s * rec;
+    const double t_now = (double)(*b.step_dev + k_off + s) * b.base.p.dt + b.base.p.t0;
+    double td;
+    const int j = axis_cell(run.g.tg, run.g.nt, t_now, &td);
+    const_cast<double4*>(run.g.slice)[k] = slice_coeffs(run.g, j, td, rr / 3, (int)(rr % 3));
 }
 
 // the cell-order keys of every run: key = run << cell_bits | cell (the cell grid_sample reads on

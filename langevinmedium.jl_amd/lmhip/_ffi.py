@@ -82,7 +82,7 @@ class lmh_run_desc(ctypes.Structure):
 
 class lmh_batch_tuning(ctypes.Structure):
     _fields_ = [("block", c_int32), ("steps_per_launch", c_int32), ("persistent", c_int32),
-                ("graph_steps", c_int32), ("sort_every", c_int32)]
+                ("graph_steps", c_int32), ("sort_every", c_int32), ("slice_min", c_int32)]
 
 
 def make_params(*, dt, t0=0.0, DsT=0.0, psi2=0.0, gamma0=0.0, Lambda=1.0, seed=0, stream=0,

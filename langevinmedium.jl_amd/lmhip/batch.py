@@ -100,16 +100,16 @@ class Batch:
         self._check(self._lib.lmh_batch_get_particles(self._h, dptr(pos), dptr(mom)))
         return pos, mom
 
-    def set_tuning(self, block=256, steps_per_launch=1, persistent=0, graph_steps=0, sort_every=0):
+    def set_tuning(self, block=256, steps_per_launch=1, persistent=0, graph_steps=0, sort_every=0, slice_min=0):
         t = _ffi.lmh_batch_tuning(int(block), int(steps_per_launch), int(persistent), int(graph_steps),
-                                  int(sort_every))
+                                  int(sort_every), int(slice_min))
         self._check(self._lib.lmh_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _ffi.lmh_batch_tuning()
         self._check(self._lib.lmh_batch_get_tuning(self._h, ctypes.byref(t)))
         return {"block": t.block, "steps_per_launch": t.steps_per_launch, "persistent": t.persistent,
-                "graph_steps": t.graph_steps, "sort_every": t.sort_every}
+                "graph_steps": t.graph_steps, "sort_every": t.sort_every, "slice_min": t.slice_min}
 
     def boost_to_lab(self, params, step: int = 0):
         self._check(self._lib.lmh_batch_boost_to_lab(self._h, ctypes.byref(params), int(step)))
@@ -123,7 +123,7 @@ class Batch:
         self._check(self._lib.lmh_batch_autotune(self._h, ctypes.byref(params), int(trial_steps),
                                                  ctypes.byref(t), ctypes.byref(ms)))
         return {"block": t.block, "steps_per_launch": t.steps_per_launch, "persistent": t.persistent,
-                "graph_steps": t.graph_steps, "sort_every": t.sort_every}, ms.value
+                "graph_steps": t.graph_steps, "sort_every": t.sort_every, "slice_min": t.slice_min}, ms.value
 
     def observables(self, nbins=40, pt_min=0.0, pt_max=10.0, sigma=0.0):
         spec = _ffi.lmh_obs_spec(int(nbins), -1, float(pt_min), float(pt_max), float(sigma))

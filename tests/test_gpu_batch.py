@@ -61,7 +61,7 @@ def _load_batch(b, runs):
     b.set_runs(desc, pos, mom)
 
 
-def _alone(runs, collision, steps, boost=True, **kw):
+def _alone(runs, collision, steps, boost=True, slice_min=0, **kw):
     out = []
     for k, r in enumerate(runs):
         if r["n"] == 0:
@@ -70,8 +70,10 @@ def _alone(runs, collision, steps, boost=True, **kw):
         p = _params(collision, DsT=r["DsT"], stream=STREAM0 + k, **kw)
         g = r["g"]
         with lmhip.Context(0) as ctx:
-            # the batch kernels interpolate each corner in tau (the unsliced form)
-            ctx.set_tuning(slice=0)
+            # the batch kernels interpolate each corner in tau (the unsliced form), except for
+            # the runs of at least slice_min quarks, which read per-step slices as lmh_run's
+            # one-step launches do
+            ctx.set_tuning(slice=1 if 0 < slice_min <= r["n"] else 0)
             ctx.set_medium_grid2p1(g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, g.tgrid, g.T, g.vx, g.vy)
             ctx.set_time_table(_ffi.LMH_TABLE_DRAG, 0, *r["drag"])
             if r["rt"] is not None:
@@ -126,6 +128,40 @@ def test_batch_run_against_oracle(gpu_ok):
     sp = O.species(keep, [r["m"]], [r["n"]], drag=[r["drag"]], rta=[r["rt"]], jt=[(inv, nU, nT, Tt, idt)])
     opos, omom = r["pos"].copy(order="F"), r["mom"].copy(order="F")
     rc, bad = O.run_grid(p3, gm, sp, opos, omom, 1, steps, noise)
+    assert rc == 0, (rc, bad)
+    assert np.all(rel_dev(got[3][1], omom, 1e-6) <= 1e-12)
+    assert np.all(rel_dev(got[3][0], opos, 1e-6) <= 1e-12)
+
+
+@pytest.mark.parametrize("spl,graph_steps,sort_every", [(1, 0, 0), (4, 0, 5), (3, 4, 0)])
+def test_batch_medium_slices_equal_runs_alone(gpu_ok, spl, graph_steps, sort_every):
+    """slice_min 500: runs 3 (2500 quarks) and 4 (777) read per-step medium slices (a ring of one
+    slice per step of a launch), runs 0 and 1 the corners; every run bit for bit the same run
+    alone with the matching medium form, and the largest sliced run against the oracle at 1e-12"""
+    runs = _runs(seed=17)
+    steps = 22
+    ref = _alone(runs, _ffi.LMH_LANGEVIN_RTA, steps, slice_min=500)
+    with Batch(0) as b:
+        b.set_tuning(block=128, steps_per_launch=spl, graph_steps=graph_steps, sort_every=sort_every,
+                     slice_min=500)
+        assert b.get_tuning()["slice_min"] == 500
+        _load_batch(b, runs)
+        p = _params(_ffi.LMH_LANGEVIN_RTA)
+        b.boost_to_lab(p, 0)
+        b.run(p, 1, steps)
+        got = _split(*b.get_particles(), runs)
+    for k, ((gp, gm), (rp, rm)) in enumerate(zip(got, ref)):
+        assert np.array_equal(gp, rp) and np.array_equal(gm, rm), f"run {k}"
+    r = runs[3]
+    p3 = _params(_ffi.LMH_LANGEVIN_RTA, DsT=r["DsT"], stream=STREAM0 + 3)
+    keep = O.Keep()
+    gm_ = O.grid_medium(keep, r["g"])
+    inv, nU, nT, Tt, idt = r["jt"]
+    sp = O.species(keep, [r["m"]], [r["n"]], drag=[r["drag"]], rta=[r["rt"]], jt=[(inv, nU, nT, Tt, idt)])
+    opos, omom = r["pos"].copy(order="F"), r["mom"].copy(order="F")
+    assert O.boost_to_lab_grid(p3, gm_, sp, opos, omom, 0) == 0
+    rc, bad = O.run_grid(p3, gm_, sp, opos, omom, 1, steps,
+                         np.ascontiguousarray(O.noise_record(p3, 2, 3, r["n"], 1, steps)))
     assert rc == 0, (rc, bad)
     assert np.all(rel_dev(got[3][1], omom, 1e-6) <= 1e-12)
     assert np.all(rel_dev(got[3][0], opos, 1e-6) <= 1e-12)
@@ -216,6 +252,8 @@ def test_batch_autotune_keeps_state_and_results(gpu_ok):
         assert tune["block"] in (64, 128, 256) and tune["persistent"] in (0, 1)
         assert tune["steps_per_launch"] in (1, 2, 4) and ms > 0.0
         assert tune["sort_every"] in (0, 50, 200)
+        # no run reaches 2^18 quarks here: either threshold slices none
+        assert tune["slice_min"] in (0, 1 << 18)
         assert b.get_tuning() == tune
         now = b.get_particles()
         assert np.array_equal(now[0], ref_before[0]) and np.array_equal(now[1], ref_before[1])
