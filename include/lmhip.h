@@ -237,8 +237,10 @@ typedef struct lmh_batch_tuning {
     int32_t slice_min;         /* > 0: the runs of at least slice_min quarks read their medium
                                   from per-step slices (the grid interpolated in tau once per
                                   step and run, as lmh_run's one-step launches do, with the
-                                  same results as such a run alone); the other runs interpolate
-                                  each corner in tau per quark; 0: no slices                  */
+                                  same results as such a run alone; in the specialised
+                                  configuration they step in a kernel of their own, one step
+                                  per launch); the other runs interpolate each corner in tau
+                                  per quark; 0: no slices                                      */
 } lmh_batch_tuning;
 
 typedef struct lmh_ctx lmh_ctx;
@@ -431,7 +433,7 @@ int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step);
 /* steps step_begin .. step_begin+nsteps-1 of every run; p->DsT is ignored (each run's own)  */
 int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps);
 /* times trial_steps steps of every launch shape (block 64/128/256 x persistent 0/1 x
- * steps_per_launch 1/2/4 x slice_min 0/2^18) on this batch, then 200 steps at the fastest shape
+ * steps_per_launch 1/2/4 x slice_min 0/2^16/2^18) on this batch, then 200 steps at the fastest shape
  * for each sort cadence (sort_every 0, 50, 200), keeps the fastest (graph_steps unchanged) and
  * restores the state; the results do not depend on the block, persistence, steps per launch or
  * cadence (slice_min selects each run's form of the medium interpolation, whose results agree

@@ -935,6 +935,31 @@ __device__ __forceinline__ double4 slice_coeffs(const GridMedium& g, int j, doub
     return make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
 }
 
+// the three records of cell k (T, vx, vy; each the value slice_coeffs gives), from the cell's
+// four corners at the two times read as whole 32-byte nodes
+__device__ __forceinline__ void slice_cell_records(const GridMedium& g, int j, double td, int64_t k, double4* out) {
+    const int64_t ncx = g.nx - 1;
+    const int64_t ix = k % ncx, iy = k / ncx;
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const double4* s0 = g.nodes + nxy * j + ix + (int64_t)g.nx * iy;
+    const double4* s1 = s0 + nxy;
+    const double4 a00 = ld_global4(s0), a10 = ld_global4(s0 + 1), a01 = ld_global4(s0 + g.nx),
+                  a11 = ld_global4(s0 + g.nx + 1);
+    const double4 b00 = ld_global4(s1), b10 = ld_global4(s1 + 1), b01 = ld_global4(s1 + g.nx),
+                  b11 = ld_global4(s1 + g.nx + 1);
+    const double u = 1.0 - td;
+#define LMH_SLICE_FIELD(F, o)                                                                            \
+    {                                                                                                    \
+        const double c00 = a00.F * u + b00.F * td, c10 = a10.F * u + b10.F * td;                         \
+        const double c01 = a01.F * u + b01.F * td, c11 = a11.F * u + b11.F * td;                         \
+        out[o] = make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));                    \
+    }
+    LMH_SLICE_FIELD(x, 0)
+    LMH_SLICE_FIELD(y, 1)
+    LMH_SLICE_FIELD(z, 2)
+#undef LMH_SLICE_FIELD
+}
+
 // the slices of steps step0 .. step0 + nk - 1 (time cell of each step as the step kernels find it)
 __global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* slices, int64_t step0,
                                                         int32_t nk, double dt, double t0) {
@@ -1460,16 +1485,18 @@ __device__ __forceinline__ void obs_block(const double* __restrict__ px, const d
         }
     } else {
         int64_t i = i0 + blk * kObsBlock + t;
-        // four particles in flight per thread
-        for (; i + 3 * stride < i1; i += 4 * stride) {
-            double a[4], b[4];
+        // eight particles in flight per thread (sixteen loads: enough bytes in flight per CU to
+        // cover HBM latency with the grid's four waves per SIMD)
+        constexpr int U = 8;
+        for (; i + (U - 1) * stride < i1; i += U * stride) {
+            double a[U], b[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 a[u] = px[i + u * stride];
                 b[u] = py[i + u * stride];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 double p;
                 m.add(a[u], b[u], &p);
                 if (nb > 0) {
@@ -3555,6 +3582,10 @@ struct lmh_batch {
     int32_t* d_srun = nullptr;
     int32_t nsr = 0;
     int64_t max_rec = 0;              // the largest slice (records) of a sliced run
+    // the tiles of the sliced runs and of the others, for the launches that split them
+    int32_t* d_tile_sl = nullptr;
+    int32_t* d_tile_co = nullptr;
+    int64_t ntiles_sl = 0, ntiles_co = 0;
 
     lmh_batch_tuning tune = {256, 1, 0, 0, 0, 0};
 
@@ -3712,7 +3743,7 @@ int batch_layout(lmh_batch* b, int block) {
     if (b->tiles_block == block && b->layout_slice_min == smin && b->layout_spl == K && b->d_runs) return LMH_OK;
     const int R = (int)b->runs.size();
     std::vector<BatchRun> h(R);
-    std::vector<int32_t> tr, srun;
+    std::vector<int32_t> tr, srun, tsl, tco;
     int64_t t0 = 0;
     // the slice rings: K steps of 3 records per cell for every run of at least slice_min quarks
     int64_t nrec = 0, max_rec = 0;
@@ -3774,18 +3805,30 @@ int batch_layout(lmh_batch* b, int block) {
             soff += K * o.slice_stride;
         }
         const int64_t nt = (d.count + block - 1) / block;
-        for (int64_t k = 0; k < nt; ++k) tr.push_back(r);
+        for (int64_t k = 0; k < nt; ++k) {
+            tr.push_back(r);
+            (o.g.slice ? tsl : tco).push_back((int32_t)(t0 + k));
+        }
         t0 += nt;
     }
     batch_drop_graph(b);
     dfree(b->d_runs);
     dfree(b->d_tile_run);
     dfree(b->d_srun);
+    dfree(b->d_tile_sl);
+    dfree(b->d_tile_co);
     b->nsr = 0;
+    b->ntiles_sl = b->ntiles_co = 0;
     if (hipMalloc((void**)&b->d_runs, std::max<size_t>(1, R) * sizeof(BatchRun)) != hipSuccess ||
         hipMalloc((void**)&b->d_tile_run, std::max<size_t>(1, tr.size()) * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void**)&b->d_srun, std::max<size_t>(1, srun.size()) * sizeof(int32_t)) != hipSuccess)
+        hipMalloc((void**)&b->d_srun, std::max<size_t>(1, srun.size()) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_tile_sl, std::max<size_t>(1, tsl.size()) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_tile_co, std::max<size_t>(1, tco.size()) * sizeof(int32_t)) != hipSuccess)
         return b->fail(LMH_E_NOMEM, "batch: device allocation failed");
+    if (!tsl.empty()) LMB_HIP(b, hipMemcpy(b->d_tile_sl, tsl.data(), tsl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!tco.empty()) LMB_HIP(b, hipMemcpy(b->d_tile_co, tco.data(), tco.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    b->ntiles_sl = (int64_t)tsl.size();
+    b->ntiles_co = (int64_t)tco.size();
     if (R) LMB_HIP(b, hipMemcpy(b->d_runs, h.data(), R * sizeof(BatchRun), hipMemcpyHostToDevice));
     if (!tr.empty()) LMB_HIP(b, hipMemcpy(b->d_tile_run, tr.data(), tr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     if (!srun.empty()) LMB_HIP(b, hipMemcpy(b->d_srun, srun.data(), srun.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -3866,21 +3909,56 @@ void batch_launch_p(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, i
     else batch_launch_s<PDIM, PERSIST, false>(b, a, coll, k_off, n, grid);
 }
 
-// steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps
-void batch_enqueue(lmh_batch* b, const BatchArgs& a, int coll, int64_t nsteps) {
+bool batch_spec(const lmh_params& p) {
+    return p.relativistic && p.integrator == LMH_PREPOINT && p.drag_model == LMH_DRAG_TABLE &&
+           !p.bjorken_redshift && !p.position_diffusion;
+}
+
+template <int PDIM>
+void batch_launch_sl(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off) {
+    const dim3 G((unsigned)a.ntiles), B(b->tune.block);
+    switch (coll) {
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_FLUID>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
+        default: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
+    }
+}
+
+// steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps.
+// In the specialised configuration the runs with medium slices take k_batch_grid_sl, one step per
+// launch after their slices; the others k_batch_grid over their own tiles.
+void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) {
     const int32_t k = std::max(1, b->tune.steps_per_launch);
-    unsigned grid = (unsigned)b->ntiles;
+    const bool split = b->nsr > 0 && batch_spec(a0.base.p);
+    BatchArgs a = a0;
+    if (split) {
+        a.tile_list = b->d_tile_co;
+        a.ntiles = b->ntiles_co;
+    }
+    unsigned grid = (unsigned)a.ntiles;
     if (b->tune.persistent) {
         // enough blocks to fill every CU at the kernel's occupancy, walking the tiles
         const int per_cu = std::max(1, 2048 / std::max(64, b->tune.block));
-        grid = (unsigned)std::min<int64_t>(b->ntiles, (int64_t)b->cus * per_cu);
+        grid = (unsigned)std::min<int64_t>(a.ntiles, (int64_t)b->cus * per_cu);
     }
-    for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
+    if (split) {
+        BatchArgs as = a0;
+        as.tile_list = b->d_tile_sl;
+        as.ntiles = b->ntiles_sl;
+        const dim3 G((unsigned)((b->max_rec / 3 + kBlock - 1) / kBlock), (unsigned)b->nsr, 1u);
+        for (int64_t s = 0; s < nsteps; ++s) {
+            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a0, a0.runs, b->d_srun, (int32_t)s);
+            if (b->pdim == 2) batch_launch_sl<2>(b, as, coll, (int32_t)s);
+            else batch_launch_sl<3>(b, as, coll, (int32_t)s);
+        }
+    }
+    for (int64_t s0 = 0; s0 < nsteps && a.ntiles > 0; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
-        if (b->nsr > 0) {
+        if (b->nsr > 0 && !split) {
             // the sliced runs' medium for the launch's n steps
-            const dim3 G((unsigned)((b->max_rec * n + kBlock - 1) / kBlock), (unsigned)b->nsr);
-            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, b->d_srun, (int32_t)s0, n);
+            const dim3 G((unsigned)((b->max_rec / 3 + kBlock - 1) / kBlock), (unsigned)b->nsr, (unsigned)n);
+            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun, (int32_t)s0);
         }
         if (b->pdim == 2) {
             if (b->tune.persistent) batch_launch_p<2, true>(b, a, coll, (int32_t)s0, n, grid);
@@ -4008,6 +4086,8 @@ void lmh_batch_destroy(lmh_batch* b) {
     dfree(b->d_runs);
     dfree(b->d_tile_run);
     dfree(b->d_srun);
+    dfree(b->d_tile_sl);
+    dfree(b->d_tile_co);
     dfree(b->d_bslice);
     dfree(b->d_obs);
     dfree(b->d_save);
@@ -4131,7 +4211,10 @@ int lmh_batch_set_runs(lmh_batch* b, const lmh_run_desc* runs, int32_t nruns, co
     dfree(b->d_runs);
     dfree(b->d_tile_run);
     dfree(b->d_srun);
+    dfree(b->d_tile_sl);
+    dfree(b->d_tile_co);
     b->nsr = 0;
+    b->ntiles_sl = b->ntiles_co = 0;
     batch_free_order(b);
     b->tiles_block = 0;
     b->runs.assign(runs, runs + nruns);
@@ -4245,11 +4328,11 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     LMB_HIP(b, hipEventCreate(&e1));
     const int blocks[3] = {64, 128, 256};
     const int spls[3] = {1, 2, 4};
-    // medium slices for no run, or for the runs of at least 2^18 quarks
-    const int slice_mins[2] = {0, 1 << 18};
+    // medium slices for no run, or for the runs of at least 2^16 or 2^18 quarks
+    const int slice_mins[3] = {0, 1 << 16, 1 << 18};
     for (int bi = 0; bi < 3 && rc == LMH_OK; ++bi)
         for (int pe = 0; pe < 2 && rc == LMH_OK; ++pe)
-            for (int si = 0; si < 3 * 2 && rc == LMH_OK; ++si) {
+            for (int si = 0; si < 3 * 3 && rc == LMH_OK; ++si) {
                 b->tune = lmh_batch_tuning{blocks[bi], spls[si % 3], pe, 0, 0, slice_mins[si / 3]};
                 // one untimed launch for the code object and caches, then the timed steps
                 if ((rc = batch_run_impl(b, &q, 1, spls[si % 3]))) break;

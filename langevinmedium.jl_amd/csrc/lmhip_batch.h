@@ -22,7 +22,8 @@ struct BatchArgs {
     StepArgs base;              // state rows over all runs, the shared parameters
     const BatchRun* runs;
     const int32_t* tile_run;
-    int64_t ntiles;
+    int64_t ntiles;             // tiles the launch walks: all, or the entries of tile_list
+    const int32_t* tile_list;   // null: tile t is tile t; else tile t is tile_list[t]
     const int64_t* step_dev;
 };
 
@@ -57,7 +58,9 @@ __device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& r
 
 // the fused step of k_cart_grid without deferral (Philox noise, the medium interpolated in tau
 // per corner), the same device functions in the same order
-template <int PDIM, int COLL>
+// SLICED: the run reads the step's slice g.slice (the sliced-run kernel); otherwise the form is
+// chosen per run at run time
+template <int PDIM, int COLL, bool SLICED = false>
 __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
                                                 double* x0, double* x1, double* q) {
@@ -65,14 +68,14 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
     constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m;
     const double t_now = (double)step * p.dt + p.t0;
-    double td;
-    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    double td = 0.0;
+    const int j = SLICED ? 0 : axis_cell(g.tg, g.nt, t_now, &td);
     double Tm, vx, vy;
     // the medium's reads are issued at wave priority 1, ahead of the other waves' arithmetic (the
     // batch kernel is issue bound too: 1.8 % per step on 256 runs): the step's slice record where
     // the run has slices (g.slice, wave-uniform), else the corners interpolated in tau
     __builtin_amdgcn_s_setprio(1);
-    if (g.slice) grid_sample<true>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    if (SLICED || g.slice) grid_sample<true>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
     else grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
     __builtin_amdgcn_s_setprio(0);
     const FlowBoost fb = flow_boost(vx, vy);
@@ -103,7 +106,7 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }
-    if (g.slice) grid_push<PDIM, LMH_NOISE_PHILOX, true>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+    if (SLICED || g.slice) grid_push<PDIM, LMH_NOISE_PHILOX, true>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
     else grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
 }
 
@@ -117,7 +120,8 @@ template <int PDIM, int COLL, bool PERSIST, bool SPEC>
 __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b, int32_t k_off, int32_t nsteps) {
     const int64_t step0 = *b.step_dev + k_off;
     const int64_t stride = PERSIST ? (int64_t)gridDim.x : b.ntiles;
-    for (int64_t tile = blockIdx.x; tile < b.ntiles; tile += stride) {
+    for (int64_t t = blockIdx.x; t < b.ntiles; t += stride) {
+        const int64_t tile = b.tile_list ? (int64_t)b.tile_list[t] : t;
         const int r = __builtin_amdgcn_readfirstlane(b.tile_run[tile]);
         const BatchRun run = uniform_read(&b.runs[r]);
         const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
@@ -149,21 +153,60 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
 }
 
 // The slices of the sliced runs for the launch's steps *step_dev + k_off .. + nk - 1: blockIdx.y
-// picks the sliced run srun[y], the x threads its records of every step (step s at slice_stride
-// records past step s - 1), each record slice_coeffs of the run's grid at the step's time, the
-// step kernels' cell and fraction in tau
-__global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const int32_t* __restrict__ srun,
-                                                         int32_t k_off, int32_t nk) {
+// picks the sliced run srun[y], blockIdx.z the step s (slice_stride records past step s - 1), the
+// x threads the cells, each writing its three records (slice_cell_records: the values
+// slice_coeffs gives) at the step's cell and fraction in tau, found once per block (the run and
+// the step are the block's)
+__global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const BatchRun* __restrict__ runs,
+                                                         const int32_t* __restrict__ srun, int32_t k_off) {
     const int r = srun[blockIdx.y];
-    const BatchRun run = uniform_read(&b.runs[r]);
+    const BatchRun& run = runs[r];
     const int64_t rec = run.slice_stride;
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= rec * nk) return;
-    const int64_t s = k / rec, rr = k - s * rec;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (3 * cell >= rec) return;
+    const int64_t s = blockIdx.z;
     const double t_now = (double)(*b.step_dev + k_off + s) * b.base.p.dt + b.base.p.t0;
     double td;
     const int j = axis_cell(run.g.tg, run.g.nt, t_now, &td);
-    const_cast<double4*>(run.g.slice)[k] = slice_coeffs(run.g, j, td, rr / 3, (int)(rr % 3));
+    double4 o[3];
+    slice_cell_records(run.g, j, td, cell, o);
+    double4* dst = const_cast<double4*>(run.g.slice) + s * rec + 3 * cell;
+    dst[0] = o[0];
+    dst[1] = o[1];
+    dst[2] = o[2];
+}
+
+// The sliced runs' step in the specialised configuration, one step per launch: block b takes the
+// tile tiles[b] (a tile of a run of at least slice_min quarks), whose run reads the step's slice
+// (the run's ring holds the one step).  fused_grid_step on the slice, at the single-run sliced
+// kernel's occupancy; the other runs take k_batch_grid.
+// The run table comes in as a read-only argument of its own, so that the block-uniform record is
+// read with scalar loads.
+template <int PDIM, int COLL>
+__global__ __launch_bounds__(kBlock, kGridWaves) void k_batch_grid_sl(BatchArgs b, const BatchRun* __restrict__ runs,
+                                                                      const int32_t* __restrict__ tiles,
+                                                                      const int32_t* __restrict__ tile_run,
+                                                                      int32_t k_off) {
+    const int64_t step = *b.step_dev + k_off;
+    const int64_t tile = tiles[blockIdx.x];
+    const int r = tile_run[tile];
+    const BatchRun& run = runs[r];
+    const int64_t i = (tile - run.tile0) * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= run.count) return;
+    StepArgs a;
+    batch_view(b, run, r, &a);
+    const lmh_params p = step_params<true>(a.p);
+    double x0 = a.pos[0][i], x1 = a.pos[1][i];
+    double q[PDIM];
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+    const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
+    fused_grid_step<PDIM, COLL, true>(a, p, run.g, run.sp, pid, step, &x0, &x1, q);
+    a.pos[0][i] = x0;
+    a.pos[1][i] = x1;
+#pragma unroll
+    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    if (!(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) atomicMin(b.base.bad, (unsigned long long)step);
 }
 
 // the cell-order keys of every run: key = run << cell_bits | cell (the cell grid_sample reads on
