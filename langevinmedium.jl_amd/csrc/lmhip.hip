@@ -1485,9 +1485,8 @@ __device__ __forceinline__ void obs_block(const double* __restrict__ px, const d
         }
     } else {
         int64_t i = i0 + blk * kObsBlock + t;
-        // eight particles in flight per thread (sixteen loads: enough bytes in flight per CU to
-        // cover HBM latency with the grid's four waves per SIMD)
-        constexpr int U = 8;
+        // four particles in flight per thread (eight was no faster)
+        constexpr int U = 4;
         for (; i + (U - 1) * stride < i1; i += U * stride) {
             double a[U], b[U];
 #pragma unroll
@@ -3582,6 +3581,7 @@ struct lmh_batch {
     int32_t* d_srun = nullptr;
     int32_t nsr = 0;
     int64_t max_rec = 0;              // the largest slice (records) of a sliced run
+    int64_t max_slice_tiles = 0;      // ... and its number of k_batch_slices cell tiles
     // the tiles of the sliced runs and of the others, for the launches that split them
     int32_t* d_tile_sl = nullptr;
     int32_t* d_tile_co = nullptr;
@@ -3746,7 +3746,7 @@ int batch_layout(lmh_batch* b, int block) {
     std::vector<int32_t> tr, srun, tsl, tco;
     int64_t t0 = 0;
     // the slice rings: K steps of 3 records per cell for every run of at least slice_min quarks
-    int64_t nrec = 0, max_rec = 0;
+    int64_t nrec = 0, max_rec = 0, max_tiles = 0;
     for (int r = 0; r < R; ++r) {
         const lmh_run_desc& d = b->runs[r];
         if (smin <= 0 || d.count < smin) continue;
@@ -3754,6 +3754,8 @@ int batch_layout(lmh_batch* b, int block) {
         const int64_t rec = 3 * (int64_t)(md.nx - 1) * (md.ny - 1);
         nrec += K * rec;
         max_rec = std::max(max_rec, rec);
+        max_tiles = std::max<int64_t>(max_tiles, (int64_t)((md.nx - 1 + kSliceTile - 1) / kSliceTile) *
+                                                     ((md.ny - 1 + kSliceTile - 1) / kSliceTile));
         srun.push_back(r);
     }
     if ((size_t)nrec > b->bslice_cap) {
@@ -3838,6 +3840,7 @@ int batch_layout(lmh_batch* b, int block) {
     b->layout_spl = K;
     b->nsr = (int32_t)srun.size();
     b->max_rec = max_rec;
+    b->max_slice_tiles = max_tiles;
     return LMH_OK;
 }
 
@@ -3946,7 +3949,7 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) 
         BatchArgs as = a0;
         as.tile_list = b->d_tile_sl;
         as.ntiles = b->ntiles_sl;
-        const dim3 G((unsigned)((b->max_rec / 3 + kBlock - 1) / kBlock), (unsigned)b->nsr, 1u);
+        const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, 1u);
         for (int64_t s = 0; s < nsteps; ++s) {
             hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a0, a0.runs, b->d_srun, (int32_t)s);
             if (b->pdim == 2) batch_launch_sl<2>(b, as, coll, (int32_t)s);
@@ -3957,7 +3960,7 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) 
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
         if (b->nsr > 0 && !split) {
             // the sliced runs' medium for the launch's n steps
-            const dim3 G((unsigned)((b->max_rec / 3 + kBlock - 1) / kBlock), (unsigned)b->nsr, (unsigned)n);
+            const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, (unsigned)n);
             hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun, (int32_t)s0);
         }
         if (b->pdim == 2) {
@@ -4328,11 +4331,11 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     LMB_HIP(b, hipEventCreate(&e1));
     const int blocks[3] = {64, 128, 256};
     const int spls[3] = {1, 2, 4};
-    // medium slices for no run, or for the runs of at least 2^16 or 2^18 quarks
-    const int slice_mins[3] = {0, 1 << 16, 1 << 18};
+    // medium slices for no run, or for the runs of at least 2^14, 2^16 or 2^18 quarks
+    const int slice_mins[4] = {0, 1 << 14, 1 << 16, 1 << 18};
     for (int bi = 0; bi < 3 && rc == LMH_OK; ++bi)
         for (int pe = 0; pe < 2 && rc == LMH_OK; ++pe)
-            for (int si = 0; si < 3 * 3 && rc == LMH_OK; ++si) {
+            for (int si = 0; si < 3 * 4 && rc == LMH_OK; ++si) {
                 b->tune = lmh_batch_tuning{blocks[bi], spls[si % 3], pe, 0, 0, slice_mins[si / 3]};
                 // one untimed launch for the code object and caches, then the timed steps
                 if ((rc = batch_run_impl(b, &q, 1, spls[si % 3]))) break;

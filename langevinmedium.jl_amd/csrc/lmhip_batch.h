@@ -153,27 +153,60 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
 }
 
 // The slices of the sliced runs for the launch's steps *step_dev + k_off .. + nk - 1: blockIdx.y
-// picks the sliced run srun[y], blockIdx.z the step s (slice_stride records past step s - 1), the
-// x threads the cells, each writing its three records (slice_cell_records: the values
-// slice_coeffs gives) at the step's cell and fraction in tau, found once per block (the run and
-// the step are the block's)
+// picks the sliced run srun[y], blockIdx.z the step s (slice_stride records past step s - 1), and
+// blockIdx.x a tile of kSliceTile x kSliceTile cells.  The block stages the tile's corner nodes at
+// the step's two times (T, vx, vy) in LDS, each read once from memory (a cell's four corners are
+// shared with its neighbours), then every thread writes its cell's three records: the values
+// slice_coeffs gives, in its order of operations.  The cell and fraction in tau are the step
+// kernels', found once per block.
+constexpr int kSliceTile = 16;
+static_assert(kSliceTile * kSliceTile == kBlock, "one thread per cell of a slice tile");
+
 __global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const BatchRun* __restrict__ runs,
                                                          const int32_t* __restrict__ srun, int32_t k_off) {
+    constexpr int NT = kSliceTile + 1;   // nodes per tile side
+    __shared__ double nd[2][3][NT * NT];
     const int r = srun[blockIdx.y];
     const BatchRun& run = runs[r];
-    const int64_t rec = run.slice_stride;
-    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (3 * cell >= rec) return;
+    const GridMedium& g = run.g;
+    const int ncx = g.nx - 1, ncy = g.ny - 1;
+    const int tiles_x = (ncx + kSliceTile - 1) / kSliceTile;
+    const int ntile = tiles_x * ((ncy + kSliceTile - 1) / kSliceTile);
+    if ((int)blockIdx.x >= ntile) return;   // a smaller grid than the launch's largest
+    const int ix0 = ((int)blockIdx.x % tiles_x) * kSliceTile, iy0 = ((int)blockIdx.x / tiles_x) * kSliceTile;
     const int64_t s = blockIdx.z;
     const double t_now = (double)(*b.step_dev + k_off + s) * b.base.p.dt + b.base.p.t0;
     double td;
-    const int j = axis_cell(run.g.tg, run.g.nt, t_now, &td);
-    double4 o[3];
-    slice_cell_records(run.g, j, td, cell, o);
-    double4* dst = const_cast<double4*>(run.g.slice) + s * rec + 3 * cell;
-    dst[0] = o[0];
-    dst[1] = o[1];
-    dst[2] = o[2];
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const double* n0 = reinterpret_cast<const double*>(g.nodes + nxy * j);
+    const double* n1 = n0 + 4 * nxy;
+    for (int k = threadIdx.x; k < NT * NT; k += kBlock) {
+        const int ax = ix0 + k % NT, ay = iy0 + k / NT;
+        if (ax < g.nx && ay < g.ny) {
+            const int64_t o = 4 * (ax + (int64_t)g.nx * ay);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                nd[0][f][k] = ld_global(n0 + o + f);
+                nd[1][f][k] = ld_global(n1 + o + f);
+            }
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kSliceTile, ty = threadIdx.x / kSliceTile;
+    const int ix = ix0 + tx, iy = iy0 + ty;
+    if (ix >= ncx || iy >= ncy) return;
+    const int k00 = tx + NT * ty;
+    double4* dst = const_cast<double4*>(g.slice) + s * run.slice_stride + 3 * (ix + (int64_t)ncx * iy);
+    const double u = 1.0 - td;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const double c00 = nd[0][f][k00] * u + nd[1][f][k00] * td;
+        const double c10 = nd[0][f][k00 + 1] * u + nd[1][f][k00 + 1] * td;
+        const double c01 = nd[0][f][k00 + NT] * u + nd[1][f][k00 + NT] * td;
+        const double c11 = nd[0][f][k00 + NT + 1] * u + nd[1][f][k00 + NT + 1] * td;
+        dst[f] = make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
+    }
 }
 
 // The sliced runs' step in the specialised configuration, one step per launch: block b takes the

@@ -252,8 +252,8 @@ def test_batch_autotune_keeps_state_and_results(gpu_ok):
         assert tune["block"] in (64, 128, 256) and tune["persistent"] in (0, 1)
         assert tune["steps_per_launch"] in (1, 2, 4) and ms > 0.0
         assert tune["sort_every"] in (0, 50, 200)
-        # no run reaches 2^16 quarks here: every threshold slices none
-        assert tune["slice_min"] in (0, 1 << 16, 1 << 18)
+        # no run reaches 2^14 quarks here: every threshold slices none
+        assert tune["slice_min"] in (0, 1 << 14, 1 << 16, 1 << 18)
         assert b.get_tuning() == tune
         now = b.get_particles()
         assert np.array_equal(now[0], ref_before[0]) and np.array_equal(now[1], ref_before[1])
