@@ -444,13 +444,73 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_observables(args):
+    """The on-device observables of 10^7 quarks: N, sum p_T, sum p_T^2, sum cos 2phi,
+    sum p_T cos 2phi and a 40-bin p_T histogram on [0, 10] GeV (lmh_observables_noalloc, the result
+    in a device buffer), on the headline workload's initial state (70 % charm, 30 % bottom, the
+    p_T spectrum of media.pt_rapidity_momenta, rapidity in [-1, 1]); `--steps` calls timed with HIP
+    events on the library's stream after `--warmup` untimed ones.  Algorithmic bytes: px and py,
+    16 B per quark."""
+    import ctypes
+    import numpy as np
+    import torch
+    import lmhip
+    from lmhip import _ffi, media
+
+    N = args.n
+    g, masses, counts, pos, mom = media.hydro_ensemble(N)
+    ctx = lmhip.Context(0)
+    ctx.set_particles(pos, mom, masses, counts)
+    del pos, mom
+    out = torch.zeros(45, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    spec = _ffi.lmh_obs_spec(40, -1, 0.0, 10.0, 0.0)
+    L = _ffi.lib()
+
+    def call():
+        rc = L.lmh_observables_noalloc(ctx._h, ctypes.byref(spec), ctypes.c_void_p(out.data_ptr()))
+        if rc != 0:
+            raise RuntimeError(L.lmh_last_error(ctx._h))
+
+    for _ in range(max(1, args.warmup // 10)):
+        call()
+    ctx.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        call()
+    ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    res = out.cpu().numpy()
+    ctx.close()
+    if res[0] != N:
+        raise RuntimeError(f"observables counted {res[0]} quarks, not {N}")
+    achieved = 16.0 * N / (ms / 1e3) / 1e9
+    print(json.dumps({
+        "metric": "on-device observables of 1e7 quarks (<p_T>, <cos 2 phi>, 40-bin spectrum) per call",
+        "value": round(ms, 5), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": max(1, args.warmup // 10),
+        "ms_per_step": round(ms, 5), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{N} quarks of the headline's initial state, 40 bins on [0, 10] GeV, "
+                               "lmh_observables_noalloc (partials and final sums, result on the device)",
+                   "mean_pT": round(res[1] / res[0], 6)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_obs_partial + k_obs_final", "bytes_per_launch": 16 * N},
+        "cpu_baseline": None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="grid", choices=["grid", "bjorken", "batch", "adjoint"],
+    ap.add_argument("--workload", default="grid", choices=["grid", "bjorken", "batch", "adjoint", "observables"],
                     help="grid: the headline 10^7-quark step (default); bjorken: the Bjorken medium in "
                          "Milne coordinates, 10^6 charm, 1942 steps; batch: 256 independent runs "
-                         "in one launch; adjoint: gradient / forward time of the pathwise adjoint")
+                         "in one launch; adjoint: gradient / forward time of the pathwise adjoint; "
+                         "observables: the on-device observables of 10^7 quarks")
     ap.add_argument("--bj-n", type=int, default=1_000_000, help="bjorken: quarks")
     ap.add_argument("--runs", type=int, default=256, help="batch: number of runs")
     ap.add_argument("--slice-min", type=int, default=-1,
@@ -493,6 +553,8 @@ def main():
         return bench_bjorken(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "observables":
+        return bench_observables(args)
 
     import numpy as np
     import torch
