@@ -3586,6 +3586,8 @@ struct lmh_batch {
     int32_t* d_tile_sl = nullptr;
     int32_t* d_tile_co = nullptr;
     int64_t ntiles_sl = 0, ntiles_co = 0;
+    double2* d_tcell = nullptr;       // runs x steps_per_launch tau cells (k_batch_tcell)
+    size_t tcell_cap = 0;
 
     lmh_batch_tuning tune = {256, 1, 0, 0, 0, 0};
 
@@ -3757,6 +3759,13 @@ int batch_layout(lmh_batch* b, int block) {
         max_tiles = std::max<int64_t>(max_tiles, (int64_t)((md.nx - 1 + kSliceTile - 1) / kSliceTile) *
                                                      ((md.ny - 1 + kSliceTile - 1) / kSliceTile));
         srun.push_back(r);
+    }
+    if ((size_t)R * K > b->tcell_cap) {
+        dfree(b->d_tcell);
+        b->tcell_cap = 0;
+        if (hipMalloc((void**)&b->d_tcell, (size_t)R * K * sizeof(double2)) != hipSuccess)
+            return b->fail(LMH_E_NOMEM, "batch: allocation failed");
+        b->tcell_cap = (size_t)R * K;
     }
     if ((size_t)nrec > b->bslice_cap) {
         dfree(b->d_bslice);
@@ -3931,6 +3940,15 @@ void batch_launch_sl(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off) 
 // steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps.
 // In the specialised configuration the runs with medium slices take k_batch_grid_sl, one step per
 // launch after their slices; the others k_batch_grid over their own tiles.
+// the tau cells of every run for the launch's n steps from k_off, into a->tcell
+void batch_tcell(lmh_batch* b, BatchArgs* a, int32_t k_off, int32_t n) {
+    const int32_t R = (int32_t)b->runs.size();
+    hipLaunchKernelGGL(k_batch_tcell, dim3((unsigned)(((int64_t)R * n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       b->stream, *a, a->runs, R, k_off, n, b->d_tcell);
+    a->tcell = b->d_tcell;
+    a->tcell_k = n;
+}
+
 void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) {
     const int32_t k = std::max(1, b->tune.steps_per_launch);
     const bool split = b->nsr > 0 && batch_spec(a0.base.p);
@@ -3951,17 +3969,20 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) 
         as.ntiles = b->ntiles_sl;
         const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, 1u);
         for (int64_t s = 0; s < nsteps; ++s) {
-            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a0, a0.runs, b->d_srun, (int32_t)s);
+            BatchArgs at = a0;
+            batch_tcell(b, &at, (int32_t)s, 1);
+            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, at, at.runs, b->d_srun);
             if (b->pdim == 2) batch_launch_sl<2>(b, as, coll, (int32_t)s);
             else batch_launch_sl<3>(b, as, coll, (int32_t)s);
         }
     }
     for (int64_t s0 = 0; s0 < nsteps && a.ntiles > 0; s0 += k) {
         const int32_t n = (int32_t)std::min<int64_t>(k, nsteps - s0);
+        batch_tcell(b, &a, (int32_t)s0, n);
         if (b->nsr > 0 && !split) {
             // the sliced runs' medium for the launch's n steps
             const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, (unsigned)n);
-            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun, (int32_t)s0);
+            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun);
         }
         if (b->pdim == 2) {
             if (b->tune.persistent) batch_launch_p<2, true>(b, a, coll, (int32_t)s0, n, grid);
@@ -4092,6 +4113,7 @@ void lmh_batch_destroy(lmh_batch* b) {
     dfree(b->d_tile_sl);
     dfree(b->d_tile_co);
     dfree(b->d_bslice);
+    dfree(b->d_tcell);
     dfree(b->d_obs);
     dfree(b->d_save);
     batch_free_order(b);

@@ -25,6 +25,10 @@ struct BatchArgs {
     int64_t ntiles;             // tiles the launch walks: all, or the entries of tile_list
     const int32_t* tile_list;   // null: tile t is tile t; else tile t is tile_list[t]
     const int64_t* step_dev;
+    // the tau cell and fraction of run r at the launch's step s, tcell[r * tcell_k + s] = (j, td),
+    // written by k_batch_tcell before the launch (null: each thread finds them itself)
+    const double2* tcell;
+    int32_t tcell_k;
 };
 
 // A block-uniform struct read from device memory into scalar registers.  The compiler issues
@@ -60,16 +64,27 @@ __device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& r
 // per corner), the same device functions in the same order
 // SLICED: the run reads the step's slice g.slice (the sliced-run kernel); otherwise the form is
 // chosen per run at run time
+// tc: the step's precomputed tau cell and fraction (k_batch_tcell), or null
 template <int PDIM, int COLL, bool SLICED = false>
 __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                                 const SpeciesArgs& sp, int64_t i, int64_t step,
-                                                double* x0, double* x1, double* q) {
+                                                double* x0, double* x1, double* q,
+                                                const double2* tc = nullptr) {
     constexpr bool HAS_KICK = COLL == LMH_LANGEVIN || COLL == LMH_LANGEVIN_RTA;
     constexpr bool HAS_RTA = COLL == LMH_RTA || COLL == LMH_LANGEVIN_RTA;
     const double m = sp.m;
     const double t_now = (double)step * p.dt + p.t0;
     double td = 0.0;
-    const int j = SLICED ? 0 : axis_cell(g.tg, g.nt, t_now, &td);
+    int j = 0;
+    if (!SLICED) {
+        if (tc) {
+            const double2 v = *tc;
+            j = (int)v.x;
+            td = v.y;
+        } else {
+            j = axis_cell(g.tg, g.nt, t_now, &td);
+        }
+    }
     double Tm, vx, vy;
     // the medium's reads are issued at wave priority 1, ahead of the other waves' arithmetic (the
     // batch kernel is issue bound too: 1.8 % per step on 256 runs): the step's slice record where
@@ -139,8 +154,9 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
         const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
         int64_t bad = -1;
         GridMedium gs = run.g;
+        const double2* tc = b.tcell ? b.tcell + (int64_t)r * b.tcell_k : nullptr;
         for (int s = 0; s < nsteps; ++s) {
-            fused_grid_step<PDIM, COLL>(a, p, gs, sp, pid, step0 + s, &x0, &x1, q);
+            fused_grid_step<PDIM, COLL>(a, p, gs, sp, pid, step0 + s, &x0, &x1, q, tc ? tc + s : nullptr);
             if (gs.slice) gs.slice += run.slice_stride;
             if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
         }
@@ -152,18 +168,31 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
     }
 }
 
+// the tau cell and fraction (axis_cell, as every step kernel finds them) of every run r < R at
+// the launch's steps s = 0 .. nk - 1: out[r * nk + s]
+__global__ __launch_bounds__(kBlock) void k_batch_tcell(BatchArgs b, const BatchRun* __restrict__ runs, int32_t R,
+                                                        int32_t k_off, int32_t nk, double2* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= (int64_t)R * nk) return;
+    const int r = (int)(k / nk), s = (int)(k % nk);
+    const GridMedium& g = runs[r].g;
+    const double t_now = (double)(*b.step_dev + k_off + s) * b.base.p.dt + b.base.p.t0;
+    double td;
+    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    out[k] = make_double2((double)j, td);
+}
+
 // The slices of the sliced runs for the launch's steps *step_dev + k_off .. + nk - 1: blockIdx.y
 // picks the sliced run srun[y], blockIdx.z the step s (slice_stride records past step s - 1), and
-// blockIdx.x a tile of kSliceTile x kSliceTile cells.  The block stages the tile's corner nodes at
+// blockIdx.x a tile of kSliceTile x kSliceTile cells; the tau cell and fraction come from b.tcell.  The block stages the tile's corner nodes at
 // the step's two times (T, vx, vy) in LDS, each read once from memory (a cell's four corners are
 // shared with its neighbours), then every thread writes its cell's three records: the values
-// slice_coeffs gives, in its order of operations.  The cell and fraction in tau are the step
-// kernels', found once per block.
+// slice_coeffs gives, in its order of operations.
 constexpr int kSliceTile = 16;
 static_assert(kSliceTile * kSliceTile == kBlock, "one thread per cell of a slice tile");
 
 __global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const BatchRun* __restrict__ runs,
-                                                         const int32_t* __restrict__ srun, int32_t k_off) {
+                                                         const int32_t* __restrict__ srun) {
     constexpr int NT = kSliceTile + 1;   // nodes per tile side
     __shared__ double nd[2][3][NT * NT];
     const int r = srun[blockIdx.y];
@@ -175,9 +204,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const Batc
     if ((int)blockIdx.x >= ntile) return;   // a smaller grid than the launch's largest
     const int ix0 = ((int)blockIdx.x % tiles_x) * kSliceTile, iy0 = ((int)blockIdx.x / tiles_x) * kSliceTile;
     const int64_t s = blockIdx.z;
-    const double t_now = (double)(*b.step_dev + k_off + s) * b.base.p.dt + b.base.p.t0;
-    double td;
-    const int j = axis_cell(g.tg, g.nt, t_now, &td);
+    const double2 jt = b.tcell[(int64_t)r * b.tcell_k + s];
+    const int j = (int)jt.x;
+    const double td = jt.y;
     const int64_t nxy = (int64_t)g.nx * g.ny;
     const double* n0 = reinterpret_cast<const double*>(g.nodes + nxy * j);
     const double* n1 = n0 + 4 * nxy;
