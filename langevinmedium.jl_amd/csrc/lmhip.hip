@@ -161,6 +161,7 @@ struct RadialMedium {
     const double* V2;
     int32_t nx, nt;
     double r_axis_eps, r_max;   // kernels_cpu.jl:528-534
+    double xinv, tinv;          // (n - 1)/(g[n-1] - g[0]) of an equally spaced grid, else 0
 };
 
 struct GridMedium {
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
         const int64_t step = a.step_begin + s_off;
         const double t_now = (double)step * p.dt + p.t0;
         double yd;
-        const int j = axis_cell(med.tg, med.nt, t_now, &yd);
+        const int j = axis_cell_guess(med.tg, med.nt, med.tinv, t_now, &yd);
         double xin[DIMS], qin[PDIM];
         if (fo && !frozen) {
 #pragma unroll
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             if (!(milne && d == 2)) r2 += x[d] * x[d];
         const double r = sqrt(r2);
         double xd;
-        const int ir = axis_cell(med.xg, med.nx, r, &xd);
+        const int ir = axis_cell_guess(med.xg, med.nx, med.xinv, r, &xd);
         const bool skip = r < kEps;
         double v = 0.0;
         if (!skip) {
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             double rf = RADIAL ? x[0] : sqrt(x[0] * x[0] + x[1 % DIMS] * x[1 % DIMS]);
             if (!(rf < kEps)) {
                 double xdf;
-                int irf = axis_cell(med.xg, med.nx, rf, &xdf);
+                int irf = axis_cell_guess(med.xg, med.nx, med.xinv, rf, &xdf);
                 double vf = bilinear(med.V, med.nx, irf, j, xdf, yd);
                 if (!p.relativistic) {
                     if (RADIAL) {
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             double dr = (pr / E) * p.dt;
             if (p.position_diffusion) {
                 double xdd;
-                int ird = axis_cell(med.xg, med.nx, r_safe, &xdd);
+                int ird = axis_cell_guess(med.xg, med.nx, med.xinv, r_safe, &xdd);
                 double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
                 double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
                 if (D > 0.0) {
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
                 for (int d = 0; d < DIMS; ++d) rr += x[d] * x[d];
                 double r_now = sqrt(rr);
                 double xdd;
-                int ird = axis_cell(med.xg, med.nx, r_now, &xdd);
+                int ird = axis_cell_guess(med.xg, med.nx, med.xinv, r_now, &xdd);
                 double Td = bilinear(med.T, med.nx, ird, j, xdd, yd);
                 double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
                 if (D > 0.0) {
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_cart_radial(StepArgs a, RadialMedium
             for (int d = 0; d < DIMS; ++d) rr += x[d] * x[d];
             const double rf = RADIAL ? fabs(x[0]) : sqrt(rr);
             double xdf;
-            const int irf = axis_cell(med.xg, med.nx, rf, &xdf);
+            const int irf = axis_cell_guess(med.xg, med.nx, med.xinv, rf, &xdf);
             const double Tf = bilinear(med.T, med.nx, irf, j, xdf, yd);
             freeze_latch<DIMS, PDIM>(a, i, Tf, t_now, (double)(step - 1) * p.dt + p.t0, xin, qin, x, q,
                                      &frozen, &lastT);
@@ -594,7 +595,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_milne(StepArgs a, RadialMedium me
         const int64_t step = a.step_begin + s_off;
         const double t_now = (double)step * p.dt + p.t0;
         double yd;
-        const int j = axis_cell(med.tg, med.nt, t_now, &yd);
+        const int j = axis_cell_guess(med.tg, med.nt, med.tinv, t_now, &yd);
         double xi[3];
         double ucol = 1.0, usamp_p = 0.0;
         noise_kick<3, NOISE>(a, i, step, s_off, xi, &ucol, &usamp_p);
@@ -604,7 +605,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_milne(StepArgs a, RadialMedium me
         r2 += x[1] * x[1];
         const double r = sqrt(r2);
         double xd;
-        const int ir = axis_cell(med.xg, med.nx, r, &xd);
+        const int ir = axis_cell_guess(med.xg, med.nx, med.xinv, r, &xd);
         const bool skip = r < kEps;
         double v = 0.0, gam = 1.0, rhx = 0.0, rhy = 0.0;
         if (!skip) {
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_milne(StepArgs a, RadialMedium me
             double rf = sqrt(x[0] * x[0] + x[1] * x[1]);
             if (!(rf < kEps)) {
                 double xdf;
-                int irf = axis_cell(med.xg, med.nx, rf, &xdf);
+                int irf = axis_cell_guess(med.xg, med.nx, med.xinv, rf, &xdf);
                 double vf = bilinear(med.V, med.nx, irf, j, xdf, yd);
                 vf = jl_clamp(vf, -vmax_(), vmax_());
                 double gam = 1.0 / sqrt(1.0 - vf * vf + 1e-10);
@@ -684,8 +685,8 @@ __global__ __launch_bounds__(kBlock) void k_boost_lab_radial(StepArgs a, RadialM
     if (r < kEps) return;
     const double t_now = (double)step * p.dt + p.t0;
     double yd, xd;
-    const int j = axis_cell(med.tg, med.nt, t_now, &yd);
-    const int ir = axis_cell(med.xg, med.nx, r, &xd);
+    const int j = axis_cell_guess(med.tg, med.nt, med.tinv, t_now, &yd);
+    const int ir = axis_cell_guess(med.xg, med.nx, med.xinv, r, &xd);
     double v = bilinear(med.V, med.nx, ir, j, xd, yd);
     if (DIMS >= 2 && med.V2 != nullptr) {
         double v2m = bilinear(med.V2, med.nx, ir, j, xd, yd);
@@ -1590,6 +1591,7 @@ struct lmh_ctx {
     double* d_V2 = nullptr;
     int nr = 0, nt = 0;
     double r_axis_eps = 1e-6, r_max = 0.0;
+    double r_inv = 0.0, t_inv = 0.0;  // the radial grids' equal_spacing_inv
     double4* d_nodes = nullptr;
     double4* d_slice = nullptr;
     int gx = 0, gy = 0;
@@ -1825,6 +1827,8 @@ RadialMedium make_radial(const lmh_ctx* c) {
     m.nt = c->nt;
     m.r_axis_eps = c->r_axis_eps;
     m.r_max = c->r_max;
+    m.xinv = c->r_inv;
+    m.tinv = c->t_inv;
     return m;
 }
 
@@ -2136,6 +2140,17 @@ int lmh_synchronize(lmh_ctx* c) {
     return LMH_OK;
 }
 
+// (n - 1)/(g[n-1] - g[0]) when every node lies within 1e-9 of a spacing of its place on an
+// equally spaced grid (the kernels' first guess of a cell is then off by at most one), else 0
+static double equal_spacing_inv(const double* g, int32_t n) {
+    const double span = g[n - 1] - g[0];
+    if (!(span > 0.0) || !std::isfinite(span)) return 0.0;
+    const double h = span / (double)(n - 1);
+    for (int32_t k = 0; k < n; ++k)
+        if (!(std::fabs(g[k] - (g[0] + (double)k * h)) <= 1e-9 * h)) return 0.0;
+    return (double)(n - 1) / span;
+}
+
 int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const double* tgrid,
                           int32_t nt, const double* T, const double* v, const double* v2) {
     if (!c) return LMH_E_INVALID;
@@ -2162,6 +2177,8 @@ int lmh_set_medium_radial(lmh_ctx* c, const double* rgrid, int32_t nr, const dou
     c->nt = nt;
     c->r_axis_eps = std::max(1e-12, 0.5 * std::fabs(rgrid[1] - rgrid[0]));
     c->r_max = rgrid[nr - 1];
+    c->r_inv = equal_spacing_inv(rgrid, nr);
+    c->t_inv = equal_spacing_inv(tgrid, nt);
     c->medium = 0;
     return LMH_OK;
 }

@@ -44,6 +44,35 @@ __device__ __forceinline__ int axis_cell(const double* __restrict__ g, int n, do
     return i - 1;
 }
 
+// axis_cell with a first guess for a grid of equal spacing (inv = (n - 1)/(g[n-1] - g[0]), the
+// host's check; inv = 0 for any other grid): the guessed cell l is taken where its nodes show it
+// to be the cell searchsortedlast finds -- g[l] <= q < g[l+1] (or l the last cell and g[l] <= q) --
+// and the search runs otherwise, so the result is axis_cell's for every q.  Two independent loads
+// instead of a chain of log2(n) dependent ones.
+__device__ __forceinline__ int axis_cell_guess(const double* __restrict__ g, int n, double inv, double q,
+                                               double* frac) {
+    const double qc = jl_clamp(q, g[0], g[n - 1]);
+    int i = 0;   // searchsortedlast clamped to [1, n - 1] (1-based)
+    bool found = false;
+    if (inv > 0.0) {
+        const double f = floor((qc - g[0]) * inv);
+        const int l = (int)fmin(fmax(f, 0.0), (double)(n - 2));   // a NaN query takes cell 0 here
+        const double a = g[l], b = g[l + 1];
+        if (a <= qc && (qc < b || l == n - 2)) {
+            i = l + 1;
+            found = true;
+        }
+    }
+    if (!found) {
+        i = searchsortedlast(g, n, qc);
+        i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
+    }
+    const double a = g[i - 1], b = g[i];
+    const double d = b - a;
+    *frac = fabs(d) < kEps ? 0.0 : (qc - a) / d;
+    return i - 1;
+}
+
 // The same without the query clamp and the eps guard (kernels_cpu_GC.jl:38-51)
 __device__ __forceinline__ int axis_cell_gc(const double* __restrict__ g, int n, double q, double* frac) {
     int i = searchsortedlast(g, n, q);

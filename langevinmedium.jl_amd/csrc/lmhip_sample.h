@@ -165,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_thermal_pz(const double* __restrict_
     double r2 = 0.0;
     for (int d = 0; d < dims; ++d) r2 += pos[(int64_t)dims * k + d] * pos[(int64_t)dims * k + d];
     double xd, yd;
-    const int ir = axis_cell(med.xg, med.nx, sqrt(r2), &xd);
-    const int jt = axis_cell(med.tg, med.nt, t0, &yd);
+    const int ir = axis_cell_guess(med.xg, med.nx, med.xinv, sqrt(r2), &xd);
+    const int jt = axis_cell_guess(med.tg, med.nt, med.tinv, t0, &yd);
     const double T = jl_max(bilinear(med.T, med.nx, ir, jt, xd, yd), 0.0);
     const double px = mom[2 * k], py = mom[2 * k + 1];
     const double mT = sqrt(m * m + px * px + py * py);
