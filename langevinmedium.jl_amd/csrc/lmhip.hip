@@ -746,6 +746,28 @@ __device__ __forceinline__ void slice_combine(const double4& cT, const double4& 
     *vy = fma(fma(cy.w, xd, cy.z), yd, fma(cy.y, xd, cy.x));
 }
 
+// The medium at (x, y) from a node slice (the batch's form): g.slice holds (T, vx, vy) at the
+// step's time for every grid node, one double4 per node in the nodes' row-major order.  The
+// cell's coefficients are formed from its four nodes in the order the record slices are written
+// (k_grid_slices), so the result equals grid_sample<true> on the same step's records, bit for
+// bit; the slice is a third of the records' size for a few more subtractions per quark.
+__device__ __forceinline__ void node_sample(const GridMedium& g, double x, double y, double* T, double* vx,
+                                            double* vy) {
+    int ix, iy;
+    double xd, yd;
+    cell_frac(x, g.x0, g.inv_dx, g.nx, &ix, &xd);
+    cell_frac(y, g.y0, g.inv_dy, g.ny, &iy, &yd);
+    // 32-bit byte offsets (the batch slices only grids of at most 2^27 nodes)
+    const uint32_t off = (uint32_t)(ix + g.nx * iy) * 32u;
+    const double4* r0 = (const double4*)((const char*)g.slice + off);
+    const double4* r1 = (const double4*)((const char*)g.slice + off + (uint32_t)g.nx * 32u);
+    const double4 c00 = r0[0], c10 = r0[1], c01 = r1[0], c11 = r1[1];
+    const double4 cT = make_double4(c00.x, c10.x - c00.x, c01.x - c00.x, (c11.x - c10.x) - (c01.x - c00.x));
+    const double4 cx = make_double4(c00.y, c10.y - c00.y, c01.y - c00.y, (c11.y - c10.y) - (c01.y - c00.y));
+    const double4 cy = make_double4(c00.z, c10.z - c00.z, c01.z - c00.z, (c11.z - c10.z) - (c01.z - c00.z));
+    slice_combine(cT, cx, cy, xd, yd, T, vx, vy);
+}
+
 template <bool SLICE>
 __device__ __forceinline__ void grid_sample(const GridMedium& g, double x, double y, int j,
                                             double td, double* T, double* vx, double* vy) {
@@ -860,8 +882,9 @@ __device__ __forceinline__ void glue_to_flow(double* q, double m, double vx, dou
 }
 
 // position push (kernels_cpu.jl:537-541, 586-607) with the optional position diffusion; i is
-// the particle's identity (its noise counters, the record column in fixed-noise mode)
-template <int PDIM, int NOISE, bool SLICE>
+// the particle's identity (its noise counters, the record column in fixed-noise mode).  SLICE:
+// the medium of the diffusion from the corners (0), the step's record slice (1) or node slice (2)
+template <int PDIM, int NOISE, int SLICE>
 __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p, const GridMedium& g,
                                           int64_t i, int64_t step, int s_off, int j, double td,
                                           double m, double* x0, double* x1, const double* q) {
@@ -885,7 +908,8 @@ __device__ __forceinline__ void grid_push(const StepArgs& a, const lmh_params& p
         *x1 = *x1 + p.dt * q[1] / E;
 
         double Td, ux, uy;
-        grid_sample<SLICE>(g, *x0, *x1, j, td, &Td, &ux, &uy);
+        if (SLICE == 2) node_sample(g, *x0, *x1, &Td, &ux, &uy);
+        else grid_sample<SLICE == 1>(g, *x0, *x1, j, td, &Td, &ux, &uy);
         double D = (p.DsT / jl_max(Td, kEps)) / kFmGeV;
         if (D > 0.0) {
             double xp[2];
@@ -3591,14 +3615,14 @@ struct lmh_batch {
     int tiles_block = 0;              // block size the tile table was built for
     int layout_slice_min = 0;         // ... and slice_min and steps per launch (the slice rings)
     int layout_spl = 0;
-    // per-step medium slices of the runs of at least slice_min quarks: run srun[k]'s ring of
-    // steps_per_launch slices in d_bslice, filled by k_batch_slices before each step launch
+    // per-step node slices of the runs of at least slice_min quarks: run srun[k]'s ring of
+    // batch_ring(steps_per_launch) slices in d_bslice, filled by k_batch_nodes ahead of the step
+    // launches that read them
     double4* d_bslice = nullptr;
-    size_t bslice_cap = 0;            // double4 records
+    size_t bslice_cap = 0;            // double4 nodes
     int32_t* d_srun = nullptr;
     int32_t nsr = 0;
-    int64_t max_rec = 0;              // the largest slice (records) of a sliced run
-    int64_t max_slice_tiles = 0;      // ... and its number of k_batch_slices cell tiles
+    int64_t max_node_blocks = 0;      // k_batch_nodes blocks of the largest sliced grid
     // the tiles of the sliced runs and of the others, for the launches that split them
     int32_t* d_tile_sl = nullptr;
     int32_t* d_tile_co = nullptr;
@@ -3755,41 +3779,50 @@ int batch_dev(lmh_batch* b, double** ptr, size_t* cap, size_t n, const char* wha
     return LMH_OK;
 }
 
+// the slots of a sliced run's ring: the steps of one corner-path launch, or of one k_batch_nodes
+// launch of the split path (kSliceRing steps)
+int batch_ring(int steps_per_launch) { return std::max(std::max(1, steps_per_launch), kSliceRing); }
+
+// a run reads node slices when it holds at least slice_min quarks and its grid fits node_sample's
+// 32-bit byte offsets
+bool batch_run_sliced(const lmh_run_desc& d, const lmh_batch::Med& md, int smin) {
+    return smin > 0 && d.count >= smin && (int64_t)md.nx * md.ny <= ((int64_t)1 << 27);
+}
+
 // tile table and per-run records for the current block size
 int batch_layout(lmh_batch* b, int block) {
     const int K = std::max(1, b->tune.steps_per_launch);
     const int smin = b->tune.slice_min;
     if (b->tiles_block == block && b->layout_slice_min == smin && b->layout_spl == K && b->d_runs) return LMH_OK;
     const int R = (int)b->runs.size();
+    const int ring = batch_ring(K);
     std::vector<BatchRun> h(R);
     std::vector<int32_t> tr, srun, tsl, tco;
     int64_t t0 = 0;
-    // the slice rings: K steps of 3 records per cell for every run of at least slice_min quarks
-    int64_t nrec = 0, max_rec = 0, max_tiles = 0;
+    // the slice rings: `ring` steps of one double4 per grid node for every sliced run
+    int64_t nnode = 0, max_blocks = 0;
     for (int r = 0; r < R; ++r) {
         const lmh_run_desc& d = b->runs[r];
-        if (smin <= 0 || d.count < smin) continue;
         const lmh_batch::Med& md = b->media[d.medium];
-        const int64_t rec = 3 * (int64_t)(md.nx - 1) * (md.ny - 1);
-        nrec += K * rec;
-        max_rec = std::max(max_rec, rec);
-        max_tiles = std::max<int64_t>(max_tiles, (int64_t)((md.nx - 1 + kSliceTile - 1) / kSliceTile) *
-                                                     ((md.ny - 1 + kSliceTile - 1) / kSliceTile));
+        if (!batch_run_sliced(d, md, smin)) continue;
+        const int64_t nxy = (int64_t)md.nx * md.ny;
+        nnode += ring * nxy;
+        max_blocks = std::max<int64_t>(max_blocks, (nxy + kBlock - 1) / kBlock);
         srun.push_back(r);
     }
-    if ((size_t)R * K > b->tcell_cap) {
+    if ((size_t)R * ring > b->tcell_cap) {
         dfree(b->d_tcell);
         b->tcell_cap = 0;
-        if (hipMalloc((void**)&b->d_tcell, (size_t)R * K * sizeof(double2)) != hipSuccess)
+        if (hipMalloc((void**)&b->d_tcell, (size_t)R * ring * sizeof(double2)) != hipSuccess)
             return b->fail(LMH_E_NOMEM, "batch: allocation failed");
-        b->tcell_cap = (size_t)R * K;
+        b->tcell_cap = (size_t)R * ring;
     }
-    if ((size_t)nrec > b->bslice_cap) {
+    if ((size_t)nnode > b->bslice_cap) {
         dfree(b->d_bslice);
         b->bslice_cap = 0;
-        if (hipMalloc((void**)&b->d_bslice, (size_t)nrec * sizeof(double4)) != hipSuccess)
+        if (hipMalloc((void**)&b->d_bslice, (size_t)nnode * sizeof(double4)) != hipSuccess)
             return b->fail(LMH_E_NOMEM, "batch: medium slice allocation failed");
-        b->bslice_cap = (size_t)nrec;
+        b->bslice_cap = (size_t)nnode;
     }
     int64_t soff = 0;
     for (int r = 0; r < R; ++r) {
@@ -3827,10 +3860,10 @@ int batch_layout(lmh_batch* b, int block) {
         }
         o.sp.shared_axis = tables_share_axis(o.sp.drag, o.sp.rta);
         o.DsT = d.DsT;
-        if (smin > 0 && d.count >= smin) {
-            o.slice_stride = 3 * (int64_t)(md.nx - 1) * (md.ny - 1);
+        if (batch_run_sliced(d, md, smin)) {
+            o.slice_stride = (int64_t)md.nx * md.ny;
             o.g.slice = b->d_bslice + soff;
-            soff += K * o.slice_stride;
+            soff += ring * o.slice_stride;
         }
         const int64_t nt = (d.count + block - 1) / block;
         for (int64_t k = 0; k < nt; ++k) {
@@ -3865,8 +3898,7 @@ int batch_layout(lmh_batch* b, int block) {
     b->layout_slice_min = smin;
     b->layout_spl = K;
     b->nsr = (int32_t)srun.size();
-    b->max_rec = max_rec;
-    b->max_slice_tiles = max_tiles;
+    b->max_node_blocks = max_blocks;
     return LMH_OK;
 }
 
@@ -3944,19 +3976,16 @@ bool batch_spec(const lmh_params& p) {
 }
 
 template <int PDIM>
-void batch_launch_sl(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off) {
+void batch_launch_sl(lmh_batch* b, const BatchArgs& a, int coll, int32_t k_off, int32_t slot) {
     const dim3 G((unsigned)a.ntiles), B(b->tune.block);
     switch (coll) {
-        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
-        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
-        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_FLUID>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
-        default: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off); break;
+        case LMH_LANGEVIN: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off, slot); break;
+        case LMH_RTA: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off, slot); break;
+        case LMH_FLUID: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_FLUID>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off, slot); break;
+        default: hipLaunchKernelGGL((k_batch_grid_sl<PDIM, LMH_LANGEVIN_RTA>), G, B, 0, b->stream, a, a.runs, a.tile_list, a.tile_run, k_off, slot); break;
     }
 }
 
-// steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps.
-// In the specialised configuration the runs with medium slices take k_batch_grid_sl, one step per
-// launch after their slices; the others k_batch_grid over their own tiles.
 // the tau cells of every run for the launch's n steps from k_off, into a->tcell
 void batch_tcell(lmh_batch* b, BatchArgs* a, int32_t k_off, int32_t n) {
     const int32_t R = (int32_t)b->runs.size();
@@ -3966,6 +3995,10 @@ void batch_tcell(lmh_batch* b, BatchArgs* a, int32_t k_off, int32_t n) {
     a->tcell_k = n;
 }
 
+// steps *d_step + 0 .. nsteps-1 as launches of steps_per_launch steps, then *d_step += nsteps.
+// In the specialised configuration the runs with medium slices take k_batch_grid_sl, one step per
+// launch, reading the node slices that one k_batch_nodes launch wrote for up to kSliceRing steps;
+// the others k_batch_grid over their own tiles.
 void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) {
     const int32_t k = std::max(1, b->tune.steps_per_launch);
     const bool split = b->nsr > 0 && batch_spec(a0.base.p);
@@ -3984,13 +4017,16 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) 
         BatchArgs as = a0;
         as.tile_list = b->d_tile_sl;
         as.ntiles = b->ntiles_sl;
-        const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, 1u);
-        for (int64_t s = 0; s < nsteps; ++s) {
+        const dim3 G((unsigned)b->max_node_blocks, (unsigned)b->nsr, 1u);
+        for (int64_t s0 = 0; s0 < nsteps; s0 += kSliceRing) {
+            const int32_t n = (int32_t)std::min<int64_t>(kSliceRing, nsteps - s0);
             BatchArgs at = a0;
-            batch_tcell(b, &at, (int32_t)s, 1);
-            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, at, at.runs, b->d_srun);
-            if (b->pdim == 2) batch_launch_sl<2>(b, as, coll, (int32_t)s);
-            else batch_launch_sl<3>(b, as, coll, (int32_t)s);
+            batch_tcell(b, &at, (int32_t)s0, n);
+            hipLaunchKernelGGL(k_batch_nodes, G, dim3(kBlock), 0, b->stream, at, at.runs, b->d_srun, n);
+            for (int32_t s = 0; s < n; ++s) {
+                if (b->pdim == 2) batch_launch_sl<2>(b, as, coll, (int32_t)(s0 + s), s);
+                else batch_launch_sl<3>(b, as, coll, (int32_t)(s0 + s), s);
+            }
         }
     }
     for (int64_t s0 = 0; s0 < nsteps && a.ntiles > 0; s0 += k) {
@@ -3998,8 +4034,8 @@ void batch_enqueue(lmh_batch* b, const BatchArgs& a0, int coll, int64_t nsteps) 
         batch_tcell(b, &a, (int32_t)s0, n);
         if (b->nsr > 0 && !split) {
             // the sliced runs' medium for the launch's n steps
-            const dim3 G((unsigned)b->max_slice_tiles, (unsigned)b->nsr, (unsigned)n);
-            hipLaunchKernelGGL(k_batch_slices, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun);
+            const dim3 G((unsigned)b->max_node_blocks, (unsigned)b->nsr, 1u);
+            hipLaunchKernelGGL(k_batch_nodes, G, dim3(kBlock), 0, b->stream, a, a.runs, b->d_srun, n);
         }
         if (b->pdim == 2) {
             if (b->tune.persistent) batch_launch_p<2, true>(b, a, coll, (int32_t)s0, n, grid);

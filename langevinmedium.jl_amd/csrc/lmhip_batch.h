@@ -11,8 +11,8 @@
 
 struct BatchRun {
     int64_t begin, count, tile0;
-    GridMedium g;               // g.slice: the run's ring of per-step medium slices (one per step
-                                // of a launch, slice_stride records apart), or null
+    GridMedium g;               // g.slice: the run's ring of per-step node slices (node_sample;
+                                // one per step, slice_stride nodes apart), or null
     SpeciesArgs sp;
     double DsT;
     int64_t slice_stride;
@@ -87,10 +87,11 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
     }
     double Tm, vx, vy;
     // the medium's reads are issued at wave priority 1, ahead of the other waves' arithmetic (the
-    // batch kernel is issue bound too: 1.8 % per step on 256 runs): the step's slice record where
-    // the run has slices (g.slice, wave-uniform), else the corners interpolated in tau
+    // batch kernel is issue bound too: 1.8 % per step on 256 runs): the cell's four nodes of the
+    // step's node slice where the run has slices (g.slice, wave-uniform), else the corners
+    // interpolated in tau
     __builtin_amdgcn_s_setprio(1);
-    if (SLICED || g.slice) grid_sample<true>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
+    if (SLICED || g.slice) node_sample(g, *x0, *x1, &Tm, &vx, &vy);
     else grid_sample<false>(g, *x0, *x1, j, td, &Tm, &vx, &vy);
     __builtin_amdgcn_s_setprio(0);
     const FlowBoost fb = flow_boost(vx, vy);
@@ -121,8 +122,8 @@ __device__ __forceinline__ void fused_grid_step(const StepArgs& a, const lmh_par
         }
         apply_boost<PDIM>(q, m, fb, +1, p.relativistic);
     }
-    if (SLICED || g.slice) grid_push<PDIM, LMH_NOISE_PHILOX, true>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
-    else grid_push<PDIM, LMH_NOISE_PHILOX, false>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+    if (SLICED || g.slice) grid_push<PDIM, LMH_NOISE_PHILOX, 2>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
+    else grid_push<PDIM, LMH_NOISE_PHILOX, 0>(a, p, g, i, step, 0, j, td, m, x0, x1, q);
 }
 
 // PERSIST: a fixed grid of blocks walks the tiles (tile = blockIdx.x + k * gridDim.x);
@@ -182,65 +183,42 @@ __global__ __launch_bounds__(kBlock) void k_batch_tcell(BatchArgs b, const Batch
     out[k] = make_double2((double)j, td);
 }
 
-// The slices of the sliced runs for the launch's steps *step_dev + k_off .. + nk - 1: blockIdx.y
-// picks the sliced run srun[y], blockIdx.z the step s (slice_stride records past step s - 1), and
-// blockIdx.x a tile of kSliceTile x kSliceTile cells; the tau cell and fraction come from b.tcell.  The block stages the tile's corner nodes at
-// the step's two times (T, vx, vy) in LDS, each read once from memory (a cell's four corners are
-// shared with its neighbours), then every thread writes its cell's three records: the values
-// slice_coeffs gives, in its order of operations.
-constexpr int kSliceTile = 16;
-static_assert(kSliceTile * kSliceTile == kBlock, "one thread per cell of a slice tile");
-
-__global__ __launch_bounds__(kBlock) void k_batch_slices(BatchArgs b, const BatchRun* __restrict__ runs,
-                                                         const int32_t* __restrict__ srun) {
-    constexpr int NT = kSliceTile + 1;   // nodes per tile side
-    __shared__ double nd[2][3][NT * NT];
+// The node slices of the sliced runs for nk steps from the launch's first: blockIdx.y picks the
+// sliced run srun[y], thread k of the x dimension the grid node k.  For each step s the node's
+// (T, vx, vy) at the step's time, interpolated in tau between the two time layers of the step's
+// tau cell (b.tcell) as the record slices interpolate them, into slot s of the run's ring.  A
+// thread reloads the layers only where the tau cell changes from one step to the next, so a
+// launch of nk steps reads each layer about once and writes 32 bytes per node and step.
+__global__ __launch_bounds__(kBlock) void k_batch_nodes(BatchArgs b, const BatchRun* __restrict__ runs,
+                                                        const int32_t* __restrict__ srun, int32_t nk) {
     const int r = srun[blockIdx.y];
     const BatchRun& run = runs[r];
     const GridMedium& g = run.g;
-    const int ncx = g.nx - 1, ncy = g.ny - 1;
-    const int tiles_x = (ncx + kSliceTile - 1) / kSliceTile;
-    const int ntile = tiles_x * ((ncy + kSliceTile - 1) / kSliceTile);
-    if ((int)blockIdx.x >= ntile) return;   // a smaller grid than the launch's largest
-    const int ix0 = ((int)blockIdx.x % tiles_x) * kSliceTile, iy0 = ((int)blockIdx.x / tiles_x) * kSliceTile;
-    const int64_t s = blockIdx.z;
-    const double2 jt = b.tcell[(int64_t)r * b.tcell_k + s];
-    const int j = (int)jt.x;
-    const double td = jt.y;
     const int64_t nxy = (int64_t)g.nx * g.ny;
-    const double* n0 = reinterpret_cast<const double*>(g.nodes + nxy * j);
-    const double* n1 = n0 + 4 * nxy;
-    for (int k = threadIdx.x; k < NT * NT; k += kBlock) {
-        const int ax = ix0 + k % NT, ay = iy0 + k / NT;
-        if (ax < g.nx && ay < g.ny) {
-            const int64_t o = 4 * (ax + (int64_t)g.nx * ay);
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                nd[0][f][k] = ld_global(n0 + o + f);
-                nd[1][f][k] = ld_global(n1 + o + f);
-            }
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nxy) return;   // a smaller grid than the launch's largest
+    const double2* tc = b.tcell + (int64_t)r * b.tcell_k;
+    double4* dst = const_cast<double4*>(g.slice) + k;
+    int jl = -1;
+    double4 n0 = make_double4(0.0, 0.0, 0.0, 0.0), n1 = n0;
+    for (int s = 0; s < nk; ++s) {
+        const double2 jt = tc[s];
+        const int j = (int)jt.x;
+        const double td = jt.y;
+        if (j != jl) {
+            n0 = ld_global4(g.nodes + nxy * j + k);
+            n1 = ld_global4(g.nodes + nxy * (j + 1) + k);
+            jl = j;
         }
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % kSliceTile, ty = threadIdx.x / kSliceTile;
-    const int ix = ix0 + tx, iy = iy0 + ty;
-    if (ix >= ncx || iy >= ncy) return;
-    const int k00 = tx + NT * ty;
-    double4* dst = const_cast<double4*>(g.slice) + s * run.slice_stride + 3 * (ix + (int64_t)ncx * iy);
-    const double u = 1.0 - td;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        const double c00 = nd[0][f][k00] * u + nd[1][f][k00] * td;
-        const double c10 = nd[0][f][k00 + 1] * u + nd[1][f][k00 + 1] * td;
-        const double c01 = nd[0][f][k00 + NT] * u + nd[1][f][k00 + NT] * td;
-        const double c11 = nd[0][f][k00 + NT + 1] * u + nd[1][f][k00 + NT + 1] * td;
-        dst[f] = make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
+        const double u = 1.0 - td;
+        dst[s * run.slice_stride] = make_double4(n0.x * u + n1.x * td, n0.y * u + n1.y * td,
+                                                 n0.z * u + n1.z * td, 0.0);
     }
 }
 
 // The sliced runs' step in the specialised configuration, one step per launch: block b takes the
-// tile tiles[b] (a tile of a run of at least slice_min quarks), whose run reads the step's slice
-// (the run's ring holds the one step).  fused_grid_step on the slice, at the single-run sliced
+// tile tiles[b] (a tile of a run of at least slice_min quarks), whose run reads the step's node
+// slice, slot `slot` of its ring.  fused_grid_step on the slice, at the single-run sliced
 // kernel's occupancy; the other runs take k_batch_grid.
 // The run table comes in as a read-only argument of its own, so that the block-uniform record is
 // read with scalar loads.
@@ -248,7 +226,7 @@ template <int PDIM, int COLL>
 __global__ __launch_bounds__(kBlock, kGridWaves) void k_batch_grid_sl(BatchArgs b, const BatchRun* __restrict__ runs,
                                                                       const int32_t* __restrict__ tiles,
                                                                       const int32_t* __restrict__ tile_run,
-                                                                      int32_t k_off) {
+                                                                      int32_t k_off, int32_t slot) {
     const int64_t step = *b.step_dev + k_off;
     const int64_t tile = tiles[blockIdx.x];
     const int r = tile_run[tile];
@@ -263,7 +241,9 @@ __global__ __launch_bounds__(kBlock, kGridWaves) void k_batch_grid_sl(BatchArgs 
 #pragma unroll
     for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
     const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
-    fused_grid_step<PDIM, COLL, true>(a, p, run.g, run.sp, pid, step, &x0, &x1, q);
+    GridMedium g = run.g;
+    g.slice += slot * run.slice_stride;
+    fused_grid_step<PDIM, COLL, true>(a, p, g, run.sp, pid, step, &x0, &x1, q);
     a.pos[0][i] = x0;
     a.pos[1][i] = x1;
 #pragma unroll
