@@ -234,13 +234,14 @@ typedef struct lmh_batch_tuning {
                                   cell, sorting them on the device before its first step and
                                   then every sort_every steps (rounded up to whole graph
                                   replays); the results do not depend on it; 0: never        */
-    int32_t slice_min;         /* > 0: the runs of at least slice_min quarks read their medium
-                                  from per-step slices (the grid interpolated in tau once per
-                                  step and run, as lmh_run's one-step launches do, with the
-                                  same results as such a run alone; in the specialised
-                                  configuration they step in a kernel of their own, one step
-                                  per launch); the other runs interpolate each corner in tau
-                                  per quark; 0: no slices                                      */
+    int32_t slice_min;         /* > 0: the runs of at least slice_min quarks (on grids of at
+                                  most 2^27 nodes) read their medium from per-step node slices
+                                  (each grid node interpolated in tau once per step and run,
+                                  the cell's coefficients then formed as lmh_run's one-step
+                                  launches form them, with the same results as such a run
+                                  alone; in the specialised configuration they step in a kernel
+                                  of their own, one step per launch); the other runs
+                                  interpolate each corner in tau per quark; 0: no slices       */
 } lmh_batch_tuning;
 
 typedef struct lmh_ctx lmh_ctx;
