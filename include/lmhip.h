@@ -435,7 +435,7 @@ int lmh_batch_boost_to_lab(lmh_batch* b, const lmh_params* p, int64_t step);
 int lmh_batch_run(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_t nsteps);
 /* times trial_steps steps of every launch shape (block 64/128/256 x persistent 0/1 x
  * steps_per_launch 1/2/4 x slice_min 0/2^14/2^16/2^18) on this batch, then 200 steps at the fastest shape
- * for each sort cadence (sort_every 0, 50, 200), keeps the fastest (graph_steps unchanged) and
+ * for each sort cadence (sort_every 0, 50, 100, 200), keeps the fastest (graph_steps unchanged) and
  * restores the state; the results do not depend on the block, persistence, steps per launch or
  * cadence (slice_min selects each run's form of the medium interpolation, whose results agree
  * to rounding).  Synchronises.                                                              */

@@ -3634,8 +3634,9 @@ struct lmh_batch {
 
     // cell order (tune.sort_every > 0): each run's quarks ordered by grid cell within the run's
     // segment; d_id[s] is the original slot of the quark in slot s (its Philox counter is d_id[s]
-    // minus the run's first slot).  A sort permutes into the alternate buffers and copies back,
-    // so the state's address, which a captured graph holds, never changes.
+    // minus the run's first slot).  A sort permutes into the alternate buffers and swaps them
+    // with the current ones, so the state alternates between two addresses (a captured graph
+    // for each).
     uint32_t* d_id = nullptr;
     uint32_t* d_id_alt = nullptr;
     double* d_state_alt = nullptr;
@@ -3645,10 +3646,12 @@ struct lmh_batch {
     bool is_sorted = false;
     int64_t steps_since_sort = 0;
 
-    // captured step loop: valid for `gkey` (parameters, tuning, steps per replay)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<unsigned char> gkey;
+    // captured step loops, one for each of the state's two addresses: slot k valid for gkey[k]
+    // (parameters, tuning, the state and identity addresses); glast the slot replayed last
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<unsigned char> gkey[2];
+    int glast = 0;
 
     double* d_obs = nullptr;
     size_t obs_cap = 0;
@@ -3671,12 +3674,17 @@ namespace {
             return (b)->fail(LMH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+void batch_drop_graph_slot(lmh_batch* b, int k) {
+    if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->gexec[k] = nullptr;
+    b->graph[k] = nullptr;
+    b->gkey[k].clear();
+}
+
 void batch_drop_graph(lmh_batch* b) {
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-    b->gkey.clear();
+    batch_drop_graph_slot(b, 0);
+    batch_drop_graph_slot(b, 1);
 }
 
 void batch_free_order(lmh_batch* b) {
@@ -3715,7 +3723,8 @@ int batch_sort_alloc(lmh_batch* b) {
 }
 
 // sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit), permute the
-// state and the identities through the sorted values into the alternate buffers and copy them back
+// state and the identities through the sorted values into the alternate buffers, which then
+// become the current ones
 int batch_sort_apply(lmh_batch* b, int end_bit) {
     const int64_t N = b->N;
     uint32_t* k = b->d_keys;
@@ -3726,9 +3735,8 @@ int batch_sort_apply(lmh_batch* b, int end_bit) {
     hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_state, b->d_state_alt,
                        (int32_t)rows, N, k + 3 * N, b->d_id, b->d_id_alt);
     LMB_HIP(b, hipGetLastError());
-    LMB_HIP(b, hipMemcpyAsync(b->d_state, b->d_state_alt, (size_t)rows * N * sizeof(double), hipMemcpyDeviceToDevice,
-                              b->stream));
-    LMB_HIP(b, hipMemcpyAsync(b->d_id, b->d_id_alt, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, b->stream));
+    std::swap(b->d_state, b->d_state_alt);
+    std::swap(b->d_id, b->d_id_alt);
     return LMH_OK;
 }
 
@@ -4054,26 +4062,36 @@ int batch_steps(lmh_batch* b, const lmh_params* p, const BatchArgs& a, int64_t n
     const int64_t S = b->tune.graph_steps;
     int64_t done = 0;
     if (S > 0 && nsteps >= S) {
-        // the key: parameters, tuning and steps per replay; the graph holds the kernel arguments
-        std::vector<unsigned char> key(sizeof(lmh_params) + sizeof(lmh_batch_tuning));
-        memcpy(key.data(), p, sizeof(lmh_params));
-        memcpy(key.data() + sizeof(lmh_params), &b->tune, sizeof(lmh_batch_tuning));
-        if (!b->gexec || key != b->gkey) {
-            batch_drop_graph(b);
+        // the key: parameters, tuning (steps per replay included) and the state's and identities'
+        // addresses; the graph holds the kernel arguments
+        std::vector<unsigned char> key(sizeof(lmh_params) + sizeof(lmh_batch_tuning) + 2 * sizeof(void*));
+        unsigned char* kp = key.data();
+        memcpy(kp, p, sizeof(lmh_params));
+        memcpy(kp += sizeof(lmh_params), &b->tune, sizeof(lmh_batch_tuning));
+        memcpy(kp += sizeof(lmh_batch_tuning), &b->d_state, sizeof(void*));
+        memcpy(kp + sizeof(void*), &b->d_id, sizeof(void*));
+        int k = -1;
+        for (int c = 0; c < 2; ++c)
+            if (b->gexec[c] && b->gkey[c] == key) k = c;
+        if (k < 0) {
+            // an empty slot, else the one not replayed last
+            k = !b->gexec[0] ? 0 : (!b->gexec[1] ? 1 : b->glast ^ 1);
+            batch_drop_graph_slot(b, k);
             LMB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             batch_enqueue(b, a, p->collision, S);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (e != hipSuccess) return b->fail(LMH_E_HIP, std::string("batch: graph capture: ") + hipGetErrorString(e));
-            b->graph = g;
-            e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            b->graph[k] = g;
+            e = hipGraphInstantiate(&b->gexec[k], g, nullptr, nullptr, 0);
             if (e != hipSuccess) {
-                batch_drop_graph(b);
+                batch_drop_graph_slot(b, k);
                 return b->fail(LMH_E_HIP, std::string("batch: graph instantiation: ") + hipGetErrorString(e));
             }
-            b->gkey = key;
+            b->gkey[k] = key;
         }
-        for (; done + S <= nsteps; done += S) LMB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+        b->glast = k;
+        for (; done + S <= nsteps; done += S) LMB_HIP(b, hipGraphLaunch(b->gexec[k], b->stream));
     }
     if (done < nsteps) batch_enqueue(b, a, p->collision, nsteps - done);
     LMB_HIP(b, hipGetLastError());
@@ -4084,7 +4102,6 @@ int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_
     int rc = batch_layout(b, b->tune.block);
     if (rc) return rc;
     if (b->ntiles == 0 || nsteps == 0) return LMH_OK;
-    const BatchArgs a = batch_args(b, p);
     LMB_HIP(b, hipMemcpyAsync(b->d_step, &step_begin, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     LMB_HIP(b, hipStreamSynchronize(b->stream));
     // cell order: a sort before the first step and whenever sort_every steps have passed, checked
@@ -4101,6 +4118,8 @@ int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_
             }
             chunk = std::min<int64_t>(chunk, ((E - b->steps_since_sort + S - 1) / S) * S);
         }
+        // the arguments after the sort, which moves the state to the other buffer
+        const BatchArgs a = batch_args(b, p);
         if ((rc = batch_steps(b, p, a, chunk))) return rc;
         done += chunk;
         b->steps_since_sort += chunk;
@@ -4435,10 +4454,10 @@ int lmh_batch_autotune(lmh_batch* b, const lmh_params* p, int32_t trial_steps, l
     // the sort cadence at the fastest shape: kTuneSortSteps steps from the same state for each
     // cadence, the sorts included (the first before the first step)
     constexpr int64_t kTuneSortSteps = 200;
-    const int cadences[3] = {0, 50, 200};
+    const int cadences[4] = {0, 50, 100, 200};
     double pick_sort_ms = 0.0;
     int pick_sort = 0;
-    for (int ci = 0; ci < 3 && rc == LMH_OK && b->d_id; ++ci) {
+    for (int ci = 0; ci < 4 && rc == LMH_OK && b->d_id; ++ci) {
         b->tune = pick;
         b->tune.graph_steps = 0;
         b->tune.sort_every = cadences[ci];

@@ -133,11 +133,13 @@ def test_batch_run_against_oracle(gpu_ok):
     assert np.all(rel_dev(got[3][0], opos, 1e-6) <= 1e-12)
 
 
-@pytest.mark.parametrize("spl,graph_steps,sort_every", [(1, 0, 0), (4, 0, 5), (3, 4, 0)])
+@pytest.mark.parametrize("spl,graph_steps,sort_every", [(1, 0, 0), (4, 0, 5), (3, 4, 0), (2, 3, 4)])
 def test_batch_medium_slices_equal_runs_alone(gpu_ok, spl, graph_steps, sort_every):
-    """slice_min 500: runs 3 (2500 quarks) and 4 (777) read per-step medium slices (a ring of one
-    slice per step of a launch), runs 0 and 1 the corners; every run bit for bit the same run
-    alone with the matching medium form, and the largest sliced run against the oracle at 1e-12"""
+    """slice_min 500: runs 3 (2500 quarks) and 4 (777) read per-step node slices (22 direct steps:
+    slices written for 16 steps, then 6; or graph replays, with sorts that alternate the state
+    between its two buffers and so between two captured graphs), runs 0 and 1 the corners; every
+    run bit for bit the same run alone with the record slices, and the largest sliced run against
+    the oracle at 1e-12"""
     runs = _runs(seed=17)
     steps = 22
     ref = _alone(runs, _ffi.LMH_LANGEVIN_RTA, steps, slice_min=500)
@@ -251,7 +253,7 @@ def test_batch_autotune_keeps_state_and_results(gpu_ok):
         tune, ms = b.autotune(p, trial_steps=2)
         assert tune["block"] in (64, 128, 256) and tune["persistent"] in (0, 1)
         assert tune["steps_per_launch"] in (1, 2, 4) and ms > 0.0
-        assert tune["sort_every"] in (0, 50, 200)
+        assert tune["sort_every"] in (0, 50, 100, 200)
         # no run reaches 2^14 quarks here: every threshold slices none
         assert tune["slice_min"] in (0, 1 << 14, 1 << 16, 1 << 18)
         assert b.get_tuning() == tune
