@@ -3634,9 +3634,9 @@ struct lmh_batch {
 
     // cell order (tune.sort_every > 0): each run's quarks ordered by grid cell within the run's
     // segment; d_id[s] is the original slot of the quark in slot s (its Philox counter is d_id[s]
-    // minus the run's first slot).  A sort permutes into the alternate buffers and swaps them
-    // with the current ones, so the state alternates between two addresses (a captured graph
-    // for each).
+    // minus the run's first slot).  The step after a sort reads through the sort's permutation
+    // and writes into the alternate buffers, which then become the current ones, so the state
+    // alternates between two addresses (a captured graph for each).
     uint32_t* d_id = nullptr;
     uint32_t* d_id_alt = nullptr;
     double* d_state_alt = nullptr;
@@ -3645,6 +3645,9 @@ struct lmh_batch {
     size_t sort_tmp_bytes = 0;
     bool is_sorted = false;
     int64_t steps_since_sort = 0;
+    // a sort's permutation (the sorted values d_keys[3N, 4N)) not yet applied: the next step
+    // launch reads through it (batch_gather_step), or k_permute applies it
+    bool perm_pending = false;
 
     // captured step loops, one for each of the state's two addresses: slot k valid for gkey[k]
     // (parameters, tuning, the state and identity addresses); glast the slot replayed last
@@ -3699,6 +3702,7 @@ void batch_free_order(lmh_batch* b) {
     b->d_sort_tmp = nullptr;
     b->sort_tmp_bytes = 0;
     b->is_sorted = false;
+    b->perm_pending = false;
     b->steps_since_sort = 0;
 }
 
@@ -3722,22 +3726,31 @@ int batch_sort_alloc(lmh_batch* b) {
     return LMH_OK;
 }
 
-// sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit), permute the
-// state and the identities through the sorted values into the alternate buffers, which then
-// become the current ones
-int batch_sort_apply(lmh_batch* b, int end_bit) {
+// permute the state and the identities through a pending sort's values into the alternate
+// buffers, which then become the current ones
+int batch_finish_permutation(lmh_batch* b) {
+    if (!b->perm_pending) return LMH_OK;
+    const int64_t N = b->N;
+    const int rows = 2 + b->pdim;
+    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_state, b->d_state_alt,
+                       (int32_t)rows, N, b->d_keys + 3 * N, b->d_id, b->d_id_alt);
+    LMB_HIP(b, hipGetLastError());
+    std::swap(b->d_state, b->d_state_alt);
+    std::swap(b->d_id, b->d_id_alt);
+    b->perm_pending = false;
+    return LMH_OK;
+}
+
+// sort the keys d_keys[0, N) with the values d_keys[2N, 3N) over bits [0, end_bit); the sorted
+// values are the permutation, applied by the next step launch (defer) or here
+int batch_sort_apply(lmh_batch* b, int end_bit, bool defer) {
     const int64_t N = b->N;
     uint32_t* k = b->d_keys;
     size_t bytes = b->sort_tmp_bytes;
     LMB_HIP(b, hipcub::DeviceRadixSort::SortPairs(b->d_sort_tmp, bytes, k, k + N, k + 2 * N, k + 3 * N, (int)N, 0,
                                                    end_bit, b->stream));
-    const int rows = 2 + b->pdim;
-    hipLaunchKernelGGL(k_permute, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_state, b->d_state_alt,
-                       (int32_t)rows, N, k + 3 * N, b->d_id, b->d_id_alt);
-    LMB_HIP(b, hipGetLastError());
-    std::swap(b->d_state, b->d_state_alt);
-    std::swap(b->d_id, b->d_id_alt);
-    return LMH_OK;
+    b->perm_pending = true;
+    return defer ? LMH_OK : batch_finish_permutation(b);
 }
 
 BatchArgs batch_args(lmh_batch* b, const lmh_params* p);
@@ -3755,7 +3768,7 @@ int batch_sort(lmh_batch* b, const lmh_params* p) {
     hipLaunchKernelGGL(k_batch_sort_keys, dim3(G), dim3(b->tune.block), 0, b->stream, a, (int32_t)cell_bits,
                        b->d_keys, b->d_keys + 2 * b->N);
     LMB_HIP(b, hipGetLastError());
-    if ((rc = batch_sort_apply(b, end_bit))) return rc;
+    if ((rc = batch_sort_apply(b, end_bit, true))) return rc;
     b->is_sorted = true;
     b->steps_since_sort = 0;
     return LMH_OK;
@@ -3763,12 +3776,13 @@ int batch_sort(lmh_batch* b, const lmh_params* p) {
 
 // restore the caller's order (a no-op unless the runs are cell-sorted)
 int batch_ensure_unsorted(lmh_batch* b) {
+    if (int rc = batch_finish_permutation(b)) return rc;
     if (!b->is_sorted) return LMH_OK;
     const int64_t N = b->N;
     hipLaunchKernelGGL(k_unsort_keys, dim3(grid_blocks(N)), dim3(kBlock), 0, b->stream, b->d_id, N, b->d_keys,
                        b->d_keys + 2 * N);
     LMB_HIP(b, hipGetLastError());
-    int rc = batch_sort_apply(b, bits_for((uint64_t)N));
+    int rc = batch_sort_apply(b, bits_for((uint64_t)N), false);
     if (rc) return rc;
     b->is_sorted = false;
     return LMH_OK;
@@ -4117,6 +4131,26 @@ int batch_run_impl(lmh_batch* b, const lmh_params* p, int64_t step_begin, int64_
                 if ((rc = batch_sort(b, p))) return rc;
             }
             chunk = std::min<int64_t>(chunk, ((E - b->steps_since_sort + S - 1) / S) * S);
+        }
+        if (b->perm_pending) {
+            // the first step after the sort, launched directly, reads every quark through the
+            // permutation and writes the new order into the alternate buffers, which then become
+            // the current ones
+            BatchArgs g = batch_args(b, p);
+            for (int d = 0; d < 3; ++d) {
+                if (g.base.wpos[d]) g.base.wpos[d] = b->d_state_alt + (g.base.pos[d] - b->d_state);
+                if (g.base.wmom[d]) g.base.wmom[d] = b->d_state_alt + (g.base.mom[d] - b->d_state);
+            }
+            g.base.perm = b->d_keys + 3 * b->N;
+            g.base.wid = b->d_id_alt;
+            batch_enqueue(b, g, p->collision, 1);
+            LMB_HIP(b, hipGetLastError());
+            std::swap(b->d_state, b->d_state_alt);
+            std::swap(b->d_id, b->d_id_alt);
+            b->perm_pending = false;
+            done += 1;
+            b->steps_since_sort += 1;
+            if (--chunk == 0) continue;
         }
         // the arguments after the sort, which moves the state to the other buffer
         const BatchArgs a = batch_args(b, p);

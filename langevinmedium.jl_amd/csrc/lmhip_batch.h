@@ -47,13 +47,15 @@ __device__ __forceinline__ T uniform_read(const T* p) {
     return out;
 }
 
-// this tile's run as a single-run view: state rows and Philox stream of the run
+// this tile's run as a single-run view: state rows (read and written) and Philox stream of the run
 __device__ __forceinline__ void batch_view(const BatchArgs& b, const BatchRun& run, int r, StepArgs* a) {
     *a = b.base;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (a->pos[d]) a->pos[d] += run.begin;
         if (a->mom[d]) a->mom[d] += run.begin;
+        if (a->wpos[d]) a->wpos[d] += run.begin;
+        if (a->wmom[d]) a->wmom[d] += run.begin;
     }
     a->N = run.count;
     a->p.DsT = run.DsT;
@@ -146,13 +148,17 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
         batch_view(b, run, r, &a);
         const lmh_params p = step_params<SPEC>(a.p);
         const SpeciesArgs sp = run.sp;
-        double x0 = a.pos[0][i], x1 = a.pos[1][i];
+        // the launch after a sort (b.base.perm set, wave-uniform) reads slot i's quark from slot
+        // src of the run's previous order and writes it, with its identity, to slot i of the
+        // other buffers; every other launch reads and writes slot i in place
+        const int64_t src = b.base.perm ? (int64_t)b.base.perm[run.begin + i] - run.begin : i;
+        double x0 = a.pos[0][src], x1 = a.pos[1][src];
         double q[PDIM];
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
+        for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][src];
         // the quark's identity keys its noise: its original slot in the run (the slot itself
         // unless the run is ordered by grid cell)
-        const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
+        const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + src] - run.begin : src;
         int64_t bad = -1;
         GridMedium gs = run.g;
         const double2* tc = b.tcell ? b.tcell + (int64_t)r * b.tcell_k : nullptr;
@@ -161,10 +167,11 @@ __global__ __launch_bounds__(kBlock, kBatchWaves) void k_batch_grid(BatchArgs b,
             if (gs.slice) gs.slice += run.slice_stride;
             if (bad < 0 && !(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) bad = step0 + s;
         }
-        a.pos[0][i] = x0;
-        a.pos[1][i] = x1;
+        a.wpos[0][i] = x0;
+        a.wpos[1][i] = x1;
 #pragma unroll
-        for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+        for (int d = 0; d < PDIM; ++d) a.wmom[d][i] = q[d];
+        if (b.base.perm) b.base.wid[run.begin + i] = (uint32_t)(pid + run.begin);
         if (bad >= 0) atomicMin(b.base.bad, (unsigned long long)bad);
     }
 }
@@ -236,18 +243,21 @@ __global__ __launch_bounds__(kBlock, kGridWaves) void k_batch_grid_sl(BatchArgs 
     StepArgs a;
     batch_view(b, run, r, &a);
     const lmh_params p = step_params<true>(a.p);
-    double x0 = a.pos[0][i], x1 = a.pos[1][i];
+    // read through the sort's permutation after a sort, as in k_batch_grid
+    const int64_t src = b.base.perm ? (int64_t)b.base.perm[run.begin + i] - run.begin : i;
+    double x0 = a.pos[0][src], x1 = a.pos[1][src];
     double q[PDIM];
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][i];
-    const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + i] - run.begin : i;
+    for (int d = 0; d < PDIM; ++d) q[d] = a.mom[d][src];
+    const int64_t pid = b.base.id ? (int64_t)b.base.id[run.begin + src] - run.begin : src;
     GridMedium g = run.g;
     g.slice += slot * run.slice_stride;
     fused_grid_step<PDIM, COLL, true>(a, p, g, run.sp, pid, step, &x0, &x1, q);
-    a.pos[0][i] = x0;
-    a.pos[1][i] = x1;
+    a.wpos[0][i] = x0;
+    a.wpos[1][i] = x1;
 #pragma unroll
-    for (int d = 0; d < PDIM; ++d) a.mom[d][i] = q[d];
+    for (int d = 0; d < PDIM; ++d) a.wmom[d][i] = q[d];
+    if (b.base.perm) b.base.wid[run.begin + i] = (uint32_t)(pid + run.begin);
     if (!(isfinite(x0) && isfinite(x1) && finite_all(q, PDIM))) atomicMin(b.base.bad, (unsigned long long)step);
 }
 
