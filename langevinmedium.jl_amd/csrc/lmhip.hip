@@ -960,31 +960,6 @@ __device__ __forceinline__ double4 slice_coeffs(const GridMedium& g, int j, doub
     return make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));
 }
 
-// the three records of cell k (T, vx, vy; each the value slice_coeffs gives), from the cell's
-// four corners at the two times read as whole 32-byte nodes
-__device__ __forceinline__ void slice_cell_records(const GridMedium& g, int j, double td, int64_t k, double4* out) {
-    const int64_t ncx = g.nx - 1;
-    const int64_t ix = k % ncx, iy = k / ncx;
-    const int64_t nxy = (int64_t)g.nx * g.ny;
-    const double4* s0 = g.nodes + nxy * j + ix + (int64_t)g.nx * iy;
-    const double4* s1 = s0 + nxy;
-    const double4 a00 = ld_global4(s0), a10 = ld_global4(s0 + 1), a01 = ld_global4(s0 + g.nx),
-                  a11 = ld_global4(s0 + g.nx + 1);
-    const double4 b00 = ld_global4(s1), b10 = ld_global4(s1 + 1), b01 = ld_global4(s1 + g.nx),
-                  b11 = ld_global4(s1 + g.nx + 1);
-    const double u = 1.0 - td;
-#define LMH_SLICE_FIELD(F, o)                                                                            \
-    {                                                                                                    \
-        const double c00 = a00.F * u + b00.F * td, c10 = a10.F * u + b10.F * td;                         \
-        const double c01 = a01.F * u + b01.F * td, c11 = a11.F * u + b11.F * td;                         \
-        out[o] = make_double4(c00, c10 - c00, c01 - c00, (c11 - c10) - (c01 - c00));                    \
-    }
-    LMH_SLICE_FIELD(x, 0)
-    LMH_SLICE_FIELD(y, 1)
-    LMH_SLICE_FIELD(z, 2)
-#undef LMH_SLICE_FIELD
-}
-
 // the slices of steps step0 .. step0 + nk - 1 (time cell of each step as the step kernels find it)
 __global__ __launch_bounds__(kBlock) void k_grid_slices(GridMedium g, double4* slices, int64_t step0,
                                                         int32_t nk, double dt, double t0) {

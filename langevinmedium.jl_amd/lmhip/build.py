@@ -18,7 +18,7 @@ REPO_ROOT = os.path.dirname(PKG_ROOT)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 LIB_PATH = os.path.join(PKG_ROOT, "liblmhip.so")
 SOURCES = [os.path.join(CSRC, "lmhip.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("lmhip_device.h", "lmhip_adjoint.h", "lmhip_batch.h")] + [
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("lmhip_device.h", "lmhip_adjoint.h", "lmhip_batch.h", "lmhip_sample.h")] + [
     os.path.join(REPO_ROOT, "include", "lmhip.h")]
 
 HIPCC_FLAGS = [
